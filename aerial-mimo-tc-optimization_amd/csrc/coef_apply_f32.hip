@@ -1,0 +1,202 @@
+// coef_apply_f32.hip -- FP32 coefficient apply with the antenna-serial fused multiply-add chain, and the
+// FP32 Gram product. Both reproduce the rounding sequence of the reference scalar kernels exactly
+// (coef_apply_v1_scalar, aerial_coef_wmma.cu:71-104; gram_scalar, aerial_bottleneck.cu:56-80): every output
+// is one chain over a = 0..A-1 in ascending order.
+//
+// Layout of the work (gfx950, wave64): a workgroup of 256 lanes owns SCB subcarriers. Each lane owns a 4x4
+// register tile (4 layers x 4 symbols) of one subcarrier, so a subcarrier needs ceil(L/4)*ceil(S/4) lanes
+// (6 at L=8, S=11). C and Y_rx rows are staged through LDS in chunks of KC antennas with coalesced
+// 128-bit loads (each 128-bit load is two complex samples), so every HBM byte is read once and each staged
+// sample is reused by 4 tiles from LDS instead of being re-fetched through the vector cache.
+#include "pe_device.hpp"
+#include "pe_internal.hpp"
+
+namespace pe {
+
+namespace {
+
+constexpr int kF32Threads = 256;
+constexpr int kF32MaxLds = 64 * 1024;
+
+template <int KC, int NPB>
+__global__ __launch_bounds__(kF32Threads) void coef_apply_f32_kernel(
+    const float2 *__restrict__ C, const float2 *__restrict__ Y, float2 *__restrict__ O,
+    const float *__restrict__ noise_inv, float *__restrict__ llr, int nSC, int L, int S, int A, int nLG,
+    int nSG, int SCB) {
+    constexpr int RS = KC + 2;  // LDS row stride in complex samples (padding staggers LDS banks)
+    extern __shared__ float4 lds_raw[];
+    float2 *lds = reinterpret_cast<float2 *>(lds_raw);
+
+    const int R = L + S;  // staged rows per subcarrier: L coefficient rows then S sample rows
+    const int tid = threadIdx.x;
+    const int tps = nLG * nSG;
+    const int sc0 = blockIdx.x * SCB;
+    const int nsc = min(SCB, nSC - sc0);
+
+    const int scl = tid / tps;
+    const int rem = tid - scl * tps;
+    const int lg = rem / nSG;
+    const int sg = rem - lg * nSG;
+    const bool active = scl < nsc;
+
+    int crow[4], yrow[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        crow[i] = min(4 * lg + i, L - 1);
+        yrow[i] = L + min(4 * sg + i, S - 1);
+    }
+
+    float2 acc[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] = make_float2(0.f, 0.f);
+
+    constexpr int F4_PER_ROW = KC / 2;
+    const int nf4 = nsc * R * F4_PER_ROW;
+    for (int k0 = 0; k0 < A; k0 += KC) {
+        const int kn = min(KC, A - k0);  // multiple of 4 (A % 4 == 0)
+        for (int q = tid; q < nf4; q += kF32Threads) {
+            const int row = q / F4_PER_ROW;
+            const int part = q - row * F4_PER_ROW;
+            const int s_l = row / R;
+            const int rr = row - s_l * R;
+            const size_t sc = (size_t)(sc0 + s_l);
+            float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+            if (2 * part < kn) {
+                const float2 *src = (rr < L) ? C + (sc * L + rr) * A : Y + (sc * S + (rr - L)) * A;
+                v = *reinterpret_cast<const float4 *>(src + k0 + 2 * part);
+            }
+            *reinterpret_cast<float4 *>(lds + (s_l * R + rr) * RS + 2 * part) = v;
+        }
+        __syncthreads();
+        if (active) {
+            const float2 *base = lds + scl * R * RS;
+            if (kn == KC) {
+#pragma unroll
+                for (int k = 0; k < KC; ++k) {
+                    float2 c[4], y[4];
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) c[i] = base[crow[i] * RS + k];
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) y[j] = base[yrow[j] * RS + k];
+#pragma unroll
+                    for (int i = 0; i < 4; ++i)
+#pragma unroll
+                        for (int j = 0; j < 4; ++j) acc[i][j] = cma(c[i], y[j], acc[i][j]);
+                }
+            } else {
+                for (int k = 0; k < kn; ++k) {
+                    float2 c[4], y[4];
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) c[i] = base[crow[i] * RS + k];
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) y[j] = base[yrow[j] * RS + k];
+#pragma unroll
+                    for (int i = 0; i < 4; ++i)
+#pragma unroll
+                        for (int j = 0; j < 4; ++j) acc[i][j] = cma(c[i], y[j], acc[i][j]);
+                }
+            }
+        }
+        __syncthreads();
+    }
+    if (!active) return;
+
+    const size_t sc = (size_t)(sc0 + scl);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const int l = 4 * lg + i;
+        if (l >= L) continue;
+        if constexpr (NPB == 0) {
+            float2 *o = O + (sc * L + l) * S;
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const int s = 4 * sg + j;
+                if (s < S) o[s] = acc[i][j];
+            }
+        } else {
+            const float ni = noise_inv[sc * L + l];
+            float *dst = llr + (sc * L + l) * S * (2 * NPB);
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const int s = 4 * sg + j;
+                if (s < S) {
+                    float v[2 * NPB];
+                    qam_llr<NPB>(acc[i][j].x, acc[i][j].y, ni, v);
+                    store_llr<NPB>(dst + s * (2 * NPB), v);
+                }
+            }
+        }
+    }
+}
+
+// Gram product, one lane per (sc, r, c); M rows and H columns are read through the vector cache (every
+// element of M[sc] is used by L lanes of the same wave, every element of H[sc] by L lanes).
+__global__ __launch_bounds__(256) void gram_f32_kernel(const float2 *__restrict__ M, const float2 *__restrict__ H,
+                                                       float2 *__restrict__ G, int nSC, int L, int A) {
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int per = L * L;
+    const long sc = t / per;
+    if (sc >= nSC) return;
+    const int rc = (int)(t - sc * per);
+    const int r = rc / L;
+    const int c = rc - r * L;
+    const float2 *m = M + ((size_t)sc * L + r) * A;
+    const float2 *h = H + (size_t)sc * A * L + c;
+    float2 acc = make_float2(0.f, 0.f);
+    for (int k = 0; k < A; ++k) acc = cma(m[k], h[(size_t)k * L], acc);
+    G[((size_t)sc * L + r) * L + c] = acc;
+}
+
+template <int KC, int NPB>
+hipError_t launch_f32_impl(const float2 *C, const float2 *Y, float2 *O, const float *ni, float *llr, int nSC,
+                           int L, int S, int A, hipStream_t st) {
+    constexpr int RS = KC + 2;
+    const int nLG = (L + 3) / 4, nSG = (S + 3) / 4;
+    const int tps = nLG * nSG;
+    const int rowBytes = (L + S) * RS * (int)sizeof(float2);
+    int scb = kF32Threads / tps;
+    scb = min(scb, kF32MaxLds / rowBytes);
+    if (scb < 1) return hipErrorInvalidConfiguration;
+    const int grid = (nSC + scb - 1) / scb;
+    const size_t lds = (size_t)scb * rowBytes;
+    hipLaunchKernelGGL((coef_apply_f32_kernel<KC, NPB>), dim3(grid), dim3(kF32Threads), lds, st, C, Y, O, ni,
+                       llr, nSC, L, S, A, nLG, nSG, scb);
+    return hipGetLastError();
+}
+
+template <int NPB>
+hipError_t launch_f32_kc(const float2 *C, const float2 *Y, float2 *O, const float *ni, float *llr, int nSC, int L,
+                         int S, int A, hipStream_t st) {
+    const CoefTile &t = coef_tile_for(L, S, A, 0);
+    if (t.f32_kc == 16) return launch_f32_impl<16, NPB>(C, Y, O, ni, llr, nSC, L, S, A, st);
+    return launch_f32_impl<8, NPB>(C, Y, O, ni, llr, nSC, L, S, A, st);
+}
+
+}  // namespace
+
+hipError_t launch_coef_apply_f32(const float2 *C, const float2 *Y, float2 *O, int nSC, int L, int S, int A,
+                                 hipStream_t st) {
+    return launch_f32_kc<0>(C, Y, O, nullptr, nullptr, nSC, L, S, A, st);
+}
+
+hipError_t launch_coef_apply_demap_f32(const float2 *C, const float2 *Y, const float *noise_inv, float *llr,
+                                       int nSC, int L, int S, int A, int nPamBits, hipStream_t st) {
+    switch (nPamBits) {
+    case 1: return launch_f32_kc<1>(C, Y, nullptr, noise_inv, llr, nSC, L, S, A, st);
+    case 2: return launch_f32_kc<2>(C, Y, nullptr, noise_inv, llr, nSC, L, S, A, st);
+    case 3: return launch_f32_kc<3>(C, Y, nullptr, noise_inv, llr, nSC, L, S, A, st);
+    case 4: return launch_f32_kc<4>(C, Y, nullptr, noise_inv, llr, nSC, L, S, A, st);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t launch_gram_f32(const float2 *M, const float2 *H, float2 *G, int nSC, int L, int A, hipStream_t st) {
+    const long total = (long)nSC * L * L;
+    const int grid = (int)((total + 255) / 256);
+    hipLaunchKernelGGL(gram_f32_kernel, dim3(grid), dim3(256), 0, st, M, H, G, nSC, L, A);
+    return hipGetLastError();
+}
+
+}  // namespace pe

@@ -1,0 +1,246 @@
+// pe_api.cpp -- the C ABI (include/pusch_eq.h): argument validation, error reporting, tile-table lookup and
+// dispatch to the kernel launchers. The reference has no library layer: each benchmark main() launches its
+// kernels directly and exits on the first CUDA error (aerial_coef_wmma.cu:57-58); here every failure is a
+// status code plus a thread-local message.
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/pusch_eq.h"
+#include "pe_internal.hpp"
+
+namespace {
+
+thread_local std::string g_err;
+
+pe_status fail(pe_status s, const char *fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return s;
+}
+
+pe_status ok() {
+    g_err.clear();
+    return PE_OK;
+}
+
+pe_status from_hip(hipError_t e, const char *what) {
+    if (e == hipSuccess) return ok();
+    return fail(PE_ERR_DEVICE, "%s: %s", what, hipGetErrorString(e));
+}
+
+bool valid_prec(int p) { return p >= PE_PREC_F32 && p <= PE_PREC_BF16_SPLIT; }
+
+// ---- tile table --------------------------------------------------------------------------------------
+struct TileEntry {
+    int L, S, A, prec;  // -1 = wildcard
+    pe::CoefTile tile;
+};
+std::mutex g_table_mu;
+std::vector<TileEntry> g_table;  // most specific match wins (first match in file order)
+pe::CoefTile g_default_tile;
+
+int load_table_file(const char *path) {
+    FILE *f = fopen(path, "r");
+    if (!f) return -1;
+    std::vector<TileEntry> t;
+    char line[256];
+    while (fgets(line, sizeof(line), f)) {
+        if (line[0] == '#' || line[0] == '\n') continue;
+        TileEntry e;
+        int kc = 0, waves = 0;
+        // format: coef <L> <S> <A> <prec> <f32_kc> <mfma_waves>   (-1 = any)
+        if (sscanf(line, "coef %d %d %d %d %d %d", &e.L, &e.S, &e.A, &e.prec, &kc, &waves) == 6) {
+            e.tile.f32_kc = kc;
+            e.tile.mfma_waves = waves;
+            t.push_back(e);
+        }
+    }
+    fclose(f);
+    std::lock_guard<std::mutex> lk(g_table_mu);
+    g_table.swap(t);
+    return (int)g_table.size();
+}
+
+bool check_ptrs(const void *a, const void *b, const void *c) { return a && b && c; }
+
+}  // namespace
+
+namespace pe {
+CoefTile coef_tile_for(int L, int S, int A, int prec) {
+    std::lock_guard<std::mutex> lk(g_table_mu);
+    for (const TileEntry &e : g_table) {
+        if ((e.L < 0 || e.L == L) && (e.S < 0 || e.S == S) && (e.A < 0 || e.A == A) &&
+            (e.prec < 0 || e.prec == prec))
+            return e.tile;
+    }
+    return g_default_tile;
+}
+}  // namespace pe
+
+extern "C" {
+
+const char *pe_version(void) { return "pusch_eq 0.1 (gfx950)"; }
+
+const char *pe_last_error(void) { return g_err.c_str(); }
+
+const char *pe_status_string(pe_status s) {
+    switch (s) {
+    case PE_OK: return "ok";
+    case PE_ERR_INVALID: return "invalid argument";
+    case PE_ERR_UNSUPPORTED: return "unsupported";
+    case PE_ERR_DEVICE: return "device error";
+    case PE_ERR_NOMEM: return "out of memory";
+    case PE_ERR_CAPACITY: return "capacity exceeded";
+    }
+    return "unknown status";
+}
+
+pe_status pe_init(int device) {
+    hipError_t e = hipSetDevice(device);
+    if (e != hipSuccess) return from_hip(e, "hipSetDevice");
+    hipDeviceProp_t prop;
+    e = hipGetDeviceProperties(&prop, device);
+    if (e != hipSuccess) return from_hip(e, "hipGetDeviceProperties");
+    if (strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+        return fail(PE_ERR_UNSUPPORTED, "device %d is %s; this library is built for gfx950 only", device,
+                    prop.gcnArchName);
+    const char *path = getenv("PE_TUNE_TABLE");
+    if (path && *path && load_table_file(path) < 0)
+        return fail(PE_ERR_INVALID, "cannot read tile table %s", path);
+    return ok();
+}
+
+// Internal (not in the public header): load / clear the tile table, used by the autotuner.
+int pe_tune_load(const char *path) {
+    if (!path) {
+        std::lock_guard<std::mutex> lk(g_table_mu);
+        g_table.clear();
+        return 0;
+    }
+    return load_table_file(path);
+}
+
+static pe_status check_coef_shape(int nSC, int L, int S, int A, int prec) {
+    if (nSC < 0) return fail(PE_ERR_INVALID, "nSC = %d < 0", nSC);
+    if (L < 1 || L > 8) return fail(PE_ERR_INVALID, "nLayer = %d outside [1, 8]", L);
+    if (S < 1) return fail(PE_ERR_INVALID, "nSym = %d < 1", S);
+    if (S > 16) return fail(PE_ERR_UNSUPPORTED, "nSym = %d > 16", S);
+    if (A < 4 || A % 4 != 0) return fail(PE_ERR_INVALID, "nAnt = %d must be a positive multiple of 4", A);
+    if (!valid_prec(prec)) return fail(PE_ERR_INVALID, "unknown precision %d", prec);
+    if (prec == PE_PREC_F32 && A > 256) return fail(PE_ERR_UNSUPPORTED, "nAnt = %d > 256", A);
+    if (prec != PE_PREC_F32 && A > 128) return fail(PE_ERR_UNSUPPORTED, "nAnt = %d > 128 on MFMA", A);
+    return PE_OK;
+}
+
+pe_status pe_coef_apply(const pe_c64 *C, const pe_c64 *Y_rx, pe_c64 *Y_eq, int nSC, int nLayer, int nSym,
+                        int nAnt, pe_prec prec, void *stream) {
+    pe_status s = check_coef_shape(nSC, nLayer, nSym, nAnt, (int)prec);
+    if (s != PE_OK) return s;
+    if (nSC == 0) return ok();
+    if (!check_ptrs(C, Y_rx, Y_eq)) return fail(PE_ERR_INVALID, "null pointer");
+    hipStream_t st = (hipStream_t)stream;
+    const float2 *c = reinterpret_cast<const float2 *>(C);
+    const float2 *y = reinterpret_cast<const float2 *>(Y_rx);
+    float2 *o = reinterpret_cast<float2 *>(Y_eq);
+    hipError_t e = (prec == PE_PREC_F32)
+                       ? pe::launch_coef_apply_f32(c, y, o, nSC, nLayer, nSym, nAnt, st)
+                       : pe::launch_coef_apply_mfma(c, y, o, nullptr, nullptr, nSC, nLayer, nSym, nAnt, 0,
+                                                    (int)prec, st);
+    return from_hip(e, "coef_apply launch");
+}
+
+pe_status pe_coef_apply_v1(const pe_c64 *C, const pe_c64 *Y_rx, pe_c64 *Y_eq, int nSC, int nSym, void *stream) {
+    return pe_coef_apply(C, Y_rx, Y_eq, nSC, 8, nSym, 64, PE_PREC_F32, stream);
+}
+
+pe_status pe_coef_apply_range(const pe_c64 *C, const pe_c64 *Y_rx, pe_c64 *Y_eq, int nSC, int scOffset,
+                              int nLayer, int nSym, int nAnt, pe_prec prec, void *stream) {
+    if (scOffset < 0) return fail(PE_ERR_INVALID, "scOffset = %d < 0", scOffset);
+    pe_status s = check_coef_shape(nSC, nLayer, nSym, nAnt, (int)prec);
+    if (s != PE_OK) return s;
+    if (nSC == 0) return ok();
+    if (!check_ptrs(C, Y_rx, Y_eq)) return fail(PE_ERR_INVALID, "null pointer");
+    const size_t off = (size_t)scOffset;
+    return pe_coef_apply(C + off * nLayer * nAnt, Y_rx + off * nSym * nAnt, Y_eq + off * nLayer * nSym, nSC,
+                         nLayer, nSym, nAnt, prec, stream);
+}
+
+pe_status pe_gram(const pe_c64 *M, const pe_c64 *H, pe_c64 *G, int nSC, int nLayer, int nAnt, void *stream) {
+    if (nSC < 0) return fail(PE_ERR_INVALID, "nSC = %d < 0", nSC);
+    if (nLayer < 1 || nLayer > 8) return fail(PE_ERR_INVALID, "nLayer = %d outside [1, 8]", nLayer);
+    if (nAnt < 1 || nAnt > 1024) return fail(PE_ERR_INVALID, "nAnt = %d outside [1, 1024]", nAnt);
+    if (nSC == 0) return ok();
+    if (!check_ptrs(M, H, G)) return fail(PE_ERR_INVALID, "null pointer");
+    hipError_t e = pe::launch_gram_f32(reinterpret_cast<const float2 *>(M), reinterpret_cast<const float2 *>(H),
+                                       reinterpret_cast<float2 *>(G), nSC, nLayer, nAnt, (hipStream_t)stream);
+    return from_hip(e, "gram launch");
+}
+
+pe_status pe_gram_mmse(const pe_c64 *H, const float *sigma2, pe_c64 *C, float *noise_inv, uint8_t *sc_flags,
+                       int nSC, int nLayer, int nAnt, unsigned flags, float cond_threshold, void *stream) {
+    if (nSC < 0) return fail(PE_ERR_INVALID, "nSC = %d < 0", nSC);
+    if (nLayer < 1 || nLayer > 8) return fail(PE_ERR_INVALID, "nLayer = %d outside [1, 8]", nLayer);
+    if (nAnt < 1 || nAnt > 1024) return fail(PE_ERR_INVALID, "nAnt = %d outside [1, 1024]", nAnt);
+    if (flags & ~PE_MMSE_UNBIASED) return fail(PE_ERR_INVALID, "unknown flags 0x%x", flags);
+    if (nSC == 0) return ok();
+    if (!check_ptrs(H, sigma2, C) || !noise_inv || !sc_flags) return fail(PE_ERR_INVALID, "null pointer");
+    const float thr = cond_threshold > 0.f ? cond_threshold : 1e4f;
+    hipError_t e = pe::launch_gram_mmse(reinterpret_cast<const float2 *>(H), sigma2, reinterpret_cast<float2 *>(C),
+                                        noise_inv, sc_flags, nSC, nLayer, nAnt, (flags & PE_MMSE_UNBIASED) != 0,
+                                        thr, (hipStream_t)stream);
+    return from_hip(e, "gram_mmse launch");
+}
+
+pe_status pe_llr_demap_f16(const pe_c16 *softEst, const float *noiseInv, uint16_t *llrOut, int N, int nPamBits,
+                           void *stream) {
+    if (N < 0) return fail(PE_ERR_INVALID, "N = %d < 0", N);
+    if (nPamBits < 1 || nPamBits > 4) return fail(PE_ERR_INVALID, "nPamBits = %d outside [1, 4]", nPamBits);
+    if (N == 0) return ok();
+    if (!check_ptrs(softEst, noiseInv, llrOut)) return fail(PE_ERR_INVALID, "null pointer");
+    hipError_t e = pe::launch_llr_f16(reinterpret_cast<const uint32_t *>(softEst), noiseInv, llrOut, N, nPamBits,
+                                      (hipStream_t)stream);
+    return from_hip(e, "llr_demap_f16 launch");
+}
+
+pe_status pe_llr_demap(const pe_c64 *Y_eq, const float *noise_inv, float *llr, int nSC, int nLayer, int nSym,
+                       int nPamBits, void *stream) {
+    if (nSC < 0) return fail(PE_ERR_INVALID, "nSC = %d < 0", nSC);
+    if (nLayer < 1 || nLayer > 8) return fail(PE_ERR_INVALID, "nLayer = %d outside [1, 8]", nLayer);
+    if (nSym < 1 || nSym > 16) return fail(PE_ERR_INVALID, "nSym = %d outside [1, 16]", nSym);
+    if (nPamBits < 1 || nPamBits > 4) return fail(PE_ERR_INVALID, "nPamBits = %d outside [1, 4]", nPamBits);
+    if (nSC == 0) return ok();
+    if (!check_ptrs(Y_eq, noise_inv, llr)) return fail(PE_ERR_INVALID, "null pointer");
+    hipError_t e = pe::launch_llr_f32(reinterpret_cast<const float2 *>(Y_eq), noise_inv, llr, nSC, nLayer, nSym,
+                                      nPamBits, (hipStream_t)stream);
+    return from_hip(e, "llr_demap launch");
+}
+
+pe_status pe_coef_apply_demap(const pe_c64 *C, const pe_c64 *Y_rx, const float *noise_inv, float *llr, int nSC,
+                              int nLayer, int nSym, int nAnt, int nPamBits, pe_prec prec, void *stream) {
+    pe_status s = check_coef_shape(nSC, nLayer, nSym, nAnt, (int)prec);
+    if (s != PE_OK) return s;
+    if (nPamBits < 1 || nPamBits > 4) return fail(PE_ERR_INVALID, "nPamBits = %d outside [1, 4]", nPamBits);
+    if (nSC == 0) return ok();
+    if (!check_ptrs(C, Y_rx, noise_inv) || !llr) return fail(PE_ERR_INVALID, "null pointer");
+    hipStream_t st = (hipStream_t)stream;
+    const float2 *c = reinterpret_cast<const float2 *>(C);
+    const float2 *y = reinterpret_cast<const float2 *>(Y_rx);
+    hipError_t e = (prec == PE_PREC_F32)
+                       ? pe::launch_coef_apply_demap_f32(c, y, noise_inv, llr, nSC, nLayer, nSym, nAnt, nPamBits, st)
+                       : pe::launch_coef_apply_mfma(c, y, nullptr, noise_inv, llr, nSC, nLayer, nSym, nAnt,
+                                                    nPamBits, (int)prec, st);
+    return from_hip(e, "coef_apply_demap launch");
+}
+
+}  // extern "C"

@@ -1,0 +1,38 @@
+// pe_internal.hpp -- host-side launchers shared between the kernel translation units and the C ABI layer.
+// Arguments are validated by the caller (pe_api.cpp); launchers only pick an instantiation and launch.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <stddef.h>
+
+namespace pe {
+
+// Tile parameters chosen per shape (defaults, overridable by the tuned table).
+struct CoefTile {
+    int f32_kc = 8;         // FP32 kernel: antennas per LDS chunk (8 or 16)
+    int mfma_waves = 4;     // MFMA kernel: waves (subcarriers) per workgroup (1, 2, 4)
+};
+CoefTile coef_tile_for(int nLayer, int nSym, int nAnt, int prec);
+
+// FP32 antenna-serial coefficient apply.
+hipError_t launch_coef_apply_f32(const float2 *C, const float2 *Y, float2 *O, int nSC, int L, int S, int A,
+                                 hipStream_t st);
+// Gram product G = M @ H (FP32 antenna-serial chain).
+hipError_t launch_gram_f32(const float2 *M, const float2 *H, float2 *G, int nSC, int L, int A, hipStream_t st);
+// MFMA coefficient apply; llr == nullptr -> writes Y_eq, else fused demap into llr (Y_eq unused).
+hipError_t launch_coef_apply_mfma(const float2 *C, const float2 *Y, float2 *O, const float *noise_inv,
+                                  float *llr, int nSC, int L, int S, int A, int nPamBits, int prec,
+                                  hipStream_t st);
+// FP32 fused apply + demap with the antenna-serial chain (the PE_PREC_F32 fused path).
+hipError_t launch_coef_apply_demap_f32(const float2 *C, const float2 *Y, const float *noise_inv, float *llr,
+                                       int nSC, int L, int S, int A, int nPamBits, hipStream_t st);
+// Demappers.
+hipError_t launch_llr_f16(const uint32_t *soft, const float *noise_inv, uint16_t *out, int N, int nPamBits,
+                          hipStream_t st);
+hipError_t launch_llr_f32(const float2 *Yeq, const float *noise_inv, float *llr, int nSC, int L, int S,
+                          int nPamBits, hipStream_t st);
+// MMSE: FP32 pass (flags subcarriers for FP64) then the FP64 pass over flagged subcarriers.
+hipError_t launch_gram_mmse(const float2 *H, const float *sigma2, float2 *C, float *noise_inv, uint8_t *flags,
+                            int nSC, int L, int A, bool unbiased, float cond_threshold, hipStream_t st);
+
+}  // namespace pe

@@ -1,0 +1,179 @@
+"""ctypes binding of libpusch_eq.so, the C ABI declared in include/pusch_eq.h.
+
+Python is the test and benchmark harness here: torch supplies device memory and streams, every numeric
+operation runs in the HIP kernels behind the C ABI. Nothing in this module computes a result on the host,
+and there is no fallback: if the shared library is missing, `lib()` raises.
+
+Tensor conventions follow the reference kernels (complex64 interleaved re/im):
+    C [nSC, nLayer, nAnt], Y_rx [nSC, nSym, nAnt], Y_eq [nSC, nLayer, nSym]      (aerial_coef_wmma.cu:72-74)
+    H [nSC, nAnt, nLayer], G [nSC, nLayer, nLayer]                             (aerial_bottleneck.cu:57-59)
+    llr [nSC, nLayer, nSym, 2*q] float32, or [N, 2*q] binary16                  (aerial_llr_bench_v2.cu:117)
+"""
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libpusch_eq.so")
+
+PE_OK, PE_ERR_INVALID, PE_ERR_UNSUPPORTED, PE_ERR_DEVICE, PE_ERR_NOMEM, PE_ERR_CAPACITY = range(6)
+PREC_F32, PREC_F16, PREC_BF16, PREC_F16_SPLIT, PREC_BF16_SPLIT = range(5)
+PREC_NAMES = {"f32": PREC_F32, "f16": PREC_F16, "bf16": PREC_BF16, "f16_split": PREC_F16_SPLIT,
+              "bf16_split": PREC_BF16_SPLIT}
+MMSE_UNBIASED = 1
+SCFLAG_FP64 = 1
+SCFLAG_SINGULAR = 2
+
+# every function the header declares, with its ctypes signature
+_P = ctypes.c_void_p
+_I = ctypes.c_int
+_U = ctypes.c_uint
+_F = ctypes.c_float
+SIGNATURES = {
+    "pe_version": ([], ctypes.c_char_p),
+    "pe_last_error": ([], ctypes.c_char_p),
+    "pe_status_string": ([_I], ctypes.c_char_p),
+    "pe_init": ([_I], _I),
+    "pe_coef_apply": ([_P, _P, _P, _I, _I, _I, _I, _I, _P], _I),
+    "pe_coef_apply_v1": ([_P, _P, _P, _I, _I, _P], _I),
+    "pe_coef_apply_range": ([_P, _P, _P, _I, _I, _I, _I, _I, _I, _P], _I),
+    "pe_gram": ([_P, _P, _P, _I, _I, _I, _P], _I),
+    "pe_gram_mmse": ([_P, _P, _P, _P, _P, _I, _I, _I, _U, _F, _P], _I),
+    "pe_llr_demap_f16": ([_P, _P, _P, _I, _I, _P], _I),
+    "pe_llr_demap": ([_P, _P, _P, _I, _I, _I, _I, _P], _I),
+    "pe_coef_apply_demap": ([_P, _P, _P, _P, _I, _I, _I, _I, _I, _I, _P], _I),
+}
+
+
+class PeError(RuntimeError):
+    def __init__(self, status, msg):
+        super().__init__("pusch_eq status %d: %s" % (status, msg))
+        self.status = status
+
+
+_lib = None
+
+
+def lib():
+    """Loads libpusch_eq.so once; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("%s is missing: build it with __graft_entry__.build()" % LIB_PATH)
+        h = ctypes.CDLL(LIB_PATH)
+        for name, (args, res) in SIGNATURES.items():
+            fn = getattr(h, name)
+            fn.argtypes = args
+            fn.restype = res
+        _lib = h
+    return _lib
+
+
+def _check(status):
+    if status != PE_OK:
+        raise PeError(status, lib().pe_last_error().decode())
+
+
+def _ptr(t):
+    return ctypes.c_void_p(t.data_ptr()) if t is not None else ctypes.c_void_p(0)
+
+
+def _stream(stream):
+    import torch
+    s = stream if stream is not None else torch.cuda.current_stream()
+    return ctypes.c_void_p(s.cuda_stream)
+
+
+def _prec(p):
+    return PREC_NAMES[p] if isinstance(p, str) else int(p)
+
+
+def init(device=0):
+    _check(lib().pe_init(device))
+
+
+def version():
+    return lib().pe_version().decode()
+
+
+def coef_apply(C, Y, out=None, prec="f32", stream=None):
+    """Y_eq[sc, l, s] = sum_a C[sc, l, a] * Y[sc, s, a]."""
+    import torch
+    nSC, L, A = C.shape
+    S = Y.shape[1]
+    assert C.dtype == torch.complex64 and Y.dtype == torch.complex64 and Y.shape == (nSC, S, A)
+    assert C.is_contiguous() and Y.is_contiguous()
+    if out is None:
+        out = torch.empty((nSC, L, S), dtype=torch.complex64, device=C.device)
+    _check(lib().pe_coef_apply(_ptr(C), _ptr(Y), _ptr(out), nSC, L, S, A, _prec(prec), _stream(stream)))
+    return out
+
+
+def coef_apply_v1(C, Y, out, nSC, nSym, stream=None):
+    """The reference's exact signature (8 layers, 64 antennas)."""
+    _check(lib().pe_coef_apply_v1(_ptr(C), _ptr(Y), _ptr(out), nSC, nSym, _stream(stream)))
+    return out
+
+
+def coef_apply_range(C, Y, out, sc_offset, n_sc, prec="f32", stream=None):
+    _, L, A = C.shape
+    S = Y.shape[1]
+    _check(lib().pe_coef_apply_range(_ptr(C), _ptr(Y), _ptr(out), n_sc, sc_offset, L, S, A, _prec(prec),
+                                     _stream(stream)))
+    return out
+
+
+def gram(M, H, out=None, stream=None):
+    """G[sc] = M[sc] @ H[sc] (plain product)."""
+    import torch
+    nSC, L, A = M.shape
+    assert H.shape == (nSC, A, L)
+    if out is None:
+        out = torch.empty((nSC, L, L), dtype=torch.complex64, device=M.device)
+    _check(lib().pe_gram(_ptr(M), _ptr(H), _ptr(out), nSC, L, A, _stream(stream)))
+    return out
+
+
+def gram_mmse(H, sigma2, unbiased=True, cond_threshold=0.0, C=None, noise_inv=None, flags=None, stream=None):
+    """C = (H^H H + sigma2 I)^-1 H^H per subcarrier; returns (C, noise_inv, flags)."""
+    import torch
+    nSC, A, L = H.shape
+    dev = H.device
+    if C is None:
+        C = torch.empty((nSC, L, A), dtype=torch.complex64, device=dev)
+    if noise_inv is None:
+        noise_inv = torch.empty((nSC, L), dtype=torch.float32, device=dev)
+    if flags is None:
+        flags = torch.empty((nSC,), dtype=torch.uint8, device=dev)
+    _check(lib().pe_gram_mmse(_ptr(H), _ptr(sigma2), _ptr(C), _ptr(noise_inv), _ptr(flags), nSC, L, A,
+                              MMSE_UNBIASED if unbiased else 0, float(cond_threshold), _stream(stream)))
+    return C, noise_inv, flags
+
+
+def llr_demap_f16(soft_half, noise_inv, n_pam_bits, out=None, stream=None):
+    """soft_half: [N, 2] binary16 (re, im); returns [N, 2*q] binary16 LLRs."""
+    import torch
+    N = soft_half.shape[0]
+    if out is None:
+        out = torch.empty((N, 2 * n_pam_bits), dtype=torch.float16, device=soft_half.device)
+    _check(lib().pe_llr_demap_f16(_ptr(soft_half), _ptr(noise_inv), _ptr(out), N, n_pam_bits, _stream(stream)))
+    return out
+
+
+def llr_demap(Yeq, noise_inv, n_pam_bits, out=None, stream=None):
+    import torch
+    nSC, L, S = Yeq.shape
+    if out is None:
+        out = torch.empty((nSC, L, S, 2 * n_pam_bits), dtype=torch.float32, device=Yeq.device)
+    _check(lib().pe_llr_demap(_ptr(Yeq), _ptr(noise_inv), _ptr(out), nSC, L, S, n_pam_bits, _stream(stream)))
+    return out
+
+
+def coef_apply_demap(C, Y, noise_inv, n_pam_bits, prec="f16", out=None, stream=None):
+    import torch
+    nSC, L, A = C.shape
+    S = Y.shape[1]
+    if out is None:
+        out = torch.empty((nSC, L, S, 2 * n_pam_bits), dtype=torch.float32, device=C.device)
+    _check(lib().pe_coef_apply_demap(_ptr(C), _ptr(Y), _ptr(noise_inv), _ptr(out), nSC, L, S, A, n_pam_bits,
+                                     _prec(prec), _stream(stream)))
+    return out

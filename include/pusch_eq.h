@@ -1,0 +1,143 @@
+/*
+ * pusch_eq.h -- C ABI of the MI355X (gfx950) PUSCH receiver equalization library.
+ *
+ * The library implements the equalization hot path of a 5G NR PUSCH receiver:
+ *
+ *     H --gram_mmse--> C, noise_inv --coef_apply--> Y_eq --llr_demap--> LLR
+ *                                  \_________ coef_apply_demap (fused) ___/
+ *
+ * Tensor layouts are the ones the reference kernels use (complex64 = interleaved {re, im} float pair):
+ *
+ *     C     [nSC][nLayer][nAnt]        coefficients           (aerial_coef_wmma.cu:72)
+ *     Y_rx  [nSC][nSym][nAnt]          received samples       (aerial_coef_wmma.cu:73)
+ *     Y_eq  [nSC][nLayer][nSym]        equalized symbols      (aerial_coef_wmma.cu:74)
+ *     H     [nSC][nAnt][nLayer]        channel estimate       (aerial_bottleneck.cu:58)
+ *     M     [nSC][nLayer][nAnt]        Gram left operand      (aerial_bottleneck.cu:57)
+ *     G     [nSC][nLayer][nLayer]      Gram product           (aerial_bottleneck.cu:59)
+ *     llr   [nSC][nLayer][nSym][2*q]   max-log LLRs, bit order b0(I) b1(Q) b2(I) ... (aerial_llr_bench_v2.cu:221-227)
+ *
+ * Conventions (all entry points):
+ *   - every pointer is caller-owned device memory; nothing here allocates except pe_slot_plan_create and
+ *     pe_serve_run, which say so;
+ *   - work is enqueued on `stream` (a hipStream_t passed as void*, NULL = the null stream) with no implicit
+ *     synchronisation; calls on different streams are independent;
+ *   - errors come back as pe_status codes with a message in pe_last_error() (thread-local); nothing calls
+ *     exit() (the reference's CUDA_CHECK does, aerial_coef_wmma.cu:57-58);
+ *   - nSC == 0 is a valid empty call and returns PE_OK without launching anything.
+ */
+#ifndef PUSCH_EQ_H
+#define PUSCH_EQ_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PE_ABI_VERSION 1
+
+/* complex64: same bytes as the reference's `struct cf32 { float re, im; }` (aerial_coef_wmma.cu:60) */
+typedef struct { float re, im; } pe_c64;
+/* complex binary16 (raw IEEE half bits): same bytes as `struct half2c { __half re, im; }`
+   (aerial_llr_bench_v2.cu:66) */
+typedef struct { uint16_t re, im; } pe_c16;
+
+typedef enum {
+    PE_OK = 0,
+    PE_ERR_INVALID = 1,      /* bad argument: null pointer, negative size, shape out of range */
+    PE_ERR_UNSUPPORTED = 2,  /* valid request this build does not implement (e.g. nSym > 16 on MFMA) */
+    PE_ERR_DEVICE = 3,       /* HIP runtime error (launch failure, wrong architecture, ...) */
+    PE_ERR_NOMEM = 4,        /* a plan/serving allocation failed */
+    PE_ERR_CAPACITY = 5      /* request exceeds a stated fixed capacity */
+} pe_status;
+
+/* Arithmetic of the coefficient-apply product. */
+typedef enum {
+    PE_PREC_F32 = 0,         /* FP32 VALU, antenna-serial fused multiply-add chain in the order of the
+                                reference scalar loop (aerial_coef_wmma.cu:96-102); bit-reproducible */
+    PE_PREC_F16 = 1,         /* MFMA fp16 x fp16 -> fp32, per-subcarrier power-of-two scaling */
+    PE_PREC_BF16 = 2,        /* MFMA bf16 x bf16 -> fp32 */
+    PE_PREC_F16_SPLIT = 3,   /* MFMA fp16 hi+lo compensated product (3 MFMAs per k-step) */
+    PE_PREC_BF16_SPLIT = 4   /* MFMA bf16 hi+lo compensated product */
+} pe_prec;
+
+/* ---- library ------------------------------------------------------------------------------------- */
+
+/* No reference counterpart (the reference has no library). */
+const char *pe_version(void);
+const char *pe_last_error(void);
+const char *pe_status_string(pe_status s);
+/* Selects `device`, checks that it is a gfx950 part and loads the tile table (file named by the
+   environment variable PE_TUNE_TABLE if set). Calling it is optional; entry points work on the current
+   device without it. */
+pe_status pe_init(int device);
+
+/* ---- coefficient apply:  Y_eq[sc][l][s] = sum_a C[sc][l][a] * Y_rx[sc][s][a]  (no conjugation) ------ */
+
+/* General form. Replaces coef_apply_v1_scalar (aerial_coef_wmma.cu:71-75, nAnt/nLayer compile-time there),
+   coef_apply_v2_wmma (aerial_coef_wmma.cu:146-150), coef_apply_v3_wmma_multiw (aerial_coef_wmma.cu:250-254),
+   v1/v2 (coef_rigor.cu:38,54) and v1_scalar/v2_wmma<N_ANT,N_SYM> (coef_scaling.cu:41,77).
+   Shapes: 1 <= nLayer <= 8, 1 <= nSym <= 16, nAnt % 4 == 0 with 4 <= nAnt <= 256 (PE_PREC_F32) or
+   4 <= nAnt <= 128 (MFMA precisions). */
+pe_status pe_coef_apply(const pe_c64 *C, const pe_c64 *Y_rx, pe_c64 *Y_eq,
+                        int nSC, int nLayer, int nSym, int nAnt, pe_prec prec, void *stream);
+
+/* The reference's exact argument list (nLayer = 8, nAnt = 64 fixed as in aerial_coef_wmma.cu:42-47),
+   FP32 path. Replaces coef_apply_v1_scalar(C, Y_rx, Y_eq, nSC, nSym) (aerial_coef_wmma.cu:71-75). */
+pe_status pe_coef_apply_v1(const pe_c64 *C, const pe_c64 *Y_rx, pe_c64 *Y_eq, int nSC, int nSym,
+                           void *stream);
+
+/* Subcarrier range [scOffset, scOffset + nSC) of full-size tensors. Replaces k_coef_offset
+   (adaptive_smpartition.cu:52-54). */
+pe_status pe_coef_apply_range(const pe_c64 *C, const pe_c64 *Y_rx, pe_c64 *Y_eq, int nSC, int scOffset,
+                              int nLayer, int nSym, int nAnt, pe_prec prec, void *stream);
+
+/* ---- Gram product G[sc] = M[sc] @ H[sc] (plain product, no conjugation, FP32 antenna-serial chain) -- */
+
+/* Replaces gram_scalar(M, H, G, nSC) (aerial_bottleneck.cu:56-60). 1 <= nLayer <= 8, 1 <= nAnt <= 1024. */
+pe_status pe_gram(const pe_c64 *M, const pe_c64 *H, pe_c64 *G, int nSC, int nLayer, int nAnt, void *stream);
+
+/* ---- MMSE coefficient compute ---------------------------------------------------------------------- */
+
+#define PE_MMSE_UNBIASED 1u    /* scale row l of C by 1/mu_l (unbiased estimate, the LLR convention) */
+
+#define PE_SCFLAG_FP64 1u      /* subcarrier solved in FP64 (condition estimate above threshold) */
+#define PE_SCFLAG_SINGULAR 2u  /* Gram not positive definite even in FP64: C and noise_inv are zero */
+
+/* Per subcarrier: G = H^H H + sigma2 I (nLayer x nLayer Hermitian), Cholesky G = L L^H, Ginv = G^-1,
+   W = Ginv H^H (the MMSE matrix), mu_l = 1 - sigma2 * Ginv[l][l], noise_inv_l = mu_l / (1 - mu_l)
+   (post-equalization SINR of the unbiased estimate). C = W, or diag(1/mu) W with PE_MMSE_UNBIASED.
+   The solve runs in FP32; subcarriers whose 1-norm condition number kappa_1(G) exceeds cond_threshold
+   (<= 0 selects 1e4), or whose FP32 Cholesky breaks down, are recomputed in FP64.
+   The reference only names this stage (aerial_bottleneck.cu:51-53: eqMmseCoefCompHighMimoKernel).
+   sigma2[nSC] > 0; noise_inv[nSC][nLayer]; sc_flags[nSC] (required: it also carries the hand-over from the
+   FP32 pass to the FP64 pass). 1 <= nLayer <= 8, 1 <= nAnt <= 1024. */
+pe_status pe_gram_mmse(const pe_c64 *H, const float *sigma2, pe_c64 *C, float *noise_inv, uint8_t *sc_flags,
+                       int nSC, int nLayer, int nAnt, unsigned flags, float cond_threshold, void *stream);
+
+/* ---- max-log LLR soft demapper (computePamLlr) ----------------------------------------------------- */
+
+/* Drop-in for llr_v1_baseline(softEst, noiseInv, llrOut, N, nPamBits) (aerial_llr_bench_v2.cu:114-118,
+   aerial_llr_v3.cu:84-86): FP16 complex in, FP16 LLR out [N][2*nPamBits], noiseInv per symbol,
+   1 <= nPamBits <= 4 (QPSK..256-QAM). FP32 arithmetic, results bit-identical to the reference formula. */
+pe_status pe_llr_demap_f16(const pe_c16 *softEst, const float *noiseInv, uint16_t *llrOut, int N,
+                           int nPamBits, void *stream);
+
+/* FP32 demapper on the equalizer output: Y_eq[nSC][nLayer][nSym], noise_inv[nSC][nLayer]
+   -> llr[nSC][nLayer][nSym][2*nPamBits] float. */
+pe_status pe_llr_demap(const pe_c64 *Y_eq, const float *noise_inv, float *llr, int nSC, int nLayer, int nSym,
+                       int nPamBits, void *stream);
+
+/* ---- fused coefficient apply + demap: Y_eq stays in registers, only LLRs are written ---------------- */
+
+/* No single reference kernel: the reference runs coef apply and demap as two launches with Y_eq in HBM
+   between them (pipeline_overlap.cu:293-301). Same shapes as pe_coef_apply plus noise_inv[nSC][nLayer]. */
+pe_status pe_coef_apply_demap(const pe_c64 *C, const pe_c64 *Y_rx, const float *noise_inv, float *llr,
+                              int nSC, int nLayer, int nSym, int nAnt, int nPamBits, pe_prec prec,
+                              void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PUSCH_EQ_H */

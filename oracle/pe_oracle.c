@@ -1,0 +1,595 @@
+/*
+ * pe_oracle.c -- CPU oracle for the PUSCH equalization path (plain C, gcc).
+ *
+ * TEST INFRASTRUCTURE ONLY. Loaded by tests/, __graft_entry__.smoke() and the cpu_baseline leg of bench.py
+ * as the checker or the timed CPU baseline; never linked into or called by the product library.
+ *
+ * Build: oracle/Makefile (gcc -O2 -mfma -ffp-contract=off -fopenmp). -ffp-contract=off keeps every
+ * multiply and add that is written separately rounded separately; fused multiply-adds appear only where
+ * fmaf()/fma() are written explicitly.
+ *
+ * Reference files restated (paths relative to the reference repository):
+ *   src/aerial_coef_wmma.cu   cma() :60-64, coef_apply_v1_scalar :71-104, inputs :383-387
+ *   src/coef_rigor.cu         inputs :101-104
+ *   src/aerial_bottleneck.cu  gram_scalar :56-80
+ *   src/llr/aerial_llr_bench_v2.cu  LUT :47-64, cpu_computePamLlr_ref :72-107, llr_v1_baseline :114-156,
+ *                                   inputs :289-304
+ *   src/llr/aerial_llr_v3.cu  cpu_ref :61-79, inputs :268-273
+ */
+#include "pe_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#include <omp.h>
+
+/* ------------------------------------------------------------------------------------------------------
+ * generators
+ * ------------------------------------------------------------------------------------------------------ */
+
+/* One glibc rand() stream: out[i] = ((float)rand()/RAND_MAX - 0.5f) [* scale].
+ * aerial_coef_wmma.cu:383-387 (srand(42), no scale; C then Y from one stream, re before im) and
+ * coef_rigor.cu:101-104 / coef_scaling.cu:182-185 (scale 1.4f). Callers ask for |C| + |Y| floats in one call
+ * and split, which reproduces "all of C first, then all of Y". */
+void po_glibc_uniform(unsigned seed, float scale, int apply_scale, float *out, size_t nfloats) {
+    srand(seed);
+    for (size_t i = 0; i < nfloats; i++) {
+        float v = (float)rand() / RAND_MAX - 0.5f;
+        out[i] = apply_scale ? v * scale : v;
+    }
+}
+
+/* aerial_llr_bench_v2.cu:289-304 and aerial_llr_v3.cu:268-273: per symbol re, im = U(+-0.7) rounded to
+ * binary16, then noiseInv = powf(10, (rand() % 26) / 10) * 2. */
+void po_llr_bench_inputs(unsigned seed, int N, uint16_t *soft_half, float *noise_inv) {
+    srand(seed);
+    for (int i = 0; i < N; i++) {
+        float re = ((float)rand() / RAND_MAX - 0.5f) * 1.4f;
+        float im = ((float)rand() / RAND_MAX - 0.5f) * 1.4f;
+        soft_half[2 * i] = po_f32_to_f16(re);
+        soft_half[2 * i + 1] = po_f32_to_f16(im);
+        float snr_lin = powf(10.f, (float)(rand() % 26) / 10.f);
+        noise_inv[i] = snr_lin * 2.f;
+    }
+}
+
+/* ------------------------------------------------------------------------------------------------------
+ * IEEE binary16, round to nearest even (the rounding of __float2half)
+ * ------------------------------------------------------------------------------------------------------ */
+uint16_t po_f32_to_f16(float f) {
+    uint32_t x;
+    memcpy(&x, &f, 4);
+    uint32_t sign = (x >> 16) & 0x8000u;
+    uint32_t ax = x & 0x7fffffffu;
+    if (ax >= 0x7f800000u) { /* inf / nan */
+        if (ax == 0x7f800000u) return (uint16_t)(sign | 0x7c00u);
+        return (uint16_t)(sign | 0x7e00u | ((ax >> 13) & 0x3ffu));
+    }
+    if (ax >= 0x477ff000u) return (uint16_t)(sign | 0x7c00u); /* >= 65520 rounds to inf */
+    if (ax < 0x38800000u) {                                    /* below 2^-14: subnormal half */
+        float a;
+        memcpy(&a, &ax, 4);
+        float m = nearbyintf(a * 16777216.0f); /* exact scaling by 2^24, then RNE to an integer */
+        return (uint16_t)(sign | (uint32_t)m);
+    }
+    uint32_t mant = ax & 0x7fffffu;
+    uint32_t e = (ax >> 23) - 127u + 15u;
+    uint32_t h = (e << 10) | (mant >> 13);
+    uint32_t rem = mant & 0x1fffu;
+    if (rem > 0x1000u || (rem == 0x1000u && (h & 1u))) h++;
+    return (uint16_t)(sign | h);
+}
+
+float po_f16_to_f32(uint16_t h) {
+    uint32_t sign = ((uint32_t)h & 0x8000u) << 16;
+    uint32_t e = (h >> 10) & 0x1fu;
+    uint32_t m = h & 0x3ffu;
+    float f;
+    if (e == 0) {
+        f = (float)m * (1.0f / 16777216.0f);
+        uint32_t b;
+        memcpy(&b, &f, 4);
+        b |= sign;
+        memcpy(&f, &b, 4);
+        return f;
+    }
+    uint32_t b = (e == 31) ? (sign | 0x7f800000u | (m << 13)) : (sign | ((e - 15u + 127u) << 23) | (m << 13));
+    memcpy(&f, &b, 4);
+    return f;
+}
+
+void po_f32_to_f16_array(const float *in, uint16_t *out, size_t n) {
+    for (size_t i = 0; i < n; i++) out[i] = po_f32_to_f16(in[i]);
+}
+void po_f16_to_f32_array(const uint16_t *in, float *out, size_t n) {
+    for (size_t i = 0; i < n; i++) out[i] = po_f16_to_f32(in[i]);
+}
+
+/* ------------------------------------------------------------------------------------------------------
+ * coefficient apply  Y_eq[sc][l][s] = sum_a C[sc][l][a] * Y[sc][s][a]
+ * ------------------------------------------------------------------------------------------------------ */
+
+/* coef_apply_v1_scalar, aerial_coef_wmma.cu:71-104: antenna-serial chain a = 0..A-1 of
+ * cma(a, b, acc) = {acc.re + a.re*b.re - a.im*b.im, acc.im + a.re*b.im + a.im*b.re} (:60-64), written in
+ * the contracted form a CUDA compiler emits for that expression:
+ *   re = fma(-c.im, y.im, fma(c.re, y.re, re));  im = fma(c.im, y.re, fma(c.re, y.im, im)). */
+void po_coef_apply_v1_f32(const float *C, const float *Y, float *O, int nSC, int L, int S, int A) {
+    for (int sc = 0; sc < nSC; sc++)
+        for (int l = 0; l < L; l++) {
+            const float *c = C + 2 * ((size_t)sc * L + l) * A;
+            for (int s = 0; s < S; s++) {
+                const float *y = Y + 2 * ((size_t)sc * S + s) * A;
+                float re = 0.f, im = 0.f;
+                for (int a = 0; a < A; a++) {
+                    re = fmaf(c[2 * a], y[2 * a], re);
+                    re = fmaf(-c[2 * a + 1], y[2 * a + 1], re);
+                    im = fmaf(c[2 * a], y[2 * a + 1], im);
+                    im = fmaf(c[2 * a + 1], y[2 * a], im);
+                }
+                float *o = O + 2 * (((size_t)sc * L + l) * S + s);
+                o[0] = re;
+                o[1] = im;
+            }
+        }
+}
+
+/* Same chain with every product and sum rounded separately (the uncontracted reading of :60-64). */
+void po_coef_apply_v1_f32_nofma(const float *C, const float *Y, float *O, int nSC, int L, int S, int A) {
+    for (int sc = 0; sc < nSC; sc++)
+        for (int l = 0; l < L; l++) {
+            const float *c = C + 2 * ((size_t)sc * L + l) * A;
+            for (int s = 0; s < S; s++) {
+                const float *y = Y + 2 * ((size_t)sc * S + s) * A;
+                float re = 0.f, im = 0.f;
+                for (int a = 0; a < A; a++) {
+                    float p0 = c[2 * a] * y[2 * a];
+                    float p1 = c[2 * a + 1] * y[2 * a + 1];
+                    float p2 = c[2 * a] * y[2 * a + 1];
+                    float p3 = c[2 * a + 1] * y[2 * a];
+                    re = (re + p0) - p1;
+                    im = (im + p2) + p3;
+                }
+                float *o = O + 2 * (((size_t)sc * L + l) * S + s);
+                o[0] = re;
+                o[1] = im;
+            }
+        }
+}
+
+/* FP64 evaluation of the same product (inputs widened exactly). */
+void po_coef_apply_f64(const float *C, const float *Y, double *O, int nSC, int L, int S, int A) {
+    for (int sc = 0; sc < nSC; sc++)
+        for (int l = 0; l < L; l++) {
+            const float *c = C + 2 * ((size_t)sc * L + l) * A;
+            for (int s = 0; s < S; s++) {
+                const float *y = Y + 2 * ((size_t)sc * S + s) * A;
+                double re = 0.0, im = 0.0;
+                for (int a = 0; a < A; a++) {
+                    double cr = c[2 * a], ci = c[2 * a + 1], yr = y[2 * a], yi = y[2 * a + 1];
+                    re += cr * yr - ci * yi;
+                    im += cr * yi + ci * yr;
+                }
+                double *o = O + 2 * (((size_t)sc * L + l) * S + s);
+                o[0] = re;
+                o[1] = im;
+            }
+        }
+}
+
+/* FP64 product with FP64 coefficients (the FP64 chain: C from po_mmse_f64). */
+void po_coef_apply_f64d(const double *C, const float *Y, double *O, int nSC, int L, int S, int A) {
+    for (int sc = 0; sc < nSC; sc++)
+        for (int l = 0; l < L; l++) {
+            const double *c = C + 2 * ((size_t)sc * L + l) * A;
+            for (int s = 0; s < S; s++) {
+                const float *y = Y + 2 * ((size_t)sc * S + s) * A;
+                double re = 0.0, im = 0.0;
+                for (int a = 0; a < A; a++) {
+                    double cr = c[2 * a], ci = c[2 * a + 1], yr = y[2 * a], yi = y[2 * a + 1];
+                    re += cr * yr - ci * yi;
+                    im += cr * yi + ci * yr;
+                }
+                double *o = O + 2 * (((size_t)sc * L + l) * S + s);
+                o[0] = re;
+                o[1] = im;
+            }
+        }
+}
+
+/* Host OpenMP restatement of the scalar loop (the CPU baseline): the contracted chain of
+ * po_coef_apply_v1_f32, `omp parallel for` over (subcarrier, layer). Returns the thread count used. */
+int po_coef_apply_v1_f32_omp(const float *C, const float *Y, float *O, int nSC, int L, int S, int A,
+                             int nthreads) {
+    if (nthreads <= 0) nthreads = omp_get_max_threads();
+    long total = (long)nSC * L;
+#pragma omp parallel for num_threads(nthreads) schedule(static)
+    for (long t = 0; t < total; t++) {
+        long sc = t / L;
+        int l = (int)(t % L);
+        const float *c = C + 2 * ((size_t)sc * L + l) * A;
+        for (int s = 0; s < S; s++) {
+            const float *y = Y + 2 * ((size_t)sc * S + s) * A;
+            float re = 0.f, im = 0.f;
+            for (int a = 0; a < A; a++) {
+                re = fmaf(c[2 * a], y[2 * a], re);
+                re = fmaf(-c[2 * a + 1], y[2 * a + 1], re);
+                im = fmaf(c[2 * a], y[2 * a + 1], im);
+                im = fmaf(c[2 * a + 1], y[2 * a], im);
+            }
+            float *o = O + 2 * (((size_t)sc * L + l) * S + s);
+            o[0] = re;
+            o[1] = im;
+        }
+    }
+    return nthreads;
+}
+
+/* ------------------------------------------------------------------------------------------------------
+ * Gram product  G[sc][r][c] = sum_k M[sc][r][k] * H[sc][k][c]   (gram_scalar, aerial_bottleneck.cu:56-80)
+ * ------------------------------------------------------------------------------------------------------ */
+void po_gram_f32(const float *M, const float *H, float *G, int nSC, int L, int A) {
+    for (int sc = 0; sc < nSC; sc++)
+        for (int r = 0; r < L; r++)
+            for (int c = 0; c < L; c++) {
+                const float *m = M + 2 * ((size_t)sc * L + r) * A;
+                const float *h = H + 2 * ((size_t)sc * A * L + c);
+                float re = 0.f, im = 0.f;
+                for (int k = 0; k < A; k++) {
+                    float mr = m[2 * k], mi = m[2 * k + 1];
+                    float hr = h[2 * (size_t)k * L], hi = h[2 * (size_t)k * L + 1];
+                    re = fmaf(mr, hr, re);
+                    re = fmaf(-mi, hi, re);
+                    im = fmaf(mr, hi, im);
+                    im = fmaf(mi, hr, im);
+                }
+                float *g = G + 2 * (((size_t)sc * L + r) * L + c);
+                g[0] = re;
+                g[1] = im;
+            }
+}
+
+void po_gram_f64(const float *M, const float *H, double *G, int nSC, int L, int A) {
+    for (int sc = 0; sc < nSC; sc++)
+        for (int r = 0; r < L; r++)
+            for (int c = 0; c < L; c++) {
+                const float *m = M + 2 * ((size_t)sc * L + r) * A;
+                const float *h = H + 2 * ((size_t)sc * A * L + c);
+                double re = 0.0, im = 0.0;
+                for (int k = 0; k < A; k++) {
+                    double mr = m[2 * k], mi = m[2 * k + 1];
+                    double hr = h[2 * (size_t)k * L], hi = h[2 * (size_t)k * L + 1];
+                    re += mr * hr - mi * hi;
+                    im += mr * hi + mi * hr;
+                }
+                double *g = G + 2 * (((size_t)sc * L + r) * L + c);
+                g[0] = re;
+                g[1] = im;
+            }
+}
+
+/* ------------------------------------------------------------------------------------------------------
+ * MMSE coefficients in FP64.  The reference has no implementation (aerial_bottleneck.cu:51-53 names
+ * eqMmseCoefCompHighMimoKernel only); this is the textbook regularized solve:
+ *   G = H^H H + sigma2 I,  G = L L^H (Cholesky),  Ginv = L^-H L^-1,  W = Ginv H^H,
+ *   mu_l = 1 - sigma2 Ginv[l][l],  noise_inv_l = mu_l / (1 - mu_l) = (1 - x) / x with x = sigma2 Ginv[l][l],
+ *   cond = ||G||_1 ||Ginv||_1,  C = W or diag(1/mu) W (unbiased).
+ * H is [nSC][A][L]; C is [nSC][L][A] (complex double); noise_inv, mu are [nSC][L]; cond is [nSC].
+ * Returns the number of subcarriers whose Gram is not positive definite (their outputs are zero).
+ * ------------------------------------------------------------------------------------------------------ */
+#define PO_MAXL 8
+int po_mmse_f64(const float *H, const float *sigma2, double *C, double *noise_inv, double *mu, double *cond,
+                int nSC, int L, int A, int unbiased) {
+    int bad = 0;
+    for (int sc = 0; sc < nSC; sc++) {
+        const float *h = H + 2 * (size_t)sc * A * L;
+        double s2 = sigma2[sc];
+        double gr[PO_MAXL][PO_MAXL], gi[PO_MAXL][PO_MAXL];
+        for (int i = 0; i < L; i++)
+            for (int j = 0; j < L; j++) {
+                double re = 0.0, im = 0.0;
+                for (int a = 0; a < A; a++) {
+                    double ar = h[2 * (a * L + i)], ai = h[2 * (a * L + i) + 1];
+                    double br = h[2 * (a * L + j)], bi = h[2 * (a * L + j) + 1];
+                    re += ar * br + ai * bi; /* conj(a) * b */
+                    im += ar * bi - ai * br;
+                }
+                gr[i][j] = re + (i == j ? s2 : 0.0);
+                gi[i][j] = (i == j) ? 0.0 : im;
+            }
+        /* Cholesky, lower L */
+        double lr[PO_MAXL][PO_MAXL] = {{0}}, li[PO_MAXL][PO_MAXL] = {{0}};
+        int ok = 1;
+        for (int j = 0; j < L && ok; j++) {
+            double d = gr[j][j];
+            for (int k = 0; k < j; k++) d -= lr[j][k] * lr[j][k] + li[j][k] * li[j][k];
+            if (!(d > 0.0)) {
+                ok = 0;
+                break;
+            }
+            double ljj = sqrt(d);
+            lr[j][j] = ljj;
+            for (int i = j + 1; i < L; i++) {
+                /* L[i][j] = (G[i][j] - sum_k L[i][k] conj(L[j][k])) / L[j][j] */
+                double re = gr[i][j], im = gi[i][j];
+                for (int k = 0; k < j; k++) {
+                    re -= lr[i][k] * lr[j][k] + li[i][k] * li[j][k];
+                    im -= li[i][k] * lr[j][k] - lr[i][k] * li[j][k];
+                }
+                lr[i][j] = re / ljj;
+                li[i][j] = im / ljj;
+            }
+        }
+        double *csc = C + 2 * (size_t)sc * L * A;
+        if (!ok) {
+            bad++;
+            memset(csc, 0, sizeof(double) * 2 * (size_t)L * A);
+            for (int l = 0; l < L; l++) {
+                noise_inv[(size_t)sc * L + l] = 0.0;
+                if (mu) mu[(size_t)sc * L + l] = 0.0;
+            }
+            if (cond) cond[sc] = INFINITY;
+            continue;
+        }
+        /* W = L^-1 (lower) */
+        double wr[PO_MAXL][PO_MAXL] = {{0}}, wi[PO_MAXL][PO_MAXL] = {{0}};
+        for (int j = 0; j < L; j++) {
+            wr[j][j] = 1.0 / lr[j][j];
+            for (int i = j + 1; i < L; i++) {
+                double re = 0.0, im = 0.0;
+                for (int k = j; k < i; k++) {
+                    re += lr[i][k] * wr[k][j] - li[i][k] * wi[k][j];
+                    im += lr[i][k] * wi[k][j] + li[i][k] * wr[k][j];
+                }
+                wr[i][j] = -re / lr[i][i];
+                wi[i][j] = -im / lr[i][i];
+            }
+        }
+        /* Ginv = W^H W */
+        double vr[PO_MAXL][PO_MAXL], vi[PO_MAXL][PO_MAXL];
+        for (int i = 0; i < L; i++)
+            for (int j = 0; j < L; j++) {
+                double re = 0.0, im = 0.0;
+                int k0 = i > j ? i : j;
+                for (int k = k0; k < L; k++) {
+                    re += wr[k][i] * wr[k][j] + wi[k][i] * wi[k][j];
+                    im += wr[k][i] * wi[k][j] - wi[k][i] * wr[k][j];
+                }
+                vr[i][j] = re;
+                vi[i][j] = im;
+            }
+        if (cond) {
+            double ng = 0.0, nv = 0.0;
+            for (int j = 0; j < L; j++) {
+                double cg = 0.0, cv = 0.0;
+                for (int i = 0; i < L; i++) {
+                    cg += hypot(gr[i][j], gi[i][j]);
+                    cv += hypot(vr[i][j], vi[i][j]);
+                }
+                if (cg > ng) ng = cg;
+                if (cv > nv) nv = cv;
+            }
+            cond[sc] = ng * nv;
+        }
+        for (int l = 0; l < L; l++) {
+            double x = s2 * vr[l][l];
+            double m = 1.0 - x;
+            if (mu) mu[(size_t)sc * L + l] = m;
+            double xf = x > 1e-30 ? x : 1e-30;
+            double ni = (1.0 - x) / xf;
+            noise_inv[(size_t)sc * L + l] = ni;
+            double scale = (unbiased && m != 0.0) ? 1.0 / m : 1.0;
+            for (int a = 0; a < A; a++) {
+                double re = 0.0, im = 0.0;
+                for (int j = 0; j < L; j++) {
+                    double hr = h[2 * (a * L + j)], hi = -h[2 * (a * L + j) + 1]; /* conj(H[a][j]) */
+                    re += vr[l][j] * hr - vi[l][j] * hi;
+                    im += vr[l][j] * hi + vi[l][j] * hr;
+                }
+                csc[2 * ((size_t)l * A + a)] = re * scale;
+                csc[2 * ((size_t)l * A + a) + 1] = im * scale;
+            }
+        }
+    }
+    return bad;
+}
+
+/* ------------------------------------------------------------------------------------------------------
+ * max-log LLR (computePamLlr)
+ * ------------------------------------------------------------------------------------------------------ */
+
+/* LUT of aerial_llr_bench_v2.cu:47-54 (PAM2, PAM4, PAM8, PAM16 distances at offsets 0, 1, 3, 6), kept as the
+ * same decimal literals; the tests check them against 1/sqrt(2), {2,1}/sqrt(10), {4,2,1}/sqrt(42),
+ * {8,4,2,1}/sqrt(170). */
+static const double LUT_D[10] = {0.707106781186548, 0.632455532033676, 0.316227766016838, 0.617213399848368,
+                                 0.308606699924184, 0.154303349962092, 0.613571991077897, 0.306785995538948,
+                                 0.153392997769474, 0.076696498884737};
+static const float LUT_F[10] = {0.707106781186548f, 0.632455532033676f, 0.316227766016838f,
+                                0.617213399848368f, 0.308606699924184f, 0.154303349962092f,
+                                0.613571991077897f, 0.306785995538948f, 0.153392997769474f,
+                                0.076696498884737f};
+static const int PAM_OFF[4] = {0, 1, 3, 6};
+
+double po_llr_lut(int idx) { return (idx >= 0 && idx < 10) ? LUT_D[idx] : NAN; }
+
+/* cpu_computePamLlr_ref, aerial_llr_bench_v2.cu:72-107 (= cpu_ref, aerial_llr_v3.cu:61-79): FP64 folding
+ * chain per axis, out[i*2 + iq] as float. */
+void po_llr_ref_f64(float pamI, float pamQ, float noise_inv, int nPamBits, float *out) {
+    int off = PAM_OFF[nPamBits - 1];
+    double dist = LUT_D[off + nPamBits - 1];
+    for (int iq = 0; iq < 2; iq++) {
+        double sb[4], md[4];
+        sb[0] = iq == 0 ? (double)pamI : (double)pamQ;
+        unsigned sign = 0;
+        for (int i = 0; i < nPamBits - 1; i++) {
+            sb[i + 1] = LUT_D[off + i] - fabs(sb[i]);
+            md[i] = dist + fabs(sb[i]);
+            md[i] *= md[i];
+            if (sb[i] < 0.0) sign |= 1u << i;
+        }
+        md[nPamBits - 1] = dist + fabs(sb[nPamBits - 1]);
+        md[nPamBits - 1] *= md[nPamBits - 1];
+        if (sb[nPamBits - 1] < 0.0) sign |= 1u << (nPamBits - 1);
+        double m1 = fabs(sb[nPamBits - 1]) - dist;
+        m1 *= m1;
+        for (int i = 0; i < nPamBits; i++) {
+            double llr = (md[i] - m1) * (double)noise_inv;
+            if (sign & (1u << i)) llr = -llr;
+            if (llr < -65504.0) llr = -65504.0;
+            if (llr > 65504.0) llr = 65504.0;
+            out[i * 2 + iq] = (float)llr;
+        }
+    }
+}
+
+/* llr_v1_baseline arithmetic, aerial_llr_bench_v2.cu:134-155 (= aerial_llr_v3.cu:93-107): the same chain in
+ * FP32 with the float LUT, clamp to +-65504; out[i*2 + iq] before the binary16 conversion. */
+void po_llr_f32(float pamI, float pamQ, float noise, int nPamBits, float *out) {
+    int off = PAM_OFF[nPamBits - 1];
+    float dist = LUT_F[off + nPamBits - 1];
+    for (int iq = 0; iq < 2; iq++) {
+        float sb[4], md[4];
+        sb[0] = iq == 0 ? pamI : pamQ;
+        unsigned sign = 0;
+        for (int i = 0; i < nPamBits - 1; i++) {
+            sb[i + 1] = LUT_F[off + i] - fabsf(sb[i]);
+            md[i] = dist + fabsf(sb[i]);
+            md[i] *= md[i];
+            if (sb[i] < 0.f) sign |= 1u << i;
+        }
+        md[nPamBits - 1] = dist + fabsf(sb[nPamBits - 1]);
+        md[nPamBits - 1] *= md[nPamBits - 1];
+        if (sb[nPamBits - 1] < 0.f) sign |= 1u << (nPamBits - 1);
+        float m1 = fabsf(sb[nPamBits - 1]) - dist;
+        m1 *= m1;
+        for (int i = 0; i < nPamBits; i++) {
+            float llr = (md[i] - m1) * noise;
+            if (sign & (1u << i)) llr = -llr;
+            llr = fmaxf(-65504.f, fminf(65504.f, llr));
+            out[i * 2 + iq] = llr;
+        }
+    }
+}
+
+/* Whole llr_v1_baseline (aerial_llr_bench_v2.cu:114-156): binary16 in, FP32 math, binary16 out. */
+void po_llr_demap_f16(const uint16_t *soft_half, const float *noise_inv, uint16_t *out, int N, int nPamBits) {
+    float tmp[8];
+    for (int i = 0; i < N; i++) {
+        po_llr_f32(po_f16_to_f32(soft_half[2 * i]), po_f16_to_f32(soft_half[2 * i + 1]), noise_inv[i],
+                   nPamBits, tmp);
+        for (int k = 0; k < 2 * nPamBits; k++) out[(size_t)i * 2 * nPamBits + k] = po_f32_to_f16(tmp[k]);
+    }
+}
+
+/* The FP64 reference loop of aerial_llr_bench_v2.cu:364-371 over all symbols. */
+void po_llr_demap_ref_f64(const uint16_t *soft_half, const float *noise_inv, float *out, int N, int nPamBits) {
+    for (int i = 0; i < N; i++)
+        po_llr_ref_f64(po_f16_to_f32(soft_half[2 * i]), po_f16_to_f32(soft_half[2 * i + 1]), noise_inv[i],
+                       nPamBits, out + (size_t)i * 2 * nPamBits);
+}
+
+/* FP32 demap on the equalizer layout: Y_eq[sc][l][s] complex, noise_inv[sc][l] ->
+ * llr[sc][l][s][2q], each symbol through po_llr_f32. */
+void po_llr_demap_f32(const float *Yeq, const float *noise_inv, float *llr, int nSC, int L, int S,
+                      int nPamBits) {
+    for (int sc = 0; sc < nSC; sc++)
+        for (int l = 0; l < L; l++) {
+            float ni = noise_inv[(size_t)sc * L + l];
+            for (int s = 0; s < S; s++) {
+                size_t e = ((size_t)sc * L + l) * S + s;
+                po_llr_f32(Yeq[2 * e], Yeq[2 * e + 1], ni, nPamBits, llr + e * 2 * nPamBits);
+            }
+        }
+}
+
+/* FP64 folding chain on double inputs (the FP64 end of the chain), same layout as po_llr_demap_f32. */
+void po_llr_demap_f64(const double *Yeq, const double *noise_inv, double *llr, int nSC, int L, int S,
+                      int nPamBits) {
+    int off = PAM_OFF[nPamBits - 1];
+    double dist = LUT_D[off + nPamBits - 1];
+    for (int sc = 0; sc < nSC; sc++)
+        for (int l = 0; l < L; l++) {
+            double ni = noise_inv[(size_t)sc * L + l];
+            for (int s = 0; s < S; s++) {
+                size_t e = ((size_t)sc * L + l) * S + s;
+                for (int iq = 0; iq < 2; iq++) {
+                    double sb[4], md[4];
+                    sb[0] = Yeq[2 * e + iq];
+                    unsigned sign = 0;
+                    for (int i = 0; i < nPamBits - 1; i++) {
+                        sb[i + 1] = LUT_D[off + i] - fabs(sb[i]);
+                        md[i] = dist + fabs(sb[i]);
+                        md[i] *= md[i];
+                        if (sb[i] < 0.0) sign |= 1u << i;
+                    }
+                    md[nPamBits - 1] = dist + fabs(sb[nPamBits - 1]);
+                    md[nPamBits - 1] *= md[nPamBits - 1];
+                    if (sb[nPamBits - 1] < 0.0) sign |= 1u << (nPamBits - 1);
+                    double m1 = fabs(sb[nPamBits - 1]) - dist;
+                    m1 *= m1;
+                    for (int i = 0; i < nPamBits; i++) {
+                        double v = (md[i] - m1) * ni;
+                        if (sign & (1u << i)) v = -v;
+                        if (v < -65504.0) v = -65504.0;
+                        if (v > 65504.0) v = 65504.0;
+                        llr[e * 2 * nPamBits + i * 2 + iq] = v;
+                    }
+                }
+            }
+        }
+}
+
+/* Brute-force max-log LLR of one PAM axis under the TS 38.211 section 5.1 Gray mapping: the 2^m levels are
+ * t_0 / sqrt(2(4^m - 1)/3) with t_{m-1} = 1 - 2c_{m-1}, t_i = (1 - 2c_i)(2^{m-1-i} - t_{i+1}); bit c_i of
+ * the axis is 38.211 bit b_{2i} (I) or b_{2i+1} (Q).  out[i] = (min_{c_i=1} d^2 - min_{c_i=0} d^2) * noise. */
+void po_llr_bruteforce_f64(double x, double noise, int nPamBits, double *out) {
+    int m = nPamBits;
+    double norm = sqrt(2.0 * (pow(4.0, m) - 1.0) / 3.0);
+    double best0[4], best1[4];
+    for (int i = 0; i < m; i++) best0[i] = best1[i] = INFINITY;
+    for (int code = 0; code < (1 << m); code++) {
+        int c[4] = {0, 0, 0, 0};
+        for (int i = 0; i < m; i++) c[i] = (code >> i) & 1;
+        double t = 1.0 - 2.0 * c[m - 1];
+        for (int i = m - 2; i >= 0; i--) t = (1.0 - 2.0 * c[i]) * ((double)(1 << (m - 1 - i)) - t);
+        double amp = t / norm;
+        double d2 = (x - amp) * (x - amp);
+        for (int i = 0; i < m; i++) {
+            if (c[i]) {
+                if (d2 < best1[i]) best1[i] = d2;
+            } else if (d2 < best0[i])
+                best0[i] = d2;
+        }
+    }
+    for (int i = 0; i < m; i++) out[i] = (best1[i] - best0[i]) * noise;
+}
+
+/* ------------------------------------------------------------------------------------------------------
+ * CPU baseline of the fused stage: the scalar coef-apply chain then the FP32 demapper per symbol.
+ * ------------------------------------------------------------------------------------------------------ */
+int po_coef_apply_demap_f32_omp(const float *C, const float *Y, const float *noise_inv, float *llr,
+                                int nSC, int L, int S, int A, int nPamBits, int nthreads) {
+    if (nthreads <= 0) nthreads = omp_get_max_threads();
+    long total = (long)nSC * L;
+#pragma omp parallel for num_threads(nthreads) schedule(static)
+    for (long t = 0; t < total; t++) {
+        long sc = t / L;
+        int l = (int)(t % L);
+        const float *c = C + 2 * ((size_t)sc * L + l) * A;
+        float ni = noise_inv[t];
+        for (int s = 0; s < S; s++) {
+            const float *y = Y + 2 * ((size_t)sc * S + s) * A;
+            float re = 0.f, im = 0.f;
+            for (int a = 0; a < A; a++) {
+                re = fmaf(c[2 * a], y[2 * a], re);
+                re = fmaf(-c[2 * a + 1], y[2 * a + 1], re);
+                im = fmaf(c[2 * a], y[2 * a + 1], im);
+                im = fmaf(c[2 * a + 1], y[2 * a], im);
+            }
+            size_t e = ((size_t)sc * L + l) * S + s;
+            po_llr_f32(re, im, ni, nPamBits, llr + e * 2 * nPamBits);
+        }
+    }
+    return nthreads;
+}
+
+int po_omp_max_threads(void) { return omp_get_max_threads(); }

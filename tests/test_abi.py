@@ -1,0 +1,79 @@
+"""CPU tests of the C ABI boundary: the library loads, exports every symbol include/pusch_eq.h declares, and
+rejects bad arguments before touching a device. No compute call is made (no GPU here)."""
+import ctypes
+import os
+import re
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "pusch_eq.h")
+
+
+def declared_functions():
+    src = open(HEADER).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    return sorted(set(re.findall(r"\b(pe_[a-z0-9_]+)\s*\(", src)))
+
+
+def test_header_declares_the_operator_surface():
+    names = declared_functions()
+    for required in ("pe_coef_apply", "pe_gram_mmse", "pe_llr_demap_f16", "pe_llr_demap", "pe_coef_apply_demap",
+                     "pe_gram", "pe_coef_apply_v1"):
+        assert required in names
+
+
+def test_library_exports_every_declared_symbol(pe):
+    h = pe.lib()
+    missing = [n for n in declared_functions() if not hasattr(h, n)]
+    assert not missing, missing
+
+
+def test_binding_covers_every_declared_symbol(pe):
+    assert set(declared_functions()) <= set(pe.SIGNATURES)
+
+
+def test_version_and_status_strings(pe):
+    assert "gfx950" in pe.version()
+    h = pe.lib()
+    assert h.pe_status_string(0) == b"ok"
+    assert h.pe_status_string(1) == b"invalid argument"
+
+
+def _call_coef(pe, L=8, S=11, A=64, nsc=4, prec=0, ptr=1):
+    p = ctypes.c_void_p(ptr)
+    return pe.lib().pe_coef_apply(p, p, p, nsc, L, S, A, prec, None)
+
+
+@pytest.mark.parametrize("kw,status", [
+    (dict(L=0), 1), (dict(L=9), 1), (dict(S=0), 1), (dict(S=17), 2), (dict(A=6), 1), (dict(A=0), 1),
+    (dict(A=260), 2), (dict(A=132, prec=1), 2), (dict(prec=7), 1), (dict(nsc=-1), 1), (dict(ptr=0), 1),
+])
+def test_coef_apply_rejects_bad_arguments_without_a_device(pe, kw, status):
+    assert _call_coef(pe, **kw) == status
+    assert pe.lib().pe_last_error()  # message set
+
+
+def test_empty_calls_succeed_without_launching(pe):
+    assert _call_coef(pe, nsc=0, ptr=0) == 0
+    h = pe.lib()
+    n = ctypes.c_void_p(0)
+    assert h.pe_llr_demap_f16(n, n, n, 0, 4, None) == 0
+    assert h.pe_llr_demap(n, n, n, 0, 8, 11, 4, None) == 0
+    assert h.pe_gram_mmse(n, n, n, n, n, 0, 8, 64, 1, 0.0, None) == 0
+    assert h.pe_coef_apply_demap(n, n, n, n, 0, 8, 11, 64, 4, 1, None) == 0
+
+
+def test_other_entry_points_validate(pe):
+    h = pe.lib()
+    p = ctypes.c_void_p(1)
+    n = ctypes.c_void_p(0)
+    assert h.pe_llr_demap_f16(p, p, p, 10, 5, None) == 1
+    assert h.pe_llr_demap_f16(p, p, p, -1, 4, None) == 1
+    assert h.pe_llr_demap(p, p, p, 4, 8, 11, 0, None) == 1
+    assert h.pe_gram_mmse(p, p, p, p, n, 4, 8, 64, 1, 0.0, None) == 1   # flags buffer required
+    assert h.pe_gram_mmse(p, p, p, p, p, 4, 9, 64, 1, 0.0, None) == 1
+    assert h.pe_gram_mmse(p, p, p, p, p, 4, 8, 64, 4, 0.0, None) == 1   # unknown flag bit
+    assert h.pe_gram(p, p, p, 4, 8, 0, None) == 1
+    assert h.pe_coef_apply_demap(p, p, p, p, 4, 8, 11, 64, 5, 1, None) == 1
+    assert h.pe_coef_apply_range(p, p, p, 4, -1, 8, 11, 64, 0, None) == 1

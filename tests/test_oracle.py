@@ -1,0 +1,242 @@
+"""CPU tests that pin the oracle: against the reference's published outputs, against closed forms and
+independent numpy/scipy routines, and against brute force on small inputs. No GPU needed."""
+import numpy as np
+import pytest
+
+# Published outputs of the reference (A100 runs of the reference programs):
+#   results/coef_rigor_results.txt:5  EVM(V2 vs V1) = 0.0307% (-70.3 dB) on srand(7), U(+-0.7), 1584x8x11x64
+#   README.md:152-154                 cos 0.99999997, max abs 0.001289 on srand(42), U(+-0.5), same shape
+RIGOR_EVM_DB = -70.3
+README_COS = 0.99999997
+README_MAXABS = 0.001289
+NSC, L, S, A = 1584, 8, 11, 64
+
+
+def f16_round(po, x):
+    return po.f16_bits_to_f32(po.f32_to_f16_bits(x))
+
+
+def fp16_product(po, C, Y):
+    """The reference V2 arithmetic up to accumulation order: operands rounded to binary16 (round to nearest
+    even, __float2half), products summed at high precision."""
+    Ch = (f16_round(po, C.real) + 1j * f16_round(po, C.imag)).astype(np.complex128)
+    Yh = (f16_round(po, Y.real) + 1j * f16_round(po, Y.imag)).astype(np.complex128)
+    return np.einsum("nla,nsa->nls", Ch, Yh)
+
+
+def test_llr_lut_is_closed_form(po):
+    lut = po.llr_lut()
+    closed = np.array([1 / np.sqrt(2), 2 / np.sqrt(10), 1 / np.sqrt(10), 4 / np.sqrt(42), 2 / np.sqrt(42),
+                       1 / np.sqrt(42), 8 / np.sqrt(170), 4 / np.sqrt(170), 2 / np.sqrt(170), 1 / np.sqrt(170)])
+    assert np.max(np.abs(lut - closed)) < 1e-14
+    assert np.array_equal(lut.astype(np.float32), closed.astype(np.float32))
+
+
+def test_f16_rounding_matches_numpy(po):
+    rng = np.random.default_rng(3)
+    x = np.concatenate([rng.standard_normal(20000).astype(np.float32) * 10 ** rng.uniform(-9, 5, 20000),
+                        np.array([65504, 65519.9, 65520, -65520, 6.1e-5, 5.96e-8, 2.98e-8, 0.0, -0.0],
+                                 dtype=np.float32)]).astype(np.float32)
+    ours = po.f32_to_f16_bits(x)
+    with np.errstate(over="ignore"):
+        ref = x.astype(np.float16).view(np.uint16)
+    assert np.array_equal(ours, ref)
+    assert np.array_equal(po.f16_bits_to_f32(ref), ref.view(np.float16).astype(np.float32))
+
+
+def test_glibc_stream_reproduces_published_rigor_evm(po):
+    """srand(7), rf() = ((float)rand()/RAND_MAX - 0.5f)*1.4f, all of C then all of Y (coef_rigor.cu:101-104),
+    V1 FP32 chain vs binary16 operands -> the published -70.3 dB."""
+    C, Y = po.reference_coef_inputs(7, NSC, L, S, A, scale=1.4)
+    assert np.abs(C).max() <= 0.7 * np.sqrt(2) + 1e-6
+    v1 = po.coef_apply_v1_f32(C, Y)
+    v2 = fp16_product(po, C, Y)
+    r, db = po.evm(v1, v2)
+    assert abs(db - RIGOR_EVM_DB) < 0.1, db
+    assert abs(r * 100 - 0.0307) < 0.0005
+
+
+def test_glibc_stream_reproduces_published_cos_maxabs(po):
+    """srand(42), U(+-0.5) (aerial_coef_wmma.cu:383-387) -> cos 0.99999997, max abs 0.001289."""
+    C, Y = po.reference_coef_inputs(42, NSC, L, S, A, scale=None)
+    v1 = po.coef_apply_v1_f32(C, Y)
+    v2 = fp16_product(po, C, Y)
+    assert round(po.cosine(v1, v2), 8) == pytest.approx(README_COS, abs=1.1e-8)
+    assert po.max_abs_err(v1, v2) == pytest.approx(README_MAXABS, rel=2e-3)
+
+
+def test_a_wrong_generator_misses_the_published_numbers(po):
+    """Negative control: a mistake in the stream (Y before C) or in the product (conjugated C) moves the
+    metrics far away from the published ones, so the two tests above can tell them apart."""
+    C, Y = po.reference_coef_inputs(42, NSC, L, S, A, scale=None)
+    v1 = po.coef_apply_v1_f32(C, Y)
+    v1_conj = po.coef_apply_v1_f32(np.conj(C), Y)
+    assert po.cosine(v1, v1_conj) < 0.1
+    Cs = Y[:, :L, :].copy()  # a shifted stream gives different values
+    assert po.max_abs_err(po.coef_apply_v1_f32(Cs, Y), v1) > 0.1
+
+
+def test_coef_apply_forms_agree_and_match_numpy(po):
+    rng = np.random.default_rng(1)
+    C = po.cn(rng, (24, 8, 64))
+    Y = po.cn(rng, (24, 11, 64))
+    ref = np.einsum("nla,nsa->nls", C.astype(np.complex128), Y.astype(np.complex128))
+    f64 = po.coef_apply_f64(C, Y)
+    assert np.max(np.abs(f64 - ref)) < 1e-12
+    for contracted in (True, False):
+        f32 = po.coef_apply_v1_f32(C, Y, contracted=contracted)
+        assert po.norm_rel_err(f64, f32) < 2e-6
+    # the two FP32 readings differ only in rounding
+    a = po.coef_apply_v1_f32(C, Y, True)
+    b = po.coef_apply_v1_f32(C, Y, False)
+    assert not np.array_equal(a, b)
+
+
+def test_coef_apply_invariants(po):
+    rng = np.random.default_rng(2)
+    Y = po.cn(rng, (12, 11, 64))
+    # C = [I_8 | 0] picks antennas 0..7 of every symbol, transposed, exactly
+    C = np.zeros((12, 8, 64), dtype=np.complex64)
+    for l in range(8):
+        C[:, l, l] = 1.0
+    out = po.coef_apply_v1_f32(C, Y)
+    assert np.array_equal(out, np.transpose(Y[:, :, :8], (0, 2, 1)))
+    assert np.array_equal(po.coef_apply_v1_f32(np.zeros_like(C), Y), np.zeros((12, 8, 11), np.complex64))
+    # linearity in C (FP64 form)
+    C1, C2 = po.cn(rng, (12, 8, 64)), po.cn(rng, (12, 8, 64))
+    lhs = po.coef_apply_f64(C1.astype(np.complex128) * 2 + C2.astype(np.complex128), Y)
+    rhs = 2 * po.coef_apply_f64(C1, Y) + po.coef_apply_f64(C2, Y)
+    assert np.max(np.abs(lhs - rhs)) < 1e-10
+
+
+def test_coef_apply_omp_equals_serial(po):
+    rng = np.random.default_rng(5)
+    C = po.cn(rng, (100, 8, 64))
+    Y = po.cn(rng, (100, 11, 64))
+    ser = po.coef_apply_v1_f32(C, Y)
+    par, used = po.coef_apply_v1_f32_omp(C, Y, nthreads=4)
+    assert used == 4 and np.array_equal(ser, par)
+
+
+def test_gram_matches_matmul(po):
+    rng = np.random.default_rng(4)
+    M = po.cn(rng, (30, 8, 64))
+    H = po.cn(rng, (30, 64, 8))
+    ref = np.matmul(M.astype(np.complex128), H.astype(np.complex128))
+    g64 = po.gram_f64(M, H)
+    assert np.max(np.abs(g64 - ref)) < 1e-12
+    g32 = po.gram_f32(M, H)
+    assert po.norm_rel_err(ref, g32) < 2e-6
+
+
+def _mmse_numpy(H, s2):
+    Hh = np.conj(np.transpose(H, (0, 2, 1))).astype(np.complex128)
+    G = Hh @ H.astype(np.complex128) + s2[:, None, None] * np.eye(H.shape[2])
+    return np.linalg.solve(G, Hh), G
+
+
+def test_mmse_matches_solve_and_identities(po):
+    rng = np.random.default_rng(6)
+    nsc, a, l = 64, 64, 8
+    H = po.rayleigh_h(rng, nsc, a, l)
+    s2 = np.full(nsc, 0.01, dtype=np.float32)
+    C, ni, mu, cond, bad = po.mmse_f64(H, s2, unbiased=False)
+    assert bad == 0
+    W, G = _mmse_numpy(H, s2.astype(np.float64))
+    assert np.max(np.abs(C - W)) < 1e-10 * np.max(np.abs(W))
+    # push-through identity: (H^H H + s2 I)^-1 H^H = H^H (H H^H + s2 I)^-1
+    Hd = H.astype(np.complex128)
+    Hh = np.conj(np.transpose(Hd, (0, 2, 1)))
+    s2v = float(s2[0])  # the float32 value the oracle used
+    W2 = Hh @ np.linalg.inv(Hd @ Hh + s2v * np.eye(a))
+    assert np.max(np.abs(C - W2)) < 1e-9
+    # C H = I - s2 G^-1, mu = diag(C H)
+    Gi = np.linalg.inv(G)
+    CH = C @ Hd
+    assert np.max(np.abs(CH - (np.eye(l) - s2v * Gi))) < 1e-12
+    assert np.max(np.abs(mu - np.real(np.diagonal(CH, axis1=1, axis2=2)))) < 1e-10
+    assert np.max(np.abs(ni - mu / (1 - mu))) < 1e-6 * np.max(ni)
+    assert np.allclose(cond, np.linalg.cond(G, 1), rtol=1e-8)
+    Cu, niu, _, _, _ = po.mmse_f64(H, s2, unbiased=True)
+    assert np.allclose(Cu, C / mu[:, :, None], rtol=1e-12, atol=1e-14)
+    assert np.array_equal(niu, ni)
+
+
+def test_mmse_zero_forcing_limit_and_singular(po):
+    rng = np.random.default_rng(7)
+    H = po.rayleigh_h(rng, 16, 32, 4)
+    C, _, _, _, bad = po.mmse_f64(H, np.full(16, 1e-12, np.float32), unbiased=False)
+    assert bad == 0
+    assert np.max(np.abs(C @ H.astype(np.complex128) - np.eye(4))) < 1e-8
+    Hs = H.copy()
+    Hs[:, :, 3] = 0  # a dead layer with sigma2 = 0: the last pivot is exactly 0, not positive definite
+    C, ni, _, cond, bad = po.mmse_f64(Hs, np.zeros(16, np.float32), unbiased=False)
+    assert bad == 16 and np.all(C == 0) and np.all(ni == 0) and np.all(np.isinf(cond))
+
+
+@pytest.mark.parametrize("q", [1, 2, 3, 4])
+def test_llr_folding_is_exact_maxlog(po, q):
+    """The folding chain of computePamLlr equals brute-force max-log over the 2^q Gray-mapped levels."""
+    xs = np.linspace(-1.6, 1.6, 2001)
+    noise = 3.7
+    for x in xs:
+        out = np.empty(2 * q, np.float32)
+        import ctypes
+        po.lib().po_llr_ref_f64(float(np.float32(x)), float(np.float32(-x)), noise, q,
+                                out.ctypes.data_as(ctypes.c_void_p))
+        bf_i = po.llr_bruteforce(float(np.float32(x)), noise, q)
+        bf_q = po.llr_bruteforce(float(np.float32(-x)), noise, q)
+        bf = np.empty(2 * q)
+        bf[0::2], bf[1::2] = bf_i, bf_q
+        bf = np.clip(bf, -65504, 65504)
+        assert np.max(np.abs(out - bf)) <= 1e-5 * max(1.0, np.max(np.abs(bf)))
+
+
+def test_llr_qpsk_closed_form(po):
+    d = 1 / np.sqrt(2)
+    for x in [-0.9, -0.2, 0.0, 0.3, 1.1]:
+        bf = po.llr_bruteforce(x, 2.0, 1)[0]
+        assert bf == pytest.approx(4 * d * x * 2.0, abs=1e-12)
+
+
+@pytest.mark.parametrize("q", [1, 2, 3, 4])
+def test_qam_map_and_demap_recover_bits(po, q):
+    """Noise-free 38.211 symbols demap to LLRs whose signs give back the bits (positive = bit 0)."""
+    rng = np.random.default_rng(10 + q)
+    bits = rng.integers(0, 2, size=(2, 3, 5, 2 * q), dtype=np.uint8)
+    x = po.qam_map(bits, q)
+    assert abs(np.mean(np.abs(po.qam_map(np.array(
+        [[(c >> i) & 1 for i in range(2 * q)] for c in range(4 ** q)]), q)) ** 2) - 1) < 1e-12
+    llr = po.llr_demap_f64(x, np.ones((2, 3)), q)
+    assert np.array_equal((llr < 0).astype(np.uint8), bits)
+
+
+def test_qam_map_known_points(po):
+    assert po.qam_map(np.array([0, 0, 0, 0]), 2) == pytest.approx((1 + 1j) / np.sqrt(10))
+    assert po.qam_map(np.array([1, 0, 1, 0]), 2) == pytest.approx((-3 + 1j) / np.sqrt(10))
+    assert po.qam_map(np.array([0, 1]), 1) == pytest.approx((1 - 1j) / np.sqrt(2))
+    assert po.qam_map(np.zeros(8, np.uint8), 4) == pytest.approx((5 + 5j) / np.sqrt(170))
+    assert po.qam_map(np.array([0, 0, 1, 1, 1, 1, 1, 1]), 4) == pytest.approx((15 + 15j) / np.sqrt(170))
+    assert po.qam_map(np.array([0, 0, 0, 0, 0, 0]), 3) == pytest.approx((3 + 3j) / np.sqrt(42))
+
+
+@pytest.mark.parametrize("q", [2, 3, 4])
+def test_llr_f16_pipeline_close_to_f64_ref(po, q):
+    """The reference's own accuracy check (aerial_llr_bench_v2.cu:379-394): binary16 output vs FP64 ref."""
+    soft, ni = po.llr_bench_inputs(42, 139392)
+    out = po.f16_bits_to_f32(po.llr_demap_f16(soft, ni, q))
+    ref = po.llr_demap_ref_f64(soft, ni, q)
+    assert po.cosine(ref, out) > 0.99999
+    # FP32 arithmetic then binary16 rounding: within a binary16 ulp of the FP64 result
+    assert np.all(np.abs(out - ref) <= np.maximum(np.abs(ref) * 2.0 ** -10, 1e-3))
+
+
+def test_metrics_helpers(po):
+    v = np.array([5, 1, 4, 2, 3, 10, 7, 6, 9, 8], dtype=np.float64)
+    assert po.percentile_nearest_rank(v, 50) == 5
+    assert po.percentile_nearest_rank(v, 99) == 10
+    assert po.percentile_nearest_rank(v, 5) == 1
+    assert po.median_upper(v) == 6
+    ref = np.array([1 + 1j, 2 - 1j])
+    r, db = po.evm(ref, ref * 1.001)
+    assert r == pytest.approx(0.001) and db == pytest.approx(-60.0)
