@@ -1,0 +1,318 @@
+// apply_device.hpp -- per-subcarrier MFMA coefficient apply (+ fused demap) as device functions, shared by the
+// single-cell kernel (coef_apply_mfma.hip) and the multi-cell persistent kernels (multicell.hip).
+//
+// Complex product as one real GEMM per subcarrier ("4-real" decomposition in real-embedding form):
+//   D[16 x 16] = A'[16 x 2A] * B'[2A x 16]
+//   B'[(a,0)][n] = Re Y[n][a],  B'[(a,1)][n] = Im Y[n][a]
+//   A'[l][(a,*)]   = ( Re C[l][a], -Im C[l][a])  -> D[l][n]   = Re Y_eq[l][n]   (rows 0..7)
+//   A'[8+l][(a,*)] = ( Im C[l][a],  Re C[l][a])  -> D[8+l][n] = Im Y_eq[l][n]   (rows 8..15)
+// so M = 16 rows are filled by the 8 layers (re and im), N = 16 columns hold up to 16 symbols (11 used),
+// K = 2A runs in steps of 32 through v_mfma_f32_16x16x32_{f16,bf16}. One wave owns one subcarrier at a time.
+//
+// Data path per subcarrier (one wave):
+//   1. the C block [L][A] and the Y block [S][A] are contiguous in HBM; lane i loads 16-byte chunks i, i+64, ...
+//      so every instruction reads 1 KB contiguous and every byte is read by exactly one lane;
+//   2. per-subcarrier power-of-two range scales of C and of Y (wave max-reduction, exact scaling), skipped when
+//      binary16 already covers the data;
+//   3. each lane rounds its chunks to 16 bits once and writes them into the LDS operand layout (the real-part
+//      and imaginary-part rows of A' from the same C chunk, via a sign xor and a half swap on packed pairs);
+//   4. MFMA: lane (g = lane/16, r = lane%16) reads its A' row r / B' column r, antennas [g*4KS + 4t, +4) for
+//      k-step t, with one 128-bit LDS read per operand;
+//   5. epilogue: undo the scale, pair real and imaginary parts with one xor-32 lane exchange, store Y_eq or run
+//      the max-log demapper in registers and store LLRs.
+// The split variants add the rounding residuals: A*B ~ Ahi*Bhi + (Ahi*Blo + Alo*Bhi).
+#pragma once
+#include "pe_device.hpp"
+
+namespace pe {
+namespace mfma {
+
+template <bool BF>
+struct Op16;
+template <>
+struct Op16<false> {
+    typedef f16x8 vec;
+    static __device__ __forceinline__ vec cvt(const float (&v)[8]) {
+        vec r;
+#pragma unroll
+        for (int i = 0; i < 8; ++i) r[i] = (_Float16)v[i];
+        return r;
+    }
+    static __device__ __forceinline__ vec cvt_lo(const float (&v)[8], const vec &hi) {
+        vec r;
+#pragma unroll
+        for (int i = 0; i < 8; ++i) r[i] = (_Float16)(v[i] - (float)hi[i]);
+        return r;
+    }
+    static __device__ __forceinline__ f32x4 mfma(const vec &a, const vec &b, const f32x4 &c) {
+        return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0);
+    }
+};
+template <>
+struct Op16<true> {
+    typedef bf16x8 vec;
+    static __device__ __forceinline__ vec cvt(const float (&v)[8]) {
+        vec r;
+#pragma unroll
+        for (int i = 0; i < 8; ++i) r[i] = (__bf16)v[i];
+        return r;
+    }
+    static __device__ __forceinline__ vec cvt_lo(const float (&v)[8], const vec &hi) {
+        vec r;
+#pragma unroll
+        for (int i = 0; i < 8; ++i) r[i] = (__bf16)(v[i] - (float)hi[i]);
+        return r;
+    }
+    static __device__ __forceinline__ f32x4 mfma(const vec &a, const vec &b, const f32x4 &c) {
+        return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+    }
+};
+
+// two floats -> packed 16-bit pair (element 0 in the low half), round to nearest even
+template <bool BF>
+__device__ __forceinline__ uint32_t pack2(float a, float b) {
+    if constexpr (BF) {
+        typedef __bf16 v2 __attribute__((ext_vector_type(2)));
+        v2 v = {(__bf16)a, (__bf16)b};
+        return __builtin_bit_cast(uint32_t, v);
+    } else {
+        typedef _Float16 v2 __attribute__((ext_vector_type(2)));
+        v2 v = {(_Float16)a, (_Float16)b};
+        return __builtin_bit_cast(uint32_t, v);
+    }
+}
+template <bool BF>
+__device__ __forceinline__ float2 unpack2(uint32_t p) {
+    if constexpr (BF) {
+        typedef __bf16 v2 __attribute__((ext_vector_type(2)));
+        const v2 v = __builtin_bit_cast(v2, p);
+        return make_float2((float)v[0], (float)v[1]);
+    } else {
+        typedef _Float16 v2 __attribute__((ext_vector_type(2)));
+        const v2 v = __builtin_bit_cast(v2, p);
+        return make_float2((float)v[0], (float)v[1]);
+    }
+}
+
+__device__ __forceinline__ float max3abs(float m, float a, float b) {
+    return fmaxf(fmaxf(m, fabsf(a)), fabsf(b));
+}
+
+// exponent e such that m * 2^-e lies in [0.5, 1); 0 for m == 0
+__device__ __forceinline__ int scale_exp(float m) {
+    int e = 0;
+    if (m > 0.f) (void)frexpf(m, &e);
+    return e;
+}
+
+// Raw complex64 data of one subcarrier, loaded cooperatively: the C block [L][A] and the Y block [S][A] of a
+// subcarrier are contiguous, so lane i loads 16-byte chunks i, i+64, ... of each (RC / RY rounds at most).
+template <int KS>
+struct RawSC {
+    static constexpr int RC = KS;      // ceil(8 * 16KS / 128): C rounds for L <= 8, A <= 16KS
+    static constexpr int RY = 2 * KS;  // Y rounds for S <= 16
+    float4 c[RC];
+    float4 y[RY];
+};
+
+template <int KS>
+__device__ __forceinline__ void load_raw(RawSC<KS> &R, const float2 *__restrict__ C, const float2 *__restrict__ Y,
+                                         size_t sc, int L, int S, int A, int lane, int nC, int nY, int rc,
+                                         int ry) {
+    const float4 *cb = reinterpret_cast<const float4 *>(C + sc * L * A);
+    const float4 *yb = reinterpret_cast<const float4 *>(Y + sc * S * A);
+    // chunks past the block end re-read chunk 0 of the same block: the per-subcarrier maximum is unchanged
+    // and they are not written to LDS
+#pragma unroll
+    for (int j = 0; j < RawSC<KS>::RC; ++j)
+        if (j < rc) {
+            const int c = lane + 64 * j;
+            R.c[j] = cb[c < nC ? c : 0];
+        }
+#pragma unroll
+    for (int j = 0; j < RawSC<KS>::RY; ++j)
+        if (j < ry) {
+            const int c = lane + 64 * j;
+            R.y[j] = yb[c < nY ? c : 0];
+        }
+}
+
+__device__ __forceinline__ float max4abs(float m, float4 v) {
+    return max3abs(max3abs(m, v.x, v.y), v.z, v.w);
+}
+
+__device__ __forceinline__ uint32_t swap_halves(uint32_t p) { return __builtin_amdgcn_perm(p, p, 0x01000302u); }
+
+// LDS operand layout of one wave: A [16 rows][2*16KS halfs], B [16 cols][2*16KS halfs], rows padded by 16 bytes.
+//   A row l      : (Re C[l][a], -Im C[l][a]) for a = 0..A-1          (real part of Y_eq row l)
+//   A row 8 + l  : (Im C[l][a],  Re C[l][a])                          (imaginary part)
+//   B col s      : (Re Y[s][a],  Im Y[s][a])
+// Rows past L / columns past S are never written; they only feed outputs that are never stored.
+template <int KS>
+struct LdsLayout {
+    static constexpr int RB = 64 * KS + 16;  // bytes per operand row
+    static constexpr int REGION = 16 * RB;   // one operand matrix
+    // A and B (hi), plus A and B (lo) for the split precisions
+    static constexpr int bytes(int prec) { return (prec == 3 || prec == 4 ? 4 : 2) * REGION; }
+};
+
+// Antennas past A are never written by process_sc; zero every operand row once before a subcarrier range
+// whose A is below the 16*KS the layout holds.
+template <int KS, int PREC>
+__device__ __forceinline__ void zero_operands(unsigned char *lds, int lane) {
+    for (int i = lane; i < LdsLayout<KS>::bytes(PREC) / 16; i += 64)
+        reinterpret_cast<uint4 *>(lds)[i] = make_uint4(0u, 0u, 0u, 0u);
+    __syncthreads();
+}
+
+// One subcarrier: range scaling, operand conversion into LDS, MFMA, epilogue to Y_eq or LLRs.
+template <int KS, int PREC, int NPB>
+__device__ __forceinline__ void process_sc(RawSC<KS> &R, size_t sc, unsigned char *lds, float2 *__restrict__ O,
+                                           const float *__restrict__ noise_inv, float *__restrict__ llr, int L,
+                                           int S, int A, int lane, int nC, int nY, int rc, int ry) {
+    constexpr bool BF = (PREC == 2 || PREC == 4);
+    constexpr bool SPLIT = (PREC == 3 || PREC == 4);
+    constexpr int AG = 4 * KS;
+    constexpr int RB = LdsLayout<KS>::RB;
+    constexpr int REG = LdsLayout<KS>::REGION;
+    typedef Op16<BF> Op;
+    typedef typename Op::vec vec;
+    const int g = lane >> 4;
+    const int r = lane & 15;
+    const int cpr = A >> 1;  // 16-byte chunks per row
+
+    // per-subcarrier power-of-two scales of C and Y
+    float mc = 0.f, my = 0.f;
+#pragma unroll
+    for (int j = 0; j < RawSC<KS>::RC; ++j)
+        if (j < rc) mc = max4abs(mc, R.c[j]);
+#pragma unroll
+    for (int j = 0; j < RawSC<KS>::RY; ++j)
+        if (j < ry) my = max4abs(my, R.y[j]);
+    mc = wave_max_xor(mc, 1, 64);
+    my = wave_max_xor(my, 1, 64);
+    int ec = scale_exp(mc);
+    int ey = scale_exp(my);
+    // binary16 keeps full precision on [2^-14, 65504]: when both maxima lie in [2^-9, 2^14) the operands
+    // are already in range and the (uniform) scaling step is skipped; bf16 has the fp32 exponent range
+    bool scaled = true;
+    if constexpr (!SPLIT) scaled = !BF && (ec < -8 || ec > 14 || ey < -8 || ey > 14);
+    if (!scaled) ec = ey = 0;
+    const float sc_c = ldexpf(1.f, -ec), sc_y = ldexpf(1.f, -ey);
+
+    // convert and write the operands
+#pragma unroll
+    for (int j = 0; j < RawSC<KS>::RC; ++j) {
+        const int c = lane + 64 * j;
+        if (j < rc && c < nC) {
+            float4 v = R.c[j];
+            if (scaled) { v.x *= sc_c; v.y *= sc_c; v.z *= sc_c; v.w *= sc_c; }
+            const int row = c / cpr, part = c - row * cpr;
+            const uint32_t p0 = pack2<BF>(v.x, v.y), p1 = pack2<BF>(v.z, v.w);
+            *reinterpret_cast<uint2 *>(lds + row * RB + part * 8) = make_uint2(p0 ^ 0x80000000u, p1 ^ 0x80000000u);
+            *reinterpret_cast<uint2 *>(lds + (row + 8) * RB + part * 8) = make_uint2(swap_halves(p0), swap_halves(p1));
+            if constexpr (SPLIT) {
+                const float2 h0 = unpack2<BF>(p0), h1 = unpack2<BF>(p1);
+                const uint32_t q0 = pack2<BF>(v.x - h0.x, v.y - h0.y), q1 = pack2<BF>(v.z - h1.x, v.w - h1.y);
+                unsigned char *lo = lds + 2 * REG;
+                *reinterpret_cast<uint2 *>(lo + row * RB + part * 8) = make_uint2(q0 ^ 0x80000000u, q1 ^ 0x80000000u);
+                *reinterpret_cast<uint2 *>(lo + (row + 8) * RB + part * 8) = make_uint2(swap_halves(q0), swap_halves(q1));
+            }
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < RawSC<KS>::RY; ++j) {
+        const int c = lane + 64 * j;
+        if (j < ry && c < nY) {
+            float4 v = R.y[j];
+            if (scaled) { v.x *= sc_y; v.y *= sc_y; v.z *= sc_y; v.w *= sc_y; }
+            const int col = c / cpr, part = c - col * cpr;
+            const uint32_t p0 = pack2<BF>(v.x, v.y), p1 = pack2<BF>(v.z, v.w);
+            *reinterpret_cast<uint2 *>(lds + REG + col * RB + part * 8) = make_uint2(p0, p1);
+            if constexpr (SPLIT) {
+                const float2 h0 = unpack2<BF>(p0), h1 = unpack2<BF>(p1);
+                const uint32_t q0 = pack2<BF>(v.x - h0.x, v.y - h0.y), q1 = pack2<BF>(v.z - h1.x, v.w - h1.y);
+                *reinterpret_cast<uint2 *>(lds + 3 * REG + col * RB + part * 8) = make_uint2(q0, q1);
+            }
+        }
+    }
+    __syncthreads();  // one-wave workgroup: orders the operand writes before the cross-lane reads
+
+    f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+    f32x4 acc2 = {0.f, 0.f, 0.f, 0.f};
+    const int off = r * RB + g * AG * 4;
+#pragma unroll
+    for (int t = 0; t < KS; ++t) {
+        const vec a = __builtin_bit_cast(vec, *reinterpret_cast<const uint4 *>(lds + off + 16 * t));
+        const vec b = __builtin_bit_cast(vec, *reinterpret_cast<const uint4 *>(lds + REG + off + 16 * t));
+        acc = Op::mfma(a, b, acc);
+        if constexpr (SPLIT) {
+            const vec al = __builtin_bit_cast(vec, *reinterpret_cast<const uint4 *>(lds + 2 * REG + off + 16 * t));
+            const vec bl = __builtin_bit_cast(vec, *reinterpret_cast<const uint4 *>(lds + 3 * REG + off + 16 * t));
+            acc2 = Op::mfma(a, bl, acc2);
+            acc2 = Op::mfma(al, b, acc2);
+        }
+    }
+    if constexpr (SPLIT) acc += acc2;
+    __syncthreads();  // operand reads done before the next subcarrier rewrites LDS
+
+    // acc[j] = D[4g + j][r]; undo the scaling (exact power of two)
+    const int e = ec + ey;
+    float d[4] = {acc[0], acc[1], acc[2], acc[3]};
+    if (e != 0) {
+#pragma unroll
+        for (int j = 0; j < 4; ++j) d[j] = ldexpf(d[j], e);
+    }
+    // pair real and imaginary parts: lanes g<2 hold Re of layers 4g..4g+3, lanes g>=2 hold Im of layers
+    // 4(g-2)..4(g-2)+3; after one xor-32 exchange each lane holds two complete symbols
+    const bool lo = g < 2;
+    const float x0 = lo ? d[2] : d[0];
+    const float x1 = lo ? d[3] : d[1];
+    const float y0 = __shfl_xor(x0, 32, 64);
+    const float y1 = __shfl_xor(x1, 32, 64);
+    const int l0 = lo ? 4 * g : 4 * (g - 2) + 2;
+    const float2 s0 = lo ? make_float2(d[0], y0) : make_float2(y0, d[2]);
+    const float2 s1 = lo ? make_float2(d[1], y1) : make_float2(y1, d[3]);
+    const int n = r;
+    if (n < S) {
+        if constexpr (NPB == 0) {
+            if (l0 < L) O[(sc * L + l0) * S + n] = s0;
+            if (l0 + 1 < L) O[(sc * L + l0 + 1) * S + n] = s1;
+        } else {
+            if (l0 < L) {
+                float v[2 * NPB];
+                qam_llr<NPB>(s0.x, s0.y, noise_inv[sc * L + l0], v);
+                store_llr<NPB>(llr + ((sc * L + l0) * S + n) * (2 * NPB), v);
+            }
+            if (l0 + 1 < L) {
+                float v[2 * NPB];
+                qam_llr<NPB>(s1.x, s1.y, noise_inv[sc * L + l0 + 1], v);
+                store_llr<NPB>(llr + ((sc * L + l0 + 1) * S + n) * (2 * NPB), v);
+            }
+        }
+    }
+}
+
+// Subcarriers [first, last) of one cell by one wave (a one-wave workgroup): the next subcarrier's raw data
+// is loaded into a second register set while the current one is converted, multiplied and demapped, so the
+// wave always has one subcarrier of loads in flight.
+template <int KS, int PREC, int NPB>
+__device__ __forceinline__ void apply_range(const float2 *__restrict__ C, const float2 *__restrict__ Y,
+                                            float2 *__restrict__ O, const float *__restrict__ noise_inv,
+                                            float *__restrict__ llr, long first, long last, int L, int S, int A,
+                                            unsigned char *lds, int lane) {
+    const int nC = L * (A >> 1), nY = S * (A >> 1);
+    const int rc = (nC + 63) >> 6, ry = (nY + 63) >> 6;
+    RawSC<KS> ra, rb;
+    load_raw<KS>(ra, C, Y, (size_t)first, L, S, A, lane, nC, nY, rc, ry);
+    for (long sc = first; sc < last; sc += 2) {
+        if (sc + 1 < last) load_raw<KS>(rb, C, Y, (size_t)(sc + 1), L, S, A, lane, nC, nY, rc, ry);
+        process_sc<KS, PREC, NPB>(ra, (size_t)sc, lds, O, noise_inv, llr, L, S, A, lane, nC, nY, rc, ry);
+        if (sc + 1 >= last) break;
+        if (sc + 2 < last) load_raw<KS>(ra, C, Y, (size_t)(sc + 2), L, S, A, lane, nC, nY, rc, ry);
+        process_sc<KS, PREC, NPB>(rb, (size_t)(sc + 1), lds, O, noise_inv, llr, L, S, A, lane, nC, nY, rc, ry);
+    }
+}
+
+}  // namespace mfma
+}  // namespace pe

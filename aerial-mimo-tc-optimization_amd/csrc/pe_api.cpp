@@ -12,7 +12,7 @@
 #include <string>
 #include <vector>
 
-#include "../../include/pusch_eq.h"
+#include "pusch_eq.h"
 #include "pe_internal.hpp"
 
 namespace {
@@ -58,11 +58,12 @@ int load_table_file(const char *path) {
     while (fgets(line, sizeof(line), f)) {
         if (line[0] == '#' || line[0] == '\n') continue;
         TileEntry e;
-        int kc = 0, waves = 0;
-        // format: coef <L> <S> <A> <prec> <f32_kc> <mfma_waves>   (-1 = any)
-        if (sscanf(line, "coef %d %d %d %d %d %d", &e.L, &e.S, &e.A, &e.prec, &kc, &waves) == 6) {
+        int kc = 0, waves = 0, spw = 0;
+        // format: coef <L> <S> <A> <prec> <f32_kc> <mfma_waves> <mfma_spw>   (-1 = any)
+        if (sscanf(line, "coef %d %d %d %d %d %d %d", &e.L, &e.S, &e.A, &e.prec, &kc, &waves, &spw) == 7) {
             e.tile.f32_kc = kc;
             e.tile.mfma_waves = waves;
+            e.tile.mfma_spw = spw;
             t.push_back(e);
         }
     }
@@ -77,6 +78,21 @@ bool check_ptrs(const void *a, const void *b, const void *c) { return a && b && 
 }  // namespace
 
 namespace pe {
+void set_error(const char *msg) { g_err = msg ? msg : ""; }
+
+int device_cus() {
+    static int cus = 0;
+    if (!cus) {
+        int dev = 0, n = 0;
+        if (hipGetDevice(&dev) == hipSuccess &&
+            hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && n > 0)
+            cus = n;
+        else
+            return 256;
+    }
+    return cus;
+}
+
 CoefTile coef_tile_for(int L, int S, int A, int prec) {
     std::lock_guard<std::mutex> lk(g_table_mu);
     for (const TileEntry &e : g_table) {

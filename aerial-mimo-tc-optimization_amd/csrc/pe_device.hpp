@@ -43,8 +43,11 @@ template <> struct PamLut<4> {
 // One PAM axis: folding chain sb_{i+1} = D_i - |sb_i|, md_i = (dist + |sb_i|)^2, m1 = (|sb_last| - dist)^2,
 // llr_i = +-(md_i - m1) * noise (negative when sb_i < 0), clamped to +-65504. Same operation sequence as
 // llr_v1_baseline (aerial_llr_bench_v2.cu:134-155), so FP32 results are bit-identical to it.
+// Every operation is rounded on its own (no fused multiply-add contraction, e.g. of md_i - m1 with
+// md_i = m*m), as the source lines of the reference read; the host restatement does the same.
 template <int NPB>
 __device__ __forceinline__ void pam_llr_axis(float x, float noise, float (&out)[NPB]) {
+#pragma clang fp contract(off)
     const float dist = PamLut<NPB>::D[NPB - 1];
     float sb = x;
     unsigned sign = 0;

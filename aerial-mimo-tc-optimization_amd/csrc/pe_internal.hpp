@@ -10,9 +10,15 @@ namespace pe {
 // Tile parameters chosen per shape (defaults, overridable by the tuned table).
 struct CoefTile {
     int f32_kc = 8;         // FP32 kernel: antennas per LDS chunk (8 or 16)
-    int mfma_waves = 4;     // MFMA kernel: waves (subcarriers) per workgroup (1, 2, 4)
+    int mfma_waves = 4;     // MFMA kernel: waves per workgroup (1, 2, 4)
+    int mfma_spw = 0;       // MFMA kernel: subcarriers per wave (0 = derived from the grid size)
 };
 CoefTile coef_tile_for(int nLayer, int nSym, int nAnt, int prec);
+
+// Compute units of the current device (cached per process).
+int device_cus();
+// Sets the thread-local message returned by pe_last_error().
+void set_error(const char *msg);
 
 // FP32 antenna-serial coefficient apply.
 hipError_t launch_coef_apply_f32(const float2 *C, const float2 *Y, float2 *O, int nSC, int L, int S, int A,

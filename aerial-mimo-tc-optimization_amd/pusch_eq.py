@@ -41,7 +41,19 @@ SIGNATURES = {
     "pe_llr_demap_f16": ([_P, _P, _P, _I, _I, _P], _I),
     "pe_llr_demap": ([_P, _P, _P, _I, _I, _I, _I, _P], _I),
     "pe_coef_apply_demap": ([_P, _P, _P, _P, _I, _I, _I, _I, _I, _I, _P], _I),
+    "pe_slot_plan_create": ([_P, _I, _U, _P], _I),
+    "pe_slot_plan_update": ([_P, _P, _I, _P], _I),
+    "pe_slot_plan_launch": ([_P, _P], _I),
+    "pe_slot_plan_graph_launch": ([_P, _P], _I),
+    "pe_slot_plan_destroy": ([_P], None),
+    "pe_slot_plan_info": ([_P, _P, _P], _I),
 }
+
+
+class CellDesc(ctypes.Structure):
+    """pe_cell_desc of include/pusch_eq.h."""
+    _fields_ = [("H", _P), ("sigma2", _P), ("C", _P), ("noise_inv", _P), ("Y_rx", _P), ("llr", _P),
+                ("nSC", _I), ("nLayer", _I), ("nSym", _I), ("nAnt", _I), ("nPamBits", _I), ("prec", _I)]
 
 
 class PeError(RuntimeError):
@@ -57,6 +69,10 @@ def lib():
     """Loads libpusch_eq.so once; raises if it has not been built."""
     global _lib
     if _lib is None:
+        # torch ships its own HIP runtime (libamdhip64.so.7). Importing torch first makes this library bind
+        # to that same runtime instance, so device pointers and streams from torch are valid here; loading
+        # this library first would pull a second runtime from /opt/rocm into the process.
+        import torch  # noqa: F401
         if not os.path.exists(LIB_PATH):
             raise RuntimeError("%s is missing: build it with __graft_entry__.build()" % LIB_PATH)
         h = ctypes.CDLL(LIB_PATH)
@@ -177,3 +193,48 @@ def coef_apply_demap(C, Y, noise_inv, n_pam_bits, prec="f16", out=None, stream=N
     _check(lib().pe_coef_apply_demap(_ptr(C), _ptr(Y), _ptr(noise_inv), _ptr(out), nSC, L, S, A, n_pam_bits,
                                      _prec(prec), _stream(stream)))
     return out
+
+
+def cell_desc(C, noise_inv, Y, llr, n_pam_bits, prec="f16", H=None, sigma2=None):
+    """Builds a pe_cell_desc from torch tensors (H/sigma2 None: coefficients are inputs)."""
+    nSC, L, A = C.shape
+    return CellDesc(H.data_ptr() if H is not None else None, sigma2.data_ptr() if sigma2 is not None else None,
+                    C.data_ptr(), noise_inv.data_ptr(), Y.data_ptr(), llr.data_ptr(), nSC, L, Y.shape[1], A,
+                    n_pam_bits, _prec(prec))
+
+
+class SlotPlan:
+    """Owner of a pe_slot_plan: one slot of many cells as persistent work-queue kernels."""
+
+    def __init__(self, cells, unbiased=True):
+        arr = (CellDesc * len(cells))(*cells)
+        h = ctypes.c_void_p()
+        _check(lib().pe_slot_plan_create(arr, len(cells), MMSE_UNBIASED if unbiased else 0, ctypes.byref(h)))
+        self._h = h
+        self.n_cells = len(cells)
+
+    def update(self, cells, stream=None):
+        arr = (CellDesc * len(cells))(*cells)
+        _check(lib().pe_slot_plan_update(self._h, arr, len(cells), _stream(stream)))
+
+    def launch(self, stream=None):
+        _check(lib().pe_slot_plan_launch(self._h, _stream(stream)))
+
+    def graph_launch(self, stream=None):
+        _check(lib().pe_slot_plan_graph_launch(self._h, _stream(stream)))
+
+    def info(self):
+        a, b = ctypes.c_int(), ctypes.c_int()
+        _check(lib().pe_slot_plan_info(self._h, ctypes.byref(a), ctypes.byref(b)))
+        return a.value, b.value
+
+    def close(self):
+        if self._h:
+            lib().pe_slot_plan_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

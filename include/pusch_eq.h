@@ -137,6 +137,41 @@ pe_status pe_coef_apply_demap(const pe_c64 *C, const pe_c64 *Y_rx, const float *
                               int nSC, int nLayer, int nSym, int nAnt, int nPamBits, pe_prec prec,
                               void *stream);
 
+/* ---- multi-cell slot: one plan per set of uneven cells, run by persistent work-queue kernels -------- */
+
+typedef struct {
+    /* stage 1 (optional, skipped when H == NULL): MMSE coefficients into C / noise_inv */
+    const pe_c64 *H;          /* [nSC][nAnt][nLayer] */
+    const float *sigma2;      /* [nSC] */
+    /* stage 2: fused apply + demap */
+    pe_c64 *C;                /* [nSC][nLayer][nAnt]; written by stage 1, read by stage 2 */
+    float *noise_inv;         /* [nSC][nLayer] */
+    const pe_c64 *Y_rx;       /* [nSC][nSym][nAnt] */
+    float *llr;               /* [nSC][nLayer][nSym][2*nPamBits] */
+    int nSC, nLayer, nSym, nAnt, nPamBits;
+    int prec;                 /* pe_prec of stage 2: the MFMA precisions (PE_PREC_F16 .. PE_PREC_BF16_SPLIT) */
+} pe_cell_desc;
+
+typedef struct pe_slot_plan pe_slot_plan;
+
+/* Builds the tile lists of a slot (per kernel class, cells ordered by per-subcarrier cost, cut into
+   subcarrier tiles) and uploads descriptors, tiles and work-queue counters. Allocates its device memory
+   (descriptors, tiles, counters, per-subcarrier MMSE flags) once, here. flags: PE_MMSE_UNBIASED for stage 1.
+   Shapes per cell: nSC >= 1, 1 <= nLayer <= 8, 1 <= nSym <= 16, 4 <= nAnt <= 128 with nAnt % 4 == 0.
+   Replaces the reference's one-launch-per-cell-per-stream protocol (multicell_contention.cu:194-283,
+   multicell_evidence.cu:497-566). */
+pe_status pe_slot_plan_create(const pe_cell_desc *cells, int nCells, unsigned flags, pe_slot_plan **plan);
+/* Rewrites the per-cell pointers of an existing plan (same shapes) for the next slot, ordered on `stream`. */
+pe_status pe_slot_plan_update(pe_slot_plan *plan, const pe_cell_desc *cells, int nCells, void *stream);
+/* Enqueues the slot on `stream`: stage 1 then stage 2, each a few persistent kernels. Graph-capturable: the
+   work-queue counters reset themselves when the last wave leaves, no memset node is recorded. */
+pe_status pe_slot_plan_launch(pe_slot_plan *plan, void *stream);
+/* Captures pe_slot_plan_launch into a hipGraph on first use and replays it on `stream`. */
+pe_status pe_slot_plan_graph_launch(pe_slot_plan *plan, void *stream);
+void pe_slot_plan_destroy(pe_slot_plan *plan);
+/* Tiles and largest persistent grid of the plan (for reports). */
+pe_status pe_slot_plan_info(const pe_slot_plan *plan, int *nTiles, int *nWorkgroups);
+
 #ifdef __cplusplus
 }
 #endif

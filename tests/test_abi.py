@@ -77,3 +77,19 @@ def test_other_entry_points_validate(pe):
     assert h.pe_gram(p, p, p, 4, 8, 0, None) == 1
     assert h.pe_coef_apply_demap(p, p, p, p, 4, 8, 11, 64, 5, 1, None) == 1
     assert h.pe_coef_apply_range(p, p, p, 4, -1, 8, 11, 64, 0, None) == 1
+
+
+def test_slot_plan_rejects_bad_cells_without_a_device(pe):
+    h = pe.lib()
+    out = ctypes.c_void_p()
+    one = ctypes.c_void_p(16)
+    good = pe.CellDesc(None, None, one, one, one, one, 12, 8, 11, 64, 4, 1)
+    bad_shape = pe.CellDesc(None, None, one, one, one, one, 12, 9, 11, 64, 4, 1)
+    bad_prec = pe.CellDesc(None, None, one, one, one, one, 12, 8, 11, 64, 4, 0)
+    no_llr = pe.CellDesc(None, None, one, one, one, None, 12, 8, 11, 64, 4, 1)
+    for cell, status in ((bad_shape, 1), (bad_prec, 2), (no_llr, 1)):
+        arr = (pe.CellDesc * 2)(good, cell)
+        assert h.pe_slot_plan_create(arr, 2, 1, ctypes.byref(out)) == status
+        assert b"cell 1" in h.pe_last_error()
+    assert h.pe_slot_plan_create(None, 0, 1, ctypes.byref(out)) == 1
+    assert h.pe_slot_plan_launch(None, None) == 1
