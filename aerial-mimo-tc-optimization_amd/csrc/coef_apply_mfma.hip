@@ -12,6 +12,25 @@ using namespace mfma;
 // PREC: 1 f16, 2 bf16, 3 f16 split, 4 bf16 split.  NPB: 0 -> store Y_eq, 1..4 -> fused demap.
 // One-wave workgroups; each streams `spw` consecutive subcarriers and loads the next subcarrier's raw data
 // into a second register set while the current one is converted and multiplied.
+// One subcarrier per one-wave workgroup: the smallest register footprint, so the most waves (and loads)
+// in flight per CU.
+template <int KS, int PREC, int NPB>
+__global__ __launch_bounds__(64) void coef_apply_mfma_sc_kernel(
+    const float2 *__restrict__ C, const float2 *__restrict__ Y, float2 *__restrict__ O,
+    const float *__restrict__ noise_inv, float *__restrict__ llr, int nSC, int L, int S, int A) {
+    __shared__ __attribute__((aligned(16))) unsigned char lds[LdsLayout<KS>::bytes(PREC)];
+    const long sc = blockIdx.x;
+    if (sc >= nSC) return;
+    const int lane = threadIdx.x;
+    if (A < 16 * KS) zero_operands<KS, PREC>(lds, lane);
+    const int nC = L * (A >> 1), nY = S * (A >> 1);
+    const int rc = (nC + 63) >> 6, ry = (nY + 63) >> 6;
+    RawSC<KS> ra;
+    load_raw<KS>(ra, C, Y, (size_t)sc, L, S, A, lane, nC, nY, rc, ry);
+    process_sc<KS, PREC, NPB>(ra, (size_t)sc, lds, O, noise_inv, llr, L, S, A, lane, nC, nY, rc, ry);
+}
+
+// `spw` consecutive subcarriers per one-wave workgroup, the next one prefetched into a second register set.
 template <int KS, int PREC, int NPB>
 __global__ __launch_bounds__(64) void coef_apply_mfma_kernel(
     const float2 *__restrict__ C, const float2 *__restrict__ Y, float2 *__restrict__ O,
@@ -28,15 +47,15 @@ template <int KS, int PREC, int NPB>
 hipError_t launch3(const float2 *C, const float2 *Y, float2 *O, const float *ni, float *llr, int nSC, int L,
                    int S, int A, hipStream_t st) {
     const CoefTile t = coef_tile_for(L, S, A, PREC);
-    int spw = t.mfma_spw;
-    if (spw <= 0) {  // enough waves to fill the chip (8 per CU) before giving a wave more subcarriers
-        const long waves_target = (long)device_cus() * 8;
-        spw = (int)(nSC / waves_target);
-        spw = spw < 1 ? 1 : (spw > 8 ? 8 : spw);
+    const int spw = t.mfma_spw > 1 ? t.mfma_spw : 1;
+    if (spw == 1) {
+        hipLaunchKernelGGL((coef_apply_mfma_sc_kernel<KS, PREC, NPB>), dim3((unsigned)nSC), dim3(64), 0, st, C, Y,
+                           O, ni, llr, nSC, L, S, A);
+    } else {
+        const long grid = (nSC + spw - 1) / spw;
+        hipLaunchKernelGGL((coef_apply_mfma_kernel<KS, PREC, NPB>), dim3((unsigned)grid), dim3(64), 0, st, C, Y,
+                           O, ni, llr, nSC, L, S, A, spw);
     }
-    const long grid = (nSC + spw - 1) / spw;
-    hipLaunchKernelGGL((coef_apply_mfma_kernel<KS, PREC, NPB>), dim3((unsigned)grid), dim3(64), 0, st, C, Y, O,
-                       ni, llr, nSC, L, S, A, spw);
     return hipGetLastError();
 }
 

@@ -86,7 +86,7 @@ __global__ __launch_bounds__(64) void slot_apply_kernel(const CellDev *__restric
         case 3: apply_range<KS, PREC, 3>(cd.C, cd.Y, nullptr, cd.noise_inv, cd.llr, first, last, cd.L, cd.S, cd.A, lds, lane); break;
         default: apply_range<KS, PREC, 4>(cd.C, cd.Y, nullptr, cd.noise_inv, cd.llr, first, last, cd.L, cd.S, cd.A, lds, lane); break;
         }
-        __syncthreads();  // LDS of this tile fully consumed before the next tile may zero it
+        lds_order_wave();  // LDS of this tile consumed before the next tile may zero it
     }
     leave_queue(q, lane, nwaves);
 }

@@ -11,7 +11,7 @@ namespace pe {
 struct CoefTile {
     int f32_kc = 8;         // FP32 kernel: antennas per LDS chunk (8 or 16)
     int mfma_waves = 4;     // MFMA kernel: waves per workgroup (1, 2, 4)
-    int mfma_spw = 0;       // MFMA kernel: subcarriers per wave (0 = derived from the grid size)
+    int mfma_spw = 1;       // MFMA kernel: subcarriers per one-wave workgroup (1 = no prefetch loop)
 };
 CoefTile coef_tile_for(int nLayer, int nSym, int nAnt, int prec);
 

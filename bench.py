@@ -143,6 +143,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--latency-iters", type=int, default=300)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run 5 plain 256 MiB device copies (scale reference for the HBM byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path (cells shard across GPUs with "
@@ -150,6 +152,15 @@ def main():
         sys.exit(2)
 
     import torch
+    if args.copy_calibration:
+        a = torch.empty(256 << 20, dtype=torch.uint8, device="cuda:0")
+        b = torch.empty_like(a)
+        a.fill_(1)
+        for _ in range(5):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibration_bytes_per_copy": a.numel(), "copies": 5}))
+        return
     import pusch_eq as pe
     dev = torch.device("cuda:0")
     pe.init(0)
@@ -210,7 +221,7 @@ def main():
         if i >= 20:
             lat.append(a.elapsed_time(b))
     lat.sort()
-    pct = lambda p: lat[min(len(lat) - 1, max(0, int(-(-p * len(lat) // 100)) - 1))]
+    pct = lambda p: lat[min(len(lat) - 1, max(0, int(-(-p * len(lat) // 100)) - 1))] if lat else None
 
     cpu = None
     if not args.no_cpu:
@@ -245,7 +256,7 @@ def main():
         "plain_f32": {"value": nsc / per32, "ms_per_step": per32 * 1e3,
                       "hbm_gbs": bytes_launch / per32 / 1e9},
         "f16_split": {"value": nsc / persp, "ms_per_step": persp * 1e3, "hbm_gbs": bytes_launch / persp / 1e9},
-        "slot_latency_ms": {"p50": pct(50), "p99": pct(99), "max": lat[-1], "iters": len(lat),
+        "slot_latency_ms": {"p50": pct(50), "p99": pct(99), "max": lat[-1] if lat else None, "iters": len(lat),
                             "workload": "one cell: pe_gram_mmse + fused apply/demap, events on one stream"},
         "wall_s_timed": wall,
     }
