@@ -183,11 +183,29 @@ __device__ __forceinline__ void zero_operands(unsigned char *lds, int lane) {
     lds_order_wave();
 }
 
+// Max-log LLRs of the two symbols (layers l0, l0 + 1, symbol n) a lane holds after the re/im exchange.
+template <int NPB>
+__device__ __forceinline__ void demap_store(float2 s0, float2 s1, size_t sc, int l0, int n, int L, int S,
+                                            const float *__restrict__ noise_inv, float *__restrict__ llr) {
+    if (l0 < L) {
+        float v[2 * NPB];
+        qam_llr<NPB>(s0.x, s0.y, noise_inv[sc * L + l0], v);
+        store_llr<NPB>(llr + ((sc * L + l0) * S + n) * (2 * NPB), v);
+    }
+    if (l0 + 1 < L) {
+        float v[2 * NPB];
+        qam_llr<NPB>(s1.x, s1.y, noise_inv[sc * L + l0 + 1], v);
+        store_llr<NPB>(llr + ((sc * L + l0 + 1) * S + n) * (2 * NPB), v);
+    }
+}
+
 // One subcarrier: range scaling, operand conversion into LDS, MFMA, epilogue to Y_eq or LLRs.
+// NPB: 0 -> store Y_eq; 1..4 -> fused demap of that QAM order; -1 -> fused demap of order npb_rt.
 template <int KS, int PREC, int NPB>
 __device__ __forceinline__ void process_sc(RawSC<KS> &R, size_t sc, unsigned char *lds, float2 *__restrict__ O,
                                            const float *__restrict__ noise_inv, float *__restrict__ llr, int L,
-                                           int S, int A, int lane, int nC, int nY, int rc, int ry) {
+                                           int S, int A, int lane, int nC, int nY, int rc, int ry,
+                                           int npb_rt = 0) {
     constexpr bool BF = (PREC == 2 || PREC == 4);
     constexpr bool SPLIT = (PREC == 3 || PREC == 4);
     constexpr int AG = 4 * KS;
@@ -199,6 +217,8 @@ __device__ __forceinline__ void process_sc(RawSC<KS> &R, size_t sc, unsigned cha
     const int g = lane >> 4;
     const int r = lane & 15;
     const int cpr = A >> 1;  // 16-byte chunks per row
+    // c / cpr as a multiply-shift (error < c / 2^20 < 1 / cpr): exact for every c < 3072, cpr <= 128
+    const unsigned cpr_mag = ((1u << 20) + (unsigned)cpr - 1u) / (unsigned)cpr;
 
     // per-subcarrier power-of-two scales of C and Y
     float mc = 0.f, my = 0.f;
@@ -234,7 +254,7 @@ __device__ __forceinline__ void process_sc(RawSC<KS> &R, size_t sc, unsigned cha
         if (j < rc && c < nC) {
             float4 v = R.c[j];
             if (scaled) { v.x *= sc_c; v.y *= sc_c; v.z *= sc_c; v.w *= sc_c; }
-            const int row = c / cpr, part = c - row * cpr;
+            const int row = (int)(((unsigned)c * cpr_mag) >> 20), part = c - row * cpr;
             const uint32_t p0 = pack2<BF>(v.x, v.y), p1 = pack2<BF>(v.z, v.w);
             *reinterpret_cast<uint2 *>(lds + row * RB + part * 8) = make_uint2(p0, p1);
             if constexpr (SPLIT) {
@@ -250,7 +270,7 @@ __device__ __forceinline__ void process_sc(RawSC<KS> &R, size_t sc, unsigned cha
         if (j < ry && c < nY) {
             float4 v = R.y[j];
             if (scaled) { v.x *= sc_y; v.y *= sc_y; v.z *= sc_y; v.w *= sc_y; }
-            const int col = c / cpr, part = c - col * cpr;
+            const int col = (int)(((unsigned)c * cpr_mag) >> 20), part = c - col * cpr;
             const uint32_t p0 = pack2<BF>(v.x, v.y), p1 = pack2<BF>(v.z, v.w);
             *reinterpret_cast<uint2 *>(lds + CREG + col * RB + part * 8) = make_uint2(p0, p1);
             if constexpr (SPLIT) {
@@ -314,39 +334,58 @@ __device__ __forceinline__ void process_sc(RawSC<KS> &R, size_t sc, unsigned cha
         if constexpr (NPB == 0) {
             if (l0 < L) O[(sc * L + l0) * S + n] = s0;
             if (l0 + 1 < L) O[(sc * L + l0 + 1) * S + n] = s1;
-        } else {
-            if (l0 < L) {
-                float v[2 * NPB];
-                qam_llr<NPB>(s0.x, s0.y, noise_inv[sc * L + l0], v);
-                store_llr<NPB>(llr + ((sc * L + l0) * S + n) * (2 * NPB), v);
-            }
-            if (l0 + 1 < L) {
-                float v[2 * NPB];
-                qam_llr<NPB>(s1.x, s1.y, noise_inv[sc * L + l0 + 1], v);
-                store_llr<NPB>(llr + ((sc * L + l0 + 1) * S + n) * (2 * NPB), v);
+        } else if constexpr (NPB > 0) {
+            demap_store<NPB>(s0, s1, sc, l0, n, L, S, noise_inv, llr);
+        } else {  // QAM order chosen at run time (multi-cell kernels): only the epilogue is duplicated
+            switch (npb_rt) {
+            case 1: demap_store<1>(s0, s1, sc, l0, n, L, S, noise_inv, llr); break;
+            case 2: demap_store<2>(s0, s1, sc, l0, n, L, S, noise_inv, llr); break;
+            case 3: demap_store<3>(s0, s1, sc, l0, n, L, S, noise_inv, llr); break;
+            default: demap_store<4>(s0, s1, sc, l0, n, L, S, noise_inv, llr); break;
             }
         }
     }
 }
 
-// Subcarriers [first, last) of one cell by one wave (a one-wave workgroup): the next subcarrier's raw data
-// is loaded into a second register set while the current one is converted, multiplied and demapped, so the
-// wave always has one subcarrier of loads in flight.
+// One subcarrier as a separate function: called in a loop, it keeps the per-subcarrier registers short-lived
+// (inlined into a loop, the compiler hoists per-lane address arithmetic out of it and doubles the registers).
 template <int KS, int PREC, int NPB>
+__device__ __attribute__((noinline)) void apply_one_sc(const float2 *__restrict__ C, const float2 *__restrict__ Y,
+                                                      float2 *__restrict__ O, const float *__restrict__ noise_inv,
+                                                      float *__restrict__ llr, long sc, int L, int S, int A,
+                                                      unsigned char *lds, int lane, int npb_rt) {
+    const int nC = L * (A >> 1), nY = S * (A >> 1);
+    const int rc = (nC + 63) >> 6, ry = (nY + 63) >> 6;
+    RawSC<KS> ra;
+    load_raw<KS>(ra, C, Y, (size_t)sc, L, S, A, lane, nC, nY, rc, ry);
+    process_sc<KS, PREC, NPB>(ra, (size_t)sc, lds, O, noise_inv, llr, L, S, A, lane, nC, nY, rc, ry, npb_rt);
+}
+
+// Subcarriers [first, last) of one cell by one wave (a one-wave workgroup). PIPE: the next subcarrier's raw
+// data is loaded into a second register set while the current one is processed (one subcarrier of loads
+// always in flight, at about 3x the registers); without it the wave relies on other resident waves to
+// overlap its loads, which measured faster on MI355X.
+template <int KS, int PREC, int NPB, bool PIPE>
 __device__ __forceinline__ void apply_range(const float2 *__restrict__ C, const float2 *__restrict__ Y,
                                             float2 *__restrict__ O, const float *__restrict__ noise_inv,
                                             float *__restrict__ llr, long first, long last, int L, int S, int A,
-                                            unsigned char *lds, int lane) {
+                                            unsigned char *lds, int lane, int npb_rt = 0) {
     const int nC = L * (A >> 1), nY = S * (A >> 1);
     const int rc = (nC + 63) >> 6, ry = (nY + 63) >> 6;
+    if constexpr (!PIPE) {
+#pragma nounroll
+        for (long sc = first; sc < last; ++sc)
+            apply_one_sc<KS, PREC, NPB>(C, Y, O, noise_inv, llr, sc, L, S, A, lds, lane, npb_rt);
+        return;
+    }
     RawSC<KS> ra, rb;
     load_raw<KS>(ra, C, Y, (size_t)first, L, S, A, lane, nC, nY, rc, ry);
     for (long sc = first; sc < last; sc += 2) {
         if (sc + 1 < last) load_raw<KS>(rb, C, Y, (size_t)(sc + 1), L, S, A, lane, nC, nY, rc, ry);
-        process_sc<KS, PREC, NPB>(ra, (size_t)sc, lds, O, noise_inv, llr, L, S, A, lane, nC, nY, rc, ry);
+        process_sc<KS, PREC, NPB>(ra, (size_t)sc, lds, O, noise_inv, llr, L, S, A, lane, nC, nY, rc, ry, npb_rt);
         if (sc + 1 >= last) break;
         if (sc + 2 < last) load_raw<KS>(ra, C, Y, (size_t)(sc + 2), L, S, A, lane, nC, nY, rc, ry);
-        process_sc<KS, PREC, NPB>(rb, (size_t)(sc + 1), lds, O, noise_inv, llr, L, S, A, lane, nC, nY, rc, ry);
+        process_sc<KS, PREC, NPB>(rb, (size_t)(sc + 1), lds, O, noise_inv, llr, L, S, A, lane, nC, nY, rc, ry, npb_rt);
     }
 }
 

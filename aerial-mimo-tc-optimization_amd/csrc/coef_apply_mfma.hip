@@ -40,7 +40,7 @@ __global__ __launch_bounds__(64) void coef_apply_mfma_kernel(
     if (first >= nSC) return;  // uniform per wave
     const long last = first + spw < nSC ? first + spw : nSC;
     if (A < 16 * KS) zero_operands<KS, PREC>(lds, threadIdx.x);
-    apply_range<KS, PREC, NPB>(C, Y, O, noise_inv, llr, first, last, L, S, A, lds, threadIdx.x);
+    apply_range<KS, PREC, NPB, true>(C, Y, O, noise_inv, llr, first, last, L, S, A, lds, threadIdx.x);
 }
 
 template <int KS, int PREC, int NPB>

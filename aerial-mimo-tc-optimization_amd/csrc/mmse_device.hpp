@@ -48,6 +48,8 @@ __device__ __forceinline__ bool mmse_one(const float2 *__restrict__ h, T s2, int
     for (int k = 0; k < NT; ++k) gr[k] = gi[k] = T(0);
 
     // partial Gram over this lane's antennas: G[i][j] = sum_a conj(H[a][i]) H[a][j], i >= j
+    // (unrolled so several antenna rows are in flight at once instead of one memory latency per row)
+#pragma unroll 4
     for (int a = p; a < A; a += P) {
         T vr[L], vi[L];
         const float2 *row = h + (size_t)a * L;
@@ -189,6 +191,7 @@ __device__ __forceinline__ bool mmse_one(const float2 *__restrict__ h, T s2, int
     }
 
     // C[l][a] = s_l * sum_j Ginv[l][j] conj(H[a][j])
+#pragma unroll 4
     for (int a = p; a < A; a += P) {
         T hr[L], hi[L];
         const float2 *row = h + (size_t)a * L;

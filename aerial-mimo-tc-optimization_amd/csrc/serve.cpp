@@ -1,0 +1,317 @@
+// serve.cpp -- slot serving loop (pe_serve_run): PUSCH slot jobs from many cells arrive as a Poisson process,
+// are admitted against a device-memory budget, and are launched in continuous batches on several streams.
+//
+// The reference measures N cells launched together and checks one 0.5 ms deadline
+// (multicell_evidence.cu:497-566); it has no arrival process. Here one host thread runs the loop:
+//   1. completions: a batch whose event has completed releases its jobs' device buffers; each job's latency is
+//      completion time minus its arrival time;
+//   2. arrivals: jobs whose arrival time has passed join a FIFO;
+//   3. admission and launch: whenever a stream is free, the FIFO head is admitted while its buffers (C,
+//      noise_inv, LLR) fit the device arena (the HBM budget), up to max_batch_jobs; the batch runs as one
+//      replay of that stream's captured slot graph (MMSE, then fused apply + demap, persistent work queues).
+// Jobs that arrive while batches are on the GPU are admitted as soon as a stream and memory free up, so the
+// batches overlap on the device. Each job's channel estimate, MMSE coefficients and samples stay
+// device-resident; the coefficients are computed once per job and applied to all of its data symbols.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <deque>
+#include <map>
+#include <memory>
+#include <random>
+#include <string>
+#include <vector>
+
+#include "pe_internal.hpp"
+#include "pusch_eq.h"
+#include "slot_exec.hpp"
+
+namespace {
+
+pe_status serve_fail(pe_status s, const std::string &m) {
+    pe::set_error(m.c_str());
+    return s;
+}
+
+// First-fit allocator over one device arena (offsets only; 256-byte granules).
+class Arena {
+public:
+    explicit Arena(size_t bytes) { free_[0] = bytes; }
+    bool alloc(size_t n, size_t &off) {
+        n = (n + 255) & ~size_t(255);
+        for (auto it = free_.begin(); it != free_.end(); ++it)
+            if (it->second >= n) {
+                off = it->first;
+                const size_t rest = it->second - n;
+                free_.erase(it);
+                if (rest) free_[off + n] = rest;
+                return true;
+            }
+        return false;
+    }
+    void release(size_t off, size_t n) {
+        n = (n + 255) & ~size_t(255);
+        auto it = free_.emplace(off, n).first;
+        auto nx = std::next(it);
+        if (nx != free_.end() && it->first + it->second == nx->first) {
+            it->second += nx->second;
+            free_.erase(nx);
+        }
+        if (it != free_.begin()) {
+            auto pv = std::prev(it);
+            if (pv->first + pv->second == it->first) {
+                pv->second += it->second;
+                free_.erase(it);
+            }
+        }
+    }
+
+private:
+    std::map<size_t, size_t> free_;
+};
+
+struct JobBuf {
+    size_t off = 0, bytes = 0;
+};
+
+size_t job_bytes(const pe_cell_desc &c, bool llr_in_arena) {
+    const size_t nsc = (size_t)c.nSC;
+    size_t b = ((nsc * c.nLayer * c.nAnt * 8 + 255) & ~size_t(255)) + ((nsc * c.nLayer * 4 + 255) & ~size_t(255));
+    if (llr_in_arena) b += (nsc * c.nLayer * c.nSym * 2 * c.nPamBits * 4 + 255) & ~size_t(255);
+    return b;
+}
+
+double nearest_rank(std::vector<double> v, double p) {
+    if (v.empty()) return 0.0;
+    std::sort(v.begin(), v.end());
+    long i = (long)std::ceil(p / 100.0 * (double)v.size()) - 1;
+    i = std::max(0L, std::min((long)v.size() - 1, i));
+    return v[(size_t)i];
+}
+
+}  // namespace
+
+extern "C" pe_status pe_serve_run(const pe_serve_config *cfg, pe_serve_report *rep, double *job_latency_s,
+                                  int *job_cell) {
+    using clock = std::chrono::steady_clock;
+    if (!cfg || !rep) return serve_fail(PE_ERR_INVALID, "null config or report");
+    if (!cfg->cells || cfg->n_cells < 1) return serve_fail(PE_ERR_INVALID, "no cells");
+    if (cfg->n_jobs < 1) return serve_fail(PE_ERR_INVALID, "n_jobs < 1");
+    if (cfg->max_batch_jobs < 1 || cfg->max_inflight < 1 || cfg->max_inflight > 8)
+        return serve_fail(PE_ERR_INVALID, "max_batch_jobs >= 1 and 1 <= max_inflight <= 8 required");
+    if (!(cfg->deadline_s > 0.0)) return serve_fail(PE_ERR_INVALID, "deadline_s must be positive");
+    const bool llr_in_arena = cfg->job_llr == nullptr;
+    std::vector<pe_cell_desc> pop(cfg->cells, cfg->cells + cfg->n_cells);
+    size_t largest = 0;
+    std::string err;
+    for (int i = 0; i < cfg->n_cells; ++i) {
+        pe_cell_desc c = pop[i];
+        if (!c.H || !c.sigma2 || !c.Y_rx)
+            return serve_fail(PE_ERR_INVALID, "cell " + std::to_string(i) + ": H, sigma2 and Y_rx are required");
+        c.C = reinterpret_cast<pe_c64 *>(16);  // job buffers come from the arena; validate shapes only
+        c.noise_inv = reinterpret_cast<float *>(16);
+        c.llr = reinterpret_cast<float *>(16);
+        pe_status s = pe::check_cell_desc(c, i, err);
+        if (s != PE_OK) return serve_fail(s, err);
+        largest = std::max(largest, job_bytes(c, llr_in_arena));
+    }
+    if (largest > cfg->hbm_budget_bytes)
+        return serve_fail(PE_ERR_CAPACITY, "hbm_budget_bytes is smaller than one job of the largest cell (" +
+                                               std::to_string(largest) + " bytes)");
+
+    // ---- arrival process (virtual times from the start of the run) ------------------------------------
+    const int nj = cfg->n_jobs;
+    std::vector<double> arrival(nj);
+    std::vector<int> cellof(nj);
+    {
+        std::mt19937_64 rng(cfg->seed);
+        std::exponential_distribution<double> ex(cfg->arrival_rate_hz > 0 ? cfg->arrival_rate_hz : 1.0);
+        std::uniform_int_distribution<int> pick(0, cfg->n_cells - 1);
+        double t = 0.0;
+        for (int j = 0; j < nj; ++j) {
+            if (cfg->arrival_rate_hz > 0) t += ex(rng);
+            arrival[j] = cfg->arrival_rate_hz > 0 ? t : 0.0;
+            cellof[j] = pick(rng);
+        }
+    }
+
+    // ---- device state ------------------------------------------------------------------------------------
+    unsigned char *arena_dev = nullptr;
+    hipError_t e = hipMalloc(&arena_dev, cfg->hbm_budget_bytes);
+    if (e != hipSuccess) return serve_fail(PE_ERR_NOMEM, std::string("arena: ") + hipGetErrorString(e));
+    const int K = cfg->max_inflight;
+    std::vector<std::unique_ptr<pe::SlotExec>> ex(K);
+    std::vector<hipStream_t> st(K, nullptr);
+    std::vector<hipEvent_t> ev(K, nullptr);
+    auto cleanup = [&]() {
+        for (int k = 0; k < K; ++k) {
+            if (st[k]) (void)hipStreamSynchronize(st[k]);
+            if (ev[k]) (void)hipEventDestroy(ev[k]);
+            if (st[k]) (void)hipStreamDestroy(st[k]);
+            ex[k].reset();
+        }
+        (void)hipFree(arena_dev);
+    };
+    for (int k = 0; k < K; ++k) {
+        ex[k].reset(new pe::SlotExec());
+        pe_status s = ex[k]->init(pop.data(), (int)pop.size(), cfg->max_batch_jobs, PE_MMSE_UNBIASED, err);
+        if (s == PE_OK && hipStreamCreateWithFlags(&st[k], hipStreamNonBlocking) != hipSuccess) s = PE_ERR_DEVICE;
+        if (s == PE_OK && hipEventCreateWithFlags(&ev[k], hipEventDisableTiming) != hipSuccess) s = PE_ERR_DEVICE;
+        if (s != PE_OK) {
+            cleanup();
+            return serve_fail(s, err.empty() ? "stream/event creation failed" : err);
+        }
+    }
+    Arena arena(cfg->hbm_budget_bytes);
+    std::vector<JobBuf> buf(nj);
+    auto make_desc = [&](int j, const JobBuf &b) {
+        pe_cell_desc d = pop[cellof[j]];
+        const size_t nsc = (size_t)d.nSC;
+        unsigned char *p = arena_dev + b.off;
+        d.C = reinterpret_cast<pe_c64 *>(p);
+        p += (nsc * d.nLayer * d.nAnt * 8 + 255) & ~size_t(255);
+        d.noise_inv = reinterpret_cast<float *>(p);
+        p += (nsc * d.nLayer * 4 + 255) & ~size_t(255);
+        d.llr = llr_in_arena ? reinterpret_cast<float *>(p) : cfg->job_llr[j];
+        return d;
+    };
+    auto submit = [&](int k, const std::vector<pe_cell_desc> &cells) -> pe_status {
+        pe_status s = ex[k]->stage(cells.data(), (int)cells.size(), err);
+        if (s == PE_OK) s = ex[k]->upload(st[k], err);
+        if (s == PE_OK) s = cfg->use_graph ? ex[k]->graph_launch(st[k], err) : ex[k]->launch(st[k], err);
+        if (s == PE_OK && hipEventRecord(ev[k], st[k]) != hipSuccess) {
+            s = PE_ERR_DEVICE;
+            err = "event record failed";
+        }
+        return s;
+    };
+
+    // warm-up: one batch of the first job's cell on every stream (captures the graphs; not measured)
+    {
+        JobBuf b;
+        if (!arena.alloc(job_bytes(pop[cellof[0]], llr_in_arena), b.off)) {
+            cleanup();
+            return serve_fail(PE_ERR_CAPACITY, "arena cannot hold one job");
+        }
+        b.bytes = job_bytes(pop[cellof[0]], llr_in_arena);
+        pe_cell_desc d = make_desc(0, b);  // LLRs go to job 0's destination, rewritten by the measured run
+        std::vector<pe_cell_desc> one(1, d);
+        for (int k = 0; k < K; ++k) {
+            pe_status s = submit(k, one);
+            if (s == PE_OK && hipStreamSynchronize(st[k]) != hipSuccess) s = PE_ERR_DEVICE;
+            if (s != PE_OK) {
+                cleanup();
+                return serve_fail(s, err);
+            }
+        }
+        arena.release(b.off, b.bytes);
+    }
+
+    // ---- loop ----------------------------------------------------------------------------------------------
+    std::vector<double> lat(nj, -1.0);
+    std::vector<std::vector<int>> batch_jobs(K);
+    std::vector<bool> busy(K, false);
+    std::deque<int> ready;
+    int next = 0, finished = 0, shed = 0, batches = 0, max_inflight_seen = 0;
+    long batched_jobs = 0;
+    std::vector<pe_cell_desc> cells;
+    const auto t0 = clock::now();
+    auto now_s = [&]() { return std::chrono::duration<double>(clock::now() - t0).count(); };
+    pe_status status = PE_OK;
+    while (finished + shed < nj) {
+        // 1. completions
+        int inflight = 0;
+        for (int k = 0; k < K; ++k) {
+            if (!busy[k]) continue;
+            const hipError_t q = hipEventQuery(ev[k]);
+            if (q == hipErrorNotReady) {
+                ++inflight;
+                continue;
+            }
+            if (q != hipSuccess) {
+                status = PE_ERR_DEVICE;
+                err = std::string("batch failed: ") + hipGetErrorString(q);
+                break;
+            }
+            const double t = now_s();
+            for (int j : batch_jobs[k]) {
+                lat[j] = t - arrival[j];
+                arena.release(buf[j].off, buf[j].bytes);
+                ++finished;
+            }
+            batch_jobs[k].clear();
+            busy[k] = false;
+        }
+        if (status != PE_OK) break;
+        // 2. arrivals
+        const double now = now_s();
+        while (next < nj && arrival[next] <= now) ready.push_back(next++);
+        // 3. admission and launch on free streams
+        for (int k = 0; k < K && !ready.empty(); ++k) {
+            if (busy[k]) continue;
+            cells.clear();
+            while (!ready.empty() && (int)cells.size() < cfg->max_batch_jobs) {
+                const int j = ready.front();
+                if (cfg->shed_late && now - arrival[j] > cfg->deadline_s) {
+                    ready.pop_front();
+                    ++shed;
+                    continue;
+                }
+                JobBuf b;
+                b.bytes = job_bytes(pop[cellof[j]], llr_in_arena);
+                if (!arena.alloc(b.bytes, b.off)) break;  // budget exhausted: wait for completions (FIFO order)
+                buf[j] = b;
+                ready.pop_front();
+                cells.push_back(make_desc(j, b));
+                batch_jobs[k].push_back(j);
+            }
+            if (cells.empty()) break;
+            status = submit(k, cells);
+            if (status != PE_OK) break;
+            busy[k] = true;
+            ++batches;
+            batched_jobs += (long)cells.size();
+            ++inflight;
+        }
+        if (status != PE_OK) break;
+        max_inflight_seen = std::max(max_inflight_seen, inflight);
+    }
+    const double wall = now_s();
+    cleanup();
+    if (status != PE_OK) return serve_fail(status, err);
+
+    std::vector<double> served;
+    int missed = shed;
+    double sum = 0.0, mx = 0.0;
+    long sc_done = 0;
+    for (int j = 0; j < nj; ++j) {
+        if (job_latency_s) job_latency_s[j] = lat[j];
+        if (job_cell) job_cell[j] = cellof[j];
+        if (lat[j] < 0) continue;
+        served.push_back(lat[j]);
+        sum += lat[j];
+        mx = std::max(mx, lat[j]);
+        if (lat[j] > cfg->deadline_s) ++missed;
+        sc_done += pop[cellof[j]].nSC;
+    }
+    rep->n_jobs = nj;
+    rep->n_done = finished;
+    rep->n_missed = missed;
+    rep->n_shed = shed;
+    rep->n_batches = batches;
+    rep->wall_s = wall;
+    rep->p50_s = nearest_rank(served, 50.0);
+    rep->p99_s = nearest_rank(served, 99.0);
+    rep->max_s = mx;
+    rep->mean_s = served.empty() ? 0.0 : sum / (double)served.size();
+    rep->miss_rate = (double)missed / (double)nj;
+    rep->jobs_per_s = wall > 0 ? finished / wall : 0.0;
+    rep->subcarriers_per_s = wall > 0 ? (double)sc_done / wall : 0.0;
+    rep->mean_batch_jobs = batches ? (double)batched_jobs / batches : 0.0;
+    rep->max_inflight_seen = max_inflight_seen;
+    rep->offered_rate_hz = cfg->arrival_rate_hz;
+    pe::set_error("");
+    return PE_OK;
+}

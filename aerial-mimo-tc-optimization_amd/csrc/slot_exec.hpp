@@ -1,0 +1,99 @@
+// slot_exec.hpp -- host-side executor of multi-cell slots: a fixed set of persistent work-queue kernels
+// whose contents (cells and tiles) change from one slot (or batch) to the next. Used by the slot plans
+// (pe_slot_plan_*, fixed cells) and by the serving loop (pe_serve_run, a new batch of jobs every launch).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "pusch_eq.h"
+
+namespace pe {
+
+// Device view of one cell-job.
+struct CellDev {
+    const float2 *H;
+    const float *sigma2;
+    float2 *C;
+    float *noise_inv;
+    const float2 *Y;
+    float *llr;
+    uint8_t *flags;  // stage-1 per-subcarrier flags (executor-owned)
+    int nSC, L, S, A, npb, prec;
+};
+
+struct Tile {
+    int cell, sc0, n, pad;
+};
+
+struct Queue {
+    unsigned next;     // tiles handed out beyond the first one per wave
+    unsigned flagged;  // subcarriers the MMSE FP32 pass handed to the FP64 pass
+};
+
+// Per-launch tile range, read by the kernels from device memory so that one captured graph serves any batch.
+struct LaunchRange {
+    int tile0, ntiles;
+};
+
+class SlotExec {
+public:
+    static constexpr int kApplyTile = 8;  // subcarriers per stage-2 tile (one wave)
+    static constexpr int kMmseTile = 8;   // subcarriers per stage-1 tile (one wave: one per 8-lane group)
+
+    struct Launch {
+        int kind;  // 0 mmse fp32, 1 mmse fp64, 2 apply
+        int cls;   // KS (apply)
+        int prec;
+        int grid, block;
+        unsigned nwaves;
+    };
+
+    SlotExec() = default;
+    ~SlotExec();
+    SlotExec(const SlotExec &) = delete;
+    SlotExec &operator=(const SlotExec &) = delete;
+
+    // Sizes device state for batches of up to max_cells cell-jobs drawn from `population` (shapes and
+    // precisions; pointers ignored). flags: PE_MMSE_UNBIASED.
+    pe_status init(const pe_cell_desc *population, int n_population, int max_cells, unsigned flags,
+                   std::string &err);
+    // Builds the tiles of a batch into the pinned staging area (cells must match population classes).
+    pe_status stage(const pe_cell_desc *cells, int n, std::string &err);
+    // Copies the staged batch to the device on `st` (one asynchronous copy).
+    pe_status upload(hipStream_t st, std::string &err);
+    // Launches the kernel sequence on `st` (kernels with no tiles exit at once).
+    pe_status launch(hipStream_t st, std::string &err);
+    // Captures launch() into a graph on first use, then replays it.
+    pe_status graph_launch(hipStream_t st, std::string &err);
+
+    int tiles_staged() const { return staged_tiles_; }
+    int max_grid() const;
+    const std::vector<Launch> &launches() const { return launches_; }
+
+private:
+    int launch_index(int kind, int cls, int prec) const;
+    hipError_t launch_one(int i, hipStream_t st) const;
+
+    unsigned flags_ = 0;
+    int max_cells_ = 0, max_tiles_ = 0;
+    size_t flags_per_cell_ = 0;
+    std::vector<Launch> launches_;
+    // one contiguous block, same layout on host (pinned) and device: [LaunchRange x nL][CellDev x max_cells]
+    // [Tile x max_tiles]
+    size_t off_cells_ = 0, off_tiles_ = 0, block_bytes_ = 0, used_bytes_ = 0;
+    unsigned char *h_block_ = nullptr, *d_block_ = nullptr;
+    Queue *d_queues_ = nullptr;
+    uint8_t *d_flags_ = nullptr;
+    int staged_tiles_ = 0;
+    hipGraph_t graph_ = nullptr;
+    hipGraphExec_t exec_ = nullptr;
+};
+
+int ks_class(int A);
+// Validates one cell descriptor for the slot executors (shape limits, MFMA precision, pointers).
+pe_status check_cell_desc(const pe_cell_desc &c, int i, std::string &err);
+
+}  // namespace pe

@@ -47,6 +47,7 @@ SIGNATURES = {
     "pe_slot_plan_graph_launch": ([_P, _P], _I),
     "pe_slot_plan_destroy": ([_P], None),
     "pe_slot_plan_info": ([_P, _P, _P], _I),
+    "pe_serve_run": ([_P, _P, _P, _P], _I),
 }
 
 
@@ -203,6 +204,13 @@ def cell_desc(C, noise_inv, Y, llr, n_pam_bits, prec="f16", H=None, sigma2=None)
                     n_pam_bits, _prec(prec))
 
 
+def job_desc(H, sigma2, Y, n_pam_bits, prec="f16"):
+    """pe_cell_desc of a serving-loop cell: inputs only (each job's C / noise_inv / LLR come from the arena)."""
+    nSC, A, L = H.shape
+    return CellDesc(H.data_ptr(), sigma2.data_ptr(), None, None, Y.data_ptr(), None, nSC, L, Y.shape[1], A,
+                    n_pam_bits, _prec(prec))
+
+
 class SlotPlan:
     """Owner of a pe_slot_plan: one slot of many cells as persistent work-queue kernels."""
 
@@ -238,3 +246,40 @@ class SlotPlan:
             self.close()
         except Exception:
             pass
+
+
+class ServeConfig(ctypes.Structure):
+    """pe_serve_config of include/pusch_eq.h."""
+    _fields_ = [("cells", _P), ("n_cells", _I), ("n_jobs", _I), ("arrival_rate_hz", ctypes.c_double),
+                ("deadline_s", ctypes.c_double), ("hbm_budget_bytes", ctypes.c_size_t), ("max_batch_jobs", _I),
+                ("max_inflight", _I), ("shed_late", _I), ("use_graph", _I), ("seed", _U), ("job_llr", _P)]
+
+
+class ServeReport(ctypes.Structure):
+    """pe_serve_report of include/pusch_eq.h."""
+    _fields_ = [("n_jobs", _I), ("n_done", _I), ("n_missed", _I), ("n_shed", _I), ("n_batches", _I),
+                ("max_inflight_seen", _I), ("wall_s", ctypes.c_double), ("p50_s", ctypes.c_double),
+                ("p99_s", ctypes.c_double), ("max_s", ctypes.c_double), ("mean_s", ctypes.c_double),
+                ("miss_rate", ctypes.c_double), ("jobs_per_s", ctypes.c_double),
+                ("subcarriers_per_s", ctypes.c_double), ("mean_batch_jobs", ctypes.c_double),
+                ("offered_rate_hz", ctypes.c_double)]
+
+
+def serve_run(cells, n_jobs, rate_hz, deadline_s=0.5e-3, hbm_budget=1 << 30, max_batch=64, max_inflight=2,
+              shed_late=False, use_graph=True, seed=0, job_llr=None):
+    """Runs the slot serving loop (pe_serve_run). cells: list of CellDesc with H, sigma2, Y_rx set.
+    job_llr: optional list of n_jobs torch tensors receiving each job's LLRs. Returns (report dict, per-job
+    latencies in seconds (-1 = shed), per-job cell index)."""
+    import numpy as np
+    arr = (CellDesc * len(cells))(*cells)
+    ptrs = None
+    if job_llr is not None:
+        ptrs = (ctypes.c_void_p * n_jobs)(*[t.data_ptr() for t in job_llr])
+    cfg = ServeConfig(ctypes.cast(arr, _P), len(cells), n_jobs, float(rate_hz), float(deadline_s), int(hbm_budget),
+                      max_batch, max_inflight, 1 if shed_late else 0, 1 if use_graph else 0, seed,
+                      ctypes.cast(ptrs, _P) if ptrs is not None else None)
+    rep = ServeReport()
+    lat = np.zeros(n_jobs, dtype=np.float64)
+    cell = np.zeros(n_jobs, dtype=np.int32)
+    _check(lib().pe_serve_run(ctypes.byref(cfg), ctypes.byref(rep), lat.ctypes.data_as(_P), cell.ctypes.data_as(_P)))
+    return {f[0]: getattr(rep, f[0]) for f in ServeReport._fields_}, lat, cell

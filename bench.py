@@ -122,6 +122,112 @@ def cpu_baseline(C, Y, ni, seconds):
                       "%.1f s" % (reps, el)}
 
 
+def draw_cells(n, seed, dev, prec="f16"):
+    """Cells drawn as in the multi-cell configurations: PRB in {24,52,106,273}, antennas in {16,32,64}, layers in
+    {2,4,8}, QAM in {16,64,256}, SNR uniform in 5-25 dB, Rayleigh CN(0,1/A) channels, 11 data symbols."""
+    import numpy as np
+    import torch
+    import pusch_eq as pe
+    rng = np.random.default_rng(seed)
+    g = torch.Generator(device=dev)
+    g.manual_seed(seed)
+    cells = []
+    for _ in range(n):
+        prb = int(rng.choice([24, 52, 106, 273]))
+        a = int(rng.choice([16, 32, 64]))
+        l = int(rng.choice([2, 4, 8]))
+        q = int(rng.choice([2, 3, 4]))
+        snr = float(rng.uniform(5, 25))
+        nsc = 12 * prb
+        H = torch.randn((nsc, a, l), dtype=torch.complex64, device=dev, generator=g) * (1.0 / a) ** 0.5
+        bits = torch.randint(0, 2, (nsc, l, S, 2 * q), device=dev, generator=g, dtype=torch.uint8)
+        s2v = 10.0 ** (-snr / 10.0)
+        Y = torch.bmm(H, qam_map_torch(bits, q))
+        Y = (Y + torch.randn(Y.shape, dtype=torch.complex64, device=dev, generator=g) * s2v ** 0.5)
+        Y = Y.transpose(1, 2).contiguous()
+        s2 = torch.full((nsc,), s2v, dtype=torch.float32, device=dev)
+        C = torch.empty((nsc, l, a), dtype=torch.complex64, device=dev)
+        ni = torch.empty((nsc, l), dtype=torch.float32, device=dev)
+        llr = torch.empty((nsc, l, S, 2 * q), dtype=torch.float32, device=dev)
+        cells.append(dict(H=H, s2=s2, Y=Y, C=C, ni=ni, llr=llr, q=q, nsc=nsc, A=a, L=l,
+                          desc=pe.cell_desc(C, ni, Y, llr, q, prec, H, s2)))
+    return cells
+
+
+def slot_bytes(cells):
+    """Algorithmic HBM bytes of one slot: MMSE (H, sigma2 in; C, noise_inv, flags out) + fused apply/demap
+    (C, Y_rx, noise_inv in; LLR out)."""
+    b = 0
+    for c in cells:
+        n, a, l, q = c["nsc"], c["A"], c["L"], c["q"]
+        b += n * (a * l * 8 + 4 + l * a * 8 + l * 4 + 1)
+        b += n * (l * a * 8 + S * a * 8 + l * 4 + l * S * 2 * q * 4)
+    return b
+
+
+def multicell_bench(dev, steps):
+    """24 uneven cells, one slot = one graph replay of the slot plan (MMSE + fused apply/demap, persistent
+    work queues); two slot buffer sets alternate."""
+    import torch
+    import pusch_eq as pe
+    sets = [draw_cells(24, 77 + k, dev) for k in range(2)]
+    plans = [pe.SlotPlan([c["desc"] for c in cs]) for cs in sets]
+    for i in range(6):
+        plans[i % 2].graph_launch()
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for i in range(steps):
+        plans[i % 2].graph_launch()
+    e1.record()
+    torch.cuda.synchronize()
+    t = e0.elapsed_time(e1) / 1e3 / steps
+    nsc = sum(c["nsc"] for c in sets[0])
+    by = slot_bytes(sets[0])
+    tiles, wg = plans[0].info()
+    for p in plans:
+        p.close()
+    return {"workload": "24 cells, PRB {24,52,106,273}, antennas {16,32,64}, layers {2,4,8}, QAM {16,64,256}, "
+                        "f16; one slot = MMSE + fused apply/demap as one graph replay",
+            "subcarriers_per_slot": nsc, "slot_ms": t * 1e3, "subcarriers_per_s": nsc / t,
+            "hbm_gbs": by / t / 1e9, "hbm_frac": by / t / 1e9 / HBM_PEAK_GBS, "bytes_per_slot": by,
+            "tiles": tiles, "max_persistent_workgroups": wg}
+
+
+def serving_bench(dev, n_jobs):
+    """64 cells; jobs (one slot of one cell: MMSE + fused apply/demap of 11 symbols) arrive as a Poisson
+    process; capacity = throughput with every job queued at t = 0; then offered load 10-120% of capacity."""
+    import torch
+    import pusch_eq as pe
+    cells = draw_cells(64, 91, dev)
+    descs = [pe.job_desc(c["H"], c["s2"], c["Y"], c["q"], "f16") for c in cells]
+    kw = dict(deadline_s=0.5e-3, hbm_budget=2 << 30, max_batch=64, max_inflight=3, shed_late=False,
+              use_graph=True)
+    cap, _, _ = pe.serve_run(descs, n_jobs, 0.0, seed=1, **kw)
+    capacity = cap["jobs_per_s"]
+    sweep = []
+    for f in (0.1, 0.3, 0.5, 0.7, 0.9, 1.0, 1.1, 1.2):
+        rep, _, _ = pe.serve_run(descs, n_jobs, f * capacity, seed=2, **kw)
+        sweep.append({"load": f, "offered_jobs_per_s": f * capacity, "jobs_per_s": rep["jobs_per_s"],
+                      "p50_ms": rep["p50_s"] * 1e3, "p99_ms": rep["p99_s"] * 1e3, "max_ms": rep["max_s"] * 1e3,
+                      "miss_rate": rep["miss_rate"], "mean_batch_jobs": rep["mean_batch_jobs"]})
+    # plain FP32 path, one job at a time (per-cell launches: MMSE then fused FP32 apply/demap)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for c in cells:
+        pe.gram_mmse(c["H"], c["s2"], unbiased=True, C=c["C"], noise_inv=c["ni"])
+        pe.coef_apply_demap(c["C"], c["Y"], c["ni"], c["q"], prec="f32", out=c["llr"])
+    e1.record()
+    torch.cuda.synchronize()
+    f32_jobs_per_s = len(cells) / (e0.elapsed_time(e1) / 1e3)
+    return {"workload": "64 cells drawn as the multi-cell batch, Poisson arrivals, 0.5 ms deadline, continuous "
+                        "batching (<= 64 jobs per batch, 3 streams, 2 GiB budget), f16",
+            "jobs": n_jobs, "capacity_jobs_per_s": capacity,
+            "capacity_subcarriers_per_s": cap["subcarriers_per_s"], "capacity_p99_ms": cap["p99_s"] * 1e3,
+            "load_sweep": sweep, "plain_f32_sequential_jobs_per_s": f32_jobs_per_s}
+
+
 def load_traffic(workload_key):
     p = os.path.join(ROOT, "profiles", "traffic.json")
     if not os.path.exists(p):
@@ -143,6 +249,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--latency-iters", type=int, default=300)
+    ap.add_argument("--no-multicell", action="store_true")
+    ap.add_argument("--serve-jobs", type=int, default=2000, help="jobs per serving-loop run (0 = skip)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run 5 plain 256 MiB device copies (scale reference for the HBM byte counters)")
     args = ap.parse_args()
@@ -228,6 +336,13 @@ def main():
         log("cpu baseline (%.0f s)" % args.cpu_seconds)
         cpu = cpu_baseline(cell["C"], cell["Y"], cell["ni"], args.cpu_seconds)
 
+    multicell = None if args.no_multicell else multicell_bench(dev, 50)
+    serving = serving_bench(dev, args.serve_jobs) if args.serve_jobs > 0 else None
+    if multicell:
+        log("multicell slot %.3f ms, %.1f GB/s" % (multicell["slot_ms"], multicell["hbm_gbs"]))
+    if serving:
+        log("serving capacity %.0f jobs/s" % serving["capacity_jobs_per_s"])
+
     workload = "fused coef-apply + 256-QAM max-log demap, %d cell-slots x 1584 SC x 8 layers x 11 sym x 64 ant " \
                "per launch (complex64 C from the library MMSE stage, complex64 Y_rx, FP32 LLR out)" % n
     out = {
@@ -258,6 +373,8 @@ def main():
         "f16_split": {"value": nsc / persp, "ms_per_step": persp * 1e3, "hbm_gbs": bytes_launch / persp / 1e9},
         "slot_latency_ms": {"p50": pct(50), "p99": pct(99), "max": lat[-1] if lat else None, "iters": len(lat),
                             "workload": "one cell: pe_gram_mmse + fused apply/demap, events on one stream"},
+        "multicell_slot": multicell,
+        "serving": serving,
         "wall_s_timed": wall,
     }
     print(json.dumps(out), flush=True)

@@ -172,6 +172,40 @@ void pe_slot_plan_destroy(pe_slot_plan *plan);
 /* Tiles and largest persistent grid of the plan (for reports). */
 pe_status pe_slot_plan_info(const pe_slot_plan *plan, int *nTiles, int *nWorkgroups);
 
+/* ---- serving loop --------------------------------------------------------------------------------- */
+
+typedef struct {
+    /* Cell population. Per cell: shapes, stage-2 precision (MFMA only) and the device-resident job inputs
+       H, sigma2, Y_rx, reused by every job of that cell. C / noise_inv / llr are ignored: each job's buffers
+       come from a device arena of hbm_budget_bytes. */
+    const pe_cell_desc *cells;
+    int n_cells;
+    int n_jobs;                /* jobs to serve, each for a uniformly drawn cell */
+    double arrival_rate_hz;    /* Poisson arrival rate over all cells; <= 0: every job arrives at t = 0 */
+    double deadline_s;         /* per-job deadline after arrival, e.g. 0.5e-3 (one 30 kHz-SCS slot) */
+    size_t hbm_budget_bytes;   /* arena for in-flight job buffers (C, noise_inv and, without job_llr, LLRs) */
+    int max_batch_jobs;        /* jobs per batch launch */
+    int max_inflight;          /* batches in flight, one stream each, 1..8 */
+    int shed_late;             /* nonzero: drop jobs whose deadline passed before launch (counted as misses) */
+    int use_graph;             /* nonzero: each stream replays one captured slot graph */
+    unsigned seed;             /* arrival process and cell draws */
+    float *const *job_llr;     /* optional [n_jobs]: LLR destination of each job (sized for its cell) */
+} pe_serve_config;
+
+typedef struct {
+    int n_jobs, n_done, n_missed, n_shed, n_batches, max_inflight_seen;
+    double wall_s;             /* first arrival window to last completion */
+    double p50_s, p99_s, max_s, mean_s;   /* latency (completion - arrival) of served jobs; nearest rank */
+    double miss_rate;          /* (served late + shed) / n_jobs */
+    double jobs_per_s, subcarriers_per_s, mean_batch_jobs, offered_rate_hz;
+} pe_serve_report;
+
+/* Runs the serving loop to completion on the current device (blocking). Allocates its arena, streams,
+   events and executors, and frees them before returning. Per job (optional arrays of n_jobs): latency in
+   seconds (-1 when shed) and the cell index. The reference has no serving loop; its closest protocol is the
+   N-cell deadline check of multicell_evidence.cu:497-566. */
+pe_status pe_serve_run(const pe_serve_config *cfg, pe_serve_report *rep, double *job_latency_s, int *job_cell);
+
 #ifdef __cplusplus
 }
 #endif

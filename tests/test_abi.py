@@ -93,3 +93,25 @@ def test_slot_plan_rejects_bad_cells_without_a_device(pe):
         assert b"cell 1" in h.pe_last_error()
     assert h.pe_slot_plan_create(None, 0, 1, ctypes.byref(out)) == 1
     assert h.pe_slot_plan_launch(None, None) == 1
+
+
+def test_serve_rejects_bad_config_without_a_device(pe):
+    h = pe.lib()
+    one = ctypes.c_void_p(16)
+    good = pe.CellDesc(one, one, None, None, one, None, 12, 8, 11, 64, 4, 1)
+    no_h = pe.CellDesc(None, one, None, None, one, None, 12, 8, 11, 64, 4, 1)
+    arr = (pe.CellDesc * 1)(good)
+    rep = pe.ServeReport()
+
+    def run(cells=arr, n=1, jobs=10, budget=1 << 30, batch=8, inflight=2, deadline=5e-4):
+        cfg = pe.ServeConfig(ctypes.cast(cells, ctypes.c_void_p) if cells is not None else None, n, jobs, 1000.0,
+                             deadline, budget, batch, inflight, 0, 1, 1, None)
+        return h.pe_serve_run(ctypes.byref(cfg), ctypes.byref(rep), None, None)
+
+    assert h.pe_serve_run(None, ctypes.byref(rep), None, None) == 1
+    assert run(cells=None, n=0) == 1
+    assert run(jobs=0) == 1
+    assert run(inflight=9) == 1
+    assert run(deadline=0.0) == 1
+    assert run(budget=1000) == 5          # PE_ERR_CAPACITY: one job does not fit
+    assert run(cells=(pe.CellDesc * 1)(no_h)) == 1
