@@ -12,16 +12,18 @@ using namespace mfma;
 // PREC: 1 f16, 2 bf16, 3 f16 split, 4 bf16 split.  NPB: 0 -> store Y_eq, 1..4 -> fused demap.
 // One-wave workgroups; each streams `spw` consecutive subcarriers and loads the next subcarrier's raw data
 // into a second register set while the current one is converted and multiplied.
-// One subcarrier per one-wave workgroup: the smallest register footprint, so the most waves (and loads)
-// in flight per CU.
-template <int KS, int PREC, int NPB>
-__global__ __launch_bounds__(64) void coef_apply_mfma_sc_kernel(
+// One subcarrier per wave, WPB waves (each with its own LDS operand region) per workgroup: the smallest
+// register footprint, so the most waves (and loads) in flight per CU.
+template <int KS, int PREC, int NPB, int WPB>
+__global__ __launch_bounds__(64 * WPB) void coef_apply_mfma_sc_kernel(
     const float2 *__restrict__ C, const float2 *__restrict__ Y, float2 *__restrict__ O,
     const float *__restrict__ noise_inv, float *__restrict__ llr, int nSC, int L, int S, int A) {
-    __shared__ __attribute__((aligned(16))) unsigned char lds[LdsLayout<KS>::bytes(PREC)];
-    const long sc = blockIdx.x;
-    if (sc >= nSC) return;
-    const int lane = threadIdx.x;
+    __shared__ __attribute__((aligned(16))) unsigned char lds_all[WPB * LdsLayout<KS>::bytes(PREC)];
+    const int wave = threadIdx.x >> 6;
+    unsigned char *lds = lds_all + wave * LdsLayout<KS>::bytes(PREC);
+    const long sc = (long)blockIdx.x * WPB + wave;
+    if (sc >= nSC) return;  // uniform per wave; no workgroup barrier follows
+    const int lane = threadIdx.x & 63;
     if (A < 16 * KS) zero_operands<KS, PREC>(lds, lane);
     const int nC = L * (A >> 1), nY = S * (A >> 1);
     const int rc = (nC + 63) >> 6, ry = (nY + 63) >> 6;
@@ -49,8 +51,17 @@ hipError_t launch3(const float2 *C, const float2 *Y, float2 *O, const float *ni,
     const CoefTile t = coef_tile_for(L, S, A, PREC);
     const int spw = t.mfma_spw > 1 ? t.mfma_spw : 1;
     if (spw == 1) {
-        hipLaunchKernelGGL((coef_apply_mfma_sc_kernel<KS, PREC, NPB>), dim3((unsigned)nSC), dim3(64), 0, st, C, Y,
-                           O, ni, llr, nSC, L, S, A);
+        const int wpb = t.mfma_waves;
+        const unsigned g = (unsigned)((nSC + wpb - 1) / wpb);
+        if (wpb == 4)
+            hipLaunchKernelGGL((coef_apply_mfma_sc_kernel<KS, PREC, NPB, 4>), dim3(g), dim3(256), 0, st, C, Y, O, ni,
+                               llr, nSC, L, S, A);
+        else if (wpb == 2)
+            hipLaunchKernelGGL((coef_apply_mfma_sc_kernel<KS, PREC, NPB, 2>), dim3(g), dim3(128), 0, st, C, Y, O, ni,
+                               llr, nSC, L, S, A);
+        else
+            hipLaunchKernelGGL((coef_apply_mfma_sc_kernel<KS, PREC, NPB, 1>), dim3((unsigned)nSC), dim3(64), 0, st,
+                               C, Y, O, ni, llr, nSC, L, S, A);
     } else {
         const long grid = (nSC + spw - 1) / spw;
         hipLaunchKernelGGL((coef_apply_mfma_kernel<KS, PREC, NPB>), dim3((unsigned)grid), dim3(64), 0, st, C, Y,

@@ -49,10 +49,9 @@ __global__ __launch_bounds__(256) void gram_mmse_f64_kernel(const float2 *__rest
     if (p == 0) flags[sc] = ok ? PE_SCFLAG_FP64 : (PE_SCFLAG_FP64 | PE_SCFLAG_SINGULAR);
 }
 
-template <int L>
-hipError_t launch_mmse_L(const float2 *H, const float *sigma2, float2 *C, float *ni, uint8_t *flags, int nSC,
-                         int A, bool unbiased, float thr, hipStream_t st) {
-    constexpr int P = kMmseP;
+template <int L, int P>
+hipError_t launch_mmse_LP(const float2 *H, const float *sigma2, float2 *C, float *ni, uint8_t *flags, int nSC,
+                          int A, bool unbiased, float thr, hipStream_t st) {
     const int per_block = 256 / P;
     const int grid = (nSC + per_block - 1) / per_block;
     hipLaunchKernelGGL((gram_mmse_f32_kernel<L, P>), dim3(grid), dim3(256), 0, st, H, sigma2, C, ni, flags, nSC,
@@ -62,6 +61,14 @@ hipError_t launch_mmse_L(const float2 *H, const float *sigma2, float2 *C, float 
     hipLaunchKernelGGL((gram_mmse_f64_kernel<L, P>), dim3(grid), dim3(256), 0, st, H, sigma2, C, ni, flags, nSC,
                        A, unbiased ? 1 : 0);
     return hipGetLastError();
+}
+
+template <int L>
+hipError_t launch_mmse_L(const float2 *H, const float *sigma2, float2 *C, float *ni, uint8_t *flags, int nSC,
+                         int A, bool unbiased, float thr, hipStream_t st) {
+    if (mmse_lanes_for(L, A, nSC) == 16)
+        return launch_mmse_LP<L, 16>(H, sigma2, C, ni, flags, nSC, A, unbiased, thr, st);
+    return launch_mmse_LP<L, 8>(H, sigma2, C, ni, flags, nSC, A, unbiased, thr, st);
 }
 
 }  // namespace

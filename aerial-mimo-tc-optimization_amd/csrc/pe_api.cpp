@@ -4,6 +4,8 @@
 // status code plus a thread-local message.
 #include <hip/hip_runtime.h>
 
+#include <dlfcn.h>
+
 #include <cstdarg>
 #include <cstdio>
 #include <cstdlib>
@@ -41,36 +43,68 @@ pe_status from_hip(hipError_t e, const char *what) {
 
 bool valid_prec(int p) { return p >= PE_PREC_F32 && p <= PE_PREC_BF16_SPLIT; }
 
-// ---- tile table --------------------------------------------------------------------------------------
+// ---- tile table -----------------------------------------------------------------------------------------
+// Lines (first match wins, -1 = any):
+//   coef <L> <S> <A> <prec> <f32_kc> <mfma_wpb> <mfma_spw>
+//   mmse <L> <A> <nsc_max> <lanes_per_subcarrier>
+// Loaded from PE_TUNE_TABLE (pe_init), pe_tune_load(), or else once from tune_table.txt beside the library.
 struct TileEntry {
-    int L, S, A, prec;  // -1 = wildcard
+    int L, S, A, prec;
     pe::CoefTile tile;
 };
+struct MmseEntry {
+    int L, A, nsc_max, lanes;
+};
 std::mutex g_table_mu;
-std::vector<TileEntry> g_table;  // most specific match wins (first match in file order)
+std::vector<TileEntry> g_table;
+std::vector<MmseEntry> g_mmse;
+bool g_table_set = false;  // a table was loaded or cleared explicitly
 pe::CoefTile g_default_tile;
 
 int load_table_file(const char *path) {
     FILE *f = fopen(path, "r");
     if (!f) return -1;
     std::vector<TileEntry> t;
+    std::vector<MmseEntry> m;
     char line[256];
     while (fgets(line, sizeof(line), f)) {
         if (line[0] == '#' || line[0] == '\n') continue;
         TileEntry e;
-        int kc = 0, waves = 0, spw = 0;
-        // format: coef <L> <S> <A> <prec> <f32_kc> <mfma_waves> <mfma_spw>   (-1 = any)
-        if (sscanf(line, "coef %d %d %d %d %d %d %d", &e.L, &e.S, &e.A, &e.prec, &kc, &waves, &spw) == 7) {
-            e.tile.f32_kc = kc;
-            e.tile.mfma_waves = waves;
-            e.tile.mfma_spw = spw;
+        MmseEntry me;
+        int kc = 0, wpb = 0, spw = 0;
+        if (sscanf(line, "coef %d %d %d %d %d %d %d", &e.L, &e.S, &e.A, &e.prec, &kc, &wpb, &spw) == 7) {
+            e.tile.f32_kc = (kc == 16) ? 16 : 8;
+            e.tile.mfma_waves = (wpb == 2 || wpb == 4) ? wpb : 1;
+            e.tile.mfma_spw = spw > 1 ? spw : 1;
             t.push_back(e);
+        } else if (sscanf(line, "mmse %d %d %d %d", &me.L, &me.A, &me.nsc_max, &me.lanes) == 4) {
+            me.lanes = me.lanes == 16 ? 16 : 8;
+            m.push_back(me);
         }
     }
     fclose(f);
     std::lock_guard<std::mutex> lk(g_table_mu);
     g_table.swap(t);
-    return (int)g_table.size();
+    g_mmse.swap(m);
+    g_table_set = true;
+    return (int)(g_table.size() + g_mmse.size());
+}
+
+// tune_table.txt beside this library, read once when no table was set explicitly
+void load_default_table() {
+    static std::once_flag once;
+    std::call_once(once, [] {
+        {
+            std::lock_guard<std::mutex> lk(g_table_mu);
+            if (g_table_set) return;
+        }
+        Dl_info info;
+        if (!dladdr(reinterpret_cast<void *>(&load_default_table), &info) || !info.dli_fname) return;
+        std::string path(info.dli_fname);
+        const size_t slash = path.rfind('/');
+        path = (slash == std::string::npos ? std::string(".") : path.substr(0, slash)) + "/tune_table.txt";
+        (void)load_table_file(path.c_str());
+    });
 }
 
 bool check_ptrs(const void *a, const void *b, const void *c) { return a && b && c; }
@@ -94,6 +128,7 @@ int device_cus() {
 }
 
 CoefTile coef_tile_for(int L, int S, int A, int prec) {
+    load_default_table();
     std::lock_guard<std::mutex> lk(g_table_mu);
     for (const TileEntry &e : g_table) {
         if ((e.L < 0 || e.L == L) && (e.S < 0 || e.S == S) && (e.A < 0 || e.A == A) &&
@@ -101,6 +136,14 @@ CoefTile coef_tile_for(int L, int S, int A, int prec) {
             return e.tile;
     }
     return g_default_tile;
+}
+
+int mmse_lanes_for(int L, int A, int nSC) {
+    load_default_table();
+    std::lock_guard<std::mutex> lk(g_table_mu);
+    for (const MmseEntry &e : g_mmse)
+        if ((e.L < 0 || e.L == L) && (e.A < 0 || e.A == A) && (e.nsc_max < 0 || nSC <= e.nsc_max)) return e.lanes;
+    return 8;
 }
 }  // namespace pe
 
@@ -137,11 +180,12 @@ pe_status pe_init(int device) {
     return ok();
 }
 
-// Internal (not in the public header): load / clear the tile table, used by the autotuner.
 int pe_tune_load(const char *path) {
     if (!path) {
         std::lock_guard<std::mutex> lk(g_table_mu);
         g_table.clear();
+        g_mmse.clear();
+        g_table_set = true;
         return 0;
     }
     return load_table_file(path);

@@ -10,10 +10,14 @@ namespace pe {
 // Tile parameters chosen per shape (defaults, overridable by the tuned table).
 struct CoefTile {
     int f32_kc = 8;         // FP32 kernel: antennas per LDS chunk (8 or 16)
-    int mfma_waves = 4;     // MFMA kernel: waves per workgroup (1, 2, 4)
+    int mfma_waves = 1;     // MFMA kernel: waves (subcarriers) per workgroup (1, 2, 4)
     int mfma_spw = 1;       // MFMA kernel: subcarriers per one-wave workgroup (1 = no prefetch loop)
 };
 CoefTile coef_tile_for(int nLayer, int nSym, int nAnt, int prec);
+
+// MMSE kernel: lanes per subcarrier (8 or 16) for a shape; more lanes = lower latency per subcarrier, fewer
+// subcarriers per wave.
+int mmse_lanes_for(int nLayer, int nAnt, int nSC);
 
 // Compute units of the current device (cached per process).
 int device_cus();

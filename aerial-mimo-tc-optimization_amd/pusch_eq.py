@@ -33,6 +33,7 @@ SIGNATURES = {
     "pe_last_error": ([], ctypes.c_char_p),
     "pe_status_string": ([_I], ctypes.c_char_p),
     "pe_init": ([_I], _I),
+    "pe_tune_load": ([ctypes.c_char_p], _I),
     "pe_coef_apply": ([_P, _P, _P, _I, _I, _I, _I, _I, _P], _I),
     "pe_coef_apply_v1": ([_P, _P, _P, _I, _I, _P], _I),
     "pe_coef_apply_range": ([_P, _P, _P, _I, _I, _I, _I, _I, _I, _P], _I),

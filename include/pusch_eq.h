@@ -72,6 +72,12 @@ const char *pe_status_string(pe_status s);
    environment variable PE_TUNE_TABLE if set). Calling it is optional; entry points work on the current
    device without it. */
 pe_status pe_init(int device);
+/* Replaces the tile table with the file at `path` (NULL: no table, built-in defaults). Returns the number of
+   entries read, or -1 if the file cannot be read. Lines, first match wins, -1 = any:
+     coef <nLayer> <nSym> <nAnt> <prec> <f32_kc: 8|16> <mfma_wpb: 1|2|4> <mfma_spw: 1..8>
+     mmse <nLayer> <nAnt> <nSC_max> <lanes_per_subcarrier: 8|16>
+   Without an explicit table the library reads tune_table.txt beside itself (written by autotune.py). */
+int pe_tune_load(const char *path);
 
 /* ---- coefficient apply:  Y_eq[sc][l][s] = sum_a C[sc][l][a] * Y_rx[sc][s][a]  (no conjugation) ------ */
 
