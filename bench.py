@@ -228,6 +228,80 @@ def serving_bench(dev, n_jobs):
             "load_sweep": sweep, "plain_f32_sequential_jobs_per_s": f32_jobs_per_s}
 
 
+SWEEP_PRECS = ("f32", "bf16", "bf16_split", "f16", "f16_split")
+
+
+def deployment_sweep(dev, seed=5, reps=10):
+    """One cell per launch: antennas {8,16,32,64,128} x PRB {25,51,106,133,273} x layers {2,4,8}, 11 symbols;
+    QAM order (QPSK..256-QAM) and SNR (0-30 dB) drawn per configuration; Rayleigh CN(0, 1/A) channels. Per
+    configuration: apply EVM of each MFMA precision vs the FP64 product of the same coefficients, and the
+    device time per launch of the fused apply + demap (median of `reps` single launches, each queued behind a
+    short spin so that host launch overhead is excluded). The bf16 path at the -60 dB bar is plain bf16 where
+    that meets it, else bf16_split; it is compared with the plain FP32 path."""
+    import numpy as np
+    import torch
+    import pusch_eq as pe
+    rng = np.random.default_rng(seed)
+    rows = []
+    for A_ in (8, 16, 32, 64, 128):
+        for prb in (25, 51, 106, 133, 273):
+            for L_ in (2, 4, 8):
+                q = int(rng.integers(1, 5))
+                snr = float(rng.uniform(0.0, 30.0))
+                nsc = prb * 12
+                g = torch.Generator(device=dev)
+                g.manual_seed(int(rng.integers(1 << 30)))
+                H = torch.randn((nsc, A_, L_), dtype=torch.complex64, device=dev, generator=g) * (1.0 / A_) ** 0.5
+                bits = torch.randint(0, 2, (nsc, L_, S, 2 * q), device=dev, generator=g, dtype=torch.uint8)
+                x = qam_map_torch(bits, q)
+                s2 = 10.0 ** (-snr / 10.0)
+                Y = torch.bmm(H, x) + torch.randn((nsc, A_, S), dtype=torch.complex64, device=dev,
+                                                  generator=g) * s2 ** 0.5
+                Y = Y.transpose(1, 2).contiguous()
+                sigma2 = torch.full((nsc,), s2, dtype=torch.float32, device=dev)
+                C, ni, fl = pe.gram_mmse(H, sigma2, unbiased=True)
+                yref = torch.bmm(C.to(torch.complex128), Y.to(torch.complex128).transpose(1, 2).contiguous())
+                llr = torch.empty((nsc, L_, S, 2 * q), dtype=torch.float32, device=dev)
+                row = {"A": A_, "prb": prb, "L": L_, "qam": 4 ** q, "snr_db": round(snr, 2), "nsc": nsc,
+                       "fp64_mmse_scs": int((fl != 0).sum())}
+                for prec in SWEEP_PRECS:
+                    yeq = pe.coef_apply(C, Y, prec=prec)
+                    row["evm_db_" + prec] = evm_db(yref, yeq)
+                    ts = []
+                    for i in range(reps + 2):
+                        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                        torch.cuda._sleep(100000)
+                        a.record()
+                        pe.coef_apply_demap(C, Y, ni, q, prec=prec, out=llr)
+                        b.record()
+                        b.synchronize()
+                        if i >= 2:
+                            ts.append(a.elapsed_time(b) * 1e3)
+                    ts.sort()
+                    row["us_" + prec] = ts[len(ts) // 2]
+                bf = "bf16" if row["evm_db_bf16"] <= -60.0 else "bf16_split"
+                row["bf16_at_bar"] = bf
+                row["speedup_bf16_at_bar_vs_f32"] = row["us_f32"] / row["us_" + bf]
+                row["speedup_f16_vs_f32"] = row["us_f32"] / row["us_f16"]
+                rows.append(row)
+    import statistics
+    summ = {"configs": len(rows),
+            "workload": "one cell per launch, antennas {8,16,32,64,128} x PRB {25,51,106,133,273} x layers "
+                        "{2,4,8}, 11 symbols, QAM and SNR (0-30 dB) drawn per config, fused apply + demap",
+            "evm_bar_db": -60.0}
+    for prec in SWEEP_PRECS[1:]:
+        evs = [r["evm_db_" + prec] for r in rows]
+        summ["worst_evm_db_" + prec] = max(evs)
+        summ["configs_at_bar_" + prec] = sum(e <= -60.0 for e in evs)
+    summ["bf16_at_bar_speedup_vs_f32_median"] = statistics.median(r["speedup_bf16_at_bar_vs_f32"] for r in rows)
+    summ["bf16_at_bar_speedup_vs_f32_min"] = min(r["speedup_bf16_at_bar_vs_f32"] for r in rows)
+    summ["f16_speedup_vs_f32_median"] = statistics.median(r["speedup_f16_vs_f32"] for r in rows)
+    big = [r for r in rows if r["nsc"] == 3276]
+    summ["largest_cells_bf16_at_bar_speedup_vs_f32_median"] = statistics.median(
+        r["speedup_bf16_at_bar_vs_f32"] for r in big)
+    return summ, rows
+
+
 def load_traffic(workload_key):
     p = os.path.join(ROOT, "profiles", "traffic.json")
     if not os.path.exists(p):
@@ -251,6 +325,8 @@ def main():
     ap.add_argument("--latency-iters", type=int, default=300)
     ap.add_argument("--no-multicell", action="store_true")
     ap.add_argument("--serve-jobs", type=int, default=2000, help="jobs per serving-loop run (0 = skip)")
+    ap.add_argument("--no-sweep", action="store_true", help="skip the deployment sweep")
+    ap.add_argument("--sweep-out", default=None, help="write the deployment sweep table (JSON) here")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run 5 plain 256 MiB device copies (scale reference for the HBM byte counters)")
     args = ap.parse_args()
@@ -293,6 +369,11 @@ def main():
     yref_chain = torch_reference_chain(s0["H"][c1], s0["s2"][c1], s0["Y"][c1])
     evm_apply = evm_db(yref_apply, yeq)
     evm_chain = evm_db(yref_chain, yeq)
+    d = (yeq.to(torch.complex128) - yref_apply)
+    acc = {"max_abs_err_vs_fp64": float(d.abs().max()),
+           "cosine_vs_fp64": float((torch.vdot(yref_apply.flatten(), yeq.to(torch.complex128).flatten())).real
+                                   / (yref_apply.abs().pow(2).sum() * yeq.to(torch.complex128).abs().pow(2).sum())
+                                   .sqrt())}
     log("EVM (%s apply vs FP64) %.1f dB, chain vs FP64 %.1f dB" % (args.prec, evm_apply, evm_chain))
 
     def fused(C, Y, ni, llr, prec=args.prec):
@@ -338,6 +419,14 @@ def main():
 
     multicell = None if args.no_multicell else multicell_bench(dev, 50)
     serving = serving_bench(dev, args.serve_jobs) if args.serve_jobs > 0 else None
+    sweep = None
+    if not args.no_sweep:
+        sweep, rows = deployment_sweep(dev)
+        log("deployment sweep: %d configs, bf16 at -60 dB vs f32 median speedup %.2fx" % (
+            sweep["configs"], sweep["bf16_at_bar_speedup_vs_f32_median"]))
+        if args.sweep_out:
+            with open(args.sweep_out, "w") as f:
+                json.dump({"summary": sweep, "rows": rows}, f, indent=1)
     if multicell:
         log("multicell slot %.3f ms, %.1f GB/s" % (multicell["slot_ms"], multicell["hbm_gbs"]))
     if serving:
@@ -367,6 +456,7 @@ def main():
         "hbm_gbs": achieved_gbs,
         "gflops": flops_launch / per_launch / 1e9,
         "evm_db": {"apply_vs_fp64": evm_apply, "chain_vs_fp64": evm_chain},
+        "accuracy": acc,
         "fp64_fallback_subcarriers": n_fp64,
         "plain_f32": {"value": nsc / per32, "ms_per_step": per32 * 1e3,
                       "hbm_gbs": bytes_launch / per32 / 1e9},
@@ -375,6 +465,7 @@ def main():
                             "workload": "one cell: pe_gram_mmse + fused apply/demap, events on one stream"},
         "multicell_slot": multicell,
         "serving": serving,
+        "deployment_sweep": sweep,
         "wall_s_timed": wall,
     }
     print(json.dumps(out), flush=True)
