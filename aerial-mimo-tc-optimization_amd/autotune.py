@@ -10,9 +10,12 @@ class becomes a line of tune_table.txt beside the library, which the library rea
 Candidates (see pe_tune_load in include/pusch_eq.h):
     MFMA apply: waves per workgroup {1, 2, 4} x subcarriers per wave {1, 2}
     FP32 apply: antennas per LDS chunk {8, 16}
-    MMSE:       lanes per subcarrier {8, 16}
-Timing: HIP events around 20 launches after 5 warm-up launches, alternating two buffer sets whose sum exceeds
-the 256 MB Infinity Cache, so launches stream from HBM as in production.
+    MMSE:       lanes per subcarrier {4, 8, 16} (one per shape: it fixes the summation order) x subcarriers per
+                tile {64/lanes .. 64} per batch bucket (single cell, large batch)
+    slot:       subcarriers per MMSE tile {8, 16, 32, 64} x per apply tile {4, 8, 16} (24-cell slot plans)
+Timing: HIP events around 20 launches after 5 warm-up launches, alternating two buffer sets (for the apply
+their sum exceeds the 256 MB Infinity Cache, so launches stream from HBM as in production); single-cell MMSE
+launches are timed one at a time (median of 15).
 """
 import argparse
 import itertools
@@ -99,28 +102,91 @@ def tune_apply(pe, dev, L, S, A, prec, target_bytes):
     return best, res, nsc
 
 
-def tune_mmse(pe, dev, L, A, nsc):
+def mmse_sets(dev, L, A, nsc):
     import torch
     sets = []
     for k in range(2):
         g = torch.Generator(device=dev).manual_seed(10 + k)
         H = torch.randn((nsc, A, L), dtype=torch.complex64, device=dev, generator=g) * (1.0 / A) ** 0.5
-        s2 = torch.full((nsc,), 0.01, device=dev)
+        s2 = torch.full((nsc,), 0.03, device=dev)
         C = torch.empty((nsc, L, A), dtype=torch.complex64, device=dev)
         ni = torch.empty((nsc, L), device=dev)
         fl = torch.empty((nsc,), dtype=torch.uint8, device=dev)
         sets.append((H, s2, C, ni, fl))
-    best, res = None, []
-    clock = timed_single if nsc <= 4096 else timed
-    for lanes in (8, 16):
-        with_table(pe, ["mmse %d %d -1 %d" % (L, A, lanes)])
-        t = clock(lambda H, s2, C, ni, fl: pe.gram_mmse(H, s2, True, 0.0, C, ni, fl), sets)
-        res.append({"lanes": lanes, "us": t * 1e6})
-        if best is None or t < best[0]:
-            best = (t, lanes)
-    del sets
-    torch.cuda.empty_cache()
-    return best, res
+    return sets
+
+
+def tune_mmse(pe, dev, L, A, base):
+    """Lanes per subcarrier P (fixes the summation order, so one choice per shape) and subcarriers per tile T
+    per batch bucket. P is the one with the smallest sum over buckets of time / best time of the bucket."""
+    import torch
+    buckets = ((1584, 4096), (50688, -1))
+    res = {}
+    for P in (4, 8, 16):
+        for nsc, bucket in buckets:
+            sets = mmse_sets(dev, L, A, nsc)
+            clock = timed_single if nsc <= 4096 else timed
+            T = 64 // P
+            while T <= 64:
+                with_table(pe, ["mmse %d %d -1 %d %d" % (L, A, P, T)] + base)
+                t = clock(lambda H, s2, C, ni, fl: pe.gram_mmse(H, s2, True, 0.0, C, ni, fl), sets)
+                res[(P, nsc, T)] = t
+                T *= 2
+            del sets
+            torch.cuda.empty_cache()
+    best_b = {nsc: min(t for (P, n, T), t in res.items() if n == nsc) for nsc, _ in buckets}
+    score = {}
+    for P in (4, 8, 16):
+        score[P] = sum(min(t for (p, n, T), t in res.items() if p == P and n == nsc) / best_b[nsc]
+                       for nsc, _ in buckets)
+    P = min(score, key=score.get)
+    lines, chosen = [], []
+    for nsc, bucket in buckets:
+        T, t = min(((T, t) for (p, n, T), t in res.items() if p == P and n == nsc), key=lambda x: x[1])
+        lines.append("mmse %d %d %d %d %d" % (L, A, bucket, P, T))
+        chosen.append((nsc, T, t))
+    report = [{"P": p, "nsc": n, "T": T, "us": t * 1e6} for (p, n, T), t in sorted(res.items())]
+    return lines, P, chosen, report
+
+
+def slot_cells(dev, n, seed):
+    """Cells of the multi-cell batch: PRB {24,52,106,273}, antennas {16,32,64}, layers {2,4,8}, QAM 16-256."""
+    import numpy as np
+    import torch
+    import pusch_eq as pe
+    rng = np.random.default_rng(seed)
+    g = torch.Generator(device=dev).manual_seed(seed)
+    cells, keep = [], []
+    for _ in range(n):
+        prb, a, l, q = (int(rng.choice(v)) for v in ([24, 52, 106, 273], [16, 32, 64], [2, 4, 8], [2, 3, 4]))
+        nsc = 12 * prb
+        H = torch.randn((nsc, a, l), dtype=torch.complex64, device=dev, generator=g) * (1.0 / a) ** 0.5
+        s2 = torch.full((nsc,), 10 ** (-float(rng.uniform(5, 25)) / 10), device=dev)
+        Y = torch.randn((nsc, 11, a), dtype=torch.complex64, device=dev, generator=g)
+        C = torch.empty((nsc, l, a), dtype=torch.complex64, device=dev)
+        ni = torch.empty((nsc, l), device=dev)
+        llr = torch.empty((nsc, l, 11, 2 * q), device=dev)
+        keep.append((H, s2, Y, C, ni, llr))
+        cells.append(pe.cell_desc(C, ni, Y, llr, q, "f16", H, s2))
+    return cells, keep
+
+
+def tune_slot(pe, dev, base):
+    """Subcarriers per MMSE tile and per apply tile of the multi-cell executor, on two alternating 24-cell
+    slot plans replayed as graphs."""
+    import torch
+    sets = [slot_cells(dev, 24, 77 + k) for k in range(2)]
+    res = []
+    for mt in (8, 16, 32, 64):
+        for at in (4, 8, 16):
+            with_table(pe, ["slot %d %d" % (mt, at)] + base)
+            plans = [pe.SlotPlan(c) for c, _ in sets]
+            t = timed(lambda pl: pl.graph_launch(), [(plans[0],), (plans[1],)], iters=30, warm=6)
+            for pl in plans:
+                pl.close()
+            res.append({"mmse_tile": mt, "apply_tile": at, "us": t * 1e6})
+    best = min(res, key=lambda r: r["us"])
+    return "slot %d %d" % (best["mmse_tile"], best["apply_tile"]), best, res
 
 
 def main():
@@ -128,6 +194,8 @@ def main():
     ap.add_argument("--out", default=os.path.join(HERE, "tune_table.txt"))
     ap.add_argument("--report", default=None, help="write the per-candidate timings as JSON")
     ap.add_argument("--quick", action="store_true", help="A in {16, 64}, L = 8, f16 only")
+    ap.add_argument("--keep-coef", action="store_true",
+                    help="keep the coef lines of the existing table (--out) and tune only MMSE and slot tiles")
     args = ap.parse_args()
     import torch
     import pusch_eq as pe
@@ -138,20 +206,31 @@ def main():
     precs = ("f16",) if args.quick else ("f16", "f16_split", "bf16", "bf16_split", "f32")
     S = 11
     lines = ["# tile table written by autotune.py on %s" % torch.cuda.get_device_name(0),
-             "# coef <L> <S> <A> <prec> <f32_kc> <mfma_wpb> <mfma_spw>; mmse <L> <A> <nSC_max> <lanes>"]
-    report = {"apply": [], "mmse": []}
-    for A, L, prec in itertools.product(As, Ls, precs):
+             "# coef <L> <S> <A> <prec> <f32_kc> <mfma_wpb> <mfma_spw>",
+             "# mmse <L> <A> <nSC_max> <lanes per subcarrier> <subcarriers per tile>",
+             "# slot <mmse subcarriers per tile> <apply subcarriers per tile>"]
+    report = {"apply": [], "mmse": [], "slot": []}
+    kept = []
+    if args.keep_coef and os.path.exists(args.out):
+        kept = [ln.strip() for ln in open(args.out) if ln.startswith("coef ")]
+        lines.extend(kept)
+    for A, L, prec in ([] if kept else itertools.product(As, Ls, precs)):
         (t, kc, w, s), res, nsc = tune_apply(pe, dev, L, S, A, prec, 600e6)
         lines.append("coef %d %d %d %d %d %d %d" % (L, S, A, PRECS[prec], kc, w, s))
         report["apply"].append({"L": L, "S": S, "A": A, "prec": prec, "nsc": nsc, "candidates": res})
         print("apply L=%d A=%3d %-10s best kc=%d wpb=%d spw=%d %.1f us" % (L, A, prec, kc, w, s, t * 1e6), flush=True)
+    if not kept:
+        lines.append("coef -1 -1 -1 -1 8 1 1")
     for A, L in itertools.product(As, Ls):
-        for nsc, bucket in ((1584, 4096), (50688, -1)):
-            (t, lanes), res = tune_mmse(pe, dev, L, A, nsc)
-            lines.append("mmse %d %d %d %d" % (L, A, bucket, lanes))
-            report["mmse"].append({"L": L, "A": A, "nsc": nsc, "candidates": res})
-            print("mmse  L=%d A=%3d nsc=%5d best lanes=%d %.1f us" % (L, A, nsc, lanes, t * 1e6), flush=True)
-    lines.append("coef -1 -1 -1 -1 8 1 1")
+        mlines, P, chosen, rep = tune_mmse(pe, dev, L, A, lines)
+        lines.extend(mlines)
+        report["mmse"].append({"L": L, "A": A, "P": P, "candidates": rep})
+        print("mmse  L=%d A=%3d lanes=%2d  %s" % (L, A, P, "  ".join("nsc %d: T=%d %.1f us" % (n, T, t * 1e6)
+                                                                   for n, T, t in chosen)), flush=True)
+    sline, best, rep = tune_slot(pe, dev, lines)
+    lines.append(sline)
+    report["slot"] = rep
+    print("slot  %s  %.1f us per 24-cell slot" % (sline, best["us"]), flush=True)
     with open(args.out, "w") as f:
         f.write("\n".join(lines) + "\n")
     pe.lib().pe_tune_load(args.out.encode())

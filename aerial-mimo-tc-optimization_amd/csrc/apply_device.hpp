@@ -164,15 +164,6 @@ struct LdsLayout {
     static constexpr int bytes(int prec) { return (prec == 3 || prec == 4 ? 2 : 1) * HI; }
 };
 
-// Orders this wave's LDS accesses across the point of call. The kernels using it run one-wave workgroups:
-// LDS instructions of a wave execute in order, so only compiler reordering has to be stopped -- unlike
-// __syncthreads(), this does not wait for the wave's outstanding global loads (the prefetched next
-// subcarrier stays in flight).
-__device__ __forceinline__ void lds_order_wave() {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
 
 // Antennas past A are never written by process_sc; zero every operand row once before a subcarrier range
 // whose A is below the 16*KS the layout holds.

@@ -1,5 +1,5 @@
-// gram_mmse.hip -- single-launch MMSE coefficient kernels (FP32 pass, FP64 pass); the per-subcarrier solve
-// is in mmse_device.hpp.
+// gram_mmse.hip -- single-launch MMSE coefficient kernels: the FP32 tile pass and the FP64 pass over flagged
+// subcarriers; the per-subcarrier arithmetic is in mmse_device.hpp.
 #include "mmse_device.hpp"
 #include "pe_internal.hpp"
 
@@ -9,66 +9,44 @@ namespace {
 
 using namespace mmse;
 
-template <int L, int P>
-__global__ __launch_bounds__(256) void gram_mmse_f32_kernel(const float2 *__restrict__ H,
-                                                            const float *__restrict__ sigma2,
-                                                            float2 *__restrict__ C, float *__restrict__ noise_inv,
-                                                            uint8_t *__restrict__ flags, int nSC, int A,
-                                                            int unbiased, float thr) {
-    const int lane = threadIdx.x & 63;
-    const int p = lane & (P - 1);
-    const long sc = (long)blockIdx.x * (blockDim.x / P) + threadIdx.x / P;
-    if (sc >= nSC) return;  // whole group leaves together; shuffles stay inside groups
-    const bool ok = mmse_one<float, L, P>(H + (size_t)sc * A * L, sigma2[sc], A, p, unbiased != 0, thr, true,
-                                          C + (size_t)sc * L * A, noise_inv + (size_t)sc * L);
-    if (p == 0) flags[sc] = ok ? 0 : kPendingFp64;
+// One wave per tile of T subcarriers (the last tile may be short); the wave's LDS holds T slots.
+template <int L>
+__global__ __launch_bounds__(64) void gram_mmse_f32_kernel(const float2 *__restrict__ H,
+                                                           const float *__restrict__ sigma2,
+                                                           float2 *__restrict__ C, float *__restrict__ noise_inv,
+                                                           uint8_t *__restrict__ flags, int nSC, int A, int P,
+                                                           int T, int unbiased, float thr) {
+    extern __shared__ __attribute__((aligned(16))) float lds[];
+    const long sc0 = (long)blockIdx.x * T;
+    const int n = (int)min((long)T, (long)nSC - sc0);
+    tile_fp32<L>(H, sigma2, C, noise_inv, flags, sc0, n, A, P, unbiased != 0, thr, lds, threadIdx.x, nullptr);
 }
 
-template <int L, int P>
-__global__ __launch_bounds__(256) void gram_mmse_f64_kernel(const float2 *__restrict__ H,
-                                                            const float *__restrict__ sigma2,
-                                                            float2 *__restrict__ C, float *__restrict__ noise_inv,
-                                                            uint8_t *__restrict__ flags, int nSC, int A,
-                                                            int unbiased) {
-    const int lane = threadIdx.x & 63;
-    const int p = lane & (P - 1);
-    const long sc = (long)blockIdx.x * (blockDim.x / P) + threadIdx.x / P;
-    if (sc >= nSC) return;
-    if (!(flags[sc] & kPendingFp64)) return;
-    float2 *c = C + (size_t)sc * L * A;
-    float *ni = noise_inv + (size_t)sc * L;
-    const bool ok = mmse_one<double, L, P>(H + (size_t)sc * A * L, (double)sigma2[sc], A, p, unbiased != 0,
-                                           0.0, false, c, ni);
-    if (!ok) {
-        for (int a = p; a < A; a += P)
-            for (int l = 0; l < L; ++l) c[(size_t)l * A + a] = make_float2(0.f, 0.f);
-        for (int l = 0; l < L; ++l)
-            if (p == l % P) ni[l] = 0.f;
-    }
-    // every lane of the group has read the flag before lane 0 rewrites it (one wave, lockstep)
-    if (p == 0) flags[sc] = ok ? PE_SCFLAG_FP64 : (PE_SCFLAG_FP64 | PE_SCFLAG_SINGULAR);
-}
-
-template <int L, int P>
-hipError_t launch_mmse_LP(const float2 *H, const float *sigma2, float2 *C, float *ni, uint8_t *flags, int nSC,
-                          int A, bool unbiased, float thr, hipStream_t st) {
-    const int per_block = 256 / P;
-    const int grid = (nSC + per_block - 1) / per_block;
-    hipLaunchKernelGGL((gram_mmse_f32_kernel<L, P>), dim3(grid), dim3(256), 0, st, H, sigma2, C, ni, flags, nSC,
-                       A, unbiased ? 1 : 0, thr);
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return e;
-    hipLaunchKernelGGL((gram_mmse_f64_kernel<L, P>), dim3(grid), dim3(256), 0, st, H, sigma2, C, ni, flags, nSC,
-                       A, unbiased ? 1 : 0);
-    return hipGetLastError();
+// One wave per 64 subcarriers; leaves after one flag load unless one of them was flagged.
+template <int L>
+__global__ __launch_bounds__(64) void gram_mmse_f64_kernel(const float2 *__restrict__ H,
+                                                           const float *__restrict__ sigma2,
+                                                           float2 *__restrict__ C, float *__restrict__ noise_inv,
+                                                           uint8_t *__restrict__ flags, int nSC, int A, int P,
+                                                           int unbiased) {
+    const long sc0 = (long)blockIdx.x * 64;
+    const int n = (int)min(64L, (long)nSC - sc0);
+    tile_fp64<L>(H, sigma2, C, noise_inv, flags, sc0, n, A, P, unbiased != 0, threadIdx.x);
 }
 
 template <int L>
 hipError_t launch_mmse_L(const float2 *H, const float *sigma2, float2 *C, float *ni, uint8_t *flags, int nSC,
                          int A, bool unbiased, float thr, hipStream_t st) {
-    if (mmse_lanes_for(L, A, nSC) == 16)
-        return launch_mmse_LP<L, 16>(H, sigma2, C, ni, flags, nSC, A, unbiased, thr, st);
-    return launch_mmse_LP<L, 8>(H, sigma2, C, ni, flags, nSC, A, unbiased, thr, st);
+    const int P = mmse_lanes_for(L, A);
+    const int T = mmse_tile_for(L, A, nSC);
+    const int grid = (nSC + T - 1) / T;
+    hipLaunchKernelGGL((gram_mmse_f32_kernel<L>), dim3(grid), dim3(64), (size_t)tile_lds_bytes(L, T), st, H, sigma2,
+                       C, ni, flags, nSC, A, P, T, unbiased ? 1 : 0, thr);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL((gram_mmse_f64_kernel<L>), dim3((nSC + 63) / 64), dim3(64), 0, st, H, sigma2, C, ni, flags,
+                       nSC, A, P, unbiased ? 1 : 0);
+    return hipGetLastError();
 }
 
 }  // namespace

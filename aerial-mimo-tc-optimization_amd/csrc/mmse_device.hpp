@@ -1,19 +1,29 @@
-// mmse_device.hpp -- per-subcarrier MMSE solve as a device function, shared by the single-cell kernels
+// mmse_device.hpp -- per-subcarrier MMSE solve as device functions, shared by the single-cell kernels
 // (gram_mmse.hip) and the multi-cell persistent kernels (multicell.hip).
 //
 // The reference only names this stage (eqMmseCoefCompHighMimoKernel, aerial_bottleneck.cu:51-53); the
-// arithmetic here is the regularized solve that defines C in the reference's headline workload.
+// arithmetic here is the regularized solve that defines C in the reference's headline workload:
+//   G = H^H H + sigma2 I (Hermitian L x L),  Cholesky G = R R^H,  W = R^-1,  Ginv = W^H W,
+//   C = diag(s) Ginv H^H  with s_l = 1 / (1 - sigma2 Ginv_ll) when unbiased (else 1),
+//   noise_inv_l = (1 - x_l) / x_l  with x_l = sigma2 Ginv_ll,
+//   condition estimate kappa_1(G) = ||G||_1 ||Ginv||_1; above the threshold, or when the factorization
+//   breaks down, the FP32 pass hands the subcarrier to an FP64 pass.
 //
-// Mapping: a group of P consecutive lanes owns one subcarrier (64/P subcarriers per wave). Lane p of a
-// group reads antenna rows a = p, p+P, ... of H[sc] (L complex each, contiguous, so the group's loads are
-// contiguous) and accumulates the lower triangle of the Hermitian Gram; a butterfly of xor-shuffles inside
-// the group sums the P partial Grams, after which every lane of the group holds the same G. The L x L
-// algebra (Cholesky G = L L^H, W = L^-1, Ginv = W^H W, 1-norm condition number) then runs redundantly in
-// each lane's registers with fully unrolled static indexing, and each lane writes its own antennas' columns
-// of C = diag(s) Ginv H^H.
-//
-// Precision: pass 1 runs in FP32 and flags subcarriers whose kappa_1(G) exceeds the threshold or whose
-// Cholesky breaks down; pass 2 (same code in FP64) recomputes only the flagged subcarriers.
+// Lanes and subcarriers. P lanes (P = 4, 8 or 16) share a subcarrier wherever antennas are summed: lane p of
+// the group takes antenna rows a = p, p + P, ... of H[sc] (rows of L complex values, contiguous, so a group's
+// loads are contiguous). The partial Gram matrices are summed across the group by a DPP butterfly, after
+// which every lane of the group holds the same G.
+//   * tile solve (FP32 pass): one wave owns a tile of up to 64 subcarriers. Gram: 64/P subcarriers per
+//     round, the group sum of each written to the wave's LDS slot of that subcarrier. Algebra: one lane per
+//     subcarrier reads its G from LDS and writes Ginv, the output scales and an ok flag back. Coefficients:
+//     one lane per antenna (up to 64 lanes per subcarrier), reading Ginv from LDS, so stores cover whole C
+//     rows. With 64/P subcarriers per tile the Gram is a single round (lowest latency); with 64 the algebra
+//     runs on every lane without redundancy. The tile size changes no result.
+//   * group solve (FP64 pass): the P lanes of a group run the whole solve for one flagged subcarrier, the
+//     L x L algebra redundantly in each lane.
+// Results depend on P only through the Gram summation order; every path takes P from the same per-(L, A)
+// choice, and the arithmetic is written with explicit fused multiply-adds and contraction off, so the
+// single-cell kernels and the multi-cell kernels produce bit-identical coefficients.
 #pragma once
 #include "pe_device.hpp"
 #include "pusch_eq.h"
@@ -21,203 +31,417 @@
 namespace pe {
 namespace mmse {
 
-constexpr int kMmseP = 8;              // lanes per subcarrier
 constexpr uint8_t kPendingFp64 = 0x80;  // internal flag: FP32 pass handed the subcarrier to the FP64 pass
 
 __host__ __device__ constexpr int tri(int i, int j) { return i * (i + 1) / 2 + j; }  // i >= j
+// Floats per subcarrier slot in LDS: Ginv (2 NT) + scales (L) + ok flag, rounded up to 4 x an odd number so
+// that 16 lanes reading 16-byte vectors from consecutive slots hit distinct LDS banks.
+__host__ __device__ constexpr int slot_quads(int L) { return (L * (L + 1) + L + 4) / 4; }
+__host__ __device__ constexpr int slot_floats(int L) {
+    return 4 * (slot_quads(L) % 2 ? slot_quads(L) : slot_quads(L) + 1);
+}
+// LDS bytes of a tile of T subcarriers with L layers
+__host__ __device__ constexpr int tile_lds_bytes(int L, int T) { return 4 * T * slot_floats(L); }
+template <int L> struct Dim {
+    static constexpr int NT = L * (L + 1) / 2;  // stored entries of a Hermitian L x L (lower triangle)
+    static constexpr int SLOT = slot_floats(L);
+};
 
-template <typename T>
-__device__ __forceinline__ T dev_sqrt(T x);
-template <>
-__device__ __forceinline__ float dev_sqrt<float>(float x) { return sqrtf(x); }
-template <>
-__device__ __forceinline__ double dev_sqrt<double>(double x) { return sqrt(x); }
+template <typename T> __device__ __forceinline__ T dev_sqrt(T x);
+template <> __device__ __forceinline__ float dev_sqrt<float>(float x) { return __builtin_amdgcn_sqrtf(x); }
+template <> __device__ __forceinline__ double dev_sqrt<double>(double x) { return __builtin_sqrt(x); }
+template <typename T> __device__ __forceinline__ T dev_rcp(T x);
+template <> __device__ __forceinline__ float dev_rcp<float>(float x) { return __builtin_amdgcn_rcpf(x); }
+template <> __device__ __forceinline__ double dev_rcp<double>(double x) { return 1.0 / x; }
+__device__ __forceinline__ float abs_(float a) { return __builtin_fabsf(a); }
+__device__ __forceinline__ double abs_(double a) { return __builtin_fabs(a); }
+__device__ __forceinline__ float fma_(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+__device__ __forceinline__ double fma_(double a, double b, double c) { return __builtin_fma(a, b, c); }
 
-template <typename T>
-__device__ __forceinline__ T shfl_xor_t(T v, int m) { return __shfl_xor(v, m, 64); }
+// ---- group sums -------------------------------------------------------------------------------------------
+template <int CTRL> __device__ __forceinline__ float dpp(float v) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), CTRL, 0xF, 0xF,
+                                                                 true));
+}
+template <int CTRL> __device__ __forceinline__ double dpp(double v) {
+    const unsigned long long u = __builtin_bit_cast(unsigned long long, v);
+    const unsigned lo = (unsigned)__builtin_amdgcn_update_dpp(0, (int)(unsigned)u, CTRL, 0xF, 0xF, true);
+    const unsigned hi = (unsigned)__builtin_amdgcn_update_dpp(0, (int)(unsigned)(u >> 32), CTRL, 0xF, 0xF, true);
+    return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
+}
+// Sum over each aligned group of P lanes (P = 4, 8, 16; P is wave-uniform), result in every lane of the
+// group: xor-1 and xor-2 quad permutes, then the half-row mirror (lane i <-> 7 - i) and the row mirror
+// (i <-> 15 - i), each pairing the two halves of the next larger group. a + b == b + a, so every lane ends
+// with the same bits.
+template <typename T> __device__ __forceinline__ T group_sum(T v, int P) {
+    v += dpp<0xB1>(v);
+    v += dpp<0x4E>(v);
+    if (P >= 8) v += dpp<0x141>(v);
+    if (P >= 16) v += dpp<0x140>(v);
+    return v;
+}
 
-// Solves one subcarrier. T = float (pass 1) or double (pass 2). Returns false when the factorization or
-// the condition test fails (pass 1: the subcarrier is handed to pass 2).
-template <typename T, int L, int P>
-__device__ __forceinline__ bool mmse_one(const float2 *__restrict__ h, T s2, int A, int p, bool unbiased,
-                                         T thr, bool check_cond, float2 *__restrict__ c_out,
-                                         float *__restrict__ ni_out) {
-    constexpr int NT = L * (L + 1) / 2;
-    T gr[NT], gi[NT];
+#pragma clang fp contract(off)
+
+// ---- Gram: this lane's partial sum G[i][j] = sum_a conj(H[a][i]) H[a][j], i >= j ---------------------------
+template <typename T, int L>
+__device__ __forceinline__ void load_row(const float2 *__restrict__ row, T (&vr)[L], T (&vi)[L]) {
+    if constexpr (L % 2 == 0) {
 #pragma unroll
-    for (int k = 0; k < NT; ++k) gr[k] = gi[k] = T(0);
+        for (int k = 0; k < L / 2; ++k) {
+            const float4 q = reinterpret_cast<const float4 *>(row)[k];
+            vr[2 * k] = (T)q.x;
+            vi[2 * k] = (T)q.y;
+            vr[2 * k + 1] = (T)q.z;
+            vi[2 * k + 1] = (T)q.w;
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < L; ++k) {
+            const float2 q = row[k];
+            vr[k] = (T)q.x;
+            vi[k] = (T)q.y;
+        }
+    }
+}
 
-    // partial Gram over this lane's antennas: G[i][j] = sum_a conj(H[a][i]) H[a][j], i >= j
-    // (unrolled so several antenna rows are in flight at once instead of one memory latency per row)
+template <typename T, int L>
+__device__ __forceinline__ void gram_group(const float2 *__restrict__ h, int A, int p, int P,
+                                           T (&gr)[Dim<L>::NT], T (&gi)[Dim<L>::NT]) {
+#pragma unroll
+    for (int k = 0; k < Dim<L>::NT; ++k) gr[k] = gi[k] = T(0);
+    // unrolled so several antenna rows are in flight at once
 #pragma unroll 4
     for (int a = p; a < A; a += P) {
         T vr[L], vi[L];
-        const float2 *row = h + (size_t)a * L;
-        if constexpr (L % 2 == 0) {
-#pragma unroll
-            for (int k = 0; k < L / 2; ++k) {
-                const float4 q = reinterpret_cast<const float4 *>(row)[k];
-                vr[2 * k] = q.x; vi[2 * k] = q.y; vr[2 * k + 1] = q.z; vi[2 * k + 1] = q.w;
-            }
-        } else {
-#pragma unroll
-            for (int k = 0; k < L; ++k) {
-                const float2 q = row[k];
-                vr[k] = q.x; vi[k] = q.y;
-            }
-        }
+        load_row<T, L>(h + (size_t)a * L, vr, vi);
 #pragma unroll
         for (int i = 0; i < L; ++i)
 #pragma unroll
             for (int j = 0; j <= i; ++j) {
-                gr[tri(i, j)] += vr[i] * vr[j] + vi[i] * vi[j];
-                gi[tri(i, j)] += vr[i] * vi[j] - vi[i] * vr[j];
+                gr[tri(i, j)] = fma_(vr[i], vr[j], fma_(vi[i], vi[j], gr[tri(i, j)]));
+                gi[tri(i, j)] = fma_(vr[i], vi[j], fma_(-vi[i], vr[j], gi[tri(i, j)]));
             }
     }
 #pragma unroll
-    for (int m = 1; m < P; m <<= 1)
+    for (int k = 0; k < Dim<L>::NT; ++k) {
+        gr[k] = group_sum(gr[k], P);
+        gi[k] = group_sum(gi[k], P);
+    }
+}
+
+// ---- the L x L algebra --------------------------------------------------------------------------------------
+// 1-norm of a Hermitian matrix stored as its lower triangle.
+template <typename T, int L>
+__device__ __forceinline__ T herm_norm1(const T (&mr)[Dim<L>::NT], const T (&mi)[Dim<L>::NT]) {
+    T mag[Dim<L>::NT];
 #pragma unroll
-        for (int k = 0; k < NT; ++k) {
-            gr[k] += shfl_xor_t(gr[k], m);
-            gi[k] += shfl_xor_t(gi[k], m);
+    for (int i = 0; i < L; ++i)
+#pragma unroll
+        for (int j = 0; j <= i; ++j) {
+            const int k = tri(i, j);
+            mag[k] = (i == j) ? abs_(mr[k]) : dev_sqrt<T>(fma_(mr[k], mr[k], mi[k] * mi[k]));
         }
+    T nrm = T(0);
+#pragma unroll
+    for (int j = 0; j < L; ++j) {
+        T cs = T(0);
+#pragma unroll
+        for (int i = 0; i < L; ++i) cs += mag[i >= j ? tri(i, j) : tri(j, i)];
+        nrm = cs > nrm ? cs : nrm;
+    }
+    return nrm;
+}
+
+// On entry g holds the summed Gram matrix (without sigma2); on return v holds Ginv (lower triangle), scl the
+// output scale of each layer and x_l = sigma2 Ginv_ll. Returns false when the factorization breaks down or,
+// with check_cond, when kappa_1 exceeds thr or is not finite.
+template <typename T, int L>
+__device__ __forceinline__ bool solve(T (&gr)[Dim<L>::NT], T (&gi)[Dim<L>::NT], T s2, bool unbiased, T thr,
+                                      bool check_cond, T (&vr)[Dim<L>::NT], T (&vi)[Dim<L>::NT], T (&scl)[L],
+                                      T (&x)[L]) {
+    constexpr int NT = Dim<L>::NT;
 #pragma unroll
     for (int i = 0; i < L; ++i) {
         gr[tri(i, i)] += s2;
         gi[tri(i, i)] = T(0);
     }
+    T normG = check_cond ? herm_norm1<T, L>(gr, gi) : T(0);
+    // pin ||G||_1 here: computed next to its use at the end, it would keep all of G alive through the solve
+    asm volatile("" : "+v"(normG));
 
-    // ||G||_1 (Hermitian: column j sums |G[i][j]| over the stored lower triangle and its mirror)
-    T normG = T(0);
-#pragma unroll
-    for (int j = 0; j < L; ++j) {
-        T cs = T(0);
-#pragma unroll
-        for (int i = 0; i < L; ++i) {
-            const int k = i >= j ? tri(i, j) : tri(j, i);
-            cs += dev_sqrt<T>(gr[k] * gr[k] + gi[k] * gi[k]);
-        }
-        normG = cs > normG ? cs : normG;
-    }
-
-    // Cholesky in place: lower L overwrites the lower triangle of G
+    // Cholesky in place: R (lower) overwrites G; rd[j] = 1 / R_jj
     bool ok = true;
+    T rd[L];
 #pragma unroll
     for (int j = 0; j < L; ++j) {
         T d = gr[tri(j, j)];
 #pragma unroll
-        for (int k = 0; k < j; ++k) d -= gr[tri(j, k)] * gr[tri(j, k)] + gi[tri(j, k)] * gi[tri(j, k)];
+        for (int k = 0; k < j; ++k) d = fma_(-gr[tri(j, k)], gr[tri(j, k)], fma_(-gi[tri(j, k)], gi[tri(j, k)], d));
         ok = ok && (d > T(0));
-        const T ljj = dev_sqrt<T>(d > T(0) ? d : T(1));
-        const T inv = T(1) / ljj;
-        gr[tri(j, j)] = ljj;
+        const T rjj = dev_sqrt<T>(d > T(0) ? d : T(1));
+        rd[j] = dev_rcp<T>(rjj);
+        gr[tri(j, j)] = rjj;
 #pragma unroll
         for (int i = j + 1; i < L; ++i) {
             T re = gr[tri(i, j)], im = gi[tri(i, j)];
 #pragma unroll
-            for (int k = 0; k < j; ++k) {
-                re -= gr[tri(i, k)] * gr[tri(j, k)] + gi[tri(i, k)] * gi[tri(j, k)];
-                im -= gi[tri(i, k)] * gr[tri(j, k)] - gr[tri(i, k)] * gi[tri(j, k)];
+            for (int k = 0; k < j; ++k) {  // R_ij -= R_ik conj(R_jk)
+                re = fma_(-gr[tri(i, k)], gr[tri(j, k)], fma_(-gi[tri(i, k)], gi[tri(j, k)], re));
+                im = fma_(-gi[tri(i, k)], gr[tri(j, k)], fma_(gr[tri(i, k)], gi[tri(j, k)], im));
             }
-            gr[tri(i, j)] = re * inv;
-            gi[tri(i, j)] = im * inv;
+            gr[tri(i, j)] = re * rd[j];
+            gi[tri(i, j)] = im * rd[j];
         }
     }
 
-    // W = L^-1 (lower)
+    // W = R^-1 (lower), from W R = I: W_jj = 1 / R_jj, W_ij = -(sum_{k=j+1}^{i} W_ik R_kj) / R_jj. Columns
+    // are formed from the last to the first and rows from the bottom up, so column j of R is dead once
+    // column j of W is done (R and W never need to be held at once).
     T wr[NT], wi[NT];
 #pragma unroll
-    for (int j = 0; j < L; ++j) {
-        wr[tri(j, j)] = T(1) / gr[tri(j, j)];
+    for (int j = L - 1; j >= 0; --j) {
+        wr[tri(j, j)] = rd[j];
         wi[tri(j, j)] = T(0);
+#pragma unroll
+        for (int i = L - 1; i > j; --i) {
+            T re = wr[tri(i, i)] * gr[tri(i, j)], im = wr[tri(i, i)] * gi[tri(i, j)];
+#pragma unroll
+            for (int k = j + 1; k < i; ++k) {
+                re = fma_(wr[tri(i, k)], gr[tri(k, j)], fma_(-wi[tri(i, k)], gi[tri(k, j)], re));
+                im = fma_(wr[tri(i, k)], gi[tri(k, j)], fma_(wi[tri(i, k)], gr[tri(k, j)], im));
+            }
+            wr[tri(i, j)] = -re * rd[j];
+            wi[tri(i, j)] = -im * rd[j];
+        }
     }
+
+    // Ginv = W^H W, lower triangle: V_ij = sum_{k >= i} conj(W_ki) W_kj (i >= j; W_ii is real), formed
+    // column by column so that column j of W is dead once column j of V is done.
 #pragma unroll
     for (int j = 0; j < L; ++j)
 #pragma unroll
-        for (int i = j + 1; i < L; ++i) {
-            T re = T(0), im = T(0);
+        for (int i = j; i < L; ++i) {
+            T re = wr[tri(i, i)] * wr[tri(i, j)], im = wr[tri(i, i)] * wi[tri(i, j)];
 #pragma unroll
-            for (int k = j; k < i; ++k) {
-                re += gr[tri(i, k)] * wr[tri(k, j)] - gi[tri(i, k)] * wi[tri(k, j)];
-                im += gr[tri(i, k)] * wi[tri(k, j)] + gi[tri(i, k)] * wr[tri(k, j)];
+            for (int k = i + 1; k < L; ++k) {
+                re = fma_(wr[tri(k, i)], wr[tri(k, j)], fma_(wi[tri(k, i)], wi[tri(k, j)], re));
+                im = fma_(wr[tri(k, i)], wi[tri(k, j)], fma_(-wi[tri(k, i)], wr[tri(k, j)], im));
             }
-            wr[tri(i, j)] = -re * wr[tri(i, i)];
-            wi[tri(i, j)] = -im * wr[tri(i, i)];
-        }
-
-    // Ginv = W^H W, lower triangle: V[i][j] = sum_{k >= i} conj(W[k][i]) W[k][j]   (i >= j)
-    T vr_[NT], vi_[NT];
-#pragma unroll
-    for (int i = 0; i < L; ++i)
-#pragma unroll
-        for (int j = 0; j <= i; ++j) {
-            T re = T(0), im = T(0);
-#pragma unroll
-            for (int k = i; k < L; ++k) {
-                re += wr[tri(k, i)] * wr[tri(k, j)] + wi[tri(k, i)] * wi[tri(k, j)];
-                im += wr[tri(k, i)] * wi[tri(k, j)] - wi[tri(k, i)] * wr[tri(k, j)];
-            }
-            vr_[tri(i, j)] = re;
-            vi_[tri(i, j)] = (i == j) ? T(0) : im;
+            vr[tri(i, j)] = re;
+            vi[tri(i, j)] = (i == j) ? T(0) : im;
         }
 
     if (check_cond) {
-        T normV = T(0);
-#pragma unroll
-        for (int j = 0; j < L; ++j) {
-            T cs = T(0);
-#pragma unroll
-            for (int i = 0; i < L; ++i) {
-                const int k = i >= j ? tri(i, j) : tri(j, i);
-                cs += dev_sqrt<T>(vr_[k] * vr_[k] + vi_[k] * vi_[k]);
-            }
-            normV = cs > normV ? cs : normV;
-        }
-        const T kappa = normG * normV;
+        const T kappa = normG * herm_norm1<T, L>(vr, vi);
         ok = ok && (kappa <= thr);  // false for NaN
     }
-    if (!ok) return false;
-
-    T scl[L];
 #pragma unroll
     for (int l = 0; l < L; ++l) {
-        const T x = s2 * vr_[tri(l, l)];
-        const T mu = T(1) - x;
-        scl[l] = (unbiased && mu != T(0)) ? T(1) / mu : T(1);
-        if (p == (l % P)) {
-            const T xf = x > T(1e-30) ? x : T(1e-30);
-            ni_out[l] = (float)((T(1) - x) / xf);
-        }
+        x[l] = s2 * vr[tri(l, l)];
+        const T mu = T(1) - x[l];
+        scl[l] = (unbiased && mu != T(0)) ? dev_rcp<T>(mu) : T(1);
     }
+    return ok;
+}
 
-    // C[l][a] = s_l * sum_j Ginv[l][j] conj(H[a][j])
-#pragma unroll 4
+template <typename T>
+__device__ __forceinline__ float noise_inv_of(T x) {
+    const T xf = x > T(1e-30) ? x : T(1e-30);
+    return (float)((T(1) - x) * dev_rcp<T>(xf));
+}
+
+// ---- coefficients: C[l][a] = s_l sum_j Ginv[l][j] conj(H[a][j]) for this lane's antennas --------------------
+template <typename T, int L>
+__device__ __forceinline__ void write_c(const float2 *__restrict__ h, int A, int p, int P,
+                                        const T (&vr)[Dim<L>::NT], const T (&vi)[Dim<L>::NT], const T (&scl)[L],
+                                        float2 *__restrict__ c_out, bool store = true) {
+#pragma unroll 2
     for (int a = p; a < A; a += P) {
         T hr[L], hi[L];
-        const float2 *row = h + (size_t)a * L;
-#pragma unroll
-        for (int k = 0; k < L; ++k) {
-            const float2 q = row[k];
-            hr[k] = q.x;
-            hi[k] = q.y;
-        }
+        load_row<T, L>(h + (size_t)a * L, hr, hi);
 #pragma unroll
         for (int l = 0; l < L; ++l) {
             T re = T(0), im = T(0);
 #pragma unroll
             for (int j = 0; j < L; ++j) {
-                // Ginv[l][j]: stored lower; for j > l use conj(V[j][l])
-                const T gr_ = j <= l ? vr_[tri(l, j)] : vr_[tri(j, l)];
-                const T gi_ = j <= l ? vi_[tri(l, j)] : -vi_[tri(j, l)];
-                // (gr_ + i gi_) * (hr - i hi)
-                re += gr_ * hr[j] + gi_ * hi[j];
-                im += gi_ * hr[j] - gr_ * hi[j];
+                // Ginv[l][j]: stored lower; for j > l it is conj(V[j][l])
+                const T g_r = j <= l ? vr[tri(l, j)] : vr[tri(j, l)];
+                const T g_i = j <= l ? vi[tri(l, j)] : -vi[tri(j, l)];
+                re = fma_(g_r, hr[j], fma_(g_i, hi[j], re));   // (g_r + i g_i)(h_r - i h_i)
+                im = fma_(g_i, hr[j], fma_(-g_r, hi[j], im));
             }
-            c_out[(size_t)l * A + a] = make_float2((float)(re * scl[l]), (float)(im * scl[l]));
+            if (store) c_out[(size_t)l * A + a] = make_float2((float)(re * scl[l]), (float)(im * scl[l]));
         }
     }
-    return true;
 }
+
+// ---- LDS slot access (16-byte vectors; SLOT is a multiple of 4 floats, slots 16-byte aligned) ----------------
+template <int K>
+__device__ __forceinline__ void slot_store(float *__restrict__ s, const float (&v)[K]) {
+#pragma unroll
+    for (int k = 0; k + 3 < K; k += 4)
+        *reinterpret_cast<float4 *>(s + k) = make_float4(v[k], v[k + 1], v[k + 2], v[k + 3]);
+#pragma unroll
+    for (int k = K / 4 * 4; k < K; ++k) s[k] = v[k];
+}
+template <int K>
+__device__ __forceinline__ void slot_load(const float *__restrict__ s, float (&v)[K]) {
+#pragma unroll
+    for (int k = 0; k + 3 < K; k += 4) {
+        const float4 q = *reinterpret_cast<const float4 *>(s + k);
+        v[k] = q.x;
+        v[k + 1] = q.y;
+        v[k + 2] = q.z;
+        v[k + 3] = q.w;
+    }
+#pragma unroll
+    for (int k = K / 4 * 4; k < K; ++k) v[k] = s[k];
+}
+
+// ---- FP32 pass over one tile: subcarriers sc0 .. sc0 + n - 1 (n <= 64) of one cell -------------------------
+// lds: this wave's region of at least n * Dim<L>::SLOT floats. flags[sc] = 0 or kPendingFp64 (C and noise_inv
+// of a flagged subcarrier are left to the FP64 pass); `flagged` (optional) counts flagged subcarriers.
+template <int L>
+__device__ __forceinline__ void tile_fp32(const float2 *__restrict__ H, const float *__restrict__ sigma2,
+                                          float2 *__restrict__ C, float *__restrict__ noise_inv,
+                                          uint8_t *__restrict__ flags, long sc0, int n, int A, int P,
+                                          bool unbiased, float thr, float *__restrict__ lds, int lane,
+                                          unsigned *flagged) {
+    constexpr int NT = Dim<L>::NT;
+    constexpr int SLOT = Dim<L>::SLOT;
+    const int lgP = __builtin_ctz((unsigned)P);
+    const int p = lane & (P - 1), g = lane >> lgP, G = 64 >> lgP;
+
+    // Gram, 64 / P subcarriers per round (lanes past n repeat the last subcarrier and store nothing)
+    for (int t0 = 0; t0 < n; t0 += G) {
+        const int t = min(t0 + g, n - 1);
+        float gr[NT], gi[NT];
+        gram_group<float, L>(H + (size_t)(sc0 + t) * A * L, A, p, P, gr, gi);
+        if (p == 0 && t0 + g < n) {
+            float buf[2 * NT];
+#pragma unroll
+            for (int k = 0; k < NT; ++k) {
+                buf[k] = gr[k];
+                buf[NT + k] = gi[k];
+            }
+            slot_store<2 * NT>(lds + t * SLOT, buf);
+        }
+    }
+    lds_order_wave();
+
+    // algebra, one lane per subcarrier
+    if (lane < n) {
+        const long sc = sc0 + lane;
+        float buf[2 * NT];
+        slot_load<2 * NT>(lds + lane * SLOT, buf);
+        float gr[NT], gi[NT], vr[NT], vi[NT], scl[L], x[L];
+#pragma unroll
+        for (int k = 0; k < NT; ++k) {
+            gr[k] = buf[k];
+            gi[k] = buf[NT + k];
+        }
+        const bool ok = solve<float, L>(gr, gi, sigma2[sc], unbiased, thr, true, vr, vi, scl, x);
+        float out[2 * NT + L + 1];
+#pragma unroll
+        for (int k = 0; k < NT; ++k) {
+            out[k] = vr[k];
+            out[NT + k] = vi[k];
+        }
+#pragma unroll
+        for (int l = 0; l < L; ++l) out[2 * NT + l] = scl[l];
+        out[2 * NT + L] = ok ? 1.f : 0.f;
+        slot_store<2 * NT + L + 1>(lds + lane * SLOT, out);
+        if (ok) {
+#pragma unroll
+            for (int l = 0; l < L; ++l) noise_inv[sc * L + l] = noise_inv_of(x[l]);
+        }
+        flags[sc] = ok ? 0 : kPendingFp64;
+        if (!ok && flagged) atomicAdd(flagged, 1u);
+    }
+    lds_order_wave();
+
+    // coefficients: Q = min(64, 2^ceil(log2 A)) lanes per subcarrier, lane q writing antennas q, q + Q, ..., so
+    // a store instruction covers up to 512 contiguous bytes of a C row whatever P is (C[l][a] is computed by
+    // one lane in a fixed order either way). Lanes past n repeat the last subcarrier and store nothing; two
+    // rounds per iteration keep two subcarriers' loads in flight.
+    const int lgQ = A >= 64 ? 6 : A <= 1 ? 0 : 32 - __builtin_clz((unsigned)A - 1u);
+    const int q = lane & ((1 << lgQ) - 1), g3 = lane >> lgQ, R3 = 64 >> lgQ;
+#pragma unroll 2
+    for (int t0 = 0; t0 < n; t0 += R3) {
+        const int t = min(t0 + g3, n - 1);
+        float in[2 * NT + L + 1];
+        slot_load<2 * NT + L + 1>(lds + t * SLOT, in);
+        const bool store = t0 + g3 < n && in[2 * NT + L] != 0.f;  // flagged: the FP64 pass writes it
+        float vr[NT], vi[NT], scl[L];
+#pragma unroll
+        for (int k = 0; k < NT; ++k) {
+            vr[k] = in[k];
+            vi[k] = in[NT + k];
+        }
+#pragma unroll
+        for (int l = 0; l < L; ++l) scl[l] = in[2 * NT + l];
+        write_c<float, L>(H + (size_t)(sc0 + t) * A * L, A, q, 1 << lgQ, vr, vi, scl,
+                          C + (size_t)(sc0 + t) * L * A, store);
+    }
+    lds_order_wave();  // slot reads done before the wave's next tile rewrites them
+}
+
+// ---- FP64 pass for one flagged subcarrier, by one group of P lanes -------------------------------------------
+// Writes C, noise_inv and the final flag (FP64, plus SINGULAR with zero C / noise_inv if the FP64
+// factorization breaks down too).
+template <int L>
+__device__ __forceinline__ void group_fp64(const float2 *__restrict__ h, float s2f, float2 *__restrict__ c,
+                                           float *__restrict__ ni, uint8_t *__restrict__ flag, int A, int p,
+                                           int P, bool unbiased) {
+    constexpr int NT = Dim<L>::NT;
+    double gr[NT], gi[NT], vr[NT], vi[NT], scl[L], x[L];
+    gram_group<double, L>(h, A, p, P, gr, gi);
+    const bool ok = solve<double, L>(gr, gi, (double)s2f, unbiased, 0.0, false, vr, vi, scl, x);
+    if (ok) {
+        if (p == 0) {
+#pragma unroll
+            for (int l = 0; l < L; ++l) ni[l] = noise_inv_of(x[l]);
+        }
+        write_c<double, L>(h, A, p, P, vr, vi, scl, c);
+    } else {
+        for (int a = p; a < A; a += P)
+            for (int l = 0; l < L; ++l) c[(size_t)l * A + a] = make_float2(0.f, 0.f);
+        if (p == 0)
+            for (int l = 0; l < L; ++l) ni[l] = 0.f;
+    }
+    if (p == 0) *flag = ok ? PE_SCFLAG_FP64 : (PE_SCFLAG_FP64 | PE_SCFLAG_SINGULAR);
+}
+
+// ---- FP64 pass over one tile: every flagged subcarrier of sc0 .. sc0 + n - 1 (n <= 64) --------------------
+// One flag load per lane; a wave whose tile holds no flagged subcarrier leaves at once. Flagged subcarriers
+// are taken 64 / P at a time, one per lane group.
+template <int L>
+__device__ __forceinline__ void tile_fp64(const float2 *__restrict__ H, const float *__restrict__ sigma2,
+                                          float2 *__restrict__ C, float *__restrict__ noise_inv,
+                                          uint8_t *__restrict__ flags, long sc0, int n, int A, int P,
+                                          bool unbiased, int lane) {
+    const bool pend = lane < n && (flags[sc0 + lane] & kPendingFp64);
+    unsigned long long m = __ballot(pend);
+    const int lgP = __builtin_ctz((unsigned)P);
+    const int p = lane & (P - 1), g = lane >> lgP, G = 64 >> lgP;
+    while (m) {  // m is wave-uniform
+        unsigned long long mm = m;
+        int idx = -1;
+        for (int k = 0; k <= g && mm; ++k) {  // the g-th lowest set bit goes to group g
+            if (k == g) idx = __builtin_ctzll(mm);
+            mm &= mm - 1;
+        }
+        for (int k = 0; k < G && m; ++k) m &= m - 1;
+        if (idx >= 0) {  // uniform per lane group
+            const long sc = sc0 + idx;
+            group_fp64<L>(H + (size_t)sc * A * L, sigma2[sc], C + (size_t)sc * L * A, noise_inv + (size_t)sc * L,
+                          flags + sc, A, p, P, unbiased);
+        }
+    }
+}
+
+#pragma clang fp contract(fast)  // the HIP default again for code after this header
 
 }  // namespace mmse
 }  // namespace pe

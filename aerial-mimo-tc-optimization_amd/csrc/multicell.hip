@@ -76,64 +76,46 @@ __global__ __launch_bounds__(64) void slot_apply_kernel(const unsigned char *__r
     }
 }
 
-// Stage 1, one subcarrier per 8-lane group: a tile is up to 8 subcarriers of one cell, one per group.
-template <int L, bool FP64>
-__device__ __forceinline__ void mmse_tile(const CellDev &cd, const Tile &tl, int p, int grp, int unbiased,
+// Stage 1: a tile is up to mmse_tile subcarriers of one cell, solved by one wave (FP32 tile solve; the FP64
+// pass revisits the tile's flagged subcarriers). Lanes per subcarrier are the cell shape's choice, the same
+// as in the single-cell kernels, so the coefficients are bit-identical to the cell solved alone.
+template <bool FP64>
+__device__ __forceinline__ void mmse_tile(const CellDev &cd, const Tile &tl, float *lds, int lane, int unbiased,
                                           float thr, Queue *q) {
     using namespace mmse;
-    constexpr int P = kMmseP;
-    if (grp >= tl.n) return;  // whole lane group
-    const int isc = tl.sc0 + grp;
-    const size_t sc = (size_t)isc;
-    float2 *c = cd.C + sc * L * cd.A;
-    float *ni = cd.noise_inv + sc * L;
-    const float2 *h = cd.H + sc * cd.A * L;
-    if constexpr (!FP64) {
-        const bool ok = mmse_one<float, L, P>(h, cd.sigma2[sc], cd.A, p, unbiased != 0, thr, true, c, ni);
-        if (p == 0) {
-            cd.flags[isc] = ok ? 0 : kPendingFp64;
-            if (!ok) atomicAdd(&q->flagged, 1u);
-        }
-    } else {
-        if (!(cd.flags[isc] & kPendingFp64)) return;  // uniform per lane group
-        const bool ok = mmse_one<double, L, P>(h, (double)cd.sigma2[sc], cd.A, p, unbiased != 0, 0.0, false, c, ni);
-        if (!ok) {
-            for (int a = p; a < cd.A; a += P)
-                for (int l = 0; l < L; ++l) c[(size_t)l * cd.A + a] = make_float2(0.f, 0.f);
-            for (int l = 0; l < L; ++l)
-                if (p == l % P) ni[l] = 0.f;
-        }
-        if (p == 0) cd.flags[isc] = ok ? PE_SCFLAG_FP64 : (PE_SCFLAG_FP64 | PE_SCFLAG_SINGULAR);
+#define PE_MMSE_L(LL)                                                                                            \
+    case LL:                                                                                                     \
+        if constexpr (FP64)                                                                                      \
+            tile_fp64<LL>(cd.H, cd.sigma2, cd.C, cd.noise_inv, cd.flags, tl.sc0, tl.n, cd.A, cd.mmse_p,          \
+                          unbiased != 0, lane);                                                                  \
+        else                                                                                                     \
+            tile_fp32<LL>(cd.H, cd.sigma2, cd.C, cd.noise_inv, cd.flags, tl.sc0, tl.n, cd.A, cd.mmse_p,          \
+                          unbiased != 0, thr, lds, lane, &q->flagged);                                           \
+        break;
+    switch (cd.L) {
+        PE_MMSE_L(1) PE_MMSE_L(2) PE_MMSE_L(3) PE_MMSE_L(4) PE_MMSE_L(5) PE_MMSE_L(6) PE_MMSE_L(7)
+    default:
+        PE_MMSE_L(8)
     }
+#undef PE_MMSE_L
 }
 
 // All layer counts in one persistent kernel per pass (the solve is unrolled per L; the tile's cell selects it).
+// One-wave workgroups; the FP32 pass uses the wave's dynamic LDS for its tile slots.
 template <bool FP64>
-__global__ __launch_bounds__(256) void slot_mmse_kernel(const unsigned char *__restrict__ block, int li,
-                                                        int off_cells, int off_tiles, Queue *q, unsigned nwaves,
-                                                        int unbiased, float thr) {
-    constexpr int P = mmse::kMmseP;
-    const int lane = threadIdx.x & 63;
-    const int p = lane & (P - 1);
-    const int grp = lane / P;
+__global__ __launch_bounds__(64) void slot_mmse_kernel(const unsigned char *__restrict__ block, int li,
+                                                       int off_cells, int off_tiles, Queue *q, unsigned nwaves,
+                                                       int unbiased, float thr) {
+    extern __shared__ __attribute__((aligned(16))) float lds[];
+    const int lane = threadIdx.x;
     const BlockView v = view(block, off_cells, off_tiles);
     const LaunchRange rg = v.ranges[li];
     // the FP64 pass has nothing to do unless the FP32 pass (same queue slot, li - 1) flagged a subcarrier
     if (FP64 && __atomic_load_n(&(q - 1)->flagged, __ATOMIC_RELAXED) == 0u) return;
-    const unsigned wave = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
-    for (unsigned t = wave; t < (unsigned)rg.ntiles; t = next_tile(q, lane, nwaves)) {
+    for (unsigned t = blockIdx.x; t < (unsigned)rg.ntiles; t = next_tile(q, lane, nwaves)) {
         const Tile tl = v.tiles[rg.tile0 + t];
         const CellDev cd = v.cells[tl.cell];
-        switch (cd.L) {
-        case 1: mmse_tile<1, FP64>(cd, tl, p, grp, unbiased, thr, q); break;
-        case 2: mmse_tile<2, FP64>(cd, tl, p, grp, unbiased, thr, q); break;
-        case 3: mmse_tile<3, FP64>(cd, tl, p, grp, unbiased, thr, q); break;
-        case 4: mmse_tile<4, FP64>(cd, tl, p, grp, unbiased, thr, q); break;
-        case 5: mmse_tile<5, FP64>(cd, tl, p, grp, unbiased, thr, q); break;
-        case 6: mmse_tile<6, FP64>(cd, tl, p, grp, unbiased, thr, q); break;
-        case 7: mmse_tile<7, FP64>(cd, tl, p, grp, unbiased, thr, q); break;
-        default: mmse_tile<8, FP64>(cd, tl, p, grp, unbiased, thr, q); break;
-        }
+        mmse_tile<FP64>(cd, tl, lds, lane, unbiased, thr, q);
     }
 }
 
@@ -182,14 +164,34 @@ pe_status SlotExec::init(const pe_cell_desc *pop, int npop, int max_cells, unsig
         if (std::find(apply_cls.begin(), apply_cls.end(), k) == apply_cls.end()) apply_cls.push_back(k);
     }
     std::sort(apply_cls.begin(), apply_cls.end(), std::greater<std::pair<int, int>>());
-    const int mmse_tiles_per_cell = (max_sc + kMmseTile - 1) / kMmseTile;
-    const int apply_tiles_per_cell = (max_sc + kApplyTile - 1) / kApplyTile;
+    const SlotTile tt = slot_tile();
+    apply_tile_ = tt.apply_tile > 0 ? tt.apply_tile : kApplyTileDefault;
+    if (tt.mmse_tile > 0) {
+        mmse_tile_ = tt.mmse_tile;
+    } else {  // the single-cell choice for a batch of the population's size at its largest shape
+        long total = 0;
+        int mL = 1, mA = 4;
+        for (int i = 0; i < npop; ++i) {
+            total += pop[i].nSC;
+            if (pop[i].nLayer * pop[i].nAnt > mL * mA) {
+                mL = pop[i].nLayer;
+                mA = pop[i].nAnt;
+            }
+        }
+        total = total * max_cells / std::max(1, npop);
+        mmse_tile_ = mmse_tile_for(mL, mA, (int)std::min<long>(total, 1L << 30));
+    }
+    max_layers_ = 1;
+    for (int i = 0; i < npop; ++i) max_layers_ = std::max(max_layers_, pop[i].nLayer);
+    mmse_lds_ = mmse::tile_lds_bytes(max_layers_, mmse_tile_);
+    const int mmse_tiles_per_cell = (max_sc + mmse_tile_ - 1) / mmse_tile_;
+    const int apply_tiles_per_cell = (max_sc + apply_tile_ - 1) / apply_tile_;
     max_tiles_ = max_cells * (mmse_tiles_per_cell + apply_tiles_per_cell);
     flags_per_cell_ = (size_t)max_sc;
     if (any_mmse) {
-        const int grid = std::max(1, std::min((max_cells * mmse_tiles_per_cell + 3) / 4, cus * 4));
-        launches_.push_back({0, 0, 0, grid, 256, (unsigned)grid * 4});
-        launches_.push_back({1, 0, 0, grid, 256, (unsigned)grid * 4});
+        const int grid = std::max(1, std::min(max_cells * mmse_tiles_per_cell, cus * 16));
+        launches_.push_back({0, 0, 0, grid, 64, (unsigned)grid});
+        launches_.push_back({1, 0, 0, grid, 64, (unsigned)grid});
     }
     for (const auto &k : apply_cls) {
         const int grid = std::max(1, std::min(max_cells * apply_tiles_per_cell, cus * 24));
@@ -228,7 +230,8 @@ pe_status SlotExec::stage(const pe_cell_desc *cells, int n, std::string &err) {
             err = "cell " + std::to_string(i) + " has more subcarriers than the executor was sized for";
             return PE_ERR_CAPACITY;
         }
-        if ((c.H && launch_index(0, 0, 0) < 0) || launch_index(2, ks_class(c.nAnt), c.prec) < 0) {
+        if ((c.H && launch_index(0, 0, 0) < 0) || launch_index(2, ks_class(c.nAnt), c.prec) < 0 ||
+            c.nLayer > max_layers_) {
             err = "cell " + std::to_string(i) + " has a shape class the executor was not built for";
             return PE_ERR_INVALID;
         }
@@ -246,6 +249,7 @@ pe_status SlotExec::stage(const pe_cell_desc *cells, int n, std::string &err) {
         d.A = c.nAnt;
         d.npb = c.nPamBits;
         d.prec = c.prec;
+        d.mmse_p = mmse_lanes_for(c.nLayer, c.nAnt);
     }
     int nt = 0;
     std::vector<int> order;
@@ -268,7 +272,7 @@ pe_status SlotExec::stage(const pe_cell_desc *cells, int n, std::string &err) {
         else
             std::stable_sort(order.begin(), order.end(),
                              [&](int a, int b) { return apply_cost(cells[a]) > apply_cost(cells[b]); });
-        const int tile = l.kind == 0 ? kMmseTile : kApplyTile;
+        const int tile = l.kind == 0 ? mmse_tile_ : apply_tile_;
         ranges[li].tile0 = nt;
         for (int i : order)
             for (int s = 0; s < cells[i].nSC; s += tile) {
@@ -302,7 +306,8 @@ hipError_t SlotExec::launch_one(int i, hipStream_t st) const {
     Queue *q = d_queues_ + i;
     dim3 g(l.grid), b(l.block);
     if (l.kind == 0) {
-        hipLaunchKernelGGL((slot_mmse_kernel<false>), g, b, 0, st, d_block_, i, oc, ot, q, l.nwaves, unb, thr);
+        hipLaunchKernelGGL((slot_mmse_kernel<false>), g, b, (size_t)mmse_lds_, st, d_block_, i, oc, ot, q, l.nwaves,
+                           unb, thr);
         return hipGetLastError();
     }
     if (l.kind == 1) {

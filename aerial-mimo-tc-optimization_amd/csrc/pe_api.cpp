@@ -46,18 +46,20 @@ bool valid_prec(int p) { return p >= PE_PREC_F32 && p <= PE_PREC_BF16_SPLIT; }
 // ---- tile table -----------------------------------------------------------------------------------------
 // Lines (first match wins, -1 = any):
 //   coef <L> <S> <A> <prec> <f32_kc> <mfma_wpb> <mfma_spw>
-//   mmse <L> <A> <nsc_max> <lanes_per_subcarrier>
+//   mmse <L> <A> <nsc_max> <lanes_per_subcarrier> [<subcarriers_per_tile>]
+//   slot <mmse_subcarriers_per_tile> <apply_subcarriers_per_tile>
 // Loaded from PE_TUNE_TABLE (pe_init), pe_tune_load(), or else once from tune_table.txt beside the library.
 struct TileEntry {
     int L, S, A, prec;
     pe::CoefTile tile;
 };
 struct MmseEntry {
-    int L, A, nsc_max, lanes;
+    int L, A, nsc_max, lanes, tile;  // tile 0: default for the size
 };
 std::mutex g_table_mu;
 std::vector<TileEntry> g_table;
 std::vector<MmseEntry> g_mmse;
+pe::SlotTile g_slot_tile;  // zero fields: executor defaults
 bool g_table_set = false;  // a table was loaded or cleared explicitly
 pe::CoefTile g_default_tile;
 
@@ -66,26 +68,32 @@ int load_table_file(const char *path) {
     if (!f) return -1;
     std::vector<TileEntry> t;
     std::vector<MmseEntry> m;
+    pe::SlotTile st;
     char line[256];
     while (fgets(line, sizeof(line), f)) {
         if (line[0] == '#' || line[0] == '\n') continue;
         TileEntry e;
         MmseEntry me;
-        int kc = 0, wpb = 0, spw = 0;
+        int kc = 0, wpb = 0, spw = 0, a1 = 0, a2 = 0;
         if (sscanf(line, "coef %d %d %d %d %d %d %d", &e.L, &e.S, &e.A, &e.prec, &kc, &wpb, &spw) == 7) {
             e.tile.f32_kc = (kc == 16) ? 16 : 8;
             e.tile.mfma_waves = (wpb == 2 || wpb == 4) ? wpb : 1;
             e.tile.mfma_spw = spw > 1 ? spw : 1;
             t.push_back(e);
-        } else if (sscanf(line, "mmse %d %d %d %d", &me.L, &me.A, &me.nsc_max, &me.lanes) == 4) {
-            me.lanes = me.lanes == 16 ? 16 : 8;
+        } else if (sscanf(line, "mmse %d %d %d %d %d", &me.L, &me.A, &me.nsc_max, &me.lanes, &a1) >= 4) {
+            me.lanes = (me.lanes == 4 || me.lanes == 16) ? me.lanes : 8;
+            me.tile = (a1 >= 1 && a1 <= 64) ? a1 : 0;
             m.push_back(me);
+        } else if (sscanf(line, "slot %d %d", &a1, &a2) == 2) {
+            st.mmse_tile = (a1 >= 1 && a1 <= 64) ? a1 : 0;
+            st.apply_tile = (a2 >= 1 && a2 <= 64) ? a2 : 0;
         }
     }
     fclose(f);
     std::lock_guard<std::mutex> lk(g_table_mu);
     g_table.swap(t);
     g_mmse.swap(m);
+    g_slot_tile = st;
     g_table_set = true;
     return (int)(g_table.size() + g_mmse.size());
 }
@@ -138,12 +146,36 @@ CoefTile coef_tile_for(int L, int S, int A, int prec) {
     return g_default_tile;
 }
 
-int mmse_lanes_for(int L, int A, int nSC) {
+int mmse_lanes_for(int L, int A) {
+    load_default_table();
+    {
+        std::lock_guard<std::mutex> lk(g_table_mu);
+        for (const MmseEntry &e : g_mmse)
+            if ((e.L < 0 || e.L == L) && (e.A < 0 || e.A == A)) return e.lanes;
+    }
+    return A >= 64 ? 16 : A >= 32 ? 8 : 4;
+}
+
+int mmse_tile_for(int L, int A, int nSC) {
+    load_default_table();
+    {
+        std::lock_guard<std::mutex> lk(g_table_mu);
+        for (const MmseEntry &e : g_mmse)
+            if ((e.L < 0 || e.L == L) && (e.A < 0 || e.A == A) && (e.nsc_max < 0 || nSC <= e.nsc_max) && e.tile > 0)
+                return e.tile;
+    }
+    // about four waves per SIMD, from one round of 64 / P subcarriers up to 64 subcarriers per wave
+    const int group = 64 / mmse_lanes_for(L, A);
+    const long target = (long)device_cus() * 16;
+    int t = group;
+    while (t < 64 && (long)nSC > target * t) t *= 2;
+    return t;
+}
+
+SlotTile slot_tile() {
     load_default_table();
     std::lock_guard<std::mutex> lk(g_table_mu);
-    for (const MmseEntry &e : g_mmse)
-        if ((e.L < 0 || e.L == L) && (e.A < 0 || e.A == A) && (e.nsc_max < 0 || nSC <= e.nsc_max)) return e.lanes;
-    return 8;
+    return g_slot_tile;
 }
 }  // namespace pe
 
@@ -185,6 +217,7 @@ int pe_tune_load(const char *path) {
         std::lock_guard<std::mutex> lk(g_table_mu);
         g_table.clear();
         g_mmse.clear();
+        g_slot_tile = pe::SlotTile();
         g_table_set = true;
         return 0;
     }

@@ -21,6 +21,15 @@ __device__ __forceinline__ float2 cma(float2 a, float2 b, float2 acc) {
     return make_float2(re, im);
 }
 
+// Orders this wave's LDS accesses across the point of call (each wave works in its own LDS region):
+// LDS instructions of a wave execute in order, so only compiler reordering has to be stopped -- unlike
+// __syncthreads(), this does not wait for the wave's outstanding global loads (prefetches stay in flight).
+__device__ __forceinline__ void lds_order_wave() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // ---- max-log PAM LLR (computePamLlr) ---------------------------------------------------------------
 // Distances of the PAM2/4/8/16 axes at offsets 0, 1, 3, 6, as the decimal literals of
 // aerial_llr_bench_v2.cu:47-54 rounded to float (they equal the float roundings of 1/sqrt(2),

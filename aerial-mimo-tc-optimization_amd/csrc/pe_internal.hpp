@@ -15,9 +15,18 @@ struct CoefTile {
 };
 CoefTile coef_tile_for(int nLayer, int nSym, int nAnt, int prec);
 
-// MMSE kernel: lanes per subcarrier (8 or 16) for a shape; more lanes = lower latency per subcarrier, fewer
-// subcarriers per wave.
-int mmse_lanes_for(int nLayer, int nAnt, int nSC);
+// MMSE: lanes per subcarrier (4, 8 or 16) wherever antennas are summed. It fixes the Gram summation order, so
+// it depends on the shape (nLayer, nAnt) only, never on the batch size: every path (single-cell and
+// multi-cell) solving a subcarrier of that shape produces the same bits.
+int mmse_lanes_for(int nLayer, int nAnt);
+// MMSE FP32 pass: subcarriers per one-wave tile (64/P .. 64) for a batch of nSC subcarriers; changes no result.
+int mmse_tile_for(int nLayer, int nAnt, int nSC);
+// Multi-cell executor tiles (zero: executor default).
+struct SlotTile {
+    int mmse_tile = 0;   // subcarriers per MMSE tile
+    int apply_tile = 0;  // subcarriers per apply tile
+};
+SlotTile slot_tile();
 
 // Compute units of the current device (cached per process).
 int device_cus();

@@ -22,6 +22,7 @@ struct CellDev {
     float *llr;
     uint8_t *flags;  // stage-1 per-subcarrier flags (executor-owned)
     int nSC, L, S, A, npb, prec;
+    int mmse_p;      // stage-1 lanes per subcarrier (the shape's choice, as in the single-cell path)
 };
 
 struct Tile {
@@ -40,8 +41,7 @@ struct LaunchRange {
 
 class SlotExec {
 public:
-    static constexpr int kApplyTile = 8;  // subcarriers per stage-2 tile (one wave)
-    static constexpr int kMmseTile = 8;   // subcarriers per stage-1 tile (one wave: one per 8-lane group)
+    static constexpr int kApplyTileDefault = 8;  // subcarriers per stage-2 tile (one wave)
 
     struct Launch {
         int kind;  // 0 mmse fp32, 1 mmse fp64, 2 apply
@@ -79,6 +79,9 @@ private:
 
     unsigned flags_ = 0;
     int max_cells_ = 0, max_tiles_ = 0;
+    int mmse_tile_ = 16, apply_tile_ = kApplyTileDefault;  // subcarriers per tile
+    int mmse_lds_ = 0;                                       // LDS bytes of a stage-1 wave
+    int max_layers_ = 1;                                     // largest layer count the LDS is sized for
     size_t flags_per_cell_ = 0;
     std::vector<Launch> launches_;
     // one contiguous block, same layout on host (pinned) and device: [LaunchRange x nL][CellDev x max_cells]
