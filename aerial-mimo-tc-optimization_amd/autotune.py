@@ -10,9 +10,12 @@ class becomes a line of tune_table.txt beside the library, which the library rea
 Candidates (see pe_tune_load in include/pusch_eq.h):
     MFMA apply: waves per workgroup {1, 2, 4} x subcarriers per wave {1, 2}
     FP32 apply: antennas per LDS chunk {8, 16}
-    MMSE:       lanes per subcarrier {4, 8, 16} (one per shape: it fixes the summation order) x subcarriers per
-                tile {64/lanes .. 64} per batch bucket (single cell, large batch)
-    slot:       subcarriers per MMSE tile {8, 16, 32, 64} x per apply tile {4, 8, 16} (24-cell slot plans)
+    MMSE:       lanes per subcarrier {4, 8, 16} (one per shape: it fixes the summation order) x waves per
+                workgroup {1, 2, 4, 8} x {one Gram round per wave, two, one with the antenna rows kept in
+                registers} (tile = rounds x waves x 64 / lanes <= 128 subcarriers) per batch bucket (single
+                cell, large batch)
+    slot:       MMSE (tile, waves, rows kept) in {(8,1,0) .. (32,8,1)} x subcarriers per apply tile {4, 8, 16}
+                (24-cell plans)
 Timing: HIP events around 20 launches after 5 warm-up launches, alternating two buffer sets (for the apply
 their sum exceeds the 256 MB Infinity Cache, so launches stream from HBM as in production); single-cell MMSE
 launches are timed one at a time (median of 15).
@@ -126,26 +129,30 @@ def tune_mmse(pe, dev, L, A, base):
         for nsc, bucket in buckets:
             sets = mmse_sets(dev, L, A, nsc)
             clock = timed_single if nsc <= 4096 else timed
-            T = 64 // P
-            while T <= 64:
-                with_table(pe, ["mmse %d %d -1 %d %d" % (L, A, P, T)] + base)
-                t = clock(lambda H, s2, C, ni, fl: pe.gram_mmse(H, s2, True, 0.0, C, ni, fl), sets)
-                res[(P, nsc, T)] = t
-                T *= 2
+            for W in (1, 2, 4, 8):
+                for R, K in ((1, 0), (2, 0), (1, 1)):  # Gram rounds per wave, rows kept in registers
+                    T = R * W * 64 // P
+                    if T > 128 or (K and A > 4 * P):
+                        continue
+                    with_table(pe, ["mmse %d %d -1 %d %d %d %d" % (L, A, P, T, W, K)] + base)
+                    t = clock(lambda H, s2, C, ni, fl: pe.gram_mmse(H, s2, True, 0.0, C, ni, fl), sets)
+                    res[(P, nsc, T, W, K)] = t
             del sets
             torch.cuda.empty_cache()
-    best_b = {nsc: min(t for (P, n, T), t in res.items() if n == nsc) for nsc, _ in buckets}
+    best_b = {nsc: min(t for k, t in res.items() if k[1] == nsc) for nsc, _ in buckets}
     score = {}
     for P in (4, 8, 16):
-        score[P] = sum(min(t for (p, n, T), t in res.items() if p == P and n == nsc) / best_b[nsc]
+        score[P] = sum(min(t for k, t in res.items() if k[0] == P and k[1] == nsc) / best_b[nsc]
                        for nsc, _ in buckets)
     P = min(score, key=score.get)
     lines, chosen = [], []
     for nsc, bucket in buckets:
-        T, t = min(((T, t) for (p, n, T), t in res.items() if p == P and n == nsc), key=lambda x: x[1])
-        lines.append("mmse %d %d %d %d %d" % (L, A, bucket, P, T))
-        chosen.append((nsc, T, t))
-    report = [{"P": p, "nsc": n, "T": T, "us": t * 1e6} for (p, n, T), t in sorted(res.items())]
+        (T, W, K), t = min((((k[2], k[3], k[4]), t) for k, t in res.items() if k[0] == P and k[1] == nsc),
+                           key=lambda x: x[1])
+        lines.append("mmse %d %d %d %d %d %d %d" % (L, A, bucket, P, T, W, K))
+        chosen.append((nsc, T, W, K, t))
+    report = [{"P": k[0], "nsc": k[1], "T": k[2], "W": k[3], "keep": k[4], "us": t * 1e6}
+              for k, t in sorted(res.items())]
     return lines, P, chosen, report
 
 
@@ -177,16 +184,18 @@ def tune_slot(pe, dev, base):
     import torch
     sets = [slot_cells(dev, 24, 77 + k) for k in range(2)]
     res = []
-    for mt in (8, 16, 32, 64):
+    for mt, mw, mk in ((8, 1, 0), (16, 1, 0), (16, 2, 0), (32, 2, 0), (32, 4, 0), (64, 4, 0), (64, 8, 0),
+                       (128, 8, 0), (8, 2, 1), (16, 4, 1), (32, 8, 1)):
         for at in (4, 8, 16):
-            with_table(pe, ["slot %d %d" % (mt, at)] + base)
+            with_table(pe, ["slot %d %d %d %d" % (mt, at, mw, mk)] + base)
             plans = [pe.SlotPlan(c) for c, _ in sets]
             t = timed(lambda pl: pl.graph_launch(), [(plans[0],), (plans[1],)], iters=30, warm=6)
             for pl in plans:
                 pl.close()
-            res.append({"mmse_tile": mt, "apply_tile": at, "us": t * 1e6})
+            res.append({"mmse_tile": mt, "mmse_waves": mw, "mmse_keep": mk, "apply_tile": at, "us": t * 1e6})
     best = min(res, key=lambda r: r["us"])
-    return "slot %d %d" % (best["mmse_tile"], best["apply_tile"]), best, res
+    return ("slot %d %d %d %d" % (best["mmse_tile"], best["apply_tile"], best["mmse_waves"], best["mmse_keep"]),
+            best, res)
 
 
 def main():
@@ -225,8 +234,8 @@ def main():
         mlines, P, chosen, rep = tune_mmse(pe, dev, L, A, lines)
         lines.extend(mlines)
         report["mmse"].append({"L": L, "A": A, "P": P, "candidates": rep})
-        print("mmse  L=%d A=%3d lanes=%2d  %s" % (L, A, P, "  ".join("nsc %d: T=%d %.1f us" % (n, T, t * 1e6)
-                                                                   for n, T, t in chosen)), flush=True)
+        print("mmse  L=%d A=%3d lanes=%2d  %s" % (L, A, P, "  ".join(
+            "nsc %d: T=%d W=%d keep=%d %.1f us" % (n, T, W, K, t * 1e6) for n, T, W, K, t in chosen)), flush=True)
     sline, best, rep = tune_slot(pe, dev, lines)
     lines.append(sline)
     report["slot"] = rep

@@ -9,17 +9,18 @@ namespace {
 
 using namespace mmse;
 
-// One wave per tile of T subcarriers (the last tile may be short); the wave's LDS holds T slots.
-template <int L>
-__global__ __launch_bounds__(64) void gram_mmse_f32_kernel(const float2 *__restrict__ H,
-                                                           const float *__restrict__ sigma2,
-                                                           float2 *__restrict__ C, float *__restrict__ noise_inv,
-                                                           uint8_t *__restrict__ flags, int nSC, int A, int P,
-                                                           int T, int unbiased, float thr) {
+// One workgroup of W waves per tile of T subcarriers (the last tile may be short); its LDS holds T slots.
+template <int L, int KR>
+__global__ __launch_bounds__(512) void gram_mmse_f32_kernel(const float2 *__restrict__ H,
+                                                            const float *__restrict__ sigma2,
+                                                            float2 *__restrict__ C, float *__restrict__ noise_inv,
+                                                            uint8_t *__restrict__ flags, int nSC, int A, int P,
+                                                            int T, int unbiased, float thr) {
     extern __shared__ __attribute__((aligned(16))) float lds[];
     const long sc0 = (long)blockIdx.x * T;
     const int n = (int)min((long)T, (long)nSC - sc0);
-    tile_fp32<L>(H, sigma2, C, noise_inv, flags, sc0, n, A, P, unbiased != 0, thr, lds, threadIdx.x, nullptr);
+    tile_fp32<L, KR>(H, sigma2, C, noise_inv, flags, sc0, n, A, P, unbiased != 0, thr, lds, threadIdx.x,
+                     blockDim.x >> 6, nullptr);
 }
 
 // One wave per 64 subcarriers; leaves after one flag load unless one of them was flagged.
@@ -38,10 +39,19 @@ template <int L>
 hipError_t launch_mmse_L(const float2 *H, const float *sigma2, float2 *C, float *ni, uint8_t *flags, int nSC,
                          int A, bool unbiased, float thr, hipStream_t st) {
     const int P = mmse_lanes_for(L, A);
-    const int T = mmse_tile_for(L, A, nSC);
+    const MmseTile mt = mmse_tile_for(L, A, nSC);
+    const int T = mt.tile;
     const int grid = (nSC + T - 1) / T;
-    hipLaunchKernelGGL((gram_mmse_f32_kernel<L>), dim3(grid), dim3(64), (size_t)tile_lds_bytes(L, T), st, H, sigma2,
-                       C, ni, flags, nSC, A, P, T, unbiased ? 1 : 0, thr);
+    // rows kept in registers when asked for, every wave has one Gram round and a lane has at most 4 rows
+    const bool keep = mt.keep && T <= mt.waves * (64 / P) && A <= 4 * P;
+    const dim3 g(grid), b(64 * mt.waves);
+    const size_t lds = (size_t)tile_lds_bytes(L, T);
+    if (keep)
+        hipLaunchKernelGGL((gram_mmse_f32_kernel<L, 4>), g, b, lds, st, H, sigma2, C, ni, flags, nSC, A, P, T,
+                           unbiased ? 1 : 0, thr);
+    else
+        hipLaunchKernelGGL((gram_mmse_f32_kernel<L, 0>), g, b, lds, st, H, sigma2, C, ni, flags, nSC, A, P, T,
+                           unbiased ? 1 : 0, thr);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL((gram_mmse_f64_kernel<L>), dim3((nSC + 63) / 64), dim3(64), 0, st, H, sigma2, C, ni, flags,

@@ -42,6 +42,7 @@ __host__ __device__ constexpr int slot_floats(int L) {
 }
 // LDS bytes of a tile of T subcarriers with L layers
 __host__ __device__ constexpr int tile_lds_bytes(int L, int T) { return 4 * T * slot_floats(L); }
+
 template <int L> struct Dim {
     static constexpr int NT = L * (L + 1) / 2;  // stored entries of a Hermitian L x L (lower triangle)
     static constexpr int SLOT = slot_floats(L);
@@ -102,6 +103,41 @@ __device__ __forceinline__ void load_row(const float2 *__restrict__ row, T (&vr)
             vr[k] = (T)q.x;
             vi[k] = (T)q.y;
         }
+    }
+}
+
+// Partial Gram of KR antenna rows per lane (A <= KR P), the rows kept in registers for the coefficients.
+template <int L, int KR>
+__device__ __forceinline__ void gram_group_keep(const float2 *__restrict__ h, int A, int p, int P,
+                                                float (&gr)[Dim<L>::NT], float (&gi)[Dim<L>::NT],
+                                                float (&hr)[KR][L], float (&hi)[KR][L]) {
+#pragma unroll
+    for (int k = 0; k < Dim<L>::NT; ++k) gr[k] = gi[k] = 0.f;
+#pragma unroll
+    for (int r = 0; r < KR; ++r) {
+        const int a = p + r * P;
+        if (a < A) {
+            load_row<float, L>(h + (size_t)a * L, hr[r], hi[r]);
+        } else {
+#pragma unroll
+            for (int k = 0; k < L; ++k) hr[r][k] = hi[r][k] = 0.f;
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < KR; ++r) {
+        if (p + r * P >= A) break;  // same rows, same order as gram_group: bit-identical sums
+#pragma unroll
+        for (int i = 0; i < L; ++i)
+#pragma unroll
+            for (int j = 0; j <= i; ++j) {
+                gr[tri(i, j)] = fma_(hr[r][i], hr[r][j], fma_(hi[r][i], hi[r][j], gr[tri(i, j)]));
+                gi[tri(i, j)] = fma_(hr[r][i], hi[r][j], fma_(-hi[r][i], hr[r][j], gi[tri(i, j)]));
+            }
+    }
+#pragma unroll
+    for (int k = 0; k < Dim<L>::NT; ++k) {
+        gr[k] = group_sum(gr[k], P);
+        gi[k] = group_sum(gi[k], P);
     }
 }
 
@@ -252,6 +288,26 @@ __device__ __forceinline__ float noise_inv_of(T x) {
 }
 
 // ---- coefficients: C[l][a] = s_l sum_j Ginv[l][j] conj(H[a][j]) for this lane's antennas --------------------
+// C[l][a] for all l from one antenna row of H
+template <typename T, int L>
+__device__ __forceinline__ void c_column(const T (&hr)[L], const T (&hi)[L], const T (&vr)[Dim<L>::NT],
+                                         const T (&vi)[Dim<L>::NT], const T (&scl)[L], int A, int a,
+                                         float2 *__restrict__ c_out, bool store) {
+#pragma unroll
+    for (int l = 0; l < L; ++l) {
+        T re = T(0), im = T(0);
+#pragma unroll
+        for (int j = 0; j < L; ++j) {
+            // Ginv[l][j]: stored lower; for j > l it is conj(V[j][l])
+            const T g_r = j <= l ? vr[tri(l, j)] : vr[tri(j, l)];
+            const T g_i = j <= l ? vi[tri(l, j)] : -vi[tri(j, l)];
+            re = fma_(g_r, hr[j], fma_(g_i, hi[j], re));   // (g_r + i g_i)(h_r - i h_i)
+            im = fma_(g_i, hr[j], fma_(-g_r, hi[j], im));
+        }
+        if (store) c_out[(size_t)l * A + a] = make_float2((float)(re * scl[l]), (float)(im * scl[l]));
+    }
+}
+
 template <typename T, int L>
 __device__ __forceinline__ void write_c(const float2 *__restrict__ h, int A, int p, int P,
                                         const T (&vr)[Dim<L>::NT], const T (&vi)[Dim<L>::NT], const T (&scl)[L],
@@ -260,19 +316,7 @@ __device__ __forceinline__ void write_c(const float2 *__restrict__ h, int A, int
     for (int a = p; a < A; a += P) {
         T hr[L], hi[L];
         load_row<T, L>(h + (size_t)a * L, hr, hi);
-#pragma unroll
-        for (int l = 0; l < L; ++l) {
-            T re = T(0), im = T(0);
-#pragma unroll
-            for (int j = 0; j < L; ++j) {
-                // Ginv[l][j]: stored lower; for j > l it is conj(V[j][l])
-                const T g_r = j <= l ? vr[tri(l, j)] : vr[tri(j, l)];
-                const T g_i = j <= l ? vi[tri(l, j)] : -vi[tri(j, l)];
-                re = fma_(g_r, hr[j], fma_(g_i, hi[j], re));   // (g_r + i g_i)(h_r - i h_i)
-                im = fma_(g_i, hr[j], fma_(-g_r, hi[j], im));
-            }
-            if (store) c_out[(size_t)l * A + a] = make_float2((float)(re * scl[l]), (float)(im * scl[l]));
-        }
+        c_column<T, L>(hr, hi, vr, vi, scl, A, a, c_out, store);
     }
 }
 
@@ -299,22 +343,55 @@ __device__ __forceinline__ void slot_load(const float *__restrict__ s, float (&v
     for (int k = K / 4 * 4; k < K; ++k) v[k] = s[k];
 }
 
-// ---- FP32 pass over one tile: subcarriers sc0 .. sc0 + n - 1 (n <= 64) of one cell -------------------------
-// lds: this wave's region of at least n * Dim<L>::SLOT floats. flags[sc] = 0 or kPendingFp64 (C and noise_inv
-// of a flagged subcarrier are left to the FP64 pass); `flagged` (optional) counts flagged subcarriers.
-template <int L>
+// Orders the workgroup's LDS accesses between the phases of a tile (one-wave workgroups need no barrier).
+__device__ __forceinline__ void tile_barrier(int W) {
+    if (W > 1)
+        __syncthreads();
+    else
+        lds_order_wave();
+}
+
+// ---- FP32 pass over one tile: subcarriers sc0 .. sc0 + n - 1 of one cell, by a workgroup of W waves -------
+// n <= 64 W; lds: the workgroup's region of at least n * Dim<L>::SLOT floats; tid = thread in the workgroup.
+// Gram and coefficient rounds are spread over the W waves; the algebra runs on threads 0 .. n - 1, i.e. on as
+// few waves as possible, while the others wait at the barrier and leave their issue slots to other
+// workgroups. flags[sc] = 0 or kPendingFp64 (C and noise_inv of a flagged subcarrier are left to the FP64
+// pass); `flagged` (optional) counts flagged subcarriers.
+// KR > 0: when every wave has a single Gram round (n <= 64 W / P) and A <= KR P, each lane keeps its KR antenna
+// rows in registers from the Gram to the coefficients instead of reading H a second time (the coefficient
+// stores then cover P consecutive antennas of a C row per lane group). It trades the second read of H for
+// registers (fewer waves per SIMD); which is faster depends on the shape.
+template <int L, int KR>
 __device__ __forceinline__ void tile_fp32(const float2 *__restrict__ H, const float *__restrict__ sigma2,
                                           float2 *__restrict__ C, float *__restrict__ noise_inv,
                                           uint8_t *__restrict__ flags, long sc0, int n, int A, int P,
-                                          bool unbiased, float thr, float *__restrict__ lds, int lane,
+                                          bool unbiased, float thr, float *__restrict__ lds, int tid, int W,
                                           unsigned *flagged) {
     constexpr int NT = Dim<L>::NT;
     constexpr int SLOT = Dim<L>::SLOT;
+    const int lane = tid & 63, wave = tid >> 6;
     const int lgP = __builtin_ctz((unsigned)P);
     const int p = lane & (P - 1), g = lane >> lgP, G = 64 >> lgP;
+    const bool keep = KR > 0 && A <= KR * P && n <= W * G;
+    constexpr int KR1 = KR > 0 ? KR : 1;
+    float kr[KR1][L], ki[KR1][L];
 
-    // Gram, 64 / P subcarriers per round (lanes past n repeat the last subcarrier and store nothing)
-    for (int t0 = 0; t0 < n; t0 += G) {
+    if (keep) {
+        const int t = min(wave * G + g, n - 1);
+        float gr[NT], gi[NT];
+        gram_group_keep<L, KR1>(H + (size_t)(sc0 + t) * A * L, A, p, P, gr, gi, kr, ki);
+        if (p == 0 && wave * G + g < n) {
+            float buf[2 * NT];
+#pragma unroll
+            for (int k = 0; k < NT; ++k) {
+                buf[k] = gr[k];
+                buf[NT + k] = gi[k];
+            }
+            slot_store<2 * NT>(lds + t * SLOT, buf);
+        }
+    }
+    // Gram, 64 / P subcarriers per wave round (lanes past n repeat the last subcarrier and store nothing)
+    for (int t0 = wave * G; !keep && t0 < n; t0 += W * G) {
         const int t = min(t0 + g, n - 1);
         float gr[NT], gi[NT];
         gram_group<float, L>(H + (size_t)(sc0 + t) * A * L, A, p, P, gr, gi);
@@ -328,13 +405,13 @@ __device__ __forceinline__ void tile_fp32(const float2 *__restrict__ H, const fl
             slot_store<2 * NT>(lds + t * SLOT, buf);
         }
     }
-    lds_order_wave();
+    tile_barrier(W);
 
-    // algebra, one lane per subcarrier
-    if (lane < n) {
-        const long sc = sc0 + lane;
+    // algebra, one thread per subcarrier
+    if (tid < n) {
+        const long sc = sc0 + tid;
         float buf[2 * NT];
-        slot_load<2 * NT>(lds + lane * SLOT, buf);
+        slot_load<2 * NT>(lds + tid * SLOT, buf);
         float gr[NT], gi[NT], vr[NT], vi[NT], scl[L], x[L];
 #pragma unroll
         for (int k = 0; k < NT; ++k) {
@@ -351,7 +428,7 @@ __device__ __forceinline__ void tile_fp32(const float2 *__restrict__ H, const fl
 #pragma unroll
         for (int l = 0; l < L; ++l) out[2 * NT + l] = scl[l];
         out[2 * NT + L] = ok ? 1.f : 0.f;
-        slot_store<2 * NT + L + 1>(lds + lane * SLOT, out);
+        slot_store<2 * NT + L + 1>(lds + tid * SLOT, out);
         if (ok) {
 #pragma unroll
             for (int l = 0; l < L; ++l) noise_inv[sc * L + l] = noise_inv_of(x[l]);
@@ -359,16 +436,35 @@ __device__ __forceinline__ void tile_fp32(const float2 *__restrict__ H, const fl
         flags[sc] = ok ? 0 : kPendingFp64;
         if (!ok && flagged) atomicAdd(flagged, 1u);
     }
-    lds_order_wave();
+    tile_barrier(W);
 
     // coefficients: Q = min(64, 2^ceil(log2 A)) lanes per subcarrier, lane q writing antennas q, q + Q, ..., so
     // a store instruction covers up to 512 contiguous bytes of a C row whatever P is (C[l][a] is computed by
     // one lane in a fixed order either way). Lanes past n repeat the last subcarrier and store nothing; two
     // rounds per iteration keep two subcarriers' loads in flight.
+    if (keep) {
+        const int t = min(wave * G + g, n - 1);
+        float in[2 * NT + L + 1];
+        slot_load<2 * NT + L + 1>(lds + t * SLOT, in);
+        const bool store = wave * G + g < n && in[2 * NT + L] != 0.f;  // flagged: the FP64 pass writes it
+        float vr[NT], vi[NT], scl[L];
+#pragma unroll
+        for (int k = 0; k < NT; ++k) {
+            vr[k] = in[k];
+            vi[k] = in[NT + k];
+        }
+#pragma unroll
+        for (int l = 0; l < L; ++l) scl[l] = in[2 * NT + l];
+#pragma unroll
+        for (int r = 0; r < KR1; ++r) {
+            const int a = p + r * P;
+            if (a < A) c_column<float, L>(kr[r], ki[r], vr, vi, scl, A, a, C + (size_t)(sc0 + t) * L * A, store);
+        }
+    }
     const int lgQ = A >= 64 ? 6 : A <= 1 ? 0 : 32 - __builtin_clz((unsigned)A - 1u);
     const int q = lane & ((1 << lgQ) - 1), g3 = lane >> lgQ, R3 = 64 >> lgQ;
 #pragma unroll 2
-    for (int t0 = 0; t0 < n; t0 += R3) {
+    for (int t0 = wave * R3; !keep && t0 < n; t0 += W * R3) {
         const int t = min(t0 + g3, n - 1);
         float in[2 * NT + L + 1];
         slot_load<2 * NT + L + 1>(lds + t * SLOT, in);
@@ -384,7 +480,7 @@ __device__ __forceinline__ void tile_fp32(const float2 *__restrict__ H, const fl
         write_c<float, L>(H + (size_t)(sc0 + t) * A * L, A, q, 1 << lgQ, vr, vi, scl,
                           C + (size_t)(sc0 + t) * L * A, store);
     }
-    lds_order_wave();  // slot reads done before the wave's next tile rewrites them
+    tile_barrier(W);  // slot reads done before the workgroup's next tile rewrites them
 }
 
 // ---- FP64 pass for one flagged subcarrier, by one group of P lanes -------------------------------------------

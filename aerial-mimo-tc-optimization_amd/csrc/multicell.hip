@@ -76,21 +76,24 @@ __global__ __launch_bounds__(64) void slot_apply_kernel(const unsigned char *__r
     }
 }
 
-// Stage 1: a tile is up to mmse_tile subcarriers of one cell, solved by one wave (FP32 tile solve; the FP64
-// pass revisits the tile's flagged subcarriers). Lanes per subcarrier are the cell shape's choice, the same
-// as in the single-cell kernels, so the coefficients are bit-identical to the cell solved alone.
-template <bool FP64>
-__device__ __forceinline__ void mmse_tile(const CellDev &cd, const Tile &tl, float *lds, int lane, int unbiased,
-                                          float thr, Queue *q) {
+// Stage 1: a tile is up to mmse_tile subcarriers of one cell. FP32 pass: one workgroup of W waves solves the
+// tile (tile solve). FP64 pass: one-wave workgroups revisit the tile's flagged subcarriers, 64 at a time.
+// Lanes per subcarrier are the cell shape's choice, the same as in the single-cell kernels, so the
+// coefficients are bit-identical to the cell solved alone.
+template <bool FP64, int KR>
+__device__ __forceinline__ void mmse_tile(const CellDev &cd, const Tile &tl, float *lds, int tid, int W,
+                                          int unbiased, float thr, Queue *q) {
     using namespace mmse;
 #define PE_MMSE_L(LL)                                                                                            \
     case LL:                                                                                                     \
-        if constexpr (FP64)                                                                                      \
-            tile_fp64<LL>(cd.H, cd.sigma2, cd.C, cd.noise_inv, cd.flags, tl.sc0, tl.n, cd.A, cd.mmse_p,          \
-                          unbiased != 0, lane);                                                                  \
-        else                                                                                                     \
-            tile_fp32<LL>(cd.H, cd.sigma2, cd.C, cd.noise_inv, cd.flags, tl.sc0, tl.n, cd.A, cd.mmse_p,          \
-                          unbiased != 0, thr, lds, lane, &q->flagged);                                           \
+        if constexpr (FP64) {                                                                                    \
+            for (int c = 0; c < tl.n; c += 64)                                                                   \
+                tile_fp64<LL>(cd.H, cd.sigma2, cd.C, cd.noise_inv, cd.flags, tl.sc0 + c, min(64, tl.n - c),      \
+                              cd.A, cd.mmse_p, unbiased != 0, tid);                                              \
+        } else {                                                                                                 \
+            tile_fp32<LL, KR>(cd.H, cd.sigma2, cd.C, cd.noise_inv, cd.flags, tl.sc0, tl.n, cd.A, cd.mmse_p,      \
+                              unbiased != 0, thr, lds, tid, W, &q->flagged);                                     \
+        }                                                                                                        \
         break;
     switch (cd.L) {
         PE_MMSE_L(1) PE_MMSE_L(2) PE_MMSE_L(3) PE_MMSE_L(4) PE_MMSE_L(5) PE_MMSE_L(6) PE_MMSE_L(7)
@@ -101,22 +104,44 @@ __device__ __forceinline__ void mmse_tile(const CellDev &cd, const Tile &tl, flo
 }
 
 // All layer counts in one persistent kernel per pass (the solve is unrolled per L; the tile's cell selects it).
-// One-wave workgroups; the FP32 pass uses the wave's dynamic LDS for its tile slots.
-template <bool FP64>
-__global__ __launch_bounds__(64) void slot_mmse_kernel(const unsigned char *__restrict__ block, int li,
-                                                       int off_cells, int off_tiles, Queue *q, unsigned nwaves,
-                                                       int unbiased, float thr) {
-    extern __shared__ __attribute__((aligned(16))) float lds[];
-    const int lane = threadIdx.x;
+// Workgroups pull whole tiles: the first from blockIdx.x, then from the queue (one atomic per workgroup,
+// broadcast through LDS). The FP32 pass uses the workgroup's dynamic LDS for its tile slots.
+template <bool FP64, int KR>
+__device__ __forceinline__ void slot_mmse(const unsigned char *__restrict__ block, int li, int off_cells,
+                                          int off_tiles, Queue *q, unsigned nblocks, int unbiased, float thr,
+                                          float *lds, unsigned *next) {
+    const int tid = threadIdx.x, W = blockDim.x >> 6;
     const BlockView v = view(block, off_cells, off_tiles);
     const LaunchRange rg = v.ranges[li];
     // the FP64 pass has nothing to do unless the FP32 pass (same queue slot, li - 1) flagged a subcarrier
     if (FP64 && __atomic_load_n(&(q - 1)->flagged, __ATOMIC_RELAXED) == 0u) return;
-    for (unsigned t = blockIdx.x; t < (unsigned)rg.ntiles; t = next_tile(q, lane, nwaves)) {
+    unsigned t = blockIdx.x;
+    while (t < (unsigned)rg.ntiles) {
         const Tile tl = v.tiles[rg.tile0 + t];
         const CellDev cd = v.cells[tl.cell];
-        mmse_tile<FP64>(cd, tl, lds, lane, unbiased, thr, q);
+        mmse_tile<FP64, KR>(cd, tl, lds, tid, W, unbiased, thr, q);
+        if (tid == 0) *next = atomicAdd(&q->next, 1u) + nblocks;
+        __syncthreads();
+        t = *next;
+        __syncthreads();  // every thread has read `next` before thread 0 rewrites it
     }
+}
+
+// KR = 4: antenna rows kept in registers where a tile allows it (see tile_fp32)
+template <int KR>
+__global__ __launch_bounds__(512) void slot_mmse_fp32_kernel(const unsigned char *__restrict__ block, int li,
+                                                             int off_cells, int off_tiles, Queue *q,
+                                                             unsigned nblocks, int unbiased, float thr) {
+    extern __shared__ __attribute__((aligned(16))) float lds[];
+    __shared__ unsigned next;
+    slot_mmse<false, KR>(block, li, off_cells, off_tiles, q, nblocks, unbiased, thr, lds, &next);
+}
+
+__global__ __launch_bounds__(64) void slot_mmse_fp64_kernel(const unsigned char *__restrict__ block, int li,
+                                                            int off_cells, int off_tiles, Queue *q,
+                                                            unsigned nblocks, int unbiased) {
+    __shared__ unsigned next;
+    slot_mmse<true, 0>(block, li, off_cells, off_tiles, q, nblocks, unbiased, 0.f, nullptr, &next);
 }
 
 size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
@@ -168,6 +193,8 @@ pe_status SlotExec::init(const pe_cell_desc *pop, int npop, int max_cells, unsig
     apply_tile_ = tt.apply_tile > 0 ? tt.apply_tile : kApplyTileDefault;
     if (tt.mmse_tile > 0) {
         mmse_tile_ = tt.mmse_tile;
+        mmse_waves_ = tt.mmse_waves;
+        mmse_keep_ = tt.mmse_keep;
     } else {  // the single-cell choice for a batch of the population's size at its largest shape
         long total = 0;
         int mL = 1, mA = 4;
@@ -179,7 +206,10 @@ pe_status SlotExec::init(const pe_cell_desc *pop, int npop, int max_cells, unsig
             }
         }
         total = total * max_cells / std::max(1, npop);
-        mmse_tile_ = mmse_tile_for(mL, mA, (int)std::min<long>(total, 1L << 30));
+        const MmseTile mt = mmse_tile_for(mL, mA, (int)std::min<long>(total, 1L << 30));
+        mmse_tile_ = mt.tile;
+        mmse_waves_ = mt.waves;
+        mmse_keep_ = mt.keep;
     }
     max_layers_ = 1;
     for (int i = 0; i < npop; ++i) max_layers_ = std::max(max_layers_, pop[i].nLayer);
@@ -189,9 +219,10 @@ pe_status SlotExec::init(const pe_cell_desc *pop, int npop, int max_cells, unsig
     max_tiles_ = max_cells * (mmse_tiles_per_cell + apply_tiles_per_cell);
     flags_per_cell_ = (size_t)max_sc;
     if (any_mmse) {
-        const int grid = std::max(1, std::min(max_cells * mmse_tiles_per_cell, cus * 16));
-        launches_.push_back({0, 0, 0, grid, 64, (unsigned)grid});
-        launches_.push_back({1, 0, 0, grid, 64, (unsigned)grid});
+        const int grid = std::max(1, std::min(max_cells * mmse_tiles_per_cell, cus * 16 / mmse_waves_));
+        launches_.push_back({0, 0, 0, grid, 64 * mmse_waves_, (unsigned)grid});
+        const int grid64 = std::max(1, std::min(max_cells * mmse_tiles_per_cell, cus * 8));
+        launches_.push_back({1, 0, 0, grid64, 64, (unsigned)grid64});
     }
     for (const auto &k : apply_cls) {
         const int grid = std::max(1, std::min(max_cells * apply_tiles_per_cell, cus * 24));
@@ -306,12 +337,15 @@ hipError_t SlotExec::launch_one(int i, hipStream_t st) const {
     Queue *q = d_queues_ + i;
     dim3 g(l.grid), b(l.block);
     if (l.kind == 0) {
-        hipLaunchKernelGGL((slot_mmse_kernel<false>), g, b, (size_t)mmse_lds_, st, d_block_, i, oc, ot, q, l.nwaves,
-                           unb, thr);
+        const size_t lds = (size_t)mmse_lds_;
+        if (mmse_keep_)
+            hipLaunchKernelGGL(slot_mmse_fp32_kernel<4>, g, b, lds, st, d_block_, i, oc, ot, q, l.nwaves, unb, thr);
+        else
+            hipLaunchKernelGGL(slot_mmse_fp32_kernel<0>, g, b, lds, st, d_block_, i, oc, ot, q, l.nwaves, unb, thr);
         return hipGetLastError();
     }
     if (l.kind == 1) {
-        hipLaunchKernelGGL((slot_mmse_kernel<true>), g, b, 0, st, d_block_, i, oc, ot, q, l.nwaves, unb, thr);
+        hipLaunchKernelGGL(slot_mmse_fp64_kernel, g, b, 0, st, d_block_, i, oc, ot, q, l.nwaves, unb);
         return hipGetLastError();
     }
 #define PE_APPLY_CASE(KK, PP)                                                                                    \

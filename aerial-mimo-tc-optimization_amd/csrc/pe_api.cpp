@@ -46,15 +46,16 @@ bool valid_prec(int p) { return p >= PE_PREC_F32 && p <= PE_PREC_BF16_SPLIT; }
 // ---- tile table -----------------------------------------------------------------------------------------
 // Lines (first match wins, -1 = any):
 //   coef <L> <S> <A> <prec> <f32_kc> <mfma_wpb> <mfma_spw>
-//   mmse <L> <A> <nsc_max> <lanes_per_subcarrier> [<subcarriers_per_tile>]
-//   slot <mmse_subcarriers_per_tile> <apply_subcarriers_per_tile>
+//   mmse <L> <A> <nsc_max> <lanes_per_subcarrier> [<subcarriers_per_tile> [<waves_per_workgroup> [<keep>]]]
+//        (tile <= 64 x waves and <= 128; keep 1: rows kept in registers where the tile allows it)
+//   slot <mmse_subcarriers_per_tile> <apply_subcarriers_per_tile> [<mmse_waves_per_workgroup> [<mmse_keep>]]
 // Loaded from PE_TUNE_TABLE (pe_init), pe_tune_load(), or else once from tune_table.txt beside the library.
 struct TileEntry {
     int L, S, A, prec;
     pe::CoefTile tile;
 };
 struct MmseEntry {
-    int L, A, nsc_max, lanes, tile;  // tile 0: default for the size
+    int L, A, nsc_max, lanes, tile, waves, keep;  // tile 0: default for the size
 };
 std::mutex g_table_mu;
 std::vector<TileEntry> g_table;
@@ -74,18 +75,23 @@ int load_table_file(const char *path) {
         if (line[0] == '#' || line[0] == '\n') continue;
         TileEntry e;
         MmseEntry me;
-        int kc = 0, wpb = 0, spw = 0, a1 = 0, a2 = 0;
+        int kc = 0, wpb = 0, spw = 0, a1 = 0, a2 = 0, a3 = 0, a4 = 0;
         if (sscanf(line, "coef %d %d %d %d %d %d %d", &e.L, &e.S, &e.A, &e.prec, &kc, &wpb, &spw) == 7) {
             e.tile.f32_kc = (kc == 16) ? 16 : 8;
             e.tile.mfma_waves = (wpb == 2 || wpb == 4) ? wpb : 1;
             e.tile.mfma_spw = spw > 1 ? spw : 1;
             t.push_back(e);
-        } else if (sscanf(line, "mmse %d %d %d %d %d", &me.L, &me.A, &me.nsc_max, &me.lanes, &a1) >= 4) {
+        } else if (sscanf(line, "mmse %d %d %d %d %d %d %d", &me.L, &me.A, &me.nsc_max, &me.lanes, &a1, &a3, &a4) >=
+                   4) {
+            me.keep = a4 == 1 ? 1 : 0;
             me.lanes = (me.lanes == 4 || me.lanes == 16) ? me.lanes : 8;
-            me.tile = (a1 >= 1 && a1 <= 64) ? a1 : 0;
+            me.waves = (a3 == 2 || a3 == 4 || a3 == 8) ? a3 : 1;
+            me.tile = (a1 >= 1 && a1 <= 64 * me.waves && a1 <= 128) ? a1 : 0;  // 128 x 84 floats: 43 KB of LDS
             m.push_back(me);
-        } else if (sscanf(line, "slot %d %d", &a1, &a2) == 2) {
-            st.mmse_tile = (a1 >= 1 && a1 <= 64) ? a1 : 0;
+        } else if (sscanf(line, "slot %d %d %d %d", &a1, &a2, &a3, &a4) >= 2) {
+            st.mmse_keep = a4 == 1 ? 1 : 0;
+            st.mmse_waves = (a3 == 2 || a3 == 4 || a3 == 8) ? a3 : 1;
+            st.mmse_tile = (a1 >= 1 && a1 <= 64 * st.mmse_waves && a1 <= 128) ? a1 : 0;
             st.apply_tile = (a2 >= 1 && a2 <= 64) ? a2 : 0;
         }
     }
@@ -156,19 +162,24 @@ int mmse_lanes_for(int L, int A) {
     return A >= 64 ? 16 : A >= 32 ? 8 : 4;
 }
 
-int mmse_tile_for(int L, int A, int nSC) {
+MmseTile mmse_tile_for(int L, int A, int nSC) {
     load_default_table();
     {
         std::lock_guard<std::mutex> lk(g_table_mu);
         for (const MmseEntry &e : g_mmse)
-            if ((e.L < 0 || e.L == L) && (e.A < 0 || e.A == A) && (e.nsc_max < 0 || nSC <= e.nsc_max) && e.tile > 0)
-                return e.tile;
+            if ((e.L < 0 || e.L == L) && (e.A < 0 || e.A == A) && (e.nsc_max < 0 || nSC <= e.nsc_max) &&
+                e.tile > 0) {
+                MmseTile t;
+                t.tile = e.tile;
+                t.waves = e.waves;
+                t.keep = e.keep;
+                return t;
+            }
     }
-    // about four waves per SIMD, from one round of 64 / P subcarriers up to 64 subcarriers per wave
-    const int group = 64 / mmse_lanes_for(L, A);
-    const long target = (long)device_cus() * 16;
-    int t = group;
-    while (t < 64 && (long)nSC > target * t) t *= 2;
+    // one Gram round per wave; for large batches four waves share the algebra of their subcarriers
+    MmseTile t;
+    t.waves = nSC > device_cus() * 64 ? 4 : 1;
+    t.tile = t.waves * 64 / mmse_lanes_for(L, A);
     return t;
 }
 

@@ -19,11 +19,19 @@ CoefTile coef_tile_for(int nLayer, int nSym, int nAnt, int prec);
 // it depends on the shape (nLayer, nAnt) only, never on the batch size: every path (single-cell and
 // multi-cell) solving a subcarrier of that shape produces the same bits.
 int mmse_lanes_for(int nLayer, int nAnt);
-// MMSE FP32 pass: subcarriers per one-wave tile (64/P .. 64) for a batch of nSC subcarriers; changes no result.
-int mmse_tile_for(int nLayer, int nAnt, int nSC);
+// MMSE FP32 pass launch shape for a batch of nSC subcarriers (changes no result): a workgroup of `waves`
+// waves solves a tile of `tile` subcarriers (tile <= 64 waves).
+struct MmseTile {
+    int tile = 4;
+    int waves = 1;
+    int keep = 0;  // 1: antenna rows stay in registers from the Gram to the coefficients
+};
+MmseTile mmse_tile_for(int nLayer, int nAnt, int nSC);
 // Multi-cell executor tiles (zero: executor default).
 struct SlotTile {
     int mmse_tile = 0;   // subcarriers per MMSE tile
+    int mmse_waves = 0;  // waves per MMSE workgroup
+    int mmse_keep = 0;   // MMSE keeps antenna rows in registers where the tile allows it
     int apply_tile = 0;  // subcarriers per apply tile
 };
 SlotTile slot_tile();

@@ -48,7 +48,7 @@ public:
         int cls;   // KS (apply)
         int prec;
         int grid, block;
-        unsigned nwaves;
+        unsigned nwaves;  // persistent workers: waves (apply) or workgroups (MMSE)
     };
 
     SlotExec() = default;
@@ -80,6 +80,8 @@ private:
     unsigned flags_ = 0;
     int max_cells_ = 0, max_tiles_ = 0;
     int mmse_tile_ = 16, apply_tile_ = kApplyTileDefault;  // subcarriers per tile
+    int mmse_waves_ = 1;                                     // waves per stage-1 workgroup
+    int mmse_keep_ = 0;                                      // stage 1 keeps antenna rows in registers
     int mmse_lds_ = 0;                                       // LDS bytes of a stage-1 wave
     int max_layers_ = 1;                                     // largest layer count the LDS is sized for
     size_t flags_per_cell_ = 0;

@@ -75,8 +75,13 @@ pe_status pe_init(int device);
 /* Replaces the tile table with the file at `path` (NULL: no table, built-in defaults). Returns the number of
    entries read, or -1 if the file cannot be read. Lines, first match wins, -1 = any:
      coef <nLayer> <nSym> <nAnt> <prec> <f32_kc: 8|16> <mfma_wpb: 1|2|4> <mfma_spw: 1..8>
-     mmse <nLayer> <nAnt> <nSC_max> <lanes_per_subcarrier: 8|16>
-   Without an explicit table the library reads tune_table.txt beside itself (written by autotune.py). */
+     mmse <nLayer> <nAnt> <nSC_max> <lanes_per_subcarrier: 4|8|16> [<subcarriers_per_tile> [<waves: 1|2|4|8>
+          [<keep_rows: 0|1>]]]
+     slot <mmse_subcarriers_per_tile> <apply_subcarriers_per_tile> [<mmse_waves: 1|2|4|8> [<keep_rows: 0|1>]]
+   The lanes of an mmse line set the summation order of the Gram matrix and are taken from the first line
+   matching (nLayer, nAnt) whatever nSC_max, so every entry point computes the same coefficients for a
+   subcarrier; tile sizes and waves change only speed. Without an explicit table the library reads
+   tune_table.txt beside itself (written by autotune.py). */
 int pe_tune_load(const char *path);
 
 /* ---- coefficient apply:  Y_eq[sc][l][s] = sum_a C[sc][l][a] * Y_rx[sc][s][a]  (no conjugation) ------ */
