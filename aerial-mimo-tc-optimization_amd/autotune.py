@@ -14,8 +14,8 @@ Candidates (see pe_tune_load in include/pusch_eq.h):
                 workgroup {1, 2, 4, 8} x {one Gram round per wave, two, one with the antenna rows kept in
                 registers} (tile = rounds x waves x 64 / lanes <= 128 subcarriers) per batch bucket (single
                 cell, large batch)
-    slot:       MMSE (tile, waves, rows kept) in {(8,1,0) .. (32,8,1)} x subcarriers per apply tile {4, 8, 16}
-                (24-cell plans)
+    slot:       MMSE (tile, waves, rows kept) in {(4,1,0) .. (32,8,1)} (24-cell plans; apply tile 8, used by the
+                serving loop's workgroups that stride over tiles)
 Timing: HIP events around 20 launches after 5 warm-up launches, alternating two buffer sets (for the apply
 their sum exceeds the 256 MB Infinity Cache, so launches stream from HBM as in production); single-cell MMSE
 launches are timed one at a time (median of 15).
@@ -184,9 +184,9 @@ def tune_slot(pe, dev, base):
     import torch
     sets = [slot_cells(dev, 24, 77 + k) for k in range(2)]
     res = []
-    for mt, mw, mk in ((8, 1, 0), (16, 1, 0), (16, 2, 0), (32, 2, 0), (32, 4, 0), (64, 4, 0), (64, 8, 0),
-                       (128, 8, 0), (8, 2, 1), (16, 4, 1), (32, 8, 1)):
-        for at in (4, 8, 16):
+    for mt, mw, mk in ((4, 1, 0), (8, 1, 0), (8, 2, 0), (16, 1, 0), (16, 2, 0), (16, 4, 0), (32, 2, 0), (32, 4, 0),
+                       (32, 8, 0), (64, 4, 0), (64, 8, 0), (128, 8, 0), (8, 2, 1), (16, 4, 1), (32, 8, 1)):
+        for at in (8,):  # slot plans run one subcarrier per wave whatever the tile; the serving loop uses it
             with_table(pe, ["slot %d %d %d %d" % (mt, at, mw, mk)] + base)
             plans = [pe.SlotPlan(c) for c, _ in sets]
             t = timed(lambda pl: pl.graph_launch(), [(plans[0],), (plans[1],)], iters=30, warm=6)

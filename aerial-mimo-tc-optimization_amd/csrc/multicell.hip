@@ -29,23 +29,12 @@ namespace pe {
 
 namespace {
 
-// Work distribution: wave w of a persistent grid of W waves takes tile w first (no atomic), then tiles
-// W + atomicAdd(next, 1) until none are left. Waves without a first tile exit at once, so a small batch on a
-// large grid costs no atomics. The counters are zeroed by queue_reset_kernel at the head of every slot (a
-// kernel node, so a captured graph replays with no memset node).
-__device__ __forceinline__ unsigned next_tile(Queue *q, int lane, unsigned nwaves) {
-    unsigned t = 0;
-    if (lane == 0) t = atomicAdd(&q->next, 1u);
-    return __shfl(t, 0, 64) + nwaves;
-}
-
-__global__ void queue_reset_kernel(Queue *q, int n) {
-    for (int i = threadIdx.x; i < n; i += blockDim.x) {
-        q[i].next = 0u;
-        q[i].flagged = 0u;
-    }
-}
-
+// Work distribution: workgroup g of a kernel takes tiles g, g + grid, g + 2 grid, ... of its class. For a slot
+// plan the grid is the class's exact tile count (one tile per workgroup); for the serving loop, whose batches
+// vary, it is bounded by the device's size. Tiles of one class cost about the same, so this static assignment
+// balances without atomics. Tile counts are read from device memory, so one captured graph serves every
+// batch. The FP32 pass counts flagged subcarriers; the first apply kernel, which follows both MMSE passes,
+// zeroes the counter for the next slot (inside a kernel, so a captured graph replays with no memset node).
 struct BlockView {
     const LaunchRange *ranges;
     const CellDev *cells;
@@ -57,32 +46,55 @@ __device__ __forceinline__ BlockView view(const unsigned char *block, int off_ce
                      reinterpret_cast<const Tile *>(block + off_tiles)};
 }
 
-template <int KS, int PREC>
-__global__ __launch_bounds__(64) void slot_apply_kernel(const unsigned char *__restrict__ block, int li,
-                                                        int off_cells, int off_tiles, Queue *q, unsigned nwaves) {
+// Fused apply + demap of one tile by a workgroup of 4 waves: wave w takes subcarriers sc0 + w, sc0 + w + 4, ...
+// of the tile (one subcarrier per wave at tile = 4), each in its own LDS operand region. LOOP = false (slot
+// plans, grid = tile count): one tile per workgroup, written without loops so the subcarrier code keeps the
+// registers of the single-cell kernel; LOOP = true (serving, bounded grid): workgroups stride over the tiles.
+constexpr int kApplyWaves = 4;
+template <int KS, int PREC, bool LOOP>
+__global__ __launch_bounds__(64 * kApplyWaves) void slot_apply_kernel(const unsigned char *__restrict__ block,
+                                                                      int li, int off_cells, int off_tiles,
+                                                                      Counters *reset) {
     using namespace mfma;
-    __shared__ __attribute__((aligned(16))) unsigned char lds[LdsLayout<KS>::bytes(PREC)];
-    const int lane = threadIdx.x;
+    __shared__ __attribute__((aligned(16))) unsigned char lds[kApplyWaves][LdsLayout<KS>::bytes(PREC)];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    // both MMSE passes are complete (stream order): zero their counter for the next slot
+    if (reset && blockIdx.x == 0 && threadIdx.x == 0) reset->flagged = 0u;
     const BlockView v = view(block, off_cells, off_tiles);
     const LaunchRange rg = v.ranges[li];
-    for (unsigned t = blockIdx.x; t < (unsigned)rg.ntiles; t = next_tile(q, lane, nwaves)) {
-        const Tile tl = v.tiles[rg.tile0 + t];
+    if constexpr (!LOOP) {
+        if ((int)blockIdx.x >= rg.ntiles) return;
+        const Tile tl = v.tiles[rg.tile0 + blockIdx.x];
+        const long sc = tl.sc0 + w;
+        if (sc >= (long)tl.sc0 + tl.n) return;  // uniform per wave; no workgroup barrier follows
         const CellDev cd = v.cells[tl.cell];
-        if (cd.A < 16 * KS) zero_operands<KS, PREC>(lds, lane);
-        const long first = tl.sc0, last = (long)tl.sc0 + tl.n;
-        apply_range<KS, PREC, -1, false>(cd.C, cd.Y, nullptr, cd.noise_inv, cd.llr, first, last, cd.L, cd.S, cd.A,
-                                         lds, lane, cd.npb);
-        lds_order_wave();  // LDS of this tile consumed before the next tile may zero it
+        if (cd.A < 16 * KS) zero_operands<KS, PREC>(lds[w], lane);
+        const int nC = cd.L * (cd.A >> 1), nY = cd.S * (cd.A >> 1);
+        const int rc = (nC + 63) >> 6, ry = (nY + 63) >> 6;
+        RawSC<KS> ra;
+        load_raw<KS>(ra, cd.C, cd.Y, (size_t)sc, cd.L, cd.S, cd.A, lane, nC, nY, rc, ry);
+        process_sc<KS, PREC, -1>(ra, (size_t)sc, lds[w], nullptr, cd.noise_inv, cd.llr, cd.L, cd.S, cd.A, lane, nC,
+                                 nY, rc, ry, cd.npb);
+    } else {
+        for (int t = blockIdx.x; t < rg.ntiles; t += gridDim.x) {
+            const Tile tl = v.tiles[rg.tile0 + t];
+            const CellDev cd = v.cells[tl.cell];
+            if (cd.A < 16 * KS) zero_operands<KS, PREC>(lds[w], lane);
+            for (long sc = tl.sc0 + w; sc < (long)tl.sc0 + tl.n; sc += kApplyWaves)
+                apply_one_sc<KS, PREC, -1>(cd.C, cd.Y, nullptr, cd.noise_inv, cd.llr, sc, cd.L, cd.S, cd.A, lds[w],
+                                           lane, cd.npb);
+            lds_order_wave();  // LDS of this tile consumed before the next tile may zero it
+        }
     }
 }
 
 // Stage 1: a tile is up to mmse_tile subcarriers of one cell. FP32 pass: one workgroup of W waves solves the
-// tile (tile solve). FP64 pass: one-wave workgroups revisit the tile's flagged subcarriers, 64 at a time.
+// tile (tile solve). FP64 pass: a one-wave workgroup revisits the tile's flagged subcarriers, 64 at a time.
 // Lanes per subcarrier are the cell shape's choice, the same as in the single-cell kernels, so the
 // coefficients are bit-identical to the cell solved alone.
 template <bool FP64, int KR>
 __device__ __forceinline__ void mmse_tile(const CellDev &cd, const Tile &tl, float *lds, int tid, int W,
-                                          int unbiased, float thr, Queue *q) {
+                                          int unbiased, float thr, Counters *q) {
     using namespace mmse;
 #define PE_MMSE_L(LL)                                                                                            \
     case LL:                                                                                                     \
@@ -103,45 +115,42 @@ __device__ __forceinline__ void mmse_tile(const CellDev &cd, const Tile &tl, flo
 #undef PE_MMSE_L
 }
 
-// All layer counts in one persistent kernel per pass (the solve is unrolled per L; the tile's cell selects it).
-// Workgroups pull whole tiles: the first from blockIdx.x, then from the queue (one atomic per workgroup,
-// broadcast through LDS). The FP32 pass uses the workgroup's dynamic LDS for its tile slots.
-template <bool FP64, int KR>
-__device__ __forceinline__ void slot_mmse(const unsigned char *__restrict__ block, int li, int off_cells,
-                                          int off_tiles, Queue *q, unsigned nblocks, int unbiased, float thr,
-                                          float *lds, unsigned *next) {
-    const int tid = threadIdx.x, W = blockDim.x >> 6;
+// All layer counts in one kernel (the tile's cell selects the unrolled solve). KR = 4: antenna rows kept in
+// registers where a tile allows it (see tile_fp32). LOOP as for the apply kernel: one tile per workgroup, or
+// workgroups striding over the tiles.
+template <int KR, bool LOOP>
+__global__ __launch_bounds__(512) void slot_mmse_fp32_kernel(const unsigned char *__restrict__ block, int li,
+                                                             int off_cells, int off_tiles, Counters *q,
+                                                             int unbiased, float thr) {
+    extern __shared__ __attribute__((aligned(16))) float lds[];
     const BlockView v = view(block, off_cells, off_tiles);
     const LaunchRange rg = v.ranges[li];
-    // the FP64 pass has nothing to do unless the FP32 pass (same queue slot, li - 1) flagged a subcarrier
-    if (FP64 && __atomic_load_n(&(q - 1)->flagged, __ATOMIC_RELAXED) == 0u) return;
-    unsigned t = blockIdx.x;
-    while (t < (unsigned)rg.ntiles) {
-        const Tile tl = v.tiles[rg.tile0 + t];
+    if constexpr (!LOOP) {
+        if ((int)blockIdx.x >= rg.ntiles) return;  // uniform per workgroup (tile_fp32 has barriers)
+        const Tile tl = v.tiles[rg.tile0 + blockIdx.x];
         const CellDev cd = v.cells[tl.cell];
-        mmse_tile<FP64, KR>(cd, tl, lds, tid, W, unbiased, thr, q);
-        if (tid == 0) *next = atomicAdd(&q->next, 1u) + nblocks;
-        __syncthreads();
-        t = *next;
-        __syncthreads();  // every thread has read `next` before thread 0 rewrites it
+        mmse_tile<false, KR>(cd, tl, lds, threadIdx.x, blockDim.x >> 6, unbiased, thr, q);
+    } else {
+        for (int t = blockIdx.x; t < rg.ntiles; t += gridDim.x) {  // uniform per workgroup
+            const Tile tl = v.tiles[rg.tile0 + t];
+            const CellDev cd = v.cells[tl.cell];
+            mmse_tile<false, KR>(cd, tl, lds, threadIdx.x, blockDim.x >> 6, unbiased, thr, q);
+        }
     }
 }
 
-// KR = 4: antenna rows kept in registers where a tile allows it (see tile_fp32)
-template <int KR>
-__global__ __launch_bounds__(512) void slot_mmse_fp32_kernel(const unsigned char *__restrict__ block, int li,
-                                                             int off_cells, int off_tiles, Queue *q,
-                                                             unsigned nblocks, int unbiased, float thr) {
-    extern __shared__ __attribute__((aligned(16))) float lds[];
-    __shared__ unsigned next;
-    slot_mmse<false, KR>(block, li, off_cells, off_tiles, q, nblocks, unbiased, thr, lds, &next);
-}
-
+// Same tiles, one wave each; leaves at once unless the FP32 pass (counter slot li - 1) flagged a subcarrier.
 __global__ __launch_bounds__(64) void slot_mmse_fp64_kernel(const unsigned char *__restrict__ block, int li,
-                                                            int off_cells, int off_tiles, Queue *q,
-                                                            unsigned nblocks, int unbiased) {
-    __shared__ unsigned next;
-    slot_mmse<true, 0>(block, li, off_cells, off_tiles, q, nblocks, unbiased, 0.f, nullptr, &next);
+                                                            int off_cells, int off_tiles, Counters *q,
+                                                            int unbiased) {
+    if (__atomic_load_n(&(q - 1)->flagged, __ATOMIC_RELAXED) == 0u) return;
+    const BlockView v = view(block, off_cells, off_tiles);
+    const LaunchRange rg = v.ranges[li];
+    for (int t = blockIdx.x; t < rg.ntiles; t += gridDim.x) {
+        const Tile tl = v.tiles[rg.tile0 + t];
+        const CellDev cd = v.cells[tl.cell];
+        mmse_tile<true, 0>(cd, tl, nullptr, threadIdx.x, 1, unbiased, 0.f, q);
+    }
 }
 
 size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
@@ -156,7 +165,11 @@ SlotExec::~SlotExec() {
     if (exec_) (void)hipGraphExecDestroy(exec_);
     if (graph_) (void)hipGraphDestroy(graph_);
     if (d_block_) (void)hipFree(d_block_);
-    if (d_queues_) (void)hipFree(d_queues_);
+    if (d_counters_) (void)hipFree(d_counters_);
+    for (hipEvent_t e : join_)
+        if (e) (void)hipEventDestroy(e);
+    for (hipStream_t t : side_)
+        if (t) (void)hipStreamDestroy(t);
     if (d_flags_) (void)hipFree(d_flags_);
     if (h_block_) (void)hipHostFree(h_block_);
 }
@@ -169,28 +182,27 @@ int SlotExec::max_grid() const {
 
 int SlotExec::launch_index(int kind, int cls, int prec) const {
     for (size_t i = 0; i < launches_.size(); ++i)
-        if (launches_[i].kind == kind && launches_[i].cls == cls && (kind != 2 || launches_[i].prec == prec))
-            return (int)i;
+        if (launches_[i].kind == kind && launches_[i].cls == cls && launches_[i].prec == prec) return (int)i;
     return -1;
 }
 
-pe_status SlotExec::init(const pe_cell_desc *pop, int npop, int max_cells, unsigned flags, std::string &err) {
+pe_status SlotExec::init(const pe_cell_desc *pop, int npop, int max_cells, unsigned flags, bool fixed,
+                         std::string &err) {
     flags_ = flags;
     max_cells_ = max_cells;
-    const int cus = device_cus();
+    fixed_ = fixed;
     int max_sc = 1;
-    bool any_mmse = false;
     std::vector<std::pair<int, int>> apply_cls;
     for (int i = 0; i < npop; ++i) {
         const pe_cell_desc &c = pop[i];
         max_sc = std::max(max_sc, c.nSC);
-        any_mmse = any_mmse || c.H != nullptr;
         const std::pair<int, int> k(ks_class(c.nAnt), c.prec);
         if (std::find(apply_cls.begin(), apply_cls.end(), k) == apply_cls.end()) apply_cls.push_back(k);
     }
     std::sort(apply_cls.begin(), apply_cls.end(), std::greater<std::pair<int, int>>());
     const SlotTile tt = slot_tile();
-    apply_tile_ = tt.apply_tile > 0 ? tt.apply_tile : kApplyTileDefault;
+    // a fixed batch runs one subcarrier per wave (the loop-free kernel); otherwise the table's tile
+    apply_tile_ = fixed ? kApplyWaves : tt.apply_tile > 0 ? tt.apply_tile : kApplyTileDefault;
     if (tt.mmse_tile > 0) {
         mmse_tile_ = tt.mmse_tile;
         mmse_waves_ = tt.mmse_waves;
@@ -214,28 +226,55 @@ pe_status SlotExec::init(const pe_cell_desc *pop, int npop, int max_cells, unsig
     max_layers_ = 1;
     for (int i = 0; i < npop; ++i) max_layers_ = std::max(max_layers_, pop[i].nLayer);
     mmse_lds_ = mmse::tile_lds_bytes(max_layers_, mmse_tile_);
-    const int mmse_tiles_per_cell = (max_sc + mmse_tile_ - 1) / mmse_tile_;
-    const int apply_tiles_per_cell = (max_sc + apply_tile_ - 1) / apply_tile_;
-    max_tiles_ = max_cells * (mmse_tiles_per_cell + apply_tiles_per_cell);
+    // tile capacity of a class: exact for a fixed batch, else max_cells jobs of its largest cell; the grid is
+    // the capacity for a fixed batch, else bounded by `waves_per_cu` waves per compute unit
+    const int cus = device_cus();
+    auto class_tiles = [&](auto in_class, int tile) {
+        long sum = 0, mx = 0;
+        for (int i = 0; i < npop; ++i)
+            if (in_class(pop[i])) {
+                const long t = (pop[i].nSC + tile - 1) / tile;
+                sum += t;
+                mx = std::max(mx, t);
+            }
+        return (int)std::max<long>(1, fixed ? sum : mx * max_cells);
+    };
+    auto grid_for = [&](int cap, int waves_per_wg, int waves_per_cu) {
+        return fixed ? cap : std::max(1, std::min(cap, cus * waves_per_cu / waves_per_wg));
+    };
+    // MMSE FP32 and FP64 passes over every cell with H (on the caller's stream), then one parallel branch per
+    // apply class (antenna class x precision) running its fused apply/demap
+    max_tiles_ = 0;
     flags_per_cell_ = (size_t)max_sc;
-    if (any_mmse) {
-        const int grid = std::max(1, std::min(max_cells * mmse_tiles_per_cell, cus * 16 / mmse_waves_));
-        launches_.push_back({0, 0, 0, grid, 64 * mmse_waves_, (unsigned)grid});
-        const int grid64 = std::max(1, std::min(max_cells * mmse_tiles_per_cell, cus * 8));
-        launches_.push_back({1, 0, 0, grid64, 64, (unsigned)grid64});
+    bool mm = false;
+    for (int i = 0; i < npop; ++i) mm = mm || pop[i].H != nullptr;
+    if (mm) {
+        const int cap = class_tiles([](const pe_cell_desc &c) { return c.H != nullptr; }, mmse_tile_);
+        launches_.push_back({0, 0, 0, grid_for(cap, mmse_waves_, 16), 64 * mmse_waves_, cap, 0});
+        launches_.push_back({1, 0, 0, std::min(cap, 64), 64, cap, 0});
+        max_tiles_ += cap;
     }
+    int branch = 0;
     for (const auto &k : apply_cls) {
-        const int grid = std::max(1, std::min(max_cells * apply_tiles_per_cell, cus * 24));
-        launches_.push_back({2, k.first, k.second, grid, 64, (unsigned)grid});
+        const int cap = class_tiles(
+            [&](const pe_cell_desc &c) { return ks_class(c.nAnt) == k.first && c.prec == k.second; }, apply_tile_);
+        launches_.push_back({2, k.first, k.second, grid_for(cap, kApplyWaves, 32), 64 * kApplyWaves, cap, branch});
+        max_tiles_ += cap;
+        ++branch;
     }
+    side_.assign(std::max(0, branch - 1), nullptr);
+    join_.assign(std::max(0, branch), nullptr);
     off_cells_ = align_up(sizeof(LaunchRange) * launches_.size(), 64);
     off_tiles_ = align_up(off_cells_ + sizeof(CellDev) * (size_t)max_cells, 64);
     block_bytes_ = off_tiles_ + sizeof(Tile) * (size_t)std::max(1, max_tiles_);
     hipError_t e = hipMalloc(&d_block_, block_bytes_);
     if (e == hipSuccess) e = hipHostMalloc(&h_block_, block_bytes_, hipHostMallocDefault);
-    if (e == hipSuccess) e = hipMalloc(&d_queues_, sizeof(Queue) * std::max<size_t>(1, launches_.size()));
+    if (e == hipSuccess) e = hipMalloc(&d_counters_, sizeof(Counters) * std::max<size_t>(1, launches_.size()));
     if (e == hipSuccess) e = hipMalloc(&d_flags_, flags_per_cell_ * (size_t)max_cells);
-    if (e == hipSuccess) e = hipMemset(d_queues_, 0, sizeof(Queue) * std::max<size_t>(1, launches_.size()));
+    if (e == hipSuccess) e = hipMemset(d_counters_, 0, sizeof(Counters) * std::max<size_t>(1, launches_.size()));
+    for (size_t k = 0; k < side_.size() && e == hipSuccess; ++k) e = hipStreamCreateWithFlags(&side_[k], hipStreamNonBlocking);
+    for (size_t k = 0; k < join_.size() && e == hipSuccess; ++k)
+        e = hipEventCreateWithFlags(&join_[k], hipEventDisableTiming);
     if (e == hipSuccess) e = hipMemset(d_block_, 0, block_bytes_);
     if (e == hipSuccess) e = hipMemset(d_flags_, 0, flags_per_cell_ * (size_t)max_cells);
     if (e == hipSuccess) e = hipDeviceSynchronize();
@@ -293,8 +332,7 @@ pe_status SlotExec::stage(const pe_cell_desc *cells, int n, std::string &err) {
         order.clear();
         for (int i = 0; i < n; ++i) {
             const pe_cell_desc &c = cells[i];
-            if (l.kind == 0 ? (c.H != nullptr) : (ks_class(c.nAnt) == l.cls && c.prec == l.prec))
-                order.push_back(i);
+            if (l.kind == 0 ? c.H != nullptr : (ks_class(c.nAnt) == l.cls && c.prec == l.prec)) order.push_back(i);
         }
         if (l.kind == 0)
             std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
@@ -307,7 +345,7 @@ pe_status SlotExec::stage(const pe_cell_desc *cells, int n, std::string &err) {
         ranges[li].tile0 = nt;
         for (int i : order)
             for (int s = 0; s < cells[i].nSC; s += tile) {
-                if (nt >= max_tiles_) {
+                if (nt >= max_tiles_ || nt - ranges[li].tile0 >= l.cap) {
                     err = "tile capacity exceeded";
                     return PE_ERR_CAPACITY;
                 }
@@ -334,23 +372,32 @@ hipError_t SlotExec::launch_one(int i, hipStream_t st) const {
     const int unb = (flags_ & PE_MMSE_UNBIASED) ? 1 : 0;
     const float thr = 1e4f;
     const int oc = (int)off_cells_, ot = (int)off_tiles_;
-    Queue *q = d_queues_ + i;
+    Counters *q = d_counters_ + i;
     dim3 g(l.grid), b(l.block);
     if (l.kind == 0) {
         const size_t lds = (size_t)mmse_lds_;
-        if (mmse_keep_)
-            hipLaunchKernelGGL(slot_mmse_fp32_kernel<4>, g, b, lds, st, d_block_, i, oc, ot, q, l.nwaves, unb, thr);
+        if (mmse_keep_ && fixed_)
+            hipLaunchKernelGGL((slot_mmse_fp32_kernel<4, false>), g, b, lds, st, d_block_, i, oc, ot, q, unb, thr);
+        else if (mmse_keep_)
+            hipLaunchKernelGGL((slot_mmse_fp32_kernel<4, true>), g, b, lds, st, d_block_, i, oc, ot, q, unb, thr);
+        else if (fixed_)
+            hipLaunchKernelGGL((slot_mmse_fp32_kernel<0, false>), g, b, lds, st, d_block_, i, oc, ot, q, unb, thr);
         else
-            hipLaunchKernelGGL(slot_mmse_fp32_kernel<0>, g, b, lds, st, d_block_, i, oc, ot, q, l.nwaves, unb, thr);
+            hipLaunchKernelGGL((slot_mmse_fp32_kernel<0, true>), g, b, lds, st, d_block_, i, oc, ot, q, unb, thr);
         return hipGetLastError();
     }
     if (l.kind == 1) {
-        hipLaunchKernelGGL(slot_mmse_fp64_kernel, g, b, 0, st, d_block_, i, oc, ot, q, l.nwaves, unb);
+        hipLaunchKernelGGL(slot_mmse_fp64_kernel, g, b, 0, st, d_block_, i, oc, ot, q, unb);
         return hipGetLastError();
     }
+    // the first apply kernel zeroes the MMSE FP32 pass's counter (launch 0) once both MMSE passes are done
+    Counters *rs = (l.branch == 0 && !launches_.empty() && launches_[0].kind == 0) ? d_counters_ : nullptr;
 #define PE_APPLY_CASE(KK, PP)                                                                                    \
     if (l.cls == KK && l.prec == PP) {                                                                           \
-        hipLaunchKernelGGL((slot_apply_kernel<KK, PP>), g, b, 0, st, d_block_, i, oc, ot, q, l.nwaves);          \
+        if (fixed_)                                                                                              \
+            hipLaunchKernelGGL((slot_apply_kernel<KK, PP, false>), g, b, 0, st, d_block_, i, oc, ot, rs);        \
+        else                                                                                                     \
+            hipLaunchKernelGGL((slot_apply_kernel<KK, PP, true>), g, b, 0, st, d_block_, i, oc, ot, rs);         \
         return hipGetLastError();                                                                                \
     }
     PE_APPLY_CASE(1, 1) PE_APPLY_CASE(1, 2) PE_APPLY_CASE(1, 3) PE_APPLY_CASE(1, 4)
@@ -361,18 +408,26 @@ hipError_t SlotExec::launch_one(int i, hipStream_t st) const {
     return hipErrorInvalidValue;
 }
 
+// MMSE passes on `st`, then one branch per apply class: the first on `st`, the others on side streams forked
+// from `st` and joined back into it (parallel branches of a captured graph: the classes fill each other's
+// tails).
 pe_status SlotExec::launch(hipStream_t st, std::string &err) {
-    hipLaunchKernelGGL(queue_reset_kernel, dim3(1), dim3(64), 0, st, d_queues_, (int)launches_.size());
-    if (hipGetLastError() != hipSuccess) {
-        err = "queue reset launch failed";
-        return PE_ERR_DEVICE;
+    hipError_t e = hipSuccess;
+    size_t i = 0;
+    for (; e == hipSuccess && i < launches_.size() && launches_[i].kind != 2; ++i) e = launch_one((int)i, st);
+    if (e == hipSuccess && !side_.empty()) {
+        e = hipEventRecord(join_[0], st);
+        for (size_t k = 0; e == hipSuccess && k < side_.size(); ++k) e = hipStreamWaitEvent(side_[k], join_[0], 0);
     }
-    for (size_t i = 0; i < launches_.size(); ++i) {
-        hipError_t e = launch_one((int)i, st);
-        if (e != hipSuccess) {
-            err = std::string("slot launch: ") + hipGetErrorString(e);
-            return PE_ERR_DEVICE;
-        }
+    for (; e == hipSuccess && i < launches_.size(); ++i)
+        e = launch_one((int)i, launches_[i].branch == 0 ? st : side_[launches_[i].branch - 1]);
+    for (size_t k = 0; e == hipSuccess && k < side_.size(); ++k) {
+        e = hipEventRecord(join_[k + 1], side_[k]);
+        if (e == hipSuccess) e = hipStreamWaitEvent(st, join_[k + 1], 0);
+    }
+    if (e != hipSuccess) {
+        err = std::string("slot launch: ") + hipGetErrorString(e);
+        return PE_ERR_DEVICE;
     }
     return PE_OK;
 }
@@ -469,7 +524,7 @@ pe_status pe_slot_plan_create(const pe_cell_desc *cells, int nCells, unsigned fl
     pe_slot_plan *p = new (std::nothrow) pe_slot_plan();
     if (!p) return plan_fail(PE_ERR_NOMEM, "host allocation");
     p->cells.assign(cells, cells + nCells);
-    pe_status s = p->exec.init(cells, nCells, nCells, flags, err);
+    pe_status s = p->exec.init(cells, nCells, nCells, flags, true, err);
     if (s == PE_OK) s = p->exec.stage(cells, nCells, err);
     if (s == PE_OK) s = p->exec.upload(nullptr, err);
     if (s == PE_OK && hipDeviceSynchronize() != hipSuccess) {

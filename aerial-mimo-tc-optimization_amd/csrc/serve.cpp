@@ -156,7 +156,7 @@ extern "C" pe_status pe_serve_run(const pe_serve_config *cfg, pe_serve_report *r
     };
     for (int k = 0; k < K; ++k) {
         ex[k].reset(new pe::SlotExec());
-        pe_status s = ex[k]->init(pop.data(), (int)pop.size(), cfg->max_batch_jobs, PE_MMSE_UNBIASED, err);
+        pe_status s = ex[k]->init(pop.data(), (int)pop.size(), cfg->max_batch_jobs, PE_MMSE_UNBIASED, false, err);
         if (s == PE_OK && hipStreamCreateWithFlags(&st[k], hipStreamNonBlocking) != hipSuccess) s = PE_ERR_DEVICE;
         if (s == PE_OK && hipEventCreateWithFlags(&ev[k], hipEventDisableTiming) != hipSuccess) s = PE_ERR_DEVICE;
         if (s != PE_OK) {

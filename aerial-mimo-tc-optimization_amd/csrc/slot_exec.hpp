@@ -29,9 +29,9 @@ struct Tile {
     int cell, sc0, n, pad;
 };
 
-struct Queue {
-    unsigned next;     // tiles handed out beyond the first one per wave
+struct Counters {
     unsigned flagged;  // subcarriers the MMSE FP32 pass handed to the FP64 pass
+    unsigned pad;
 };
 
 // Per-launch tile range, read by the kernels from device memory so that one captured graph serves any batch.
@@ -47,8 +47,9 @@ public:
         int kind;  // 0 mmse fp32, 1 mmse fp64, 2 apply
         int cls;   // KS (apply)
         int prec;
-        int grid, block;
-        unsigned nwaves;  // persistent workers: waves (apply) or workgroups (MMSE)
+        int grid, block;  // workgroups (each takes tiles g, g + grid, ...) and threads per workgroup
+        int cap;          // largest tile count of the class
+        int branch;       // parallel branch (apply: one per class; the MMSE passes precede the fork)
     };
 
     SlotExec() = default;
@@ -57,8 +58,9 @@ public:
     SlotExec &operator=(const SlotExec &) = delete;
 
     // Sizes device state for batches of up to max_cells cell-jobs drawn from `population` (shapes and
-    // precisions; pointers ignored). flags: PE_MMSE_UNBIASED.
-    pe_status init(const pe_cell_desc *population, int n_population, int max_cells, unsigned flags,
+    // precisions; pointers ignored). fixed: every batch is the population itself (slot plans), so launch grids
+    // are exact; else they cover max_cells jobs of any population shape. flags: PE_MMSE_UNBIASED.
+    pe_status init(const pe_cell_desc *population, int n_population, int max_cells, unsigned flags, bool fixed,
                    std::string &err);
     // Builds the tiles of a batch into the pinned staging area (cells must match population classes).
     pe_status stage(const pe_cell_desc *cells, int n, std::string &err);
@@ -78,6 +80,7 @@ private:
     hipError_t launch_one(int i, hipStream_t st) const;
 
     unsigned flags_ = 0;
+    bool fixed_ = false;  // every batch is the population (grids = tile counts, one tile per workgroup)
     int max_cells_ = 0, max_tiles_ = 0;
     int mmse_tile_ = 16, apply_tile_ = kApplyTileDefault;  // subcarriers per tile
     int mmse_waves_ = 1;                                     // waves per stage-1 workgroup
@@ -90,7 +93,9 @@ private:
     // [Tile x max_tiles]
     size_t off_cells_ = 0, off_tiles_ = 0, block_bytes_ = 0, used_bytes_ = 0;
     unsigned char *h_block_ = nullptr, *d_block_ = nullptr;
-    Queue *d_queues_ = nullptr;
+    Counters *d_counters_ = nullptr;
+    std::vector<hipStream_t> side_;   // apply classes past the first run on these (parallel graph branches)
+    std::vector<hipEvent_t> join_;    // join_[0]: fork point; join_[k]: end of side stream k
     uint8_t *d_flags_ = nullptr;
     int staged_tiles_ = 0;
     hipGraph_t graph_ = nullptr;
