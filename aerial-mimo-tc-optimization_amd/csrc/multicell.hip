@@ -47,16 +47,42 @@ __device__ __forceinline__ BlockView view(const unsigned char *block, int off_ce
 }
 
 // Fused apply + demap of one tile by a workgroup of 4 waves: wave w takes subcarriers sc0 + w, sc0 + w + 4, ...
-// of the tile (one subcarrier per wave at tile = 4), each in its own LDS operand region. LOOP = false (slot
-// plans, grid = tile count): one tile per workgroup, written without loops so the subcarrier code keeps the
-// registers of the single-cell kernel; LOOP = true (serving, bounded grid): workgroups stride over the tiles.
+// of the tile (one subcarrier per wave at tile = 4), each in its own LDS operand region. One kernel per
+// precision serves every antenna class up to KSMAX: the tile's cell selects the class (the same per-class
+// code as the single-cell kernel, so results are bit-identical). LOOP = false (slot plans, grid = tile count):
+// one tile per workgroup, written without loops so the subcarrier code keeps the single-cell kernel's
+// registers; LOOP = true (serving, bounded grid): workgroups stride over the tiles.
 constexpr int kApplyWaves = 4;
-template <int KS, int PREC, bool LOOP>
+
+__host__ __device__ __forceinline__ int ks_class_dev(int A) { return A <= 16 ? 1 : A <= 32 ? 2 : A <= 64 ? 4 : 8; }
+
+// One subcarrier of a cell through the noinline per-subcarrier function (used inside loops, where inlining
+// would make the compiler hoist per-lane address arithmetic and double the registers).
+template <int KS, int PREC>
+__device__ __forceinline__ void apply_one_sc_cell(const CellDev &cd, long sc, unsigned char *lds, int lane) {
+    if (cd.A < 16 * KS) mfma::zero_operands<KS, PREC>(lds, lane);
+    mfma::apply_one_sc<KS, PREC, -1>(cd.C, cd.Y, nullptr, cd.noise_inv, cd.llr, sc, cd.L, cd.S, cd.A, lds, lane,
+                                     cd.npb);
+}
+
+template <int KS, int PREC>
+__device__ __forceinline__ void apply_sc_class(const CellDev &cd, long sc, unsigned char *lds, int lane) {
+    using namespace mfma;
+    if (cd.A < 16 * KS) zero_operands<KS, PREC>(lds, lane);
+    const int nC = cd.L * (cd.A >> 1), nY = cd.S * (cd.A >> 1);
+    const int rc = (nC + 63) >> 6, ry = (nY + 63) >> 6;
+    RawSC<KS> ra;
+    load_raw<KS>(ra, cd.C, cd.Y, (size_t)sc, cd.L, cd.S, cd.A, lane, nC, nY, rc, ry);
+    process_sc<KS, PREC, -1>(ra, (size_t)sc, lds, nullptr, cd.noise_inv, cd.llr, cd.L, cd.S, cd.A, lane, nC, nY, rc,
+                             ry, cd.npb);
+}
+
+template <int KSMAX, int PREC, bool LOOP>
 __global__ __launch_bounds__(64 * kApplyWaves) void slot_apply_kernel(const unsigned char *__restrict__ block,
                                                                       int li, int off_cells, int off_tiles,
                                                                       Counters *reset) {
     using namespace mfma;
-    __shared__ __attribute__((aligned(16))) unsigned char lds[kApplyWaves][LdsLayout<KS>::bytes(PREC)];
+    __shared__ __attribute__((aligned(16))) unsigned char lds[kApplyWaves][LdsLayout<KSMAX>::bytes(PREC)];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     // both MMSE passes are complete (stream order): zero their counter for the next slot
     if (reset && blockIdx.x == 0 && threadIdx.x == 0) reset->flagged = 0u;
@@ -68,21 +94,22 @@ __global__ __launch_bounds__(64 * kApplyWaves) void slot_apply_kernel(const unsi
         const long sc = tl.sc0 + w;
         if (sc >= (long)tl.sc0 + tl.n) return;  // uniform per wave; no workgroup barrier follows
         const CellDev cd = v.cells[tl.cell];
-        if (cd.A < 16 * KS) zero_operands<KS, PREC>(lds[w], lane);
-        const int nC = cd.L * (cd.A >> 1), nY = cd.S * (cd.A >> 1);
-        const int rc = (nC + 63) >> 6, ry = (nY + 63) >> 6;
-        RawSC<KS> ra;
-        load_raw<KS>(ra, cd.C, cd.Y, (size_t)sc, cd.L, cd.S, cd.A, lane, nC, nY, rc, ry);
-        process_sc<KS, PREC, -1>(ra, (size_t)sc, lds[w], nullptr, cd.noise_inv, cd.llr, cd.L, cd.S, cd.A, lane, nC,
-                                 nY, rc, ry, cd.npb);
+        const int ks = ks_class_dev(cd.A);
+        if (ks == 1) apply_sc_class<1, PREC>(cd, sc, lds[w], lane);
+        else if (ks == 2) apply_sc_class<2, PREC>(cd, sc, lds[w], lane);
+        else if (KSMAX >= 4 && ks == 4) apply_sc_class<(KSMAX >= 4 ? 4 : 1), PREC>(cd, sc, lds[w], lane);
+        else if (KSMAX >= 8) apply_sc_class<(KSMAX >= 8 ? 8 : 1), PREC>(cd, sc, lds[w], lane);
     } else {
         for (int t = blockIdx.x; t < rg.ntiles; t += gridDim.x) {
             const Tile tl = v.tiles[rg.tile0 + t];
             const CellDev cd = v.cells[tl.cell];
-            if (cd.A < 16 * KS) zero_operands<KS, PREC>(lds[w], lane);
-            for (long sc = tl.sc0 + w; sc < (long)tl.sc0 + tl.n; sc += kApplyWaves)
-                apply_one_sc<KS, PREC, -1>(cd.C, cd.Y, nullptr, cd.noise_inv, cd.llr, sc, cd.L, cd.S, cd.A, lds[w],
-                                           lane, cd.npb);
+            const int ks = ks_class_dev(cd.A);
+            for (long sc = tl.sc0 + w; sc < (long)tl.sc0 + tl.n; sc += kApplyWaves) {
+                if (ks == 1) apply_one_sc_cell<1, PREC>(cd, sc, lds[w], lane);
+                else if (ks == 2) apply_one_sc_cell<2, PREC>(cd, sc, lds[w], lane);
+                else if (KSMAX >= 4 && ks == 4) apply_one_sc_cell<(KSMAX >= 4 ? 4 : 1), PREC>(cd, sc, lds[w], lane);
+                else if (KSMAX >= 8) apply_one_sc_cell<(KSMAX >= 8 ? 8 : 1), PREC>(cd, sc, lds[w], lane);
+            }
             lds_order_wave();  // LDS of this tile consumed before the next tile may zero it
         }
     }
@@ -166,10 +193,6 @@ SlotExec::~SlotExec() {
     if (graph_) (void)hipGraphDestroy(graph_);
     if (d_block_) (void)hipFree(d_block_);
     if (d_counters_) (void)hipFree(d_counters_);
-    for (hipEvent_t e : join_)
-        if (e) (void)hipEventDestroy(e);
-    for (hipStream_t t : side_)
-        if (t) (void)hipStreamDestroy(t);
     if (d_flags_) (void)hipFree(d_flags_);
     if (h_block_) (void)hipHostFree(h_block_);
 }
@@ -180,9 +203,9 @@ int SlotExec::max_grid() const {
     return g;
 }
 
-int SlotExec::launch_index(int kind, int cls, int prec) const {
+int SlotExec::launch_index(int kind, int prec) const {
     for (size_t i = 0; i < launches_.size(); ++i)
-        if (launches_[i].kind == kind && launches_[i].cls == cls && launches_[i].prec == prec) return (int)i;
+        if (launches_[i].kind == kind && (kind != 2 || launches_[i].prec == prec)) return (int)i;
     return -1;
 }
 
@@ -192,12 +215,18 @@ pe_status SlotExec::init(const pe_cell_desc *pop, int npop, int max_cells, unsig
     max_cells_ = max_cells;
     fixed_ = fixed;
     int max_sc = 1;
-    std::vector<std::pair<int, int>> apply_cls;
+    std::vector<std::pair<int, int>> apply_cls;  // (precision, largest antenna class)
     for (int i = 0; i < npop; ++i) {
         const pe_cell_desc &c = pop[i];
         max_sc = std::max(max_sc, c.nSC);
-        const std::pair<int, int> k(ks_class(c.nAnt), c.prec);
-        if (std::find(apply_cls.begin(), apply_cls.end(), k) == apply_cls.end()) apply_cls.push_back(k);
+        auto it = std::find_if(apply_cls.begin(), apply_cls.end(), [&](const std::pair<int, int> &k) {
+            return k.first == c.prec;
+        });
+        const int ks = ks_class(c.nAnt) <= 4 ? 4 : 8;
+        if (it == apply_cls.end())
+            apply_cls.emplace_back(c.prec, ks);
+        else
+            it->second = std::max(it->second, ks);
     }
     std::sort(apply_cls.begin(), apply_cls.end(), std::greater<std::pair<int, int>>());
     const SlotTile tt = slot_tile();
@@ -242,28 +271,22 @@ pe_status SlotExec::init(const pe_cell_desc *pop, int npop, int max_cells, unsig
     auto grid_for = [&](int cap, int waves_per_wg, int waves_per_cu) {
         return fixed ? cap : std::max(1, std::min(cap, cus * waves_per_cu / waves_per_wg));
     };
-    // MMSE FP32 and FP64 passes over every cell with H (on the caller's stream), then one parallel branch per
-    // apply class (antenna class x precision) running its fused apply/demap
+    // MMSE FP32 and FP64 passes over every cell with H, then one fused apply/demap launch per precision
     max_tiles_ = 0;
     flags_per_cell_ = (size_t)max_sc;
     bool mm = false;
     for (int i = 0; i < npop; ++i) mm = mm || pop[i].H != nullptr;
     if (mm) {
         const int cap = class_tiles([](const pe_cell_desc &c) { return c.H != nullptr; }, mmse_tile_);
-        launches_.push_back({0, 0, 0, grid_for(cap, mmse_waves_, 16), 64 * mmse_waves_, cap, 0});
-        launches_.push_back({1, 0, 0, std::min(cap, 64), 64, cap, 0});
+        launches_.push_back({0, 0, 0, grid_for(cap, mmse_waves_, 16), 64 * mmse_waves_, cap});
+        launches_.push_back({1, 0, 0, std::min(cap, 64), 64, cap});
         max_tiles_ += cap;
     }
-    int branch = 0;
-    for (const auto &k : apply_cls) {
-        const int cap = class_tiles(
-            [&](const pe_cell_desc &c) { return ks_class(c.nAnt) == k.first && c.prec == k.second; }, apply_tile_);
-        launches_.push_back({2, k.first, k.second, grid_for(cap, kApplyWaves, 32), 64 * kApplyWaves, cap, branch});
+    for (const auto &k : apply_cls) {  // one fused apply/demap launch per precision, every antenna class
+        const int cap = class_tiles([&](const pe_cell_desc &c) { return c.prec == k.first; }, apply_tile_);
+        launches_.push_back({2, k.second, k.first, grid_for(cap, kApplyWaves, 32), 64 * kApplyWaves, cap});
         max_tiles_ += cap;
-        ++branch;
     }
-    side_.assign(std::max(0, branch - 1), nullptr);
-    join_.assign(std::max(0, branch), nullptr);
     off_cells_ = align_up(sizeof(LaunchRange) * launches_.size(), 64);
     off_tiles_ = align_up(off_cells_ + sizeof(CellDev) * (size_t)max_cells, 64);
     block_bytes_ = off_tiles_ + sizeof(Tile) * (size_t)std::max(1, max_tiles_);
@@ -272,9 +295,6 @@ pe_status SlotExec::init(const pe_cell_desc *pop, int npop, int max_cells, unsig
     if (e == hipSuccess) e = hipMalloc(&d_counters_, sizeof(Counters) * std::max<size_t>(1, launches_.size()));
     if (e == hipSuccess) e = hipMalloc(&d_flags_, flags_per_cell_ * (size_t)max_cells);
     if (e == hipSuccess) e = hipMemset(d_counters_, 0, sizeof(Counters) * std::max<size_t>(1, launches_.size()));
-    for (size_t k = 0; k < side_.size() && e == hipSuccess; ++k) e = hipStreamCreateWithFlags(&side_[k], hipStreamNonBlocking);
-    for (size_t k = 0; k < join_.size() && e == hipSuccess; ++k)
-        e = hipEventCreateWithFlags(&join_[k], hipEventDisableTiming);
     if (e == hipSuccess) e = hipMemset(d_block_, 0, block_bytes_);
     if (e == hipSuccess) e = hipMemset(d_flags_, 0, flags_per_cell_ * (size_t)max_cells);
     if (e == hipSuccess) e = hipDeviceSynchronize();
@@ -300,7 +320,8 @@ pe_status SlotExec::stage(const pe_cell_desc *cells, int n, std::string &err) {
             err = "cell " + std::to_string(i) + " has more subcarriers than the executor was sized for";
             return PE_ERR_CAPACITY;
         }
-        if ((c.H && launch_index(0, 0, 0) < 0) || launch_index(2, ks_class(c.nAnt), c.prec) < 0 ||
+        if ((c.H && launch_index(0, 0) < 0) || launch_index(2, c.prec) < 0 ||
+            (ks_class(c.nAnt) == 8 && launches_[launch_index(2, c.prec)].cls < 8) ||
             c.nLayer > max_layers_) {
             err = "cell " + std::to_string(i) + " has a shape class the executor was not built for";
             return PE_ERR_INVALID;
@@ -323,16 +344,18 @@ pe_status SlotExec::stage(const pe_cell_desc *cells, int n, std::string &err) {
     }
     int nt = 0;
     std::vector<int> order;
+    staged_ntiles_.assign(launches_.size(), 0);
     for (size_t li = 0; li < launches_.size(); ++li) {
         const Launch &l = launches_[li];
         if (l.kind == 1) {  // FP64 pass: same tiles as the FP32 pass just before it
             ranges[li] = ranges[li - 1];
+            staged_ntiles_[li] = ranges[li].ntiles;
             continue;
         }
         order.clear();
         for (int i = 0; i < n; ++i) {
             const pe_cell_desc &c = cells[i];
-            if (l.kind == 0 ? c.H != nullptr : (ks_class(c.nAnt) == l.cls && c.prec == l.prec)) order.push_back(i);
+            if (l.kind == 0 ? c.H != nullptr : c.prec == l.prec) order.push_back(i);
         }
         if (l.kind == 0)
             std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
@@ -352,6 +375,7 @@ pe_status SlotExec::stage(const pe_cell_desc *cells, int n, std::string &err) {
                 ht[nt++] = Tile{i, s, std::min(tile, cells[i].nSC - s), 0};
             }
         ranges[li].ntiles = nt - ranges[li].tile0;
+        staged_ntiles_[li] = ranges[li].ntiles;
     }
     staged_tiles_ = nt;
     used_bytes_ = off_tiles_ + sizeof(Tile) * (size_t)nt;
@@ -367,13 +391,13 @@ pe_status SlotExec::upload(hipStream_t st, std::string &err) {
     return PE_OK;
 }
 
-hipError_t SlotExec::launch_one(int i, hipStream_t st) const {
+hipError_t SlotExec::launch_one(int i, hipStream_t st, bool exact) const {
     const Launch &l = launches_[i];
     const int unb = (flags_ & PE_MMSE_UNBIASED) ? 1 : 0;
     const float thr = 1e4f;
     const int oc = (int)off_cells_, ot = (int)off_tiles_;
     Counters *q = d_counters_ + i;
-    dim3 g(l.grid), b(l.block);
+    dim3 g(exact ? std::max(1, std::min(l.grid, staged_ntiles_[i])) : l.grid), b(l.block);
     if (l.kind == 0) {
         const size_t lds = (size_t)mmse_lds_;
         if (mmse_keep_ && fixed_)
@@ -391,7 +415,7 @@ hipError_t SlotExec::launch_one(int i, hipStream_t st) const {
         return hipGetLastError();
     }
     // the first apply kernel zeroes the MMSE FP32 pass's counter (launch 0) once both MMSE passes are done
-    Counters *rs = (l.branch == 0 && !launches_.empty() && launches_[0].kind == 0) ? d_counters_ : nullptr;
+    Counters *rs = (i > 0 && launches_[i - 1].kind != 2 && launches_[0].kind == 0) ? d_counters_ : nullptr;
 #define PE_APPLY_CASE(KK, PP)                                                                                    \
     if (l.cls == KK && l.prec == PP) {                                                                           \
         if (fixed_)                                                                                              \
@@ -400,31 +424,16 @@ hipError_t SlotExec::launch_one(int i, hipStream_t st) const {
             hipLaunchKernelGGL((slot_apply_kernel<KK, PP, true>), g, b, 0, st, d_block_, i, oc, ot, rs);         \
         return hipGetLastError();                                                                                \
     }
-    PE_APPLY_CASE(1, 1) PE_APPLY_CASE(1, 2) PE_APPLY_CASE(1, 3) PE_APPLY_CASE(1, 4)
-    PE_APPLY_CASE(2, 1) PE_APPLY_CASE(2, 2) PE_APPLY_CASE(2, 3) PE_APPLY_CASE(2, 4)
     PE_APPLY_CASE(4, 1) PE_APPLY_CASE(4, 2) PE_APPLY_CASE(4, 3) PE_APPLY_CASE(4, 4)
     PE_APPLY_CASE(8, 1) PE_APPLY_CASE(8, 2) PE_APPLY_CASE(8, 3) PE_APPLY_CASE(8, 4)
 #undef PE_APPLY_CASE
     return hipErrorInvalidValue;
 }
 
-// MMSE passes on `st`, then one branch per apply class: the first on `st`, the others on side streams forked
-// from `st` and joined back into it (parallel branches of a captured graph: the classes fill each other's
-// tails).
-pe_status SlotExec::launch(hipStream_t st, std::string &err) {
+// The whole sequence on `st`: MMSE FP32 pass, FP64 pass, one fused apply/demap launch per precision.
+pe_status SlotExec::launch(hipStream_t st, std::string &err, bool exact) {
     hipError_t e = hipSuccess;
-    size_t i = 0;
-    for (; e == hipSuccess && i < launches_.size() && launches_[i].kind != 2; ++i) e = launch_one((int)i, st);
-    if (e == hipSuccess && !side_.empty()) {
-        e = hipEventRecord(join_[0], st);
-        for (size_t k = 0; e == hipSuccess && k < side_.size(); ++k) e = hipStreamWaitEvent(side_[k], join_[0], 0);
-    }
-    for (; e == hipSuccess && i < launches_.size(); ++i)
-        e = launch_one((int)i, launches_[i].branch == 0 ? st : side_[launches_[i].branch - 1]);
-    for (size_t k = 0; e == hipSuccess && k < side_.size(); ++k) {
-        e = hipEventRecord(join_[k + 1], side_[k]);
-        if (e == hipSuccess) e = hipStreamWaitEvent(st, join_[k + 1], 0);
-    }
+    for (size_t i = 0; e == hipSuccess && i < launches_.size(); ++i) e = launch_one((int)i, st, exact);
     if (e != hipSuccess) {
         err = std::string("slot launch: ") + hipGetErrorString(e);
         return PE_ERR_DEVICE;

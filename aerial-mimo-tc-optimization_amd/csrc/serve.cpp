@@ -177,10 +177,13 @@ extern "C" pe_status pe_serve_run(const pe_serve_config *cfg, pe_serve_report *r
         d.llr = llr_in_arena ? reinterpret_cast<float *>(p) : cfg->job_llr[j];
         return d;
     };
-    auto submit = [&](int k, const std::vector<pe_cell_desc> &cells) -> pe_status {
+    // graph replay (capacity grids) for batches of at least a quarter of max_batch_jobs; smaller batches are
+    // launched eagerly with grids sized to their tiles, so a lone job does not dispatch a full batch's grid
+    auto submit = [&](int k, const std::vector<pe_cell_desc> &cells, bool force_graph) -> pe_status {
         pe_status s = ex[k]->stage(cells.data(), (int)cells.size(), err);
         if (s == PE_OK) s = ex[k]->upload(st[k], err);
-        if (s == PE_OK) s = cfg->use_graph ? ex[k]->graph_launch(st[k], err) : ex[k]->launch(st[k], err);
+        const bool graph = cfg->use_graph && (force_graph || 4 * (int)cells.size() >= cfg->max_batch_jobs);
+        if (s == PE_OK) s = graph ? ex[k]->graph_launch(st[k], err) : ex[k]->launch(st[k], err, true);
         if (s == PE_OK && hipEventRecord(ev[k], st[k]) != hipSuccess) {
             s = PE_ERR_DEVICE;
             err = "event record failed";
@@ -188,7 +191,8 @@ extern "C" pe_status pe_serve_run(const pe_serve_config *cfg, pe_serve_report *r
         return s;
     };
 
-    // warm-up: one batch of the first job's cell on every stream (captures the graphs; not measured)
+    // warm-up: one batch of the first job's cell on every stream, as a graph (captured here) and eagerly; not
+    // measured
     {
         JobBuf b;
         if (!arena.alloc(job_bytes(pop[cellof[0]], llr_in_arena), b.off)) {
@@ -199,7 +203,9 @@ extern "C" pe_status pe_serve_run(const pe_serve_config *cfg, pe_serve_report *r
         pe_cell_desc d = make_desc(0, b);  // LLRs go to job 0's destination, rewritten by the measured run
         std::vector<pe_cell_desc> one(1, d);
         for (int k = 0; k < K; ++k) {
-            pe_status s = submit(k, one);
+            pe_status s = submit(k, one, true);
+            if (s == PE_OK && hipStreamSynchronize(st[k]) != hipSuccess) s = PE_ERR_DEVICE;
+            if (s == PE_OK) s = submit(k, one, false);
             if (s == PE_OK && hipStreamSynchronize(st[k]) != hipSuccess) s = PE_ERR_DEVICE;
             if (s != PE_OK) {
                 cleanup();
@@ -268,7 +274,7 @@ extern "C" pe_status pe_serve_run(const pe_serve_config *cfg, pe_serve_report *r
                 batch_jobs[k].push_back(j);
             }
             if (cells.empty()) break;
-            status = submit(k, cells);
+            status = submit(k, cells, false);
             if (status != PE_OK) break;
             busy[k] = true;
             ++batches;

@@ -45,11 +45,10 @@ public:
 
     struct Launch {
         int kind;  // 0 mmse fp32, 1 mmse fp64, 2 apply
-        int cls;   // KS (apply)
+        int cls;   // apply: largest antenna class KS of the launch (4 or 8)
         int prec;
         int grid, block;  // workgroups (each takes tiles g, g + grid, ...) and threads per workgroup
-        int cap;          // largest tile count of the class
-        int branch;       // parallel branch (apply: one per class; the MMSE passes precede the fork)
+        int cap;          // largest tile count of the launch
     };
 
     SlotExec() = default;
@@ -66,8 +65,9 @@ public:
     pe_status stage(const pe_cell_desc *cells, int n, std::string &err);
     // Copies the staged batch to the device on `st` (one asynchronous copy).
     pe_status upload(hipStream_t st, std::string &err);
-    // Launches the kernel sequence on `st` (kernels with no tiles exit at once).
-    pe_status launch(hipStream_t st, std::string &err);
+    // Launches the kernel sequence on `st` (kernels with no tiles exit at once). exact: grids sized to the
+    // staged batch (eager launches of a small batch) instead of the capacity grids a captured graph uses.
+    pe_status launch(hipStream_t st, std::string &err, bool exact = false);
     // Captures launch() into a graph on first use, then replays it.
     pe_status graph_launch(hipStream_t st, std::string &err);
 
@@ -76,8 +76,8 @@ public:
     const std::vector<Launch> &launches() const { return launches_; }
 
 private:
-    int launch_index(int kind, int cls, int prec) const;
-    hipError_t launch_one(int i, hipStream_t st) const;
+    int launch_index(int kind, int prec) const;
+    hipError_t launch_one(int i, hipStream_t st, bool exact) const;
 
     unsigned flags_ = 0;
     bool fixed_ = false;  // every batch is the population (grids = tile counts, one tile per workgroup)
@@ -94,10 +94,9 @@ private:
     size_t off_cells_ = 0, off_tiles_ = 0, block_bytes_ = 0, used_bytes_ = 0;
     unsigned char *h_block_ = nullptr, *d_block_ = nullptr;
     Counters *d_counters_ = nullptr;
-    std::vector<hipStream_t> side_;   // apply classes past the first run on these (parallel graph branches)
-    std::vector<hipEvent_t> join_;    // join_[0]: fork point; join_[k]: end of side stream k
     uint8_t *d_flags_ = nullptr;
     int staged_tiles_ = 0;
+    std::vector<int> staged_ntiles_;  // per launch, of the staged batch
     hipGraph_t graph_ = nullptr;
     hipGraphExec_t exec_ = nullptr;
 };
