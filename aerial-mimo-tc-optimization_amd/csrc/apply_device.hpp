@@ -1,5 +1,5 @@
 // apply_device.hpp -- per-subcarrier MFMA coefficient apply (+ fused demap) as device functions, shared by the
-// single-cell kernel (coef_apply_mfma.hip) and the multi-cell persistent kernels (multicell.hip).
+// single-cell kernel (coef_apply_mfma.hip) and the multi-cell tile kernels (multicell.hip).
 //
 // Complex product as one real GEMM per subcarrier ("4-real" decomposition in real-embedding form):
 //   D[16 x 16] = A'[16 x 2A] * B'[2A x 16]

@@ -1,5 +1,5 @@
 // mmse_device.hpp -- per-subcarrier MMSE solve as device functions, shared by the single-cell kernels
-// (gram_mmse.hip) and the multi-cell persistent kernels (multicell.hip).
+// (gram_mmse.hip) and the multi-cell tile kernels (multicell.hip).
 //
 // The reference only names this stage (eqMmseCoefCompHighMimoKernel, aerial_bottleneck.cu:51-53); the
 // arithmetic here is the regularized solve that defines C in the reference's headline workload:

@@ -1,16 +1,16 @@
-// multicell.hip -- many uneven cells per launch as persistent work-queue kernels (SlotExec), and the slot-plan
-// C ABI (pe_slot_plan_*) on top of it.
+// multicell.hip -- many uneven cells per launch (SlotExec: tile kernels over a batch of cell-jobs), and the
+// slot-plan C ABI (pe_slot_plan_*) on top of it.
 //
 // The reference runs one launch per cell on its own stream (multicell_contention.cu:194-283,
 // multicell_evidence.cu:497-566). Here a batch of cell-jobs is cut into subcarrier tiles on the host and
-// executed by a short fixed sequence of persistent kernels that pull tiles from device work queues:
-//   stage 1 (cells with H): MMSE FP32 pass, then the FP64 pass over flagged subcarriers (one kernel each; the
-//                           L x L solve is unrolled per L and selected by the tile's cell);
-//   stage 2 (all cells):    fused coefficient apply + demap, one kernel per (antenna class KS, precision); the
-//                           QAM order is read per tile.
-// Tiles of a kernel are ordered by decreasing per-subcarrier cost so the queue drains with a short tail. The
-// kernels read their tile range from device memory, so one captured graph serves every batch; a one-thread
-// kernel at the head of the sequence zeroes the queue counters, so graph replays need no memset node.
+// executed by a fixed sequence of three kernels, whatever the number and shapes of the cells:
+//   MMSE FP32 pass over the cells with H (the L x L solve is unrolled per L and selected by the tile's cell),
+//   the FP64 pass over flagged subcarriers,
+//   one fused coefficient apply + demap kernel per precision (the antenna class and the QAM order are read
+//   per tile).
+// Tiles of a kernel are ordered by decreasing per-subcarrier cost. The kernels read their tile ranges from
+// device memory, so one captured graph serves every batch, and no memset node is needed (the flag counter is
+// zeroed by a kernel).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>

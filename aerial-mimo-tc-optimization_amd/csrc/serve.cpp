@@ -8,7 +8,8 @@
 //   2. arrivals: jobs whose arrival time has passed join a FIFO;
 //   3. admission and launch: whenever a stream is free, the FIFO head is admitted while its buffers (C,
 //      noise_inv, LLR) fit the device arena (the HBM budget), up to max_batch_jobs; the batch runs as one
-//      replay of that stream's captured slot graph (MMSE, then fused apply + demap, persistent work queues).
+//      launch of that stream's slot executor (MMSE, FP64 pass, fused apply + demap; eagerly with grids
+//      sized to the batch, or as a replay of its captured graph).
 // Jobs that arrive while batches are on the GPU are admitted as soon as a stream and memory free up, so the
 // batches overlap on the device. Each job's channel estimate, MMSE coefficients and samples stay
 // device-resident; the coefficients are computed once per job and applied to all of its data symbols.

@@ -1,5 +1,5 @@
-// slot_exec.hpp -- host-side executor of multi-cell slots: a fixed set of persistent work-queue kernels
-// whose contents (cells and tiles) change from one slot (or batch) to the next. Used by the slot plans
+// slot_exec.hpp -- host-side executor of multi-cell slots: a fixed sequence of tile kernels whose contents
+// (cells and tiles) change from one slot (or batch) to the next. Used by the slot plans
 // (pe_slot_plan_*, fixed cells) and by the serving loop (pe_serve_run, a new batch of jobs every launch).
 #pragma once
 #include <hip/hip_runtime.h>

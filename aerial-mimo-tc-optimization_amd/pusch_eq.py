@@ -213,7 +213,7 @@ def job_desc(H, sigma2, Y, n_pam_bits, prec="f16"):
 
 
 class SlotPlan:
-    """Owner of a pe_slot_plan: one slot of many cells as persistent work-queue kernels."""
+    """Owner of a pe_slot_plan: one slot of many cells as three tile kernels."""
 
     def __init__(self, cells, unbiased=True):
         arr = (CellDesc * len(cells))(*cells)

@@ -343,6 +343,8 @@ def main():
     ap.add_argument("--serve-jobs", type=int, default=2000, help="jobs per serving-loop run (0 = skip)")
     ap.add_argument("--no-sweep", action="store_true", help="skip the deployment sweep")
     ap.add_argument("--sweep-out", default=None, help="write the deployment sweep table (JSON) here")
+    ap.add_argument("--multicell-only", action="store_true",
+                    help="only the 24-cell slot section (graph and eager slots), printed as JSON (for profiling)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run 5 plain 256 MiB device copies (scale reference for the HBM byte counters)")
     args = ap.parse_args()
@@ -364,6 +366,9 @@ def main():
     import pusch_eq as pe
     dev = torch.device("cuda:0")
     pe.init(0)
+    if args.multicell_only:
+        print(json.dumps(multicell_bench(dev, 20)), flush=True)
+        return
     n = args.cells
     nsc = n * NSC_CELL
     log("generating %d cells x 2 buffer sets" % n)

@@ -148,7 +148,7 @@ pe_status pe_coef_apply_demap(const pe_c64 *C, const pe_c64 *Y_rx, const float *
                               int nSC, int nLayer, int nSym, int nAnt, int nPamBits, pe_prec prec,
                               void *stream);
 
-/* ---- multi-cell slot: one plan per set of uneven cells, run by persistent work-queue kernels -------- */
+/* ---- multi-cell slot: one plan per set of uneven cells, run by tile kernels ------------------------ */
 
 typedef struct {
     /* stage 1 (optional, skipped when H == NULL): MMSE coefficients into C / noise_inv */
@@ -165,22 +165,25 @@ typedef struct {
 
 typedef struct pe_slot_plan pe_slot_plan;
 
-/* Builds the tile lists of a slot (per kernel class, cells ordered by per-subcarrier cost, cut into
-   subcarrier tiles) and uploads descriptors, tiles and work-queue counters. Allocates its device memory
-   (descriptors, tiles, counters, per-subcarrier MMSE flags) once, here. flags: PE_MMSE_UNBIASED for stage 1.
+/* Builds the tile lists of a slot (cells with H for the MMSE stage, cells of each precision for the fused
+   apply/demap, ordered by per-subcarrier cost and cut into subcarrier tiles) and uploads descriptors and
+   tiles. Allocates its device memory (descriptors, tiles, the MMSE flag counter, per-subcarrier MMSE flags)
+   once, here. flags: PE_MMSE_UNBIASED for the MMSE stage.
    Shapes per cell: nSC >= 1, 1 <= nLayer <= 8, 1 <= nSym <= 16, 4 <= nAnt <= 128 with nAnt % 4 == 0.
    Replaces the reference's one-launch-per-cell-per-stream protocol (multicell_contention.cu:194-283,
    multicell_evidence.cu:497-566). */
 pe_status pe_slot_plan_create(const pe_cell_desc *cells, int nCells, unsigned flags, pe_slot_plan **plan);
 /* Rewrites the per-cell pointers of an existing plan (same shapes) for the next slot, ordered on `stream`. */
 pe_status pe_slot_plan_update(pe_slot_plan *plan, const pe_cell_desc *cells, int nCells, void *stream);
-/* Enqueues the slot on `stream`: stage 1 then stage 2, each a few persistent kernels. Graph-capturable: the
-   work-queue counters reset themselves when the last wave leaves, no memset node is recorded. */
+/* Enqueues the slot on `stream`: the MMSE FP32 pass, the FP64 pass over flagged subcarriers and one fused
+   apply/demap kernel per precision, one workgroup per tile. Graph-capturable: the flag counter is zeroed by
+   a kernel, no memset node is recorded. Each cell's results are bit-identical to the cell run alone through
+   pe_gram_mmse and pe_coef_apply_demap. */
 pe_status pe_slot_plan_launch(pe_slot_plan *plan, void *stream);
 /* Captures pe_slot_plan_launch into a hipGraph on first use and replays it on `stream`. */
 pe_status pe_slot_plan_graph_launch(pe_slot_plan *plan, void *stream);
 void pe_slot_plan_destroy(pe_slot_plan *plan);
-/* Tiles and largest persistent grid of the plan (for reports). */
+/* Tiles and largest kernel grid (workgroups) of the plan (for reports). */
 pe_status pe_slot_plan_info(const pe_slot_plan *plan, int *nTiles, int *nWorkgroups);
 
 /* ---- serving loop --------------------------------------------------------------------------------- */

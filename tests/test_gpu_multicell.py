@@ -1,4 +1,4 @@
-"""Multi-cell slot plans: uneven cells in persistent work-queue kernels. Every cell's LLRs must be identical,
+"""Multi-cell slot plans: uneven cells in one fixed sequence of tile kernels. Every cell's LLRs must be identical,
 bit for bit, to the same cell run alone through the single-cell entry points (pe_gram_mmse +
 pe_coef_apply_demap), after eager launches, graph replays and pointer updates alike."""
 import numpy as np
