@@ -9,7 +9,7 @@ class becomes a line of tune_table.txt beside the library, which the library rea
 
 Candidates (see pe_tune_load in include/pusch_eq.h):
     MFMA apply: waves per workgroup {1, 2, 4} x subcarriers per wave {1, 2}
-    FP32 apply: antennas per LDS chunk {8, 16}
+    FP32 apply: antennas per LDS chunk {4, 8, 16}
     MMSE:       lanes per subcarrier {4, 8, 16} (one per shape: it fixes the summation order) x waves per
                 workgroup {1, 2, 4, 8} x {one Gram round per wave, two, one with the antenna rows kept in
                 registers} (tile = rounds x waves x 64 / lanes <= 128 subcarriers) per batch bucket (single
@@ -89,7 +89,7 @@ def tune_apply(pe, dev, L, S, A, prec, target_bytes):
         llr = torch.empty((nsc, L, S, 8), device=dev)
         sets.append((C, Y, ni, llr))
     if prec == "f32":
-        cands = [(kc, 1, 1) for kc in (8, 16)]
+        cands = [(kc, 1, 1) for kc in (4, 8, 16)]
     else:
         cands = [(8, w, s) for w, s in itertools.product((1, 2, 4), (1, 2))]
     best = None
@@ -203,6 +203,7 @@ def main():
     ap.add_argument("--out", default=os.path.join(HERE, "tune_table.txt"))
     ap.add_argument("--report", default=None, help="write the per-candidate timings as JSON")
     ap.add_argument("--quick", action="store_true", help="A in {16, 64}, L = 8, f16 only")
+    ap.add_argument("--precs", default=None, help="comma-separated precisions to tune (default: all)")
     ap.add_argument("--keep-coef", action="store_true",
                     help="keep the coef lines of the existing table (--out) and tune only MMSE and slot tiles")
     args = ap.parse_args()
@@ -213,6 +214,8 @@ def main():
     As = (16, 64) if args.quick else (8, 16, 32, 64, 128)
     Ls = (8,) if args.quick else (2, 4, 8)
     precs = ("f16",) if args.quick else ("f16", "f16_split", "bf16", "bf16_split", "f32")
+    if args.precs:
+        precs = tuple(args.precs.split(","))
     S = 11
     lines = ["# tile table written by autotune.py on %s" % torch.cuda.get_device_name(0),
              "# coef <L> <S> <A> <prec> <f32_kc> <mfma_wpb> <mfma_spw>",

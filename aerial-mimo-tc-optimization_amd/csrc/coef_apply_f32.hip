@@ -7,7 +7,8 @@
 // register tile (4 layers x 4 symbols) of one subcarrier, so a subcarrier needs ceil(L/4)*ceil(S/4) lanes
 // (6 at L=8, S=11). C and Y_rx rows are staged through LDS in chunks of KC antennas with coalesced
 // 128-bit loads (each 128-bit load is two complex samples), so every HBM byte is read once and each staged
-// sample is reused by 4 tiles from LDS instead of being re-fetched through the vector cache.
+// sample is reused by 4 tiles from LDS instead of being re-fetched through the vector cache. The staging
+// addresses of a lane are computed once per workgroup (no divisions in the chunk loop).
 #include "pe_device.hpp"
 #include "pe_internal.hpp"
 
@@ -17,6 +18,11 @@ namespace {
 
 constexpr int kF32Threads = 256;
 constexpr int kF32MaxLds = 64 * 1024;
+// Staged 16-byte slots per lane and chunk: the 64 KB LDS budget bounds the slots of a chunk by
+// 4096 * KC / (KC + 2), so NQ slots per lane of 256 cover every workgroup the launcher forms.
+template <int KC> struct F32Q {
+    static constexpr int NQ = (4096 * KC / (KC + 2) + kF32Threads - 1) / kF32Threads;
+};
 
 template <int KC, int NPB>
 __global__ __launch_bounds__(kF32Threads) void coef_apply_f32_kernel(
@@ -24,6 +30,9 @@ __global__ __launch_bounds__(kF32Threads) void coef_apply_f32_kernel(
     const float *__restrict__ noise_inv, float *__restrict__ llr, int nSC, int L, int S, int A, int nLG,
     int nSG, int SCB) {
     constexpr int RS = KC + 2;  // LDS row stride in complex samples (padding staggers LDS banks)
+    constexpr int F4_PER_ROW = KC / 2;
+    constexpr int LG_F4 = F4_PER_ROW == 2 ? 1 : F4_PER_ROW == 4 ? 2 : 3;
+    constexpr int NQ = F32Q<KC>::NQ;
     extern __shared__ float4 lds_raw[];
     float2 *lds = reinterpret_cast<float2 *>(lds_raw);
 
@@ -46,28 +55,37 @@ __global__ __launch_bounds__(kF32Threads) void coef_apply_f32_kernel(
         yrow[i] = L + min(4 * sg + i, S - 1);
     }
 
+    // staging slots of this lane: q = tid + 256 i over [subcarrier][row][part] of a chunk. The row's source
+    // offset from the workgroup's first subcarrier is computed once (bit 31 set: a Y_rx row); the part and
+    // the LDS destination follow from q by shifts.
+    const float2 *Cw = C + (size_t)sc0 * L * A;
+    const float2 *Yw = Y + (size_t)sc0 * S * A;
+    const int nf4 = nsc * R * F4_PER_ROW;
+    unsigned soff[NQ];
+#pragma unroll
+    for (int i = 0; i < NQ; ++i) {
+        const int row = (tid + i * kF32Threads) >> LG_F4;
+        const int s_l = row / R;
+        const int rr = row - s_l * R;
+        soff[i] = rr < L ? (unsigned)((s_l * L + rr) * A) : (unsigned)((s_l * S + rr - L) * A) | 0x80000000u;
+    }
     float2 acc[4][4];
 #pragma unroll
     for (int i = 0; i < 4; ++i)
 #pragma unroll
         for (int j = 0; j < 4; ++j) acc[i][j] = make_float2(0.f, 0.f);
 
-    constexpr int F4_PER_ROW = KC / 2;
-    const int nf4 = nsc * R * F4_PER_ROW;
     for (int k0 = 0; k0 < A; k0 += KC) {
         const int kn = min(KC, A - k0);  // multiple of 4 (A % 4 == 0)
-        for (int q = tid; q < nf4; q += kF32Threads) {
-            const int row = q / F4_PER_ROW;
-            const int part = q - row * F4_PER_ROW;
-            const int s_l = row / R;
-            const int rr = row - s_l * R;
-            const size_t sc = (size_t)(sc0 + s_l);
-            float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
-            if (2 * part < kn) {
-                const float2 *src = (rr < L) ? C + (sc * L + rr) * A : Y + (sc * S + (rr - L)) * A;
-                v = *reinterpret_cast<const float4 *>(src + k0 + 2 * part);
-            }
-            *reinterpret_cast<float4 *>(lds + (s_l * R + rr) * RS + 2 * part) = v;
+#pragma unroll
+        for (int i = 0; i < NQ; ++i) {
+            const int q = tid + i * kF32Threads;
+            const int p2 = 2 * (q & (F4_PER_ROW - 1));
+            const unsigned o = soff[i];
+            const float2 *src = (o & 0x80000000u) ? Yw + (o & 0x7fffffffu) : Cw + o;
+            if (q < nf4)
+                *reinterpret_cast<float4 *>(lds + (q >> LG_F4) * RS + p2) =
+                    p2 < kn ? *reinterpret_cast<const float4 *>(src + k0 + p2) : make_float4(0.f, 0.f, 0.f, 0.f);
         }
         __syncthreads();
         if (active) {
@@ -158,6 +176,7 @@ hipError_t launch_f32_impl(const float2 *C, const float2 *Y, float2 *O, const fl
     const int rowBytes = (L + S) * RS * (int)sizeof(float2);
     int scb = kF32Threads / tps;
     scb = min(scb, kF32MaxLds / rowBytes);
+    scb = min(scb, F32Q<KC>::NQ * kF32Threads / ((L + S) * (KC / 2)));  // staging slots of one chunk
     if (scb < 1) return hipErrorInvalidConfiguration;
     const int grid = (nSC + scb - 1) / scb;
     const size_t lds = (size_t)scb * rowBytes;
@@ -171,6 +190,7 @@ hipError_t launch_f32_kc(const float2 *C, const float2 *Y, float2 *O, const floa
                          int S, int A, hipStream_t st) {
     const CoefTile &t = coef_tile_for(L, S, A, 0);
     if (t.f32_kc == 16) return launch_f32_impl<16, NPB>(C, Y, O, ni, llr, nSC, L, S, A, st);
+    if (t.f32_kc == 4) return launch_f32_impl<4, NPB>(C, Y, O, ni, llr, nSC, L, S, A, st);
     return launch_f32_impl<8, NPB>(C, Y, O, ni, llr, nSC, L, S, A, st);
 }
 

@@ -77,7 +77,7 @@ int load_table_file(const char *path) {
         MmseEntry me;
         int kc = 0, wpb = 0, spw = 0, a1 = 0, a2 = 0, a3 = 0, a4 = 0;
         if (sscanf(line, "coef %d %d %d %d %d %d %d", &e.L, &e.S, &e.A, &e.prec, &kc, &wpb, &spw) == 7) {
-            e.tile.f32_kc = (kc == 16) ? 16 : 8;
+            e.tile.f32_kc = (kc == 16 || kc == 4) ? kc : 8;
             e.tile.mfma_waves = (wpb == 2 || wpb == 4) ? wpb : 1;
             e.tile.mfma_spw = spw > 1 ? spw : 1;
             t.push_back(e);

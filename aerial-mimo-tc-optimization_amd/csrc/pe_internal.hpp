@@ -9,7 +9,7 @@ namespace pe {
 
 // Tile parameters chosen per shape (defaults, overridable by the tuned table).
 struct CoefTile {
-    int f32_kc = 8;         // FP32 kernel: antennas per LDS chunk (8 or 16)
+    int f32_kc = 8;         // FP32 kernel: antennas per LDS chunk (4, 8 or 16)
     int mfma_waves = 1;     // MFMA kernel: waves (subcarriers) per workgroup (1, 2, 4)
     int mfma_spw = 1;       // MFMA kernel: subcarriers per one-wave workgroup (1 = no prefetch loop)
 };
