@@ -73,7 +73,7 @@ const char *pe_status_string(pe_status s);
    device without it. */
 pe_status pe_init(int device);
 /* Replaces the tile table with the file at `path` (NULL: no table, built-in defaults). Returns the number of
-   entries read, or -1 if the file cannot be read. Lines, first match wins, -1 = any:
+   coef and mmse entries read (0 for NULL), or -1 if the file cannot be read; malformed lines are skipped. Lines, first match wins, -1 = any:
      coef <nLayer> <nSym> <nAnt> <prec> <f32_kc: 4|8|16> <mfma_wpb: 1|2|4> <mfma_spw: 1..8>
      mmse <nLayer> <nAnt> <nSC_max> <lanes_per_subcarrier: 4|8|16> [<subcarriers_per_tile> [<waves: 1|2|4|8>
           [<keep_rows: 0|1>]]]

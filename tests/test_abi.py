@@ -115,3 +115,24 @@ def test_serve_rejects_bad_config_without_a_device(pe):
     assert run(deadline=0.0) == 1
     assert run(budget=1000) == 5          # PE_ERR_CAPACITY: one job does not fit
     assert run(cells=(pe.CellDesc * 1)(no_h)) == 1
+
+
+def test_tune_table_parser(pe, tmp_path):
+    """pe_tune_load returns the number of coef and mmse entries it read, skips comments and malformed lines,
+    and NULL restores the built-in defaults; the committed table parses entirely."""
+    h = pe.lib()
+    t = tmp_path / "table.txt"
+    t.write_text("# comment\n"
+                 "coef 8 11 64 1 8 4 1\n"
+                 "coef -1 -1 -1 -1 16 2 1\n"
+                 "mmse 8 64 4096 16 8 2 1\n"
+                 "mmse 8 64 -1 16\n"
+                 "slot 32 8 4 0\n"
+                 "coef 1 2 3\n"          # too few fields
+                 "bogus line\n")
+    assert h.pe_tune_load(str(t).encode()) == 4
+    assert h.pe_tune_load(str(tmp_path / "missing.txt").encode()) == -1
+    assert h.pe_tune_load(None) == 0
+    committed = os.path.join(ROOT, "aerial-mimo-tc-optimization_amd", "tune_table.txt")
+    lines = [ln for ln in open(committed) if ln.startswith(("coef ", "mmse "))]
+    assert h.pe_tune_load(committed.encode()) == len(lines)
