@@ -176,17 +176,26 @@ __device__ __forceinline__ void zero_operands(unsigned char *lds, int lane) {
 
 // Max-log LLRs of the two symbols (layers l0, l0 + 1, symbol n) a lane holds after the re/im exchange.
 template <int NPB>
+// f16: llr points to binary16 storage of the same layout (the LLR wire format) instead of FP32.
 __device__ __forceinline__ void demap_store(float2 s0, float2 s1, size_t sc, int l0, int n, int L, int S,
-                                            const float *__restrict__ noise_inv, float *__restrict__ llr) {
+                                            const float *__restrict__ noise_inv, float *__restrict__ llr, int f16) {
     if (l0 < L) {
         float v[2 * NPB];
         qam_llr<NPB>(s0.x, s0.y, noise_inv[sc * L + l0], v);
-        store_llr<NPB>(llr + ((sc * L + l0) * S + n) * (2 * NPB), v);
+        const size_t e = ((sc * L + l0) * S + n) * (2 * NPB);
+        if (f16)
+            store_llr_f16<NPB>(reinterpret_cast<uint16_t *>(llr) + e, v);
+        else
+            store_llr<NPB>(llr + e, v);
     }
     if (l0 + 1 < L) {
         float v[2 * NPB];
         qam_llr<NPB>(s1.x, s1.y, noise_inv[sc * L + l0 + 1], v);
-        store_llr<NPB>(llr + ((sc * L + l0 + 1) * S + n) * (2 * NPB), v);
+        const size_t e = ((sc * L + l0 + 1) * S + n) * (2 * NPB);
+        if (f16)
+            store_llr_f16<NPB>(reinterpret_cast<uint16_t *>(llr) + e, v);
+        else
+            store_llr<NPB>(llr + e, v);
     }
 }
 
@@ -196,7 +205,7 @@ template <int KS, int PREC, int NPB>
 __device__ __forceinline__ void process_sc(RawSC<KS> &R, size_t sc, unsigned char *lds, float2 *__restrict__ O,
                                            const float *__restrict__ noise_inv, float *__restrict__ llr, int L,
                                            int S, int A, int lane, int nC, int nY, int rc, int ry,
-                                           int npb_rt = 0) {
+                                           int npb_rt = 0, int f16 = 0) {
     constexpr bool BF = (PREC == 2 || PREC == 4);
     constexpr bool SPLIT = (PREC == 3 || PREC == 4);
     constexpr int AG = 4 * KS;
@@ -326,13 +335,13 @@ __device__ __forceinline__ void process_sc(RawSC<KS> &R, size_t sc, unsigned cha
             if (l0 < L) O[(sc * L + l0) * S + n] = s0;
             if (l0 + 1 < L) O[(sc * L + l0 + 1) * S + n] = s1;
         } else if constexpr (NPB > 0) {
-            demap_store<NPB>(s0, s1, sc, l0, n, L, S, noise_inv, llr);
+            demap_store<NPB>(s0, s1, sc, l0, n, L, S, noise_inv, llr, f16);
         } else {  // QAM order chosen at run time (multi-cell kernels): only the epilogue is duplicated
             switch (npb_rt) {
-            case 1: demap_store<1>(s0, s1, sc, l0, n, L, S, noise_inv, llr); break;
-            case 2: demap_store<2>(s0, s1, sc, l0, n, L, S, noise_inv, llr); break;
-            case 3: demap_store<3>(s0, s1, sc, l0, n, L, S, noise_inv, llr); break;
-            default: demap_store<4>(s0, s1, sc, l0, n, L, S, noise_inv, llr); break;
+            case 1: demap_store<1>(s0, s1, sc, l0, n, L, S, noise_inv, llr, f16); break;
+            case 2: demap_store<2>(s0, s1, sc, l0, n, L, S, noise_inv, llr, f16); break;
+            case 3: demap_store<3>(s0, s1, sc, l0, n, L, S, noise_inv, llr, f16); break;
+            default: demap_store<4>(s0, s1, sc, l0, n, L, S, noise_inv, llr, f16); break;
             }
         }
     }
@@ -344,12 +353,12 @@ template <int KS, int PREC, int NPB>
 __device__ __attribute__((noinline)) void apply_one_sc(const float2 *__restrict__ C, const float2 *__restrict__ Y,
                                                       float2 *__restrict__ O, const float *__restrict__ noise_inv,
                                                       float *__restrict__ llr, long sc, int L, int S, int A,
-                                                      unsigned char *lds, int lane, int npb_rt) {
+                                                      unsigned char *lds, int lane, int npb_rt, int f16 = 0) {
     const int nC = L * (A >> 1), nY = S * (A >> 1);
     const int rc = (nC + 63) >> 6, ry = (nY + 63) >> 6;
     RawSC<KS> ra;
     load_raw<KS>(ra, C, Y, (size_t)sc, L, S, A, lane, nC, nY, rc, ry);
-    process_sc<KS, PREC, NPB>(ra, (size_t)sc, lds, O, noise_inv, llr, L, S, A, lane, nC, nY, rc, ry, npb_rt);
+    process_sc<KS, PREC, NPB>(ra, (size_t)sc, lds, O, noise_inv, llr, L, S, A, lane, nC, nY, rc, ry, npb_rt, f16);
 }
 
 // Subcarriers [first, last) of one cell by one wave (a one-wave workgroup). PIPE: the next subcarrier's raw
@@ -360,23 +369,25 @@ template <int KS, int PREC, int NPB, bool PIPE>
 __device__ __forceinline__ void apply_range(const float2 *__restrict__ C, const float2 *__restrict__ Y,
                                             float2 *__restrict__ O, const float *__restrict__ noise_inv,
                                             float *__restrict__ llr, long first, long last, int L, int S, int A,
-                                            unsigned char *lds, int lane, int npb_rt = 0) {
+                                            unsigned char *lds, int lane, int npb_rt = 0, int f16 = 0) {
     const int nC = L * (A >> 1), nY = S * (A >> 1);
     const int rc = (nC + 63) >> 6, ry = (nY + 63) >> 6;
     if constexpr (!PIPE) {
 #pragma nounroll
         for (long sc = first; sc < last; ++sc)
-            apply_one_sc<KS, PREC, NPB>(C, Y, O, noise_inv, llr, sc, L, S, A, lds, lane, npb_rt);
+            apply_one_sc<KS, PREC, NPB>(C, Y, O, noise_inv, llr, sc, L, S, A, lds, lane, npb_rt, f16);
         return;
     }
     RawSC<KS> ra, rb;
     load_raw<KS>(ra, C, Y, (size_t)first, L, S, A, lane, nC, nY, rc, ry);
     for (long sc = first; sc < last; sc += 2) {
         if (sc + 1 < last) load_raw<KS>(rb, C, Y, (size_t)(sc + 1), L, S, A, lane, nC, nY, rc, ry);
-        process_sc<KS, PREC, NPB>(ra, (size_t)sc, lds, O, noise_inv, llr, L, S, A, lane, nC, nY, rc, ry, npb_rt);
+        process_sc<KS, PREC, NPB>(ra, (size_t)sc, lds, O, noise_inv, llr, L, S, A, lane, nC, nY, rc, ry, npb_rt,
+                                  f16);
         if (sc + 1 >= last) break;
         if (sc + 2 < last) load_raw<KS>(ra, C, Y, (size_t)(sc + 2), L, S, A, lane, nC, nY, rc, ry);
-        process_sc<KS, PREC, NPB>(rb, (size_t)(sc + 1), lds, O, noise_inv, llr, L, S, A, lane, nC, nY, rc, ry, npb_rt);
+        process_sc<KS, PREC, NPB>(rb, (size_t)(sc + 1), lds, O, noise_inv, llr, L, S, A, lane, nC, nY, rc, ry, npb_rt,
+                                  f16);
     }
 }
 

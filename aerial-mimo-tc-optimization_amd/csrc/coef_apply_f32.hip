@@ -28,7 +28,7 @@ template <int KC, int NPB>
 __global__ __launch_bounds__(kF32Threads) void coef_apply_f32_kernel(
     const float2 *__restrict__ C, const float2 *__restrict__ Y, float2 *__restrict__ O,
     const float *__restrict__ noise_inv, float *__restrict__ llr, int nSC, int L, int S, int A, int nLG,
-    int nSG, int SCB) {
+    int nSG, int SCB, int f16) {
     constexpr int RS = KC + 2;  // LDS row stride in complex samples (padding staggers LDS banks)
     constexpr int F4_PER_ROW = KC / 2;
     constexpr int LG_F4 = F4_PER_ROW == 2 ? 1 : F4_PER_ROW == 4 ? 2 : 3;
@@ -135,14 +135,17 @@ __global__ __launch_bounds__(kF32Threads) void coef_apply_f32_kernel(
             }
         } else {
             const float ni = noise_inv[sc * L + l];
-            float *dst = llr + (sc * L + l) * S * (2 * NPB);
+            const size_t e0 = (sc * L + l) * S * (2 * NPB);
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
                 const int s = 4 * sg + j;
                 if (s < S) {
                     float v[2 * NPB];
                     qam_llr<NPB>(acc[i][j].x, acc[i][j].y, ni, v);
-                    store_llr<NPB>(dst + s * (2 * NPB), v);
+                    if (f16)  // binary16 LLR wire format, same layout
+                        store_llr_f16<NPB>(reinterpret_cast<uint16_t *>(llr) + e0 + s * (2 * NPB), v);
+                    else
+                        store_llr<NPB>(llr + e0 + s * (2 * NPB), v);
                 }
             }
         }
@@ -169,7 +172,7 @@ __global__ __launch_bounds__(256) void gram_f32_kernel(const float2 *__restrict_
 
 template <int KC, int NPB>
 hipError_t launch_f32_impl(const float2 *C, const float2 *Y, float2 *O, const float *ni, float *llr, int nSC,
-                           int L, int S, int A, hipStream_t st) {
+                           int L, int S, int A, int f16, hipStream_t st) {
     constexpr int RS = KC + 2;
     const int nLG = (L + 3) / 4, nSG = (S + 3) / 4;
     const int tps = nLG * nSG;
@@ -181,33 +184,33 @@ hipError_t launch_f32_impl(const float2 *C, const float2 *Y, float2 *O, const fl
     const int grid = (nSC + scb - 1) / scb;
     const size_t lds = (size_t)scb * rowBytes;
     hipLaunchKernelGGL((coef_apply_f32_kernel<KC, NPB>), dim3(grid), dim3(kF32Threads), lds, st, C, Y, O, ni,
-                       llr, nSC, L, S, A, nLG, nSG, scb);
+                       llr, nSC, L, S, A, nLG, nSG, scb, f16);
     return hipGetLastError();
 }
 
 template <int NPB>
 hipError_t launch_f32_kc(const float2 *C, const float2 *Y, float2 *O, const float *ni, float *llr, int nSC, int L,
-                         int S, int A, hipStream_t st) {
+                         int S, int A, int f16, hipStream_t st) {
     const CoefTile &t = coef_tile_for(L, S, A, 0);
-    if (t.f32_kc == 16) return launch_f32_impl<16, NPB>(C, Y, O, ni, llr, nSC, L, S, A, st);
-    if (t.f32_kc == 4) return launch_f32_impl<4, NPB>(C, Y, O, ni, llr, nSC, L, S, A, st);
-    return launch_f32_impl<8, NPB>(C, Y, O, ni, llr, nSC, L, S, A, st);
+    if (t.f32_kc == 16) return launch_f32_impl<16, NPB>(C, Y, O, ni, llr, nSC, L, S, A, f16, st);
+    if (t.f32_kc == 4) return launch_f32_impl<4, NPB>(C, Y, O, ni, llr, nSC, L, S, A, f16, st);
+    return launch_f32_impl<8, NPB>(C, Y, O, ni, llr, nSC, L, S, A, f16, st);
 }
 
 }  // namespace
 
 hipError_t launch_coef_apply_f32(const float2 *C, const float2 *Y, float2 *O, int nSC, int L, int S, int A,
                                  hipStream_t st) {
-    return launch_f32_kc<0>(C, Y, O, nullptr, nullptr, nSC, L, S, A, st);
+    return launch_f32_kc<0>(C, Y, O, nullptr, nullptr, nSC, L, S, A, 0, st);
 }
 
 hipError_t launch_coef_apply_demap_f32(const float2 *C, const float2 *Y, const float *noise_inv, float *llr,
-                                       int nSC, int L, int S, int A, int nPamBits, hipStream_t st) {
+                                       int nSC, int L, int S, int A, int nPamBits, hipStream_t st, int llr_f16) {
     switch (nPamBits) {
-    case 1: return launch_f32_kc<1>(C, Y, nullptr, noise_inv, llr, nSC, L, S, A, st);
-    case 2: return launch_f32_kc<2>(C, Y, nullptr, noise_inv, llr, nSC, L, S, A, st);
-    case 3: return launch_f32_kc<3>(C, Y, nullptr, noise_inv, llr, nSC, L, S, A, st);
-    case 4: return launch_f32_kc<4>(C, Y, nullptr, noise_inv, llr, nSC, L, S, A, st);
+    case 1: return launch_f32_kc<1>(C, Y, nullptr, noise_inv, llr, nSC, L, S, A, llr_f16, st);
+    case 2: return launch_f32_kc<2>(C, Y, nullptr, noise_inv, llr, nSC, L, S, A, llr_f16, st);
+    case 3: return launch_f32_kc<3>(C, Y, nullptr, noise_inv, llr, nSC, L, S, A, llr_f16, st);
+    case 4: return launch_f32_kc<4>(C, Y, nullptr, noise_inv, llr, nSC, L, S, A, llr_f16, st);
     default: return hipErrorInvalidValue;
     }
 }

@@ -15,9 +15,6 @@ namespace {
 __device__ __forceinline__ float half_bits_to_float(uint32_t b) {
     return (float)__builtin_bit_cast(_Float16, (uint16_t)(b & 0xffffu));
 }
-__device__ __forceinline__ uint32_t float_to_half_bits(float f) {
-    return (uint32_t)__builtin_bit_cast(uint16_t, (_Float16)f);
-}
 
 template <int NPB>
 __global__ __launch_bounds__(256) void llr_f16_kernel(const uint32_t *__restrict__ soft,

@@ -347,4 +347,24 @@ pe_status pe_coef_apply_demap(const pe_c64 *C, const pe_c64 *Y_rx, const float *
     return from_hip(e, "coef_apply_demap launch");
 }
 
+pe_status pe_coef_apply_demap_f16(const pe_c64 *C, const pe_c64 *Y_rx, const float *noise_inv, uint16_t *llr,
+                                  int nSC, int nLayer, int nSym, int nAnt, int nPamBits, pe_prec prec,
+                                  void *stream) {
+    pe_status s = check_coef_shape(nSC, nLayer, nSym, nAnt, (int)prec);
+    if (s != PE_OK) return s;
+    if (nPamBits < 1 || nPamBits > 4) return fail(PE_ERR_INVALID, "nPamBits = %d outside [1, 4]", nPamBits);
+    if (nSC == 0) return ok();
+    if (!check_ptrs(C, Y_rx, noise_inv) || !llr) return fail(PE_ERR_INVALID, "null pointer");
+    hipStream_t st = (hipStream_t)stream;
+    const float2 *c = reinterpret_cast<const float2 *>(C);
+    const float2 *y = reinterpret_cast<const float2 *>(Y_rx);
+    float *out = reinterpret_cast<float *>(llr);  // binary16 storage; the kernels store through uint16_t
+    hipError_t e = (prec == PE_PREC_F32)
+                       ? pe::launch_coef_apply_demap_f32(c, y, noise_inv, out, nSC, nLayer, nSym, nAnt, nPamBits,
+                                                         st, 1)
+                       : pe::launch_coef_apply_mfma(c, y, nullptr, noise_inv, out, nSC, nLayer, nSym, nAnt,
+                                                    nPamBits, (int)prec, st, 1);
+    return from_hip(e, "coef_apply_demap_f16 launch");
+}
+
 }  // extern "C"

@@ -108,6 +108,29 @@ __device__ __forceinline__ void store_llr(float *dst, const float (&v)[2 * NPB])
     }
 }
 
+// binary16 bits of a float, round to nearest even (the conversion of __float2half)
+__device__ __forceinline__ uint32_t float_to_half_bits(float f) {
+    return (uint32_t)__builtin_bit_cast(uint16_t, (_Float16)f);
+}
+
+// The 2*NPB LLRs of one symbol as binary16 (the reference's LLR wire format, aerial_llr_bench_v2.cu:221-227;
+// the values are already clamped to +-65504 by pam_llr_axis).
+template <int NPB>
+__device__ __forceinline__ void store_llr_f16(uint16_t *dst, const float (&v)[2 * NPB]) {
+    uint32_t w[NPB];
+#pragma unroll
+    for (int k = 0; k < NPB; ++k) w[k] = float_to_half_bits(v[2 * k]) | (float_to_half_bits(v[2 * k + 1]) << 16);
+    uint32_t *d = reinterpret_cast<uint32_t *>(dst);
+    if constexpr (NPB == 4) {
+        *reinterpret_cast<uint4 *>(d) = make_uint4(w[0], w[1], w[2], w[3]);
+    } else if constexpr (NPB == 2) {
+        *reinterpret_cast<uint2 *>(d) = make_uint2(w[0], w[1]);
+    } else {
+#pragma unroll
+        for (int k = 0; k < NPB; ++k) d[k] = w[k];
+    }
+}
+
 // ---- wave helpers (wave64) -------------------------------------------------------------------------
 __device__ __forceinline__ float wave_max_xor(float v, int from, int to) {
     // butterfly over lane-index bits [from, to): every lane of the group ends with the same value

@@ -47,12 +47,14 @@ hipError_t launch_coef_apply_f32(const float2 *C, const float2 *Y, float2 *O, in
 // Gram product G = M @ H (FP32 antenna-serial chain).
 hipError_t launch_gram_f32(const float2 *M, const float2 *H, float2 *G, int nSC, int L, int A, hipStream_t st);
 // MFMA coefficient apply; llr == nullptr -> writes Y_eq, else fused demap into llr (Y_eq unused).
+// llr_f16 != 0: llr is binary16 storage (the LLR wire format), same element layout.
 hipError_t launch_coef_apply_mfma(const float2 *C, const float2 *Y, float2 *O, const float *noise_inv,
                                   float *llr, int nSC, int L, int S, int A, int nPamBits, int prec,
-                                  hipStream_t st);
+                                  hipStream_t st, int llr_f16 = 0);
 // FP32 fused apply + demap with the antenna-serial chain (the PE_PREC_F32 fused path).
 hipError_t launch_coef_apply_demap_f32(const float2 *C, const float2 *Y, const float *noise_inv, float *llr,
-                                       int nSC, int L, int S, int A, int nPamBits, hipStream_t st);
+                                       int nSC, int L, int S, int A, int nPamBits, hipStream_t st,
+                                       int llr_f16 = 0);
 // Demappers.
 hipError_t launch_llr_f16(const uint32_t *soft, const float *noise_inv, uint16_t *out, int N, int nPamBits,
                           hipStream_t st);

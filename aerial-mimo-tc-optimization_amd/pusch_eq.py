@@ -42,6 +42,7 @@ SIGNATURES = {
     "pe_llr_demap_f16": ([_P, _P, _P, _I, _I, _P], _I),
     "pe_llr_demap": ([_P, _P, _P, _I, _I, _I, _I, _P], _I),
     "pe_coef_apply_demap": ([_P, _P, _P, _P, _I, _I, _I, _I, _I, _I, _P], _I),
+    "pe_coef_apply_demap_f16": ([_P, _P, _P, _P, _I, _I, _I, _I, _I, _I, _P], _I),
     "pe_slot_plan_create": ([_P, _I, _U, _P], _I),
     "pe_slot_plan_update": ([_P, _P, _I, _P], _I),
     "pe_slot_plan_launch": ([_P, _P], _I),
@@ -194,6 +195,19 @@ def coef_apply_demap(C, Y, noise_inv, n_pam_bits, prec="f16", out=None, stream=N
         out = torch.empty((nSC, L, S, 2 * n_pam_bits), dtype=torch.float32, device=C.device)
     _check(lib().pe_coef_apply_demap(_ptr(C), _ptr(Y), _ptr(noise_inv), _ptr(out), nSC, L, S, A, n_pam_bits,
                                      _prec(prec), _stream(stream)))
+    return out
+
+
+def coef_apply_demap_f16(C, Y, noise_inv, n_pam_bits, prec="f16", out=None, stream=None):
+    """Fused apply + demap writing binary16 LLRs (the decoder wire format); returns a torch.float16 tensor
+    [nSC, nLayer, nSym, 2*n_pam_bits] (pe_coef_apply_demap_f16)."""
+    import torch
+    nSC, L, A = C.shape
+    S = Y.shape[1]
+    if out is None:
+        out = torch.empty((nSC, L, S, 2 * n_pam_bits), dtype=torch.float16, device=C.device)
+    _check(lib().pe_coef_apply_demap_f16(_ptr(C), _ptr(Y), _ptr(noise_inv), _ptr(out), nSC, L, S, A,
+                                         n_pam_bits, _prec(prec), _stream(stream)))
     return out
 
 

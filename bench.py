@@ -100,26 +100,77 @@ def time_launches(fn, bufs, steps, warmup):
     return e0.elapsed_time(e1) / 1e3, wall
 
 
+def cpu_model():
+    try:
+        for ln in open("/proc/cpuinfo"):
+            if ln.startswith("model name"):
+                return ln.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
 def cpu_baseline(C, Y, ni, seconds):
     """The oracle's OpenMP restatement of the scalar coef-apply loop + FP32 demapper on one cell, repeated
-    for about `seconds`, on the host cores (OMP_NUM_THREADS or all)."""
+    for about `seconds`, on the host cores (OMP_NUM_THREADS or all), then for a quarter of that on one core."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import numpy as np
     import pe_oracle as po
     Cn, Yn, nin = C.cpu().numpy(), Y.cpu().numpy(), ni.cpu().numpy()
     out = np.empty((NSC_CELL, L, S, 2 * Q), dtype=np.float32)
-    _, used = po.coef_apply_demap_f32_omp(Cn, Yn, nin, Q, 0, out)  # warm
-    reps, t0 = 0, time.perf_counter()
-    while True:
-        po.coef_apply_demap_f32_omp(Cn, Yn, nin, Q, 0, out)
-        reps += 1
-        el = time.perf_counter() - t0
-        if el >= seconds:
-            break
-    return {"value": reps * NSC_CELL / el, "unit": "subcarriers/s", "cores": int(used), "kind": "port",
+    bytes_sc = L * A * 8 + S * A * 8 + L * S * 2 * Q * 4 + L * 4
+
+    def run(nthreads, budget):
+        _, used = po.coef_apply_demap_f32_omp(Cn, Yn, nin, Q, nthreads, out)  # warm
+        reps, t0 = 0, time.perf_counter()
+        while True:
+            po.coef_apply_demap_f32_omp(Cn, Yn, nin, Q, nthreads, out)
+            reps += 1
+            el = time.perf_counter() - t0
+            if el >= budget:
+                break
+        return reps * NSC_CELL / el, int(used), reps, el
+
+    v, used, reps, el = run(0, seconds)
+    v1, _, reps1, el1 = run(1, max(1.0, seconds / 4))
+    return {"value": v, "unit": "subcarriers/s", "cores": used, "kind": "port",
             "sample": "%d repetitions of one 1584-subcarrier cell (8 layers, 11 symbols, 64 antennas, 256-QAM), "
                       "scalar antenna-serial FP32 chain + FP32 demapper, OpenMP over (subcarrier, layer), "
-                      "%.1f s" % (reps, el)}
+                      "%.1f s; then %d repetitions on 1 thread, %.1f s" % (reps, el, reps1, el1),
+            "gbs": v * bytes_sc / 1e9, "single_thread_value": v1, "single_thread_gbs": v1 * bytes_sc / 1e9,
+            "cpu_model": cpu_model(), "host_cpus": os.cpu_count()}
+
+
+def hbm_ceiling(dev):
+    """Measured streaming ceilings: a 1 GiB device copy (read + write bytes) and a 1 GiB sum (read bytes)."""
+    import torch
+    n = 1 << 30
+    a = torch.empty(n // 4, dtype=torch.float32, device=dev)
+    b = torch.empty_like(a)
+    a.fill_(1.0)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    e0.record()
+    for _ in range(10):
+        b.copy_(a)
+    e1.record()
+    torch.cuda.synchronize()
+    copy_gbs = 2 * n * 10 / (e0.elapsed_time(e1) * 1e-3) / 1e9
+    for _ in range(3):
+        a.sum()
+    torch.cuda.synchronize()
+    e0.record()
+    for _ in range(10):
+        a.sum()
+    e1.record()
+    torch.cuda.synchronize()
+    read_gbs = n * 10 / (e0.elapsed_time(e1) * 1e-3) / 1e9
+    del a, b
+    torch.cuda.empty_cache()
+    return {"copy_gbs": copy_gbs, "read_gbs": read_gbs,
+            "method": "torch copy_ of 1 GiB (bytes read + written) and sum of 1 GiB (bytes read), 10 each"}
 
 
 def draw_cells(n, seed, dev, prec="f16"):
@@ -411,6 +462,24 @@ def main():
     achieved_gbs = bytes_launch / per_launch / 1e9
     value = nsc / per_launch
 
+    # per-launch distribution (events between back-to-back launches, same alternation), cold = from HBM
+    evs = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps + 1)]
+    evs[0].record()
+    for i in range(args.steps):
+        fused(*bufs[i % 2])
+        evs[i + 1].record()
+    torch.cuda.synchronize()
+    per = sorted(evs[i].elapsed_time(evs[i + 1]) for i in range(args.steps))
+    launch_ms = {"p50": per[len(per) // 2], "p99": per[min(len(per) - 1, int(-(-99 * len(per) // 100)) - 1)],
+                 "launches": len(per)}
+    # warm: four cells (80 MB) fit in the 256 MB Infinity Cache; the same launch repeated over them
+    c4 = slice(0, 4 * NSC_CELL)
+    wbuf = [(sets[0]["C"][c4], sets[0]["Y"][c4], sets[0]["ni"][c4], sets[0]["llr"][c4])]
+    elw, _ = time_launches(fused, wbuf, 200, 20)
+    warm = {"subcarriers_per_s": 4 * NSC_CELL / (elw / 200), "gbs": 4 * bytes_launch / n / (elw / 200) / 1e9,
+            "workload": "4 cells (80 MB) relaunched, resident in the Infinity Cache"}
+    ceiling = hbm_ceiling(dev)
+
     # plain FP32 path on the same batches
     el32, _ = time_launches(lambda C, Y, ni, llr: fused(C, Y, ni, llr, "f32"), bufs, max(20, args.steps // 4),
                             5)
@@ -419,6 +488,14 @@ def main():
     elsp, _ = time_launches(lambda C, Y, ni, llr: fused(C, Y, ni, llr, "f16_split"), bufs,
                             max(20, args.steps // 4), 5)
     persp = elsp / max(20, args.steps // 4)
+    # binary16 LLR wire format (the decoder input) on the same batches
+    llr16 = [torch.empty((nsc, L, S, 2 * Q), dtype=torch.float16, device=dev) for _ in sets]
+    bufs16 = [(s["C"], s["Y"], s["ni"], o) for s, o in zip(sets, llr16)]
+    el16, _ = time_launches(lambda C, Y, ni, o: pe.coef_apply_demap_f16(C, Y, ni, Q, prec=args.prec, out=o),
+                            bufs16, max(20, args.steps // 4), 5)
+    per16 = el16 / max(20, args.steps // 4)
+    bytes16 = nsc * (L * A * 8 + S * A * 8 + L * S * 2 * Q * 2 + L * 4)
+    del llr16, bufs16
 
     # single-cell slot latency: MMSE + fused apply/demap for one cell, one stream
     cell = {k: (v[c1].contiguous() if torch.is_tensor(v) and v.shape[0] == nsc else v) for k, v in s0.items()}
@@ -477,6 +554,10 @@ def main():
                      "frac": achieved_gbs / HBM_PEAK_GBS, "traffic": load_traffic("fused_%s" % args.prec)},
         "cpu_baseline": cpu,
         "hbm_gbs": achieved_gbs,
+        "cold_hbm": {"note": "value and roofline are cold: two 637 MB buffer sets alternate, beyond the Infinity "
+                             "Cache", "launch_ms": launch_ms},
+        "warm_infinity_cache": warm,
+        "measured_ceiling": ceiling,
         "gflops": flops_launch / per_launch / 1e9,
         "evm_db": {"apply_vs_fp64": evm_apply, "chain_vs_fp64": evm_chain},
         "accuracy": acc,
@@ -484,6 +565,9 @@ def main():
         "plain_f32": {"value": nsc / per32, "ms_per_step": per32 * 1e3,
                       "hbm_gbs": bytes_launch / per32 / 1e9},
         "f16_split": {"value": nsc / persp, "ms_per_step": persp * 1e3, "hbm_gbs": bytes_launch / persp / 1e9},
+        "llr_f16_wire": {"value": nsc / per16, "ms_per_step": per16 * 1e3, "hbm_gbs": bytes16 / per16 / 1e9,
+                         "bytes_per_launch": bytes16,
+                         "workload": "same launch, LLRs written as binary16 (pe_coef_apply_demap_f16)"},
         "slot_latency_ms": {"p50": pct(50), "p99": pct(99), "max": lat[-1] if lat else None, "iters": len(lat),
                             "workload": "one cell: pe_gram_mmse + fused apply/demap, events on one stream"},
         "multicell_slot": multicell,

@@ -148,6 +148,15 @@ pe_status pe_coef_apply_demap(const pe_c64 *C, const pe_c64 *Y_rx, const float *
                               int nSC, int nLayer, int nSym, int nAnt, int nPamBits, pe_prec prec,
                               void *stream);
 
+/* Same as pe_coef_apply_demap, with the LLRs written in the LDPC decoder's wire format: binary16
+   llr[nSC][nLayer][nSym][2*nPamBits], interleaved [b0_I, b0_Q, b1_I, ...], clamped to +-65504 and rounded
+   to nearest even, as the reference demapper stores them (aerial_llr_bench_v2.cu:117,221-227). Each value
+   is the binary16 rounding of the FP32 LLR pe_coef_apply_demap writes for the same inputs and precision.
+   llr must be 16-byte aligned (8-byte for nPamBits 2, 4-byte for 1 and 3). */
+pe_status pe_coef_apply_demap_f16(const pe_c64 *C, const pe_c64 *Y_rx, const float *noise_inv, uint16_t *llr,
+                                  int nSC, int nLayer, int nSym, int nAnt, int nPamBits, pe_prec prec,
+                                  void *stream);
+
 /* ---- multi-cell slot: one plan per set of uneven cells, run by tile kernels ------------------------ */
 
 typedef struct {

@@ -19,7 +19,7 @@ def declared_functions():
 def test_header_declares_the_operator_surface():
     names = declared_functions()
     for required in ("pe_coef_apply", "pe_gram_mmse", "pe_llr_demap_f16", "pe_llr_demap", "pe_coef_apply_demap",
-                     "pe_gram", "pe_coef_apply_v1"):
+                     "pe_coef_apply_demap_f16", "pe_gram", "pe_coef_apply_v1"):
         assert required in names
 
 
@@ -62,6 +62,7 @@ def test_empty_calls_succeed_without_launching(pe):
     assert h.pe_llr_demap(n, n, n, 0, 8, 11, 4, None) == 0
     assert h.pe_gram_mmse(n, n, n, n, n, 0, 8, 64, 1, 0.0, None) == 0
     assert h.pe_coef_apply_demap(n, n, n, n, 0, 8, 11, 64, 4, 1, None) == 0
+    assert h.pe_coef_apply_demap_f16(n, n, n, n, 0, 8, 11, 64, 4, 1, None) == 0
 
 
 def test_other_entry_points_validate(pe):
@@ -76,6 +77,8 @@ def test_other_entry_points_validate(pe):
     assert h.pe_gram_mmse(p, p, p, p, p, 4, 8, 64, 4, 0.0, None) == 1   # unknown flag bit
     assert h.pe_gram(p, p, p, 4, 8, 0, None) == 1
     assert h.pe_coef_apply_demap(p, p, p, p, 4, 8, 11, 64, 5, 1, None) == 1
+    assert h.pe_coef_apply_demap_f16(p, p, p, p, 4, 8, 11, 64, 5, 1, None) == 1
+    assert h.pe_coef_apply_demap_f16(p, p, p, n, 4, 8, 11, 64, 4, 1, None) == 1
     assert h.pe_coef_apply_range(p, p, p, 4, -1, 8, 11, 64, 0, None) == 1
 
 

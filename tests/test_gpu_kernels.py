@@ -168,6 +168,31 @@ def test_fused_equals_apply_then_demap(po, pe, dev, prec, q, L, S, A, nsc):
         assert np.array_equal(fused, ref)
 
 
+@pytest.mark.parametrize("prec", ["f32", "f16", "bf16", "f16_split", "bf16_split"])
+@pytest.mark.parametrize("q", [1, 2, 3, 4])
+@pytest.mark.parametrize("L,S,A,nsc,ni_hi", [(8, 11, 64, 300, 100.0), (3, 9, 16, 41, 1e6), (5, 3, 24, 7, 1e4)])
+def test_fused_llr_wire_format_f16(po, pe, dev, prec, q, L, S, A, nsc, ni_hi):
+    """Binary16 LLR output = round-to-nearest-even binary16 of the FP32 fused LLRs, bit for bit; the large
+    noise_inv case drives LLRs past the binary16 range, where the +-65504 clamp must give finite values."""
+    import torch
+    rng = np.random.default_rng(5 * q + L)
+    C = po.cn(rng, (nsc, L, A), 1.0 / A)
+    Y = po.cn(rng, (nsc, S, A))
+    ni = rng.uniform(1, ni_hi, (nsc, L)).astype(np.float32)
+    dC, dY, dni = dev_c64(C, dev), dev_c64(Y, dev), dev_f32(ni, dev)
+    f32 = host(pe.coef_apply_demap(dC, dY, dni, q, prec=prec))
+    out = pe.coef_apply_demap_f16(dC, dY, dni, q, prec=prec)
+    assert out.dtype == torch.float16 and tuple(out.shape) == (nsc, L, S, 2 * q)
+    bits = host(out.view(torch.int16)).view(np.uint16)
+    assert np.array_equal(bits, f32.astype(np.float16).view(np.uint16))
+    assert np.isfinite(host(out.float())).all()
+    if ni_hi >= 1e6:
+        assert (np.abs(f32) == 65504.0).any()  # the clamp was exercised
+    if prec == "f32":  # against the oracle directly, rounded by numpy (IEEE round-to-nearest-even)
+        ref, _ = po.coef_apply_demap_f32_omp(C, Y, ni, q, nthreads=1)
+        assert np.array_equal(bits, ref.astype(np.float16).view(np.uint16))
+
+
 # ------------------------------------------------------------------------------------------------------
 # Gram product and MMSE
 # ------------------------------------------------------------------------------------------------------
