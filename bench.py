@@ -142,7 +142,9 @@ def cpu_baseline(C, Y, ni, seconds):
 
 
 def hbm_ceiling(dev):
-    """Measured streaming ceilings: a 1 GiB device copy (read + write bytes) and a 1 GiB sum (read bytes)."""
+    """Streaming rates of torch's own kernels on this GPU: a 1 GiB device copy (read + write bytes) and a 1 GiB
+    sum (read bytes). A reference point for the fused kernel's rate, not an upper bound: torch's kernels do not
+    reach the HBM limit themselves."""
     import torch
     n = 1 << 30
     a = torch.empty(n // 4, dtype=torch.float32, device=dev)
@@ -170,7 +172,8 @@ def hbm_ceiling(dev):
     del a, b
     torch.cuda.empty_cache()
     return {"copy_gbs": copy_gbs, "read_gbs": read_gbs,
-            "method": "torch copy_ of 1 GiB (bytes read + written) and sum of 1 GiB (bytes read), 10 each"}
+            "method": "torch copy_ of 1 GiB (bytes read + written) and sum of 1 GiB (bytes read), 10 each; "
+                      "torch's kernels, not an upper bound"}
 
 
 def draw_cells(n, seed, dev, prec="f16"):
@@ -472,12 +475,13 @@ def main():
     per = sorted(evs[i].elapsed_time(evs[i + 1]) for i in range(args.steps))
     launch_ms = {"p50": per[len(per) // 2], "p99": per[min(len(per) - 1, int(-(-99 * len(per) // 100)) - 1)],
                  "launches": len(per)}
-    # warm: four cells (80 MB) fit in the 256 MB Infinity Cache; the same launch repeated over them
-    c4 = slice(0, 4 * NSC_CELL)
-    wbuf = [(sets[0]["C"][c4], sets[0]["Y"][c4], sets[0]["ni"][c4], sets[0]["llr"][c4])]
+    # warm: eight cells (159 MB) fit in the 256 MB Infinity Cache; the same launch repeated over them
+    c8 = slice(0, 8 * NSC_CELL)
+    wbuf = [(sets[0]["C"][c8], sets[0]["Y"][c8], sets[0]["ni"][c8], sets[0]["llr"][c8])]
     elw, _ = time_launches(fused, wbuf, 200, 20)
-    warm = {"subcarriers_per_s": 4 * NSC_CELL / (elw / 200), "gbs": 4 * bytes_launch / n / (elw / 200) / 1e9,
-            "workload": "4 cells (80 MB) relaunched, resident in the Infinity Cache"}
+    warm = {"subcarriers_per_s": 8 * NSC_CELL / (elw / 200), "gbs": 8 * bytes_launch / n / (elw / 200) / 1e9,
+            "ms_per_launch": elw / 200 * 1e3,
+            "workload": "8 cells (159 MB) relaunched over the same buffers, resident in the Infinity Cache"}
     ceiling = hbm_ceiling(dev)
 
     # plain FP32 path on the same batches
@@ -557,7 +561,7 @@ def main():
         "cold_hbm": {"note": "value and roofline are cold: two 637 MB buffer sets alternate, beyond the Infinity "
                              "Cache", "launch_ms": launch_ms},
         "warm_infinity_cache": warm,
-        "measured_ceiling": ceiling,
+        "torch_streaming_rates": ceiling,
         "gflops": flops_launch / per_launch / 1e9,
         "evm_db": {"apply_vs_fp64": evm_apply, "chain_vs_fp64": evm_chain},
         "accuracy": acc,
