@@ -367,4 +367,25 @@ pe_status pe_coef_apply_demap_f16(const pe_c64 *C, const pe_c64 *Y_rx, const flo
     return from_hip(e, "coef_apply_demap_f16 launch");
 }
 
+pe_status pe_srs_filter(pe_c16 *H_est, const pe_c16 *SRS, const pe_c16 *W, const pe_c16 *FOCC, int nAnt,
+                        int nBlocks, int K, int nPorts, int foccLen, unsigned flags, void *stream) {
+    if (nBlocks < 0) return fail(PE_ERR_INVALID, "nBlocks = %d < 0", nBlocks);
+    if (nAnt < 1 || nAnt > 4096) return fail(PE_ERR_INVALID, "nAnt = %d outside [1, 4096]", nAnt);
+    if (K < 1 || K > 48) return fail(PE_ERR_INVALID, "K = %d outside [1, 48]", K);
+    if (nPorts < 1 || nPorts > 8) return fail(PE_ERR_INVALID, "nPorts = %d outside [1, 8]", nPorts);
+    if (foccLen < 1 || foccLen > 48) return fail(PE_ERR_INVALID, "foccLen = %d outside [1, 48]", foccLen);
+    if (flags & ~(PE_SRS_MFMA | PE_SRS_CONJ_W)) return fail(PE_ERR_INVALID, "unknown flag bits 0x%x", flags);
+    const bool mfma = (flags & PE_SRS_MFMA) != 0;
+    if (mfma && K % 4 != 0) return fail(PE_ERR_UNSUPPORTED, "K = %d: PE_SRS_MFMA needs K %% 4 == 0", K);
+    if (mfma && (reinterpret_cast<uintptr_t>(SRS) & 15u))
+        return fail(PE_ERR_INVALID, "SRS must be 16-byte aligned with PE_SRS_MFMA");
+    if (nBlocks == 0) return ok();
+    if (!check_ptrs(H_est, SRS, W) || !FOCC) return fail(PE_ERR_INVALID, "null pointer");
+    hipError_t e = pe::launch_srs_filter(reinterpret_cast<uint32_t *>(H_est), reinterpret_cast<const uint32_t *>(SRS),
+                                         reinterpret_cast<const uint32_t *>(W),
+                                         reinterpret_cast<const uint32_t *>(FOCC), nAnt, nBlocks, K, nPorts, foccLen,
+                                         (flags & PE_SRS_CONJ_W) != 0, mfma, (hipStream_t)stream);
+    return from_hip(e, "srs_filter launch");
+}
+
 }  // extern "C"

@@ -43,6 +43,7 @@ SIGNATURES = {
     "pe_llr_demap": ([_P, _P, _P, _I, _I, _I, _I, _P], _I),
     "pe_coef_apply_demap": ([_P, _P, _P, _P, _I, _I, _I, _I, _I, _I, _P], _I),
     "pe_coef_apply_demap_f16": ([_P, _P, _P, _P, _I, _I, _I, _I, _I, _I, _P], _I),
+    "pe_srs_filter": ([_P, _P, _P, _P, _I, _I, _I, _I, _I, _U, _P], _I),
     "pe_slot_plan_create": ([_P, _I, _U, _P], _I),
     "pe_slot_plan_update": ([_P, _P, _I, _P], _I),
     "pe_slot_plan_launch": ([_P, _P], _I),
@@ -208,6 +209,25 @@ def coef_apply_demap_f16(C, Y, noise_inv, n_pam_bits, prec="f16", out=None, stre
         out = torch.empty((nSC, L, S, 2 * n_pam_bits), dtype=torch.float16, device=C.device)
     _check(lib().pe_coef_apply_demap_f16(_ptr(C), _ptr(Y), _ptr(noise_inv), _ptr(out), nSC, L, S, A,
                                          n_pam_bits, _prec(prec), _stream(stream)))
+    return out
+
+
+SRS_MFMA = 1
+SRS_CONJ_W = 2
+
+
+def srs_filter(SRS, W, FOCC, mfma=False, conj_w=False, out=None, stream=None):
+    """SRS filter multiply (pe_srs_filter). SRS [nBlocks, nAnt, K], W [K, K], FOCC [nPorts, foccLen] as
+    binary16 complex pairs: torch int32 tensors whose 32 bits hold {re, im} binary16 (or any 4-byte dtype
+    viewed so). Returns H_est [nBlocks, nAnt, K, nPorts] of the same dtype."""
+    import torch
+    nBlocks, nAnt, K = SRS.shape
+    nPorts, foccLen = FOCC.shape
+    if out is None:
+        out = torch.empty((nBlocks, nAnt, K, nPorts), dtype=SRS.dtype, device=SRS.device)
+    flags = (SRS_MFMA if mfma else 0) | (SRS_CONJ_W if conj_w else 0)
+    _check(lib().pe_srs_filter(_ptr(out), _ptr(SRS), _ptr(W), _ptr(FOCC), nAnt, nBlocks, K, nPorts, foccLen, flags,
+                               _stream(stream)))
     return out
 
 

@@ -264,6 +264,52 @@ def multicell_bench(dev, steps):
             "tiles": tiles, "max_workgroups": wg}
 
 
+def srs_bench(dev, iters=500, warmup=50):
+    """SRS filter multiply (pe_srs_filter) on the configurations of srs_wmma_bench.cu main (:347-353): 64
+    antennas, 66 computation blocks; binary16 chain and MFMA. Per-launch times are medians of single launches
+    bracketed by events (the reference's method); back-to-back launches give the device-side rate."""
+    import torch
+    import pusch_eq as pe
+    g = torch.Generator(device=dev)
+    g.manual_seed(42)
+    nAnt, nBlocks = 64, 66
+
+    def c16(*shape):
+        v = ((torch.rand(shape + (2,), generator=g, device=dev) - 0.5) * 1.4).half()
+        return v.view(torch.int32).reshape(shape)
+
+    rows = []
+    for K, P, FL in ((24, 4, 8), (24, 2, 8), (12, 4, 12), (12, 2, 12)):
+        S, W, F = c16(nBlocks, nAnt, K), c16(K, K), c16(P, FL)
+        outs = {}
+        row = {"K": K, "ports": P, "focc_len": FL, "antennas": nAnt, "blocks": nBlocks}
+        ops = nBlocks * nAnt * K * P * K * 8.0
+        nbytes = 4 * (nBlocks * nAnt * K + nBlocks * nAnt * K * P + K * K + P * FL)
+        for name, mf in (("f16_chain", False), ("mfma_f32acc", True)):
+            out = torch.empty((nBlocks, nAnt, K, P), dtype=torch.int32, device=dev)
+            for _ in range(warmup):
+                pe.srs_filter(S, W, F, mfma=mf, out=out)
+            torch.cuda.synchronize()
+            ts = []
+            for _ in range(iters):
+                a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                a.record()
+                pe.srs_filter(S, W, F, mfma=mf, out=out)
+                b.record()
+                b.synchronize()
+                ts.append(a.elapsed_time(b))
+            ts.sort()
+            el, _ = time_launches(lambda: pe.srs_filter(S, W, F, mfma=mf, out=out), [()], iters, 10)
+            dev_ms = el / iters * 1e3
+            row[name] = {"median_ms": ts[len(ts) // 2], "gflops_median": ops / (ts[len(ts) // 2] * 1e-3) / 1e9,
+                         "back_to_back_ms": dev_ms, "gbs_back_to_back": nbytes / (dev_ms * 1e-3) / 1e9}
+            outs[name] = out.view(torch.float16).float()
+        a, b = outs["f16_chain"].flatten(), outs["mfma_f32acc"].flatten()
+        row["cosine_chain_vs_mfma"] = float((a @ b) / (a.norm() * b.norm()))
+        rows.append(row)
+    return rows
+
+
 def serving_bench(dev, n_jobs):
     """64 cells; jobs (one slot of one cell: MMSE + fused apply/demap of 11 symbols) arrive as a Poisson
     process; capacity = throughput with every job queued at t = 0; then offered load 10-120% of capacity."""
@@ -396,6 +442,8 @@ def main():
     ap.add_argument("--no-multicell", action="store_true")
     ap.add_argument("--serve-jobs", type=int, default=2000, help="jobs per serving-loop run (0 = skip)")
     ap.add_argument("--no-sweep", action="store_true", help="skip the deployment sweep")
+    ap.add_argument("--no-srs", action="store_true", help="skip the SRS filter section")
+    ap.add_argument("--srs-only", action="store_true", help="only the SRS filter section, printed as JSON")
     ap.add_argument("--sweep-out", default=None, help="write the deployment sweep table (JSON) here")
     ap.add_argument("--multicell-only", action="store_true",
                     help="only the 24-cell slot section (graph and eager slots), printed as JSON (for profiling)")
@@ -422,6 +470,9 @@ def main():
     pe.init(0)
     if args.multicell_only:
         print(json.dumps(multicell_bench(dev, 20)), flush=True)
+        return
+    if args.srs_only:
+        print(json.dumps(srs_bench(dev)), flush=True)
         return
     n = args.cells
     nsc = n * NSC_CELL
@@ -522,6 +573,7 @@ def main():
         cpu = cpu_baseline(cell["C"], cell["Y"], cell["ni"], args.cpu_seconds)
 
     multicell = None if args.no_multicell else multicell_bench(dev, 50)
+    srs = None if args.no_srs else srs_bench(dev)
     serving = serving_bench(dev, args.serve_jobs) if args.serve_jobs > 0 else None
     sweep = None
     if not args.no_sweep:
@@ -576,6 +628,7 @@ def main():
                             "workload": "one cell: pe_gram_mmse + fused apply/demap, events on one stream"},
         "multicell_slot": multicell,
         "serving": serving,
+        "srs_filter": srs,
         "deployment_sweep": sweep,
         "wall_s_timed": wall,
     }

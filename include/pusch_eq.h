@@ -157,6 +157,24 @@ pe_status pe_coef_apply_demap_f16(const pe_c64 *C, const pe_c64 *Y_rx, const flo
                                   int nSC, int nLayer, int nSym, int nAnt, int nPamBits, pe_prec prec,
                                   void *stream);
 
+/* ---- SRS filter multiply (channel estimation upstream of the equalizer) ---------------------------- */
+
+#define PE_SRS_MFMA 1u    /* MFMA with FP32 accumulation: one binary16 rounding per output (K % 4 == 0,
+                             SRS 16-byte aligned) instead of the reference's binary16 chain */
+#define PE_SRS_CONJ_W 2u  /* multiply by conj(W[sc][k]) (the operation as srs_wmma_bench.cu:6 writes it); without
+                             it W is not conjugated, as the reference's complex_conjmul computes (:43-47) */
+
+/* Drop-in for srs_filter_v1_scalar<K, N_PORTS, FOCC_LENGTH>(H_est, SRS, W, FOCC, nAnt, nBlocks)
+   (srs_wmma_bench.cu:57-90; the WMMA variant srs_filter_v2_wmma :102-231 computes the same sum):
+     H_est[b][ant][sc][port] = sum_{k<K} (W[sc][k] * FOCC[port][k % foccLen]) * SRS[b][ant][k]
+   SRS [nBlocks][nAnt][K], W [K][K], FOCC [nPorts][foccLen], H_est [nBlocks][nAnt][K][nPorts], binary16
+   complex. flags = 0: bit-identical to the reference's binary16 arithmetic (each product and sum of W*FOCC
+   rounded, then two binary16 fused multiply-adds per part and k, in k order). 1 <= K <= 48,
+   1 <= nPorts <= 8, 1 <= foccLen <= 48, 1 <= nAnt <= 4096; H_est 16-byte aligned for nPorts 4 and 8
+   (8-byte for 2). */
+pe_status pe_srs_filter(pe_c16 *H_est, const pe_c16 *SRS, const pe_c16 *W, const pe_c16 *FOCC, int nAnt,
+                        int nBlocks, int K, int nPorts, int foccLen, unsigned flags, void *stream);
+
 /* ---- multi-cell slot: one plan per set of uneven cells, run by tile kernels ------------------------ */
 
 typedef struct {

@@ -15,6 +15,8 @@
  *   src/llr/aerial_llr_bench_v2.cu  LUT :47-64, cpu_computePamLlr_ref :72-107, llr_v1_baseline :114-156,
  *                                   inputs :289-304
  *   src/llr/aerial_llr_v3.cu  cpu_ref :61-79, inputs :268-273
+ *   src/srs/srs_wmma_bench.cu complex_conjmul :43-47, hcmadd :48-52, srs_filter_v1_scalar :57-90,
+ *                             inputs :262-268
  */
 #include "pe_oracle.h"
 
@@ -593,3 +595,137 @@ int po_coef_apply_demap_f32_omp(const float *C, const float *Y, const float *noi
 }
 
 int po_omp_max_threads(void) { return omp_get_max_threads(); }
+
+/* ------------------------------------------------------------------------------------------------------
+ * binary16 arithmetic: every operation rounded once to nearest even, the semantics of __hmul, __hadd,
+ * __hsub and __hfma. Products of two binary16 values (22 significant bits) and sums of two (at most 40 bits
+ * between 2^15 and 2^-24) are exact in double; the fused multiply-add carries the double sum's rounding
+ * error (TwoSum) into the final rounding, so ties are broken on the exact value.
+ * ------------------------------------------------------------------------------------------------------ */
+
+/* binary16 nearest to hi + lo (|lo| at most half an ulp of hi in double), ties to even */
+static uint16_t f64x2_to_f16(double hi, double lo) {
+    uint16_t sign = signbit(hi) ? 0x8000u : 0u;
+    double a = fabs(hi);
+    if (isnan(a)) return (uint16_t)(sign | 0x7e00u);
+    if (isinf(a)) return (uint16_t)(sign | 0x7c00u);
+    if (a == 0.0) return sign; /* an exact zero sum has no rounding error */
+    if (signbit(hi)) lo = -lo; /* lo relative to |hi| */
+    int e2;
+    frexp(a, &e2); /* a = f * 2^e2, f in [0.5, 1) */
+    int e = e2 - 1;   /* a in [2^e, 2^(e+1)) */
+    int sub = e < -14;
+    double q = sub ? ldexp(a, 24) : ldexp(a, 10 - e); /* exact scaling: integer part = kept significand */
+    double fl = floor(q), fr = q - fl;
+    double n;
+    if (fr > 0.5 || (fr == 0.5 && (lo > 0 || (lo == 0 && fmod(fl, 2.0) != 0.0)))) n = fl + 1.0;
+    else n = fl;
+    if (sub) return (uint16_t)(sign | (uint16_t)n); /* n == 1024 encodes the smallest normal */
+    if (n == 2048.0) { n = 1024.0; e += 1; }
+    if (e > 15) return (uint16_t)(sign | 0x7c00u);
+    return (uint16_t)(sign | ((uint32_t)(e + 15) << 10) | ((uint32_t)n - 1024u));
+}
+
+static double f16_to_f64(uint16_t h) { return (double)po_f16_to_f32(h); }
+
+uint16_t po_hmul(uint16_t a, uint16_t b) { return f64x2_to_f16(f16_to_f64(a) * f16_to_f64(b), 0.0); }
+uint16_t po_hadd(uint16_t a, uint16_t b) { return f64x2_to_f16(f16_to_f64(a) + f16_to_f64(b), 0.0); }
+uint16_t po_hsub(uint16_t a, uint16_t b) { return f64x2_to_f16(f16_to_f64(a) - f16_to_f64(b), 0.0); }
+uint16_t po_hfma(uint16_t a, uint16_t b, uint16_t c) {
+    double p = f16_to_f64(a) * f16_to_f64(b), x = f16_to_f64(c);
+    double s = p + x;
+    double bp = s - p;
+    double err = (p - (s - bp)) + (x - bp);
+    if (isinf(s) || isnan(s)) err = 0.0;
+    return f64x2_to_f16(s, err);
+}
+
+void po_half_ops(const uint16_t *a, const uint16_t *b, const uint16_t *c, uint16_t *mul, uint16_t *add,
+                 uint16_t *sub, uint16_t *fma_, size_t n) {
+    for (size_t i = 0; i < n; i++) {
+        mul[i] = po_hmul(a[i], b[i]);
+        add[i] = po_hadd(a[i], b[i]);
+        sub[i] = po_hsub(a[i], b[i]);
+        fma_[i] = po_hfma(a[i], b[i], c[i]);
+    }
+}
+
+/* ------------------------------------------------------------------------------------------------------
+ * SRS filter multiply (channel estimation from sounding reference signals)
+ *   H_est[b][ant][sc][port] = sum_k (W[sc][k] * FOCC[port][k % foccLen]) * SRS[b][ant][k]
+ * srs_wmma_bench.cu srs_filter_v1_scalar :57-90 with complex_conjmul :43-47 and hcmadd :48-52. The
+ * helper named complex_conjmul multiplies without conjugating (its real part is re*re - im*im); conj_w != 0
+ * conjugates W first, the form the operation's comment (:6) states. Complex values are binary16 pairs
+ * {re, im}.
+ * ------------------------------------------------------------------------------------------------------ */
+void po_srs_filter_v1(uint16_t *H_est, const uint16_t *SRS, const uint16_t *W, const uint16_t *FOCC, int nAnt,
+                      int nBlocks, int K, int nPorts, int foccLen, int conj_w) {
+    for (int b = 0; b < nBlocks; b++)
+        for (int ant = 0; ant < nAnt; ant++)
+            for (int sc = 0; sc < K; sc++)
+                for (int p = 0; p < nPorts; p++) {
+                    const uint16_t *srs = SRS + 2 * ((size_t)b * nAnt + ant) * K;
+                    const uint16_t *w = W + 2 * (size_t)sc * K;
+                    const uint16_t *f = FOCC + 2 * (size_t)p * foccLen;
+                    uint16_t er = 0, ei = 0;
+                    for (int k = 0; k < K; k++) {
+                        uint16_t wr = w[2 * k], wi = w[2 * k + 1];
+                        if (conj_w) wi ^= 0x8000u;
+                        uint16_t fr = f[2 * (k % foccLen)], fi = f[2 * (k % foccLen) + 1];
+                        uint16_t pr = po_hsub(po_hmul(wr, fr), po_hmul(wi, fi));
+                        uint16_t pi = po_hadd(po_hmul(wr, fi), po_hmul(wi, fr));
+                        uint16_t sr = srs[2 * k], si = srs[2 * k + 1];
+                        er = po_hfma(pr, sr, po_hfma((uint16_t)(pi ^ 0x8000u), si, er));
+                        ei = po_hfma(pr, si, po_hfma(pi, sr, ei));
+                    }
+                    size_t o = (((size_t)b * nAnt + ant) * K + sc) * nPorts + p;
+                    H_est[2 * o] = er;
+                    H_est[2 * o + 1] = ei;
+                }
+}
+
+/* Same operation in double from the binary16 inputs (no intermediate rounding); prod_f16 != 0 rounds the
+ * W * FOCC products to binary16 as the reference forms them before the sum. */
+void po_srs_filter_f64(double *H_est, const uint16_t *SRS, const uint16_t *W, const uint16_t *FOCC, int nAnt,
+                       int nBlocks, int K, int nPorts, int foccLen, int conj_w, int prod_f16) {
+    for (int b = 0; b < nBlocks; b++)
+        for (int ant = 0; ant < nAnt; ant++)
+            for (int sc = 0; sc < K; sc++)
+                for (int p = 0; p < nPorts; p++) {
+                    const uint16_t *srs = SRS + 2 * ((size_t)b * nAnt + ant) * K;
+                    const uint16_t *w = W + 2 * (size_t)sc * K;
+                    const uint16_t *f = FOCC + 2 * (size_t)p * foccLen;
+                    double er = 0, ei = 0;
+                    for (int k = 0; k < K; k++) {
+                        double wr = f16_to_f64(w[2 * k]), wi = f16_to_f64(w[2 * k + 1]);
+                        if (conj_w) wi = -wi;
+                        double fr = f16_to_f64(f[2 * (k % foccLen)]), fi = f16_to_f64(f[2 * (k % foccLen) + 1]);
+                        double pr = wr * fr - wi * fi, pi = wr * fi + wi * fr;
+                        if (prod_f16) {
+                            uint16_t wr16 = w[2 * k], wi16 = w[2 * k + 1];
+                            if (conj_w) wi16 ^= 0x8000u;
+                            uint16_t fr16 = f[2 * (k % foccLen)], fi16 = f[2 * (k % foccLen) + 1];
+                            pr = f16_to_f64(po_hsub(po_hmul(wr16, fr16), po_hmul(wi16, fi16)));
+                            pi = f16_to_f64(po_hadd(po_hmul(wr16, fi16), po_hmul(wi16, fr16)));
+                        }
+                        double sr = f16_to_f64(srs[2 * k]), si = f16_to_f64(srs[2 * k + 1]);
+                        er += pr * sr - pi * si;
+                        ei += pr * si + pi * sr;
+                    }
+                    size_t o = (((size_t)b * nAnt + ant) * K + sc) * nPorts + p;
+                    H_est[2 * o] = er;
+                    H_est[2 * o + 1] = ei;
+                }
+}
+
+/* Inputs of srs_wmma_bench.cu run_config :262-268: srand(42), then SRS, W and FOCC from one stream, each
+ * complex value {re, im} = two draws of __float2half(((float)rand()/RAND_MAX - 0.5f) * 1.4f). */
+void po_srs_bench_inputs(unsigned seed, int nAnt, int nBlocks, int K, int nPorts, int foccLen, uint16_t *SRS,
+                         uint16_t *W, uint16_t *FOCC) {
+    srand(seed);
+    size_t n[3] = {(size_t)nBlocks * nAnt * K, (size_t)K * K, (size_t)nPorts * foccLen};
+    uint16_t *dst[3] = {SRS, W, FOCC};
+    for (int t = 0; t < 3; t++)
+        for (size_t i = 0; i < 2 * n[t]; i++)
+            dst[t][i] = po_f32_to_f16(((float)rand() / RAND_MAX - 0.5f) * 1.4f);
+}

@@ -54,4 +54,20 @@ int po_coef_apply_demap_f32_omp(const float *C, const float *Y, const float *noi
                                 int nSC, int L, int S, int A, int nPamBits, int nthreads);
 
 int po_omp_max_threads(void);
+
+/* -- binary16 arithmetic, one rounding to nearest even per operation (__hmul/__hadd/__hsub/__hfma) -- */
+uint16_t po_hmul(uint16_t a, uint16_t b);
+uint16_t po_hadd(uint16_t a, uint16_t b);
+uint16_t po_hsub(uint16_t a, uint16_t b);
+uint16_t po_hfma(uint16_t a, uint16_t b, uint16_t c);
+void po_half_ops(const uint16_t *a, const uint16_t *b, const uint16_t *c, uint16_t *mul, uint16_t *add,
+                 uint16_t *sub, uint16_t *fma_, size_t n);
+
+/* -- SRS filter multiply (srs_wmma_bench.cu) -- */
+void po_srs_filter_v1(uint16_t *H_est, const uint16_t *SRS, const uint16_t *W, const uint16_t *FOCC, int nAnt,
+                      int nBlocks, int K, int nPorts, int foccLen, int conj_w);
+void po_srs_filter_f64(double *H_est, const uint16_t *SRS, const uint16_t *W, const uint16_t *FOCC, int nAnt,
+                       int nBlocks, int K, int nPorts, int foccLen, int conj_w, int prod_f16);
+void po_srs_bench_inputs(unsigned seed, int nAnt, int nBlocks, int K, int nPorts, int foccLen, uint16_t *SRS,
+                         uint16_t *W, uint16_t *FOCC);
 #endif

@@ -49,6 +49,10 @@ def lib():
             "po_llr_lut": ([I], D),
             "po_coef_apply_demap_f32_omp": ([P, P, P, P, I, I, I, I, I, I], I),
             "po_omp_max_threads": ([], I),
+            "po_half_ops": ([P, P, P, P, P, P, P, Z], None),
+            "po_srs_filter_v1": ([P, P, P, P, I, I, I, I, I, I], None),
+            "po_srs_filter_f64": ([P, P, P, P, I, I, I, I, I, I, I], None),
+            "po_srs_bench_inputs": ([U, I, I, I, I, I, P, P, P], None),
         }
         for name, (args, res) in sig.items():
             fn = getattr(h, name)
@@ -332,3 +336,57 @@ def chain_inputs(rng, nSC, A, L, S, q, snr_db):
     Y = np.einsum("nal,nls->nsa", H.astype(np.complex128), x)
     Y = (Y + cn(rng, Y.shape, sigma2)).astype(np.complex64)
     return {"H": H, "Y": Y, "bits": bits, "x": x, "sigma2": np.full(nSC, sigma2, dtype=np.float32)}
+
+
+# ---------------------------------------------------------------------------------------------------------
+# binary16 arithmetic and the SRS filter multiply (srs_wmma_bench.cu)
+# ---------------------------------------------------------------------------------------------------------
+def half_ops(a, b, c):
+    """Element-wise binary16 mul, add, sub of a, b and fma(a, b, c), each rounded once to nearest even.
+    Inputs and outputs are uint16 bit patterns."""
+    a, b, c = (np.ascontiguousarray(x, dtype=np.uint16) for x in (a, b, c))
+    outs = [np.empty_like(a) for _ in range(4)]
+    lib().po_half_ops(_p(a), _p(b), _p(c), *[_p(o) for o in outs], a.size)
+    return dict(zip(("mul", "add", "sub", "fma"), outs))
+
+
+def _srs_dims(SRS, W, FOCC):
+    nBlocks, nAnt, K, two = SRS.shape
+    nPorts, foccLen, _ = FOCC.shape
+    assert two == 2 and W.shape == (K, K, 2)
+    return nBlocks, nAnt, K, nPorts, foccLen
+
+
+def srs_filter_v1(SRS, W, FOCC, conj_w=False):
+    """srs_filter_v1_scalar (srs_wmma_bench.cu:57-90) on binary16 bit patterns: SRS [nBlocks, nAnt, K, 2],
+    W [K, K, 2], FOCC [nPorts, foccLen, 2] -> H_est [nBlocks, nAnt, K, nPorts, 2] (uint16)."""
+    SRS, W, FOCC = (np.ascontiguousarray(x, dtype=np.uint16) for x in (SRS, W, FOCC))
+    nBlocks, nAnt, K, nPorts, foccLen = _srs_dims(SRS, W, FOCC)
+    out = np.empty((nBlocks, nAnt, K, nPorts, 2), dtype=np.uint16)
+    lib().po_srs_filter_v1(_p(out), _p(SRS), _p(W), _p(FOCC), nAnt, nBlocks, K, nPorts, foccLen, int(conj_w))
+    return out
+
+
+def srs_filter_f64(SRS, W, FOCC, conj_w=False, prod_f16=False):
+    """The same sum in double from the binary16 inputs -> complex128 [nBlocks, nAnt, K, nPorts]."""
+    SRS, W, FOCC = (np.ascontiguousarray(x, dtype=np.uint16) for x in (SRS, W, FOCC))
+    nBlocks, nAnt, K, nPorts, foccLen = _srs_dims(SRS, W, FOCC)
+    out = np.empty((nBlocks, nAnt, K, nPorts, 2), dtype=np.float64)
+    lib().po_srs_filter_f64(_p(out), _p(SRS), _p(W), _p(FOCC), nAnt, nBlocks, K, nPorts, foccLen, int(conj_w),
+                            int(prod_f16))
+    return out[..., 0] + 1j * out[..., 1]
+
+
+def srs_bench_inputs(seed, nAnt, nBlocks, K, nPorts, foccLen):
+    """srs_wmma_bench.cu:262-268 inputs: (SRS, W, FOCC) as binary16 bit patterns."""
+    SRS = np.empty((nBlocks, nAnt, K, 2), dtype=np.uint16)
+    W = np.empty((K, K, 2), dtype=np.uint16)
+    FOCC = np.empty((nPorts, foccLen, 2), dtype=np.uint16)
+    lib().po_srs_bench_inputs(seed, nAnt, nBlocks, K, nPorts, foccLen, _p(SRS), _p(W), _p(FOCC))
+    return SRS, W, FOCC
+
+
+def f16_complex(bits):
+    """[..., 2] binary16 bit patterns -> complex128."""
+    v = np.ascontiguousarray(bits, dtype=np.uint16).view(np.float16).astype(np.float64)
+    return v[..., 0] + 1j * v[..., 1]

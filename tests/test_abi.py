@@ -19,7 +19,7 @@ def declared_functions():
 def test_header_declares_the_operator_surface():
     names = declared_functions()
     for required in ("pe_coef_apply", "pe_gram_mmse", "pe_llr_demap_f16", "pe_llr_demap", "pe_coef_apply_demap",
-                     "pe_coef_apply_demap_f16", "pe_gram", "pe_coef_apply_v1"):
+                     "pe_coef_apply_demap_f16", "pe_srs_filter", "pe_gram", "pe_coef_apply_v1"):
         assert required in names
 
 
@@ -80,6 +80,17 @@ def test_other_entry_points_validate(pe):
     assert h.pe_coef_apply_demap_f16(p, p, p, p, 4, 8, 11, 64, 5, 1, None) == 1
     assert h.pe_coef_apply_demap_f16(p, p, p, n, 4, 8, 11, 64, 4, 1, None) == 1
     assert h.pe_coef_apply_range(p, p, p, 4, -1, 8, 11, 64, 0, None) == 1
+    # SRS filter: (H, SRS, W, FOCC, nAnt, nBlocks, K, nPorts, foccLen, flags, stream)
+    assert h.pe_srs_filter(n, n, n, n, 64, 0, 24, 4, 8, 0, None) == 0          # empty
+    assert h.pe_srs_filter(p, p, p, p, 64, -1, 24, 4, 8, 0, None) == 1
+    assert h.pe_srs_filter(p, p, p, p, 64, 2, 49, 4, 8, 0, None) == 1         # K > 48
+    assert h.pe_srs_filter(p, p, p, p, 64, 2, 24, 9, 8, 0, None) == 1         # nPorts > 8
+    assert h.pe_srs_filter(p, p, p, p, 64, 2, 24, 4, 0, 0, None) == 1         # foccLen 0
+    assert h.pe_srs_filter(p, p, p, p, 64, 2, 24, 4, 8, 4, None) == 1         # unknown flag
+    assert h.pe_srs_filter(p, p, p, n, 64, 2, 24, 4, 8, 0, None) == 1         # null FOCC
+    a16 = ctypes.c_void_p(256)
+    assert h.pe_srs_filter(a16, a16, a16, a16, 64, 2, 10, 4, 8, 1, None) == 2  # MFMA needs K % 4 == 0
+    assert h.pe_srs_filter(a16, p, a16, a16, 64, 2, 12, 4, 8, 1, None) == 1    # MFMA needs aligned SRS
 
 
 def test_slot_plan_rejects_bad_cells_without_a_device(pe):

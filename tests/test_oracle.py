@@ -240,3 +240,146 @@ def test_metrics_helpers(po):
     ref = np.array([1 + 1j, 2 - 1j])
     r, db = po.evm(ref, ref * 1.001)
     assert r == pytest.approx(0.001) and db == pytest.approx(-60.0)
+
+
+# ------------------------------------------------------------------------------------------------------
+# binary16 arithmetic and the SRS filter multiply
+# ------------------------------------------------------------------------------------------------------
+def _finite_halves(rng, n):
+    """Random finite binary16 bit patterns, a quarter of them subnormal or zero."""
+    h = rng.integers(0, 0x7c00, n, dtype=np.uint32)
+    sub = rng.random(n) < 0.25
+    h[sub] = rng.integers(0, 0x400, int(sub.sum()), dtype=np.uint32)
+    h |= (rng.random(n) < 0.5).astype(np.uint32) << 15
+    return h.astype(np.uint16)
+
+
+def test_half_mul_add_sub_match_numpy(po):
+    """numpy evaluates binary16 operations in float32 and rounds once more; float32 carries 24 >= 2*11 + 2
+    bits, so that second rounding is exact for +, -, * and numpy is an independent correctly rounded model."""
+    rng = np.random.default_rng(0)
+    a, b = _finite_halves(rng, 400000), _finite_halves(rng, 400000)
+    c = np.zeros_like(a)
+    got = po.half_ops(a, b, c)
+    fa, fb = a.view(np.float16), b.view(np.float16)
+    with np.errstate(over="ignore", invalid="ignore"):
+        exp = {"mul": fa * fb, "add": fa + fb, "sub": fa - fb}
+    for k, v in exp.items():
+        ok = np.isfinite(v)
+        assert ok.mean() > 0.9
+        assert np.array_equal(got[k][ok], v.view(np.uint16)[ok]), k
+        assert np.all((got[k][~ok] & 0x7fff) == 0x7c00), k  # overflow to +-inf
+
+
+def _nearest_half_exact(x, table_vals, table_bits):
+    """Brute force: the binary16 value nearest to the exact rational x, ties to the even significand."""
+    from fractions import Fraction
+    ax = abs(x)
+    i = np.searchsorted(table_vals, float(ax))  # float(ax) is within an ulp of double: check neighbours exactly
+    best = None
+    for j in range(max(0, i - 2), min(len(table_vals), i + 3)):
+        d = abs(Fraction(table_vals[j]) - ax)
+        key = (d, table_bits[j] & 1)
+        if best is None or key < best[0]:
+            best = (key, table_bits[j])
+    bits = int(best[1])
+    if x < 0:
+        bits |= 0x8000
+    return bits
+
+
+def test_half_fma_single_rounding_bruteforce(po):
+    """fma(a, b, c) = the binary16 nearest to the exact a*b + c (ties to even), checked by exact rationals
+    against a table of every finite binary16, on random triples and on triples built to land on ties."""
+    from fractions import Fraction
+    pos = np.arange(0, 0x7c00, dtype=np.uint16)
+    vals = pos.view(np.float16).astype(np.float64)
+    rng = np.random.default_rng(1)
+    a, b, c = (_finite_halves(rng, 3000) for _ in range(3))
+    # ties: a*b exactly halfway between two binary16 values, c tiny (the rounding direction is c's sign)
+    ta = np.array([0x4200, 0x3c00 | 0x155, 0x4600, 0x3e00], dtype=np.uint16)  # 3, ~1.33, 6, 1.5
+    tb = np.array([0x3c03, 0x3c01, 0x3c01, 0x3c01], dtype=np.uint16)
+    for s in (0x0001, 0x8001, 0x0000, 0x8000):
+        a = np.concatenate([a, ta]); b = np.concatenate([b, tb]); c = np.concatenate([c, np.full(4, s, np.uint16)])
+    got = po.half_ops(a, b, c)["fma"]
+    fa, fb, fc = (x.view(np.float16).astype(np.float64) for x in (a, b, c))
+    checked = 0
+    for i in range(a.size):
+        x = Fraction(float(fa[i])) * Fraction(float(fb[i])) + Fraction(float(fc[i]))
+        if abs(x) >= 65520:
+            assert got[i] & 0x7fff == 0x7c00
+            continue
+        if x == 0:
+            continue  # signed zeros: checked below
+        assert got[i] == _nearest_half_exact(x, vals, pos), (hex(a[i]), hex(b[i]), hex(c[i]))
+        checked += 1
+    assert checked > 2500
+    # signed zeros follow IEEE: (+0)*x + (-0) = +0, (-0)*x + (-0) = -0, x - x = +0
+    z = po.half_ops(np.array([0x0000, 0x8000], np.uint16), np.array([0x3c00, 0x3c00], np.uint16),
+                    np.array([0x8000, 0x8000], np.uint16))
+    assert list(z["fma"]) == [0x0000, 0x8000]
+    assert po.half_ops(np.array([0x4500], np.uint16), np.array([0x4500], np.uint16),
+                       np.array([0], np.uint16))["sub"][0] == 0
+
+
+def _srs_case(rng, nBlocks, nAnt, K, P, FL):
+    to16 = lambda x: np.stack([x.real, x.imag], -1).astype(np.float16).view(np.uint16)
+    S = to16((rng.uniform(-0.7, 0.7, (nBlocks, nAnt, K)) + 1j * rng.uniform(-0.7, 0.7, (nBlocks, nAnt, K))))
+    W = to16(rng.uniform(-0.7, 0.7, (K, K)) + 1j * rng.uniform(-0.7, 0.7, (K, K)))
+    F = to16(rng.uniform(-0.7, 0.7, (P, FL)) + 1j * rng.uniform(-0.7, 0.7, (P, FL)))
+    return S, W, F
+
+
+def test_srs_oracle_identity_filter_returns_srs(po):
+    """W = I and FOCC in {1, -1, j, -j}: every product is exact, so H_est[b][a][sc][p] = FOCC[p][sc % FL] *
+    SRS[b][a][sc] bit for bit -- pins the subcarrier, port and FOCC indexing and the output layout."""
+    rng = np.random.default_rng(2)
+    nB, nA, K, P, FL = 3, 5, 12, 3, 8
+    S, _, _ = _srs_case(rng, nB, nA, K, P, FL)
+    one = np.float16(1).view(np.uint16)
+    W = np.zeros((K, K, 2), np.uint16)
+    W[np.arange(K), np.arange(K), 0] = one
+    units = [(1, 0), (-1, 0), (0, 1), (0, -1)]
+    fv = np.array([units[(3 * p + j) % 4] for p in range(P) for j in range(FL)], np.float16).reshape(P, FL, 2)
+    F = fv.view(np.uint16)
+    H = po.f16_complex(po.srs_filter_v1(S, W, F))
+    s = po.f16_complex(S)
+    f = fv[..., 0].astype(np.float64) + 1j * fv[..., 1].astype(np.float64)
+    exp = s[:, :, :, None] * f.T[np.arange(K) % FL][None, None, :, :]
+    assert np.array_equal(H, exp)
+    # a conjugated W changes nothing here (W real); a j on the diagonal turns into -j with conj_w
+    W[np.arange(K), np.arange(K), 0] = 0
+    W[np.arange(K), np.arange(K), 1] = one
+    Hj = po.f16_complex(po.srs_filter_v1(S, W, F))
+    Hc = po.f16_complex(po.srs_filter_v1(S, W, F, conj_w=True))
+    assert np.array_equal(Hj, 1j * exp) and np.array_equal(Hc, -1j * exp)
+
+
+@pytest.mark.parametrize("K,P,FL", [(24, 4, 8), (12, 2, 12), (7, 3, 5)])
+def test_srs_oracle_close_to_exact_sum(po, K, P, FL):
+    """The binary16 chain stays within its rounding bound of the exact sum (built from the same binary16
+    taps) and far from the conjugated-W or unrounded-tap alternatives only where those differ."""
+    rng = np.random.default_rng(K + P)
+    S, W, F = _srs_case(rng, 4, 16, K, P, FL)
+    H = po.f16_complex(po.srs_filter_v1(S, W, F))
+    X = po.srs_filter_f64(S, W, F, prod_f16=True)
+    rel = np.linalg.norm(H - X) / np.linalg.norm(X)
+    assert rel < 2e-3
+    # per element, against the exact sum: a rounding-error bound (2K roundings of at most 2^-11 relative to
+    # the running sums, which stay below sum|s| max|wf|)
+    bound = (2 * K + 2) * 2.0 ** -11 * np.abs(po.f16_complex(S)).sum(-1).max() * 1.0
+    assert np.abs(H - X).max() < bound
+    Xc = po.srs_filter_f64(S, W, F, conj_w=True, prod_f16=True)
+    assert np.linalg.norm(H - Xc) / np.linalg.norm(Xc) > 0.5
+    Hc = po.f16_complex(po.srs_filter_v1(S, W, F, conj_w=True))
+    assert np.linalg.norm(Hc - Xc) / np.linalg.norm(Xc) < 2e-3
+
+
+def test_srs_bench_inputs_follow_the_stream(po):
+    """srand(42) stream order: all SRS values, then W, then FOCC, {re, im} per value, scaled by 1.4."""
+    S, W, F = po.srs_bench_inputs(42, 4, 2, 12, 2, 12)
+    u = po.glibc_uniform(42, 2 * (S.size // 2 + W.size // 2 + F.size // 2), scale=1.4)
+    bits = po.f32_to_f16_bits(u)
+    n1, n2 = S.size, S.size + W.size
+    assert np.array_equal(S.ravel(), bits[:n1]) and np.array_equal(W.ravel(), bits[n1:n2])
+    assert np.array_equal(F.ravel(), bits[n2:])
