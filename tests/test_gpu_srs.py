@@ -56,16 +56,24 @@ def test_srs_f16_bitexact_ragged(po, pe, dev, nB, nA, K, P, FL, conj):
                                           (2, 100, 48, 8, 48), (4, 130, 16, 3, 5), (2, 64, 32, 5, 8)])
 @pytest.mark.parametrize("conj", [False, True])
 def test_srs_mfma_within_one_ulp_of_exact(po, pe, dev, nB, nA, K, P, FL, conj):
-    """FP32 accumulation of exact binary16 products, one rounding: within 1 binary16 ulp of the exact sum
-    (built from the same binary16 taps), element by element; tighter than the binary16 chain."""
+    """FP32 accumulation of exact binary16 products, one rounding: the binary16 rounding of the exact sum
+    (built from the same binary16 taps) for almost every element; every element within one binary16 ulp plus
+    the FP32 accumulation bound 2K * 2^-23 * sum|terms| (which matters only where the sum cancels); and
+    closer to the exact sum than the binary16 chain."""
     rng = np.random.default_rng(7 * K + P)
     S, W, F = rand_case(rng, nB, nA, K, P, FL)
     exact = po.srs_filter_f64(S, W, F, conj_w=conj, prod_f16=True)
     out = to_bits(pe.srs_filter(to_dev(S, dev), to_dev(W, dev), to_dev(F, dev), mfma=True, conj_w=conj))
     rn = np.stack([exact.real, exact.imag], -1).astype(np.float16).view(np.uint16)
     d = np.abs(ordered(out) - ordered(rn))
-    assert d.max() <= 1
-    assert (d == 0).mean() > 0.99
+    assert (d == 0).mean() > 0.99 and (d <= 1).mean() > 0.9999
+    s = po.f16_complex(S)
+    wf_max = np.abs(po.f16_complex(W)).max() * np.abs(po.f16_complex(F)).max() * (1 + 2.0 ** -9)
+    terms = 2 * np.abs(s).sum(-1) * wf_max  # >= sum of |products| in each real output
+    ex = np.stack([exact.real, exact.imag], -1)
+    ulp = np.spacing(np.abs(ex).astype(np.float16)).astype(np.float64)
+    err = np.abs(out.view(np.float16).astype(np.float64) - ex)
+    assert np.all(err <= ulp + 2 * K * 2.0 ** -23 * terms[:, :, None, None, None])
     chain = po.f16_complex(po.srs_filter_v1(S, W, F, conj_w=conj))
     got = po.f16_complex(out)
     assert np.linalg.norm(got - exact) <= np.linalg.norm(chain - exact)
