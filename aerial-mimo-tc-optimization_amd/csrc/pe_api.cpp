@@ -388,4 +388,14 @@ pe_status pe_srs_filter(pe_c16 *H_est, const pe_c16 *SRS, const pe_c16 *W, const
     return from_hip(e, "srs_filter launch");
 }
 
+pe_status pe_hbm_probe(const void *src, void *dst, size_t bytes, int read_only, int grid, void *stream) {
+    if (bytes % 16 != 0) return fail(PE_ERR_INVALID, "bytes = %zu is not a multiple of 16", bytes);
+    if (grid < 1 || grid > (1 << 20)) return fail(PE_ERR_INVALID, "grid = %d outside [1, 2^20]", grid);
+    if (bytes == 0) return ok();
+    if (!src || !dst) return fail(PE_ERR_INVALID, "null pointer");
+    if ((reinterpret_cast<uintptr_t>(src) | reinterpret_cast<uintptr_t>(dst)) & 15u)
+        return fail(PE_ERR_INVALID, "src and dst must be 16-byte aligned");
+    return from_hip(pe::launch_hbm_probe(src, dst, bytes, read_only, grid, (hipStream_t)stream), "hbm_probe launch");
+}
+
 }  // extern "C"

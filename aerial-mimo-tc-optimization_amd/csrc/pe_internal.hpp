@@ -63,6 +63,8 @@ hipError_t launch_llr_f32(const float2 *Yeq, const float *noise_inv, float *llr,
 // SRS filter multiply (binary16 pairs as uint32): the binary16 chain, or MFMA with FP32 accumulation.
 hipError_t launch_srs_filter(uint32_t *H, const uint32_t *SRS, const uint32_t *W, const uint32_t *FOCC, int nAnt,
                              int nBlocks, int K, int nPorts, int foccLen, bool conj, bool mfma, hipStream_t st);
+// Streaming probes (HBM ceiling): read_only -> reads src, else copies src to dst.
+hipError_t launch_hbm_probe(const void *src, void *dst, size_t bytes, int read_only, int grid, hipStream_t st);
 // MMSE: FP32 pass (flags subcarriers for FP64) then the FP64 pass over flagged subcarriers.
 hipError_t launch_gram_mmse(const float2 *H, const float *sigma2, float2 *C, float *noise_inv, uint8_t *flags,
                             int nSC, int L, int A, bool unbiased, float cond_threshold, hipStream_t st);

@@ -44,6 +44,7 @@ SIGNATURES = {
     "pe_coef_apply_demap": ([_P, _P, _P, _P, _I, _I, _I, _I, _I, _I, _P], _I),
     "pe_coef_apply_demap_f16": ([_P, _P, _P, _P, _I, _I, _I, _I, _I, _I, _P], _I),
     "pe_srs_filter": ([_P, _P, _P, _P, _I, _I, _I, _I, _I, _U, _P], _I),
+    "pe_hbm_probe": ([_P, _P, ctypes.c_size_t, _I, _I, _P], _I),
     "pe_slot_plan_create": ([_P, _I, _U, _P], _I),
     "pe_slot_plan_update": ([_P, _P, _I, _P], _I),
     "pe_slot_plan_launch": ([_P, _P], _I),
@@ -229,6 +230,12 @@ def srs_filter(SRS, W, FOCC, mfma=False, conj_w=False, out=None, stream=None):
     _check(lib().pe_srs_filter(_ptr(out), _ptr(SRS), _ptr(W), _ptr(FOCC), nAnt, nBlocks, K, nPorts, foccLen, flags,
                                _stream(stream)))
     return out
+
+
+def hbm_probe(src, dst, read_only=False, grid=4096, nbytes=None, stream=None):
+    """Streaming probe (pe_hbm_probe): copy src -> dst, or read src (dst: at least `grid` words)."""
+    n = src.numel() * src.element_size() if nbytes is None else nbytes
+    _check(lib().pe_hbm_probe(_ptr(src), _ptr(dst), n, 1 if read_only else 0, grid, _stream(stream)))
 
 
 def cell_desc(C, noise_inv, Y, llr, n_pam_bits, prec="f16", H=None, sigma2=None):

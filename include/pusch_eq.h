@@ -175,6 +175,14 @@ pe_status pe_coef_apply_demap_f16(const pe_c64 *C, const pe_c64 *Y_rx, const flo
 pe_status pe_srs_filter(pe_c16 *H_est, const pe_c16 *SRS, const pe_c16 *W, const pe_c16 *FOCC, int nAnt,
                         int nBlocks, int K, int nPorts, int foccLen, unsigned flags, void *stream);
 
+/* ---- HBM streaming probe ---------------------------------------------------------------------------- */
+
+/* No reference counterpart: measures the device's streaming ceiling, the yardstick for the bandwidth-bound
+   kernels above. read_only = 0: copies `bytes` from src to dst (read + write traffic); read_only != 0: reads
+   `bytes` of src and may write to dst[0..grid) (4-byte words). 16-byte nontemporal accesses over `grid`
+   workgroups of 256 lanes; bytes % 16 == 0, src and dst 16-byte aligned. */
+pe_status pe_hbm_probe(const void *src, void *dst, size_t bytes, int read_only, int grid, void *stream);
+
 /* ---- multi-cell slot: one plan per set of uneven cells, run by tile kernels ------------------------ */
 
 typedef struct {

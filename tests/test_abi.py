@@ -88,6 +88,10 @@ def test_other_entry_points_validate(pe):
     assert h.pe_srs_filter(p, p, p, p, 64, 2, 24, 4, 0, 0, None) == 1         # foccLen 0
     assert h.pe_srs_filter(p, p, p, p, 64, 2, 24, 4, 8, 4, None) == 1         # unknown flag
     assert h.pe_srs_filter(p, p, p, n, 64, 2, 24, 4, 8, 0, None) == 1         # null FOCC
+    assert h.pe_hbm_probe(p, p, 17, 0, 64, None) == 1                         # bytes % 16
+    assert h.pe_hbm_probe(p, p, 32, 0, 0, None) == 1                          # grid
+    assert h.pe_hbm_probe(n, n, 0, 0, 64, None) == 0                          # empty
+    assert h.pe_hbm_probe(p, ctypes.c_void_p(256), 32, 0, 64, None) == 1      # unaligned src
     a16 = ctypes.c_void_p(256)
     assert h.pe_srs_filter(a16, a16, a16, a16, 64, 2, 10, 4, 8, 1, None) == 2  # MFMA needs K % 4 == 0
     assert h.pe_srs_filter(a16, p, a16, a16, 64, 2, 12, 4, 8, 1, None) == 1    # MFMA needs aligned SRS

@@ -251,3 +251,18 @@ def test_mmse_singular_is_flagged(po, pe, dev):
     C, ni, fl = pe.gram_mmse(dev_c64(H, dev), dev_f32(np.zeros(16, np.float32), dev), unbiased=True)
     assert np.all(host(fl) == pe.SCFLAG_FP64 | pe.SCFLAG_SINGULAR)
     assert np.all(host(C) == 0) and np.all(host(ni) == 0)
+
+
+# ------------------------------------------------------------------------------------------------------
+# HBM probe (the measured-ceiling yardstick)
+# ------------------------------------------------------------------------------------------------------
+@pytest.mark.parametrize("nbytes,grid", [(16, 1), (4096 + 48, 3), (1 << 22, 1024)])
+def test_hbm_probe_copy_is_a_copy(pe, dev, nbytes, grid):
+    import torch
+    a = torch.randint(-2 ** 31, 2 ** 31 - 1, (nbytes // 4,), dtype=torch.int32, device=dev)
+    b = torch.zeros_like(a)
+    pe.hbm_probe(a, b, grid=grid)
+    sink = torch.zeros(grid, dtype=torch.int32, device=dev)
+    pe.hbm_probe(a, sink, read_only=True, grid=grid)
+    torch.cuda.synchronize()
+    assert torch.equal(a, b)
