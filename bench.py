@@ -178,15 +178,17 @@ def hbm_ceiling(dev):
 
 def draw_cells(n, seed, dev, prec="f16"):
     """Cells drawn as in the multi-cell configurations: PRB in {24,52,106,273}, antennas in {16,32,64}, layers in
-    {2,4,8}, QAM in {16,64,256}, SNR uniform in 5-25 dB, Rayleigh CN(0,1/A) channels, 11 data symbols."""
+    {2,4,8}, QAM in {16,64,256}, SNR uniform in 5-25 dB, Rayleigh CN(0,1/A) channels, 11 data symbols. Cell c
+    is drawn from its own seed, seed + 1000 c (the reference's per-cell convention, 42 + 1000 c,
+    multicell_evidence.cu:110-132)."""
     import numpy as np
     import torch
     import pusch_eq as pe
-    rng = np.random.default_rng(seed)
-    g = torch.Generator(device=dev)
-    g.manual_seed(seed)
     cells = []
-    for _ in range(n):
+    for ci in range(n):
+        rng = np.random.default_rng(seed + 1000 * ci)
+        g = torch.Generator(device=dev)
+        g.manual_seed(seed + 1000 * ci)
         prb = int(rng.choice([24, 52, 106, 273]))
         a = int(rng.choice([16, 32, 64]))
         l = int(rng.choice([2, 4, 8]))
@@ -225,7 +227,7 @@ def multicell_bench(dev, steps):
     Also the latency of one slot on an idle GPU (events around the call, so host submission is included)."""
     import torch
     import pusch_eq as pe
-    sets = [draw_cells(24, 77 + k, dev) for k in range(2)]
+    sets = [draw_cells(24, 42 + k, dev) for k in range(2)]
     plans = [pe.SlotPlan([c["desc"] for c in cs]) for cs in sets]
     res = {}
     for mode in ("graph_launch", "launch"):
@@ -250,13 +252,17 @@ def multicell_bench(dev, steps):
         lat.sort()
         res[mode + "_latency"] = lat[len(lat) // 2]
     t = res["graph_launch"]
-    nsc = sum(c["nsc"] for c in sets[0])
-    by = slot_bytes(sets[0])
+    # the two plans alternate: per-slot figures are their means
+    nscs = [sum(c["nsc"] for c in cs) for cs in sets]
+    bys = [slot_bytes(cs) for cs in sets]
+    nsc, by = sum(nscs) / 2, sum(bys) / 2
     tiles, wg = plans[0].info()
     for p in plans:
         p.close()
-    return {"workload": "24 cells, PRB {24,52,106,273}, antennas {16,32,64}, layers {2,4,8}, QAM {16,64,256}, "
-                        "f16; one slot = MMSE + fused apply/demap, back-to-back slots",
+    return {"workload": "24 cells per slot, PRB {24,52,106,273}, antennas {16,32,64}, layers {2,4,8}, QAM "
+                        "{16,64,256}, f16, cell c of a plan drawn from seed (42 or 43) + 1000 c; two plans "
+                        "alternate; one slot = MMSE + fused apply/demap, back-to-back slots",
+            "subcarriers_per_plan": nscs, "bytes_per_plan": bys,
             "subcarriers_per_slot": nsc, "slot_ms": t * 1e3, "subcarriers_per_s": nsc / t,
             "hbm_gbs": by / t / 1e9, "hbm_frac": by / t / 1e9 / HBM_PEAK_GBS, "bytes_per_slot": by,
             "slot_ms_eager": res["launch"] * 1e3, "hbm_gbs_eager": by / res["launch"] / 1e9,
@@ -326,7 +332,7 @@ def serving_bench(dev, n_jobs):
     process; capacity = throughput with every job queued at t = 0; then offered load 10-120% of capacity."""
     import torch
     import pusch_eq as pe
-    cells = draw_cells(64, 91, dev)
+    cells = draw_cells(64, 42, dev)
     descs = [pe.job_desc(c["H"], c["s2"], c["Y"], c["q"], "f16") for c in cells]
     kw = dict(deadline_s=0.5e-3, hbm_budget=2 << 30, max_batch=64, max_inflight=3, shed_late=False,
               use_graph=False)
