@@ -10,6 +10,14 @@
 //      noise_inv, LLR) fit the device arena (the HBM budget), up to max_batch_jobs; the batch runs as one
 //      launch of that stream's slot executor (MMSE, FP64 pass, fused apply + demap; eagerly with grids
 //      sized to the batch, or as a replay of its captured graph).
+// Shedding (shed_late): 1 drops a job whose deadline has passed before launch; 2 (deadline-aware admission)
+// drops a job whose predicted completion -- its age plus the predicted latency of the batch it would join --
+// is past its deadline. The prediction is an online model of batch latency (launch to observed completion):
+// the isolated one-job latency measured at warm-up plus a per-subcarrier cost, a running average over the
+// completed batches. Under overload this serves the jobs that can still make their deadline instead of
+// serving every job late.
+// The report carries the reference's online contention indicator (pmu_contention.cu:230-245): the mean batch
+// latency of the first 5 batches divided by the isolated one-job latency.
 // Jobs that arrive while batches are on the GPU are admitted as soon as a stream and memory free up, so the
 // batches overlap on the device. Each job's channel estimate, MMSE coefficients and samples stay
 // device-resident; the coefficients are computed once per job and applied to all of its data symbols.
@@ -77,6 +85,22 @@ struct JobBuf {
     size_t off = 0, bytes = 0;
 };
 
+// Online batch-latency model: latency(n_sc) = iso + rate * max(0, n_sc - iso_sc); rate is a running average
+// (weight 1/8) of the excess latency per subcarrier of completed batches.
+struct LatencyModel {
+    double iso = 0.0;   // one-job batch latency at warm-up (seconds)
+    double iso_sc = 0;  // its subcarriers
+    double rate = 0.0;  // seconds per subcarrier beyond iso_sc
+    int samples = 0;
+    double predict(double n_sc) const { return iso + rate * std::max(0.0, n_sc - iso_sc); }
+    void observe(double latency, double n_sc) {
+        if (n_sc <= iso_sc) return;
+        const double r = std::max(0.0, latency - iso) / (n_sc - iso_sc);
+        rate = samples == 0 ? r : rate + (r - rate) / 8.0;
+        ++samples;
+    }
+};
+
 size_t job_bytes(const pe_cell_desc &c, bool llr_in_arena) {
     const size_t nsc = (size_t)c.nSC;
     size_t b = ((nsc * c.nLayer * c.nAnt * 8 + 255) & ~size_t(255)) + ((nsc * c.nLayer * 4 + 255) & ~size_t(255));
@@ -103,6 +127,7 @@ extern "C" pe_status pe_serve_run(const pe_serve_config *cfg, pe_serve_report *r
     if (cfg->max_batch_jobs < 1 || cfg->max_inflight < 1 || cfg->max_inflight > 8)
         return serve_fail(PE_ERR_INVALID, "max_batch_jobs >= 1 and 1 <= max_inflight <= 8 required");
     if (!(cfg->deadline_s > 0.0)) return serve_fail(PE_ERR_INVALID, "deadline_s must be positive");
+    if (cfg->shed_late < 0 || cfg->shed_late > 2) return serve_fail(PE_ERR_INVALID, "shed_late must be 0, 1 or 2");
     const bool llr_in_arena = cfg->job_llr == nullptr;
     std::vector<pe_cell_desc> pop(cfg->cells, cfg->cells + cfg->n_cells);
     size_t largest = 0;
@@ -193,7 +218,8 @@ extern "C" pe_status pe_serve_run(const pe_serve_config *cfg, pe_serve_report *r
     };
 
     // warm-up: one batch of the first job's cell on every stream, as a graph (captured here) and eagerly; not
-    // measured
+    // measured. Then the isolated latency of one job (eager, idle GPU; median of 5) seeds the latency model.
+    LatencyModel model;
     {
         JobBuf b;
         if (!arena.alloc(job_bytes(pop[cellof[0]], llr_in_arena), b.off)) {
@@ -213,12 +239,29 @@ extern "C" pe_status pe_serve_run(const pe_serve_config *cfg, pe_serve_report *r
                 return serve_fail(s, err);
             }
         }
+        std::vector<double> iso;
+        for (int r = 0; r < 5; ++r) {
+            const auto a = clock::now();
+            pe_status s = submit(0, one, false);
+            while (s == PE_OK && hipEventQuery(ev[0]) == hipErrorNotReady) {
+            }
+            if (s == PE_OK && hipEventSynchronize(ev[0]) != hipSuccess) s = PE_ERR_DEVICE;
+            if (s != PE_OK) {
+                cleanup();
+                return serve_fail(s, err);
+            }
+            iso.push_back(std::chrono::duration<double>(clock::now() - a).count());
+        }
+        model.iso = nearest_rank(iso, 50.0);
+        model.iso_sc = pop[cellof[0]].nSC;
         arena.release(b.off, b.bytes);
     }
 
     // ---- loop ----------------------------------------------------------------------------------------------
     std::vector<double> lat(nj, -1.0);
     std::vector<std::vector<int>> batch_jobs(K);
+    std::vector<double> batch_t0(K, 0.0), batch_sc(K, 0.0);
+    std::vector<double> first_batches;  // latencies of the first 5 batches (contention indicator)
     std::vector<bool> busy(K, false);
     std::deque<int> ready;
     int next = 0, finished = 0, shed = 0, batches = 0, max_inflight_seen = 0;
@@ -243,6 +286,8 @@ extern "C" pe_status pe_serve_run(const pe_serve_config *cfg, pe_serve_report *r
                 break;
             }
             const double t = now_s();
+            model.observe(t - batch_t0[k], batch_sc[k]);
+            if (first_batches.size() < 5) first_batches.push_back(t - batch_t0[k]);
             for (int j : batch_jobs[k]) {
                 lat[j] = t - arrival[j];
                 arena.release(buf[j].off, buf[j].bytes);
@@ -259,9 +304,14 @@ extern "C" pe_status pe_serve_run(const pe_serve_config *cfg, pe_serve_report *r
         for (int k = 0; k < K && !ready.empty(); ++k) {
             if (busy[k]) continue;
             cells.clear();
+            double sc_batch = 0.0;
             while (!ready.empty() && (int)cells.size() < cfg->max_batch_jobs) {
                 const int j = ready.front();
-                if (cfg->shed_late && now - arrival[j] > cfg->deadline_s) {
+                const double nsc_j = pop[cellof[j]].nSC;
+                const bool late = cfg->shed_late == 1 && now - arrival[j] > cfg->deadline_s;
+                const bool predicted_late =
+                    cfg->shed_late == 2 && now - arrival[j] + model.predict(sc_batch + nsc_j) > cfg->deadline_s;
+                if (late || predicted_late) {
                     ready.pop_front();
                     ++shed;
                     continue;
@@ -273,8 +323,14 @@ extern "C" pe_status pe_serve_run(const pe_serve_config *cfg, pe_serve_report *r
                 ready.pop_front();
                 cells.push_back(make_desc(j, b));
                 batch_jobs[k].push_back(j);
+                sc_batch += nsc_j;
             }
-            if (cells.empty()) break;
+            if (cells.empty()) {
+                if (ready.empty()) continue;  // everything at the head was shed: the next stream may take more
+                break;                        // budget exhausted
+            }
+            batch_t0[k] = now_s();
+            batch_sc[k] = sc_batch;
             status = submit(k, cells, false);
             if (status != PE_OK) break;
             busy[k] = true;
@@ -319,6 +375,11 @@ extern "C" pe_status pe_serve_run(const pe_serve_config *cfg, pe_serve_report *r
     rep->mean_batch_jobs = batches ? (double)batched_jobs / batches : 0.0;
     rep->max_inflight_seen = max_inflight_seen;
     rep->offered_rate_hz = cfg->arrival_rate_hz;
+    rep->iso_job_s = model.iso;
+    double fb = 0.0;
+    for (double v : first_batches) fb += v;
+    rep->contention_index = first_batches.empty() || model.iso <= 0 ? 0.0 : fb / first_batches.size() / model.iso;
+    rep->model_s_per_sc = model.rate;
     pe::set_error("");
     return PE_OK;
 }

@@ -304,21 +304,23 @@ class ServeReport(ctypes.Structure):
                 ("p99_s", ctypes.c_double), ("max_s", ctypes.c_double), ("mean_s", ctypes.c_double),
                 ("miss_rate", ctypes.c_double), ("jobs_per_s", ctypes.c_double),
                 ("subcarriers_per_s", ctypes.c_double), ("mean_batch_jobs", ctypes.c_double),
-                ("offered_rate_hz", ctypes.c_double)]
+                ("offered_rate_hz", ctypes.c_double), ("iso_job_s", ctypes.c_double),
+                ("contention_index", ctypes.c_double), ("model_s_per_sc", ctypes.c_double)]
 
 
 def serve_run(cells, n_jobs, rate_hz, deadline_s=0.5e-3, hbm_budget=1 << 30, max_batch=64, max_inflight=2,
               shed_late=False, use_graph=True, seed=0, job_llr=None):
     """Runs the slot serving loop (pe_serve_run). cells: list of CellDesc with H, sigma2, Y_rx set.
-    job_llr: optional list of n_jobs torch tensors receiving each job's LLRs. Returns (report dict, per-job
-    latencies in seconds (-1 = shed), per-job cell index)."""
+    job_llr: optional list of n_jobs torch tensors receiving each job's LLRs. shed_late: False/0 (serve every
+    job), True/1 (drop jobs already late at launch) or 2 (deadline-aware admission). Returns (report dict,
+    per-job latencies in seconds (-1 = shed), per-job cell index)."""
     import numpy as np
     arr = (CellDesc * len(cells))(*cells)
     ptrs = None
     if job_llr is not None:
         ptrs = (ctypes.c_void_p * n_jobs)(*[t.data_ptr() for t in job_llr])
     cfg = ServeConfig(ctypes.cast(arr, _P), len(cells), n_jobs, float(rate_hz), float(deadline_s), int(hbm_budget),
-                      max_batch, max_inflight, 1 if shed_late else 0, 1 if use_graph else 0, seed,
+                      max_batch, max_inflight, int(shed_late), 1 if use_graph else 0, seed,
                       ctypes.cast(ptrs, _P) if ptrs is not None else None)
     rep = ServeReport()
     lat = np.zeros(n_jobs, dtype=np.float64)

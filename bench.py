@@ -344,6 +344,14 @@ def serving_bench(dev, n_jobs):
         sweep.append({"load": f, "offered_jobs_per_s": f * capacity, "jobs_per_s": rep["jobs_per_s"],
                       "p50_ms": rep["p50_s"] * 1e3, "p99_ms": rep["p99_s"] * 1e3, "max_ms": rep["max_s"] * 1e3,
                       "miss_rate": rep["miss_rate"], "mean_batch_jobs": rep["mean_batch_jobs"]})
+    # deadline-aware admission (shed jobs predicted to finish late) at and beyond capacity
+    admission = []
+    for f in (0.9, 1.0, 1.1, 1.2, 1.5):
+        rep, _, _ = pe.serve_run(descs, n_jobs, f * capacity, seed=2, **dict(kw, shed_late=2))
+        admission.append({"load": f, "jobs_per_s": rep["jobs_per_s"], "p50_ms": rep["p50_s"] * 1e3,
+                          "p99_ms": rep["p99_s"] * 1e3, "max_ms": rep["max_s"] * 1e3,
+                          "miss_rate": rep["miss_rate"], "shed": rep["n_shed"],
+                          "mean_batch_jobs": rep["mean_batch_jobs"]})
     # plain FP32 path, one job at a time (per-cell launches: MMSE then fused FP32 apply/demap)
     torch.cuda.synchronize()
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
@@ -358,7 +366,9 @@ def serving_bench(dev, n_jobs):
                         "batching (<= 64 jobs per batch, 3 streams, 2 GiB budget, eager launches), f16",
             "jobs": n_jobs, "capacity_jobs_per_s": capacity,
             "capacity_subcarriers_per_s": cap["subcarriers_per_s"], "capacity_p99_ms": cap["p99_s"] * 1e3,
-            "load_sweep": sweep, "plain_f32_sequential_jobs_per_s": f32_jobs_per_s}
+            "load_sweep": sweep, "deadline_aware_admission": admission,
+            "iso_job_ms": cap["iso_job_s"] * 1e3, "contention_index_first5": cap["contention_index"],
+            "plain_f32_sequential_jobs_per_s": f32_jobs_per_s}
 
 
 SWEEP_PRECS = ("f32", "bf16", "bf16_split", "f16", "f16_split")

@@ -235,7 +235,9 @@ typedef struct {
     size_t hbm_budget_bytes;   /* arena for in-flight job buffers (C, noise_inv and, without job_llr, LLRs) */
     int max_batch_jobs;        /* jobs per batch launch */
     int max_inflight;          /* batches in flight, one stream each, 1..8 */
-    int shed_late;             /* nonzero: drop jobs whose deadline passed before launch (counted as misses) */
+    int shed_late;             /* 0: serve every job; 1: drop jobs whose deadline passed before launch; 2:
+                                  deadline-aware admission, drop jobs whose predicted completion (age + predicted
+                                  batch latency, online model) is past the deadline. Dropped jobs count as misses. */
     int use_graph;             /* nonzero: each stream replays one captured slot graph */
     unsigned seed;             /* arrival process and cell draws */
     float *const *job_llr;     /* optional [n_jobs]: LLR destination of each job (sized for its cell) */
@@ -247,6 +249,10 @@ typedef struct {
     double p50_s, p99_s, max_s, mean_s;   /* latency (completion - arrival) of served jobs; nearest rank */
     double miss_rate;          /* (served late + shed) / n_jobs */
     double jobs_per_s, subcarriers_per_s, mean_batch_jobs, offered_rate_hz;
+    double iso_job_s;          /* latency of one job alone on an idle GPU (warm-up, median of 5) */
+    double contention_index;   /* mean latency of the first 5 batches / iso_job_s: the online contention
+                                  indicator of pmu_contention.cu:230-245 */
+    double model_s_per_sc;     /* final per-subcarrier cost of the batch-latency model */
 } pe_serve_report;
 
 /* Runs the serving loop to completion on the current device (blocking). Allocates its arena, streams,

@@ -121,9 +121,9 @@ def test_serve_rejects_bad_config_without_a_device(pe):
     arr = (pe.CellDesc * 1)(good)
     rep = pe.ServeReport()
 
-    def run(cells=arr, n=1, jobs=10, budget=1 << 30, batch=8, inflight=2, deadline=5e-4):
+    def run(cells=arr, n=1, jobs=10, budget=1 << 30, batch=8, inflight=2, deadline=5e-4, shed=0):
         cfg = pe.ServeConfig(ctypes.cast(cells, ctypes.c_void_p) if cells is not None else None, n, jobs, 1000.0,
-                             deadline, budget, batch, inflight, 0, 1, 1, None)
+                             deadline, budget, batch, inflight, shed, 1, 1, None)
         return h.pe_serve_run(ctypes.byref(cfg), ctypes.byref(rep), None, None)
 
     assert h.pe_serve_run(None, ctypes.byref(rep), None, None) == 1
@@ -131,6 +131,7 @@ def test_serve_rejects_bad_config_without_a_device(pe):
     assert run(jobs=0) == 1
     assert run(inflight=9) == 1
     assert run(deadline=0.0) == 1
+    assert run(shed=3) == 1 and run(shed=-1) == 1
     assert run(budget=1000) == 5          # PE_ERR_CAPACITY: one job does not fit
     assert run(cells=(pe.CellDesc * 1)(no_h)) == 1
 

@@ -55,3 +55,33 @@ def test_serve_budget_limits_inflight_jobs(po, pe, dev):
                                   max_inflight=2, seed=4)
     print(rep)
     assert rep["n_done"] == 30 and np.all(lat > 0)
+
+
+def test_serve_deadline_aware_admission_under_overload(po, pe, dev):
+    """Offered load far above capacity: deadline-aware admission (shed_late=2) sheds jobs that would finish
+    late, so the served jobs' latency stays near the deadline, while serving every job (shed_late=0) lets the
+    queue and the latency grow. Served jobs' LLRs stay bit-identical to the cells alone."""
+    import torch
+    cells, refs = population(po, pe, dev, 23)
+    descs = [c[0] for c in cells]
+    n_jobs = 400
+    cap, _, _ = pe.serve_run(descs, n_jobs, 0.0, hbm_budget=256 << 20, max_batch=16, max_inflight=2,
+                             use_graph=False, seed=5)
+    rate = 3.0 * cap["jobs_per_s"]
+    deadline = 0.25e-3
+    base, _, _ = pe.serve_run(descs, n_jobs, rate, deadline_s=deadline, hbm_budget=256 << 20, max_batch=16,
+                              max_inflight=2, use_graph=False, seed=6)
+    size = max(int(np.prod(r.shape)) for r in refs)
+    outs = [torch.full((size,), -7.0, dtype=torch.float32, device=dev) for _ in range(n_jobs)]
+    rep, lat, cell = pe.serve_run(descs, n_jobs, rate, deadline_s=deadline, hbm_budget=256 << 20, max_batch=16,
+                                  max_inflight=2, use_graph=False, seed=6, shed_late=2, job_llr=outs)
+    print("capacity", cap["jobs_per_s"], "serve-all", base, "deadline-aware", rep)
+    assert rep["iso_job_s"] > 0 and rep["contention_index"] > 0
+    assert rep["n_done"] + rep["n_shed"] == n_jobs and rep["n_shed"] > 0 and rep["n_done"] > 0
+    assert base["n_shed"] == 0 and base["p99_s"] > 2 * deadline
+    assert rep["p99_s"] < base["p99_s"]
+    assert rep["p50_s"] <= 1.5 * deadline
+    for j in np.nonzero(lat > 0)[0]:
+        r = refs[cell[j]]
+        got = outs[j][: r.size].cpu().numpy().reshape(r.shape)
+        assert np.array_equal(got, r), "job %d (cell %d)" % (j, cell[j])
