@@ -12,10 +12,8 @@
 //      sized to the batch, or as a replay of its captured graph).
 // Shedding (shed_late): 1 drops a job whose deadline has passed before launch; 2 (deadline-aware admission)
 // drops a job whose predicted completion -- its age plus the predicted latency of the batch it would join --
-// is past its deadline. The prediction is an online model of batch latency (launch to observed completion):
-// the isolated one-job latency measured at warm-up plus a per-subcarrier cost, a running average over the
-// completed batches. Under overload this serves the jobs that can still make their deadline instead of
-// serving every job late.
+// is past its deadline (the model is described at LatencyModel). Under overload this serves the jobs that
+// can still make their deadline instead of serving every job late.
 // The report carries the reference's online contention indicator (pmu_contention.cu:230-245): the mean batch
 // latency of the first 5 batches divided by the isolated one-job latency.
 // Jobs that arrive while batches are on the GPU are admitted as soon as a stream and memory free up, so the
@@ -85,19 +83,41 @@ struct JobBuf {
     size_t off = 0, bytes = 0;
 };
 
-// Online batch-latency model: latency(n_sc) = iso + rate * max(0, n_sc - iso_sc); rate is a running average
-// (weight 1/8) of the excess latency per subcarrier of completed batches.
+// Online batch-latency model for deadline-aware admission. The latencies (launch to observed completion) of
+// the last 16 completed batches, which already include the sharing of the GPU with the batches in flight at
+// the time, are fitted by least squares as latency = a + b * subcarriers; a batch of n subcarriers is
+// predicted to take a + b n plus three residual standard deviations (a margin for the spread). Until two
+// batches have completed nothing is predicted late.
 struct LatencyModel {
-    double iso = 0.0;   // one-job batch latency at warm-up (seconds)
-    double iso_sc = 0;  // its subcarriers
-    double rate = 0.0;  // seconds per subcarrier beyond iso_sc
-    int samples = 0;
-    double predict(double n_sc) const { return iso + rate * std::max(0.0, n_sc - iso_sc); }
+    static constexpr size_t kWindow = 16;
+    double iso = 0.0;  // one-job batch latency at warm-up (seconds), reported
+    std::deque<std::pair<double, double>> recent;  // (subcarriers, latency)
+    double a = 0.0, b = 0.0, margin = 0.0;
+    bool ready = false;
+    double predict(double n_sc) const { return ready ? a + b * n_sc + margin : 0.0; }
     void observe(double latency, double n_sc) {
-        if (n_sc <= iso_sc) return;
-        const double r = std::max(0.0, latency - iso) / (n_sc - iso_sc);
-        rate = samples == 0 ? r : rate + (r - rate) / 8.0;
-        ++samples;
+        if (n_sc <= 0) return;
+        recent.emplace_back(n_sc, latency);
+        if (recent.size() > kWindow) recent.pop_front();
+        const double n = (double)recent.size();
+        if (n < 2) return;
+        double sx = 0, sy = 0, sxx = 0, sxy = 0;
+        for (const auto &r : recent) {
+            sx += r.first;
+            sy += r.second;
+            sxx += r.first * r.first;
+            sxy += r.first * r.second;
+        }
+        const double var = sxx - sx * sx / n;
+        b = var > 0 ? std::max(0.0, (sxy - sx * sy / n) / var) : 0.0;
+        a = std::max(0.0, (sy - b * sx) / n);
+        double ss = 0;
+        for (const auto &r : recent) {
+            const double e = r.second - (a + b * r.first);
+            ss += e * e;
+        }
+        margin = 3.0 * std::sqrt(ss / n);
+        ready = true;
     }
 };
 
@@ -217,8 +237,9 @@ extern "C" pe_status pe_serve_run(const pe_serve_config *cfg, pe_serve_report *r
         return s;
     };
 
-    // warm-up: one batch of the first job's cell on every stream, as a graph (captured here) and eagerly; not
-    // measured. Then the isolated latency of one job (eager, idle GPU; median of 5) seeds the latency model.
+    // warm-up, not measured: one batch of the first job's cell on every stream, as a graph (captured here)
+    // and eagerly (every kernel of the executor runs, at least one workgroup each). Then the isolated latency
+    // of one job (eager, idle GPU; median of 5) seeds the latency model.
     LatencyModel model;
     {
         JobBuf b;
@@ -253,16 +274,15 @@ extern "C" pe_status pe_serve_run(const pe_serve_config *cfg, pe_serve_report *r
             iso.push_back(std::chrono::duration<double>(clock::now() - a).count());
         }
         model.iso = nearest_rank(iso, 50.0);
-        model.iso_sc = pop[cellof[0]].nSC;
         arena.release(b.off, b.bytes);
     }
 
     // ---- loop ----------------------------------------------------------------------------------------------
     std::vector<double> lat(nj, -1.0);
     std::vector<std::vector<int>> batch_jobs(K);
-    std::vector<double> batch_t0(K, 0.0), batch_sc(K, 0.0);
-    std::vector<double> first_batches;  // latencies of the first 5 batches (contention indicator)
     std::vector<bool> busy(K, false);
+    std::vector<double> batch_t0(K, 0.0), batch_sc(K, 0.0);  // per stream: launch time, subcarriers
+    std::vector<double> first_batches;  // latencies of the first 5 batches (contention indicator)
     std::deque<int> ready;
     int next = 0, finished = 0, shed = 0, batches = 0, max_inflight_seen = 0;
     long batched_jobs = 0;
@@ -305,12 +325,21 @@ extern "C" pe_status pe_serve_run(const pe_serve_config *cfg, pe_serve_report *r
             if (busy[k]) continue;
             cells.clear();
             double sc_batch = 0.0;
+            // deadline-aware admission predicts with the batch this launch will have if nothing is shed: the
+            // first max_batch_jobs jobs of the queue (a batch only grows while it is formed)
+            double pred = 0.0;
+            if (cfg->shed_late == 2) {
+                double sc_full = 0.0;
+                int m = 0;
+                for (auto it = ready.begin(); it != ready.end() && m < cfg->max_batch_jobs; ++it, ++m)
+                    sc_full += pop[cellof[*it]].nSC;
+                pred = model.predict(sc_full);
+            }
             while (!ready.empty() && (int)cells.size() < cfg->max_batch_jobs) {
                 const int j = ready.front();
                 const double nsc_j = pop[cellof[j]].nSC;
                 const bool late = cfg->shed_late == 1 && now - arrival[j] > cfg->deadline_s;
-                const bool predicted_late =
-                    cfg->shed_late == 2 && now - arrival[j] + model.predict(sc_batch + nsc_j) > cfg->deadline_s;
+                const bool predicted_late = cfg->shed_late == 2 && now - arrival[j] + pred > cfg->deadline_s;
                 if (late || predicted_late) {
                     ready.pop_front();
                     ++shed;
@@ -379,7 +408,7 @@ extern "C" pe_status pe_serve_run(const pe_serve_config *cfg, pe_serve_report *r
     double fb = 0.0;
     for (double v : first_batches) fb += v;
     rep->contention_index = first_batches.empty() || model.iso <= 0 ? 0.0 : fb / first_batches.size() / model.iso;
-    rep->model_s_per_sc = model.rate;
+    rep->model_s_per_sc = model.b;
     pe::set_error("");
     return PE_OK;
 }

@@ -252,7 +252,7 @@ typedef struct {
     double iso_job_s;          /* latency of one job alone on an idle GPU (warm-up, median of 5) */
     double contention_index;   /* mean latency of the first 5 batches / iso_job_s: the online contention
                                   indicator of pmu_contention.cu:230-245 */
-    double model_s_per_sc;     /* final per-subcarrier cost of the batch-latency model */
+    double model_s_per_sc;     /* final per-subcarrier slope of the batch-latency model (shed_late = 2) */
 } pe_serve_report;
 
 /* Runs the serving loop to completion on the current device (blocking). Allocates its arena, streams,
