@@ -145,6 +145,28 @@ __device__ __forceinline__ void load_raw(RawSC<KS> &R, const float2 *__restrict_
         }
 }
 
+// Binary16 coefficient blocks (C16 [nSC][L][A] pairs, c16_exp scaling): the C chunks hold 4 pairs each,
+// L*A/4 of them per subcarrier.
+template <int KS>
+__device__ __forceinline__ void load_raw_c16(RawSC<KS> &R, const uint32_t *__restrict__ C16,
+                                             const float2 *__restrict__ Y, size_t sc, int L, int S, int A,
+                                             int lane, int nC, int nY, int rc, int ry) {
+    const float4 *cb = reinterpret_cast<const float4 *>(C16 + sc * L * A);
+    const float4 *yb = reinterpret_cast<const float4 *>(Y + sc * S * A);
+#pragma unroll
+    for (int j = 0; j < RawSC<KS>::RC; ++j)
+        if (j < rc) {
+            const int c = lane + 64 * j;
+            R.c[j] = ld_stream(cb + (c < nC ? c : 0));
+        }
+#pragma unroll
+    for (int j = 0; j < RawSC<KS>::RY; ++j)
+        if (j < ry) {
+            const int c = lane + 64 * j;
+            R.y[j] = ld_stream(yb + (c < nY ? c : 0));
+        }
+}
+
 __device__ __forceinline__ float max4abs(float m, float4 v) {
     return max3abs(max3abs(m, v.x, v.y), v.z, v.w);
 }
@@ -201,11 +223,14 @@ __device__ __forceinline__ void demap_store(float2 s0, float2 s1, size_t sc, int
 
 // One subcarrier: range scaling, operand conversion into LDS, MFMA, epilogue to Y_eq or LLRs.
 // NPB: 0 -> store Y_eq; 1..4 -> fused demap of that QAM order; -1 -> fused demap of order npb_rt.
-template <int KS, int PREC, int NPB>
+// C16IN: R.c holds binary16 coefficient chunks (load_raw_c16; nC, rc count those) scaled by 2^ec_in; only the
+// samples are range-checked here, and only PREC 1 applies.
+template <int KS, int PREC, int NPB, bool C16IN = false>
 __device__ __forceinline__ void process_sc(RawSC<KS> &R, size_t sc, unsigned char *lds, float2 *__restrict__ O,
                                            const float *__restrict__ noise_inv, float *__restrict__ llr, int L,
                                            int S, int A, int lane, int nC, int nY, int rc, int ry,
-                                           int npb_rt = 0, int f16 = 0) {
+                                           int npb_rt = 0, int f16 = 0, int ec_in = 0) {
+    static_assert(!C16IN || PREC == 1, "binary16 coefficient input is for the binary16 product");
     constexpr bool BF = (PREC == 2 || PREC == 4);
     constexpr bool SPLIT = (PREC == 3 || PREC == 4);
     constexpr int AG = 4 * KS;
@@ -224,7 +249,7 @@ __device__ __forceinline__ void process_sc(RawSC<KS> &R, size_t sc, unsigned cha
     float mc = 0.f, my = 0.f;
 #pragma unroll
     for (int j = 0; j < RawSC<KS>::RC; ++j)
-        if (j < rc) mc = max4abs(mc, R.c[j]);
+        if (!C16IN && j < rc) mc = max4abs(mc, R.c[j]);
 #pragma unroll
     for (int j = 0; j < RawSC<KS>::RY; ++j)
         if (j < ry) my = max4abs(my, R.y[j]);
@@ -232,7 +257,9 @@ __device__ __forceinline__ void process_sc(RawSC<KS> &R, size_t sc, unsigned cha
     // the operands are already in range and scaling is skipped. The test needs only wave votes on per-lane
     // maxima; the cross-lane max reductions run only when scaling is needed. bf16 has the fp32 exponent range.
     bool scaled = true;
-    if constexpr (!SPLIT) {
+    if constexpr (C16IN) {
+        scaled = __any(my >= 0x1p14f) || !__any(my >= 0x1p-9f);  // the samples alone
+    } else if constexpr (!SPLIT) {
         if constexpr (BF) {
             scaled = false;
         } else {
@@ -241,17 +268,32 @@ __device__ __forceinline__ void process_sc(RawSC<KS> &R, size_t sc, unsigned cha
         }
     }
     int ec = 0, ey = 0;
-    if (scaled) {
+    if constexpr (C16IN) {
+        ec = ec_in;
+        if (scaled) ey = scale_exp(wave_max_xor(my, 1, 64));
+    } else if (scaled) {
         ec = scale_exp(wave_max_xor(mc, 1, 64));
         ey = scale_exp(wave_max_xor(my, 1, 64));
     }
     const float sc_c = ldexpf(1.f, -ec), sc_y = ldexpf(1.f, -ey);
 
     // convert and write the operands
+    if constexpr (C16IN) {  // binary16 chunks of 4 pairs, copied as they are
+        const int cpr4 = A >> 2;
+        const unsigned cpr4_mag = ((1u << 20) + (unsigned)cpr4 - 1u) / (unsigned)cpr4;
+#pragma unroll
+        for (int j = 0; j < RawSC<KS>::RC; ++j) {
+            const int c = lane + 64 * j;
+            if (j < rc && c < nC) {
+                const int row = (int)(((unsigned)c * cpr4_mag) >> 20), part = c - row * cpr4;
+                *reinterpret_cast<float4 *>(lds + row * RB + part * 16) = R.c[j];
+            }
+        }
+    }
 #pragma unroll
     for (int j = 0; j < RawSC<KS>::RC; ++j) {
         const int c = lane + 64 * j;
-        if (j < rc && c < nC) {
+        if (!C16IN && j < rc && c < nC) {
             float4 v = R.c[j];
             if (scaled) { v.x *= sc_c; v.y *= sc_c; v.z *= sc_c; v.w *= sc_c; }
             const int row = (int)(((unsigned)c * cpr_mag) >> 20), part = c - row * cpr;

@@ -367,6 +367,49 @@ pe_status pe_coef_apply_demap_f16(const pe_c64 *C, const pe_c64 *Y_rx, const flo
     return from_hip(e, "coef_apply_demap_f16 launch");
 }
 
+static bool aligned16(const void *p) { return (reinterpret_cast<uintptr_t>(p) & 15u) == 0; }
+
+pe_status pe_coef_to_c16(const pe_c64 *C, pe_c16 *C16, int *c_exp, int nSC, int nLayer, int nAnt, void *stream) {
+    pe_status s = check_coef_shape(nSC, nLayer, 1, nAnt, (int)PE_PREC_F16);
+    if (s != PE_OK) return s;
+    if (nSC == 0) return ok();
+    if (!check_ptrs(C, C16, c_exp)) return fail(PE_ERR_INVALID, "null pointer");
+    if (!aligned16(C) || !aligned16(C16)) return fail(PE_ERR_INVALID, "C and C16 must be 16-byte aligned");
+    return from_hip(pe::launch_coef_to_c16(reinterpret_cast<const float2 *>(C), reinterpret_cast<uint32_t *>(C16),
+                                           c_exp, nSC, nLayer, nAnt, (hipStream_t)stream),
+                    "coef_to_c16 launch");
+}
+
+static pe_status coef_apply_c16(const pe_c16 *C16, const int *c_exp, const pe_c64 *Y_rx, pe_c64 *Y_eq,
+                                const float *noise_inv, void *llr, int llr_f16, int nSC, int nLayer, int nSym,
+                                int nAnt, int nPamBits, void *stream) {
+    pe_status s = check_coef_shape(nSC, nLayer, nSym, nAnt, (int)PE_PREC_F16);
+    if (s != PE_OK) return s;
+    if (llr && (nPamBits < 1 || nPamBits > 4)) return fail(PE_ERR_INVALID, "nPamBits = %d outside [1, 4]", nPamBits);
+    if (nSC == 0) return ok();
+    if (!check_ptrs(C16, c_exp, Y_rx) || (llr ? !noise_inv : !Y_eq)) return fail(PE_ERR_INVALID, "null pointer");
+    if (!aligned16(C16) || !aligned16(Y_rx)) return fail(PE_ERR_INVALID, "C16 and Y_rx must be 16-byte aligned");
+    hipError_t e = pe::launch_coef_apply_c16(reinterpret_cast<const uint32_t *>(C16), c_exp,
+                                             reinterpret_cast<const float2 *>(Y_rx), reinterpret_cast<float2 *>(Y_eq),
+                                             noise_inv, llr, nSC, nLayer, nSym, nAnt, llr ? nPamBits : 0, llr_f16,
+                                             (hipStream_t)stream);
+    return from_hip(e, "coef_apply_c16 launch");
+}
+
+pe_status pe_coef_apply_c16(const pe_c16 *C16, const int *c_exp, const pe_c64 *Y_rx, pe_c64 *Y_eq, int nSC,
+                            int nLayer, int nSym, int nAnt, void *stream) {
+    return coef_apply_c16(C16, c_exp, Y_rx, Y_eq, nullptr, nullptr, 0, nSC, nLayer, nSym, nAnt, 0, stream);
+}
+
+pe_status pe_coef_apply_demap_c16(const pe_c16 *C16, const int *c_exp, const pe_c64 *Y_rx, const float *noise_inv,
+                                  void *llr, int nSC, int nLayer, int nSym, int nAnt, int nPamBits,
+                                  unsigned flags, void *stream) {
+    if (flags & ~PE_LLR_F16) return fail(PE_ERR_INVALID, "unknown flag bits 0x%x", flags);
+    if (!llr && nSC > 0) return fail(PE_ERR_INVALID, "null pointer");
+    return coef_apply_c16(C16, c_exp, Y_rx, nullptr, noise_inv, llr, (flags & PE_LLR_F16) ? 1 : 0, nSC, nLayer,
+                          nSym, nAnt, nPamBits, stream);
+}
+
 pe_status pe_srs_filter(pe_c16 *H_est, const pe_c16 *SRS, const pe_c16 *W, const pe_c16 *FOCC, int nAnt,
                         int nBlocks, int K, int nPorts, int foccLen, unsigned flags, void *stream) {
     if (nBlocks < 0) return fail(PE_ERR_INVALID, "nBlocks = %d < 0", nBlocks);

@@ -131,6 +131,18 @@ __device__ __forceinline__ void store_llr_f16(uint16_t *dst, const float (&v)[2 
     }
 }
 
+// ---- binary16 coefficient blocks ------------------------------------------------------------------------
+// A subcarrier's coefficient block C[L][A] stored as binary16 pairs {re, im} with one power-of-two exponent:
+// C = C16 * 2^e. e = 0 when the block's largest |Re|, |Im| lies in [2^-9, 2^14) (binary16 holds it at full
+// precision as is, the range the MFMA path converts without scaling); otherwise the frexp exponent of that
+// maximum, so that the scaled block peaks in [0.5, 1). 0 for an all-zero block.
+__device__ __forceinline__ int c16_exp(float m) {
+    if (m >= 0x1p-9f && m < 0x1p14f) return 0;
+    int e = 0;
+    if (m > 0.f) (void)frexpf(m, &e);
+    return e;
+}
+
 // ---- wave helpers (wave64) -------------------------------------------------------------------------
 __device__ __forceinline__ float wave_max_xor(float v, int from, int to) {
     // butterfly over lane-index bits [from, to): every lane of the group ends with the same value

@@ -63,6 +63,12 @@ hipError_t launch_llr_f32(const float2 *Yeq, const float *noise_inv, float *llr,
 // SRS filter multiply (binary16 pairs as uint32): the binary16 chain, or MFMA with FP32 accumulation.
 hipError_t launch_srs_filter(uint32_t *H, const uint32_t *SRS, const uint32_t *W, const uint32_t *FOCC, int nAnt,
                              int nBlocks, int K, int nPorts, int foccLen, bool conj, bool mfma, hipStream_t st);
+// Binary16 coefficient blocks (C16 pairs + per-subcarrier exponent): conversion from complex64, and the
+// binary16 MFMA apply from them (llr == nullptr -> Y_eq into O; llr_f16 -> binary16 LLRs).
+hipError_t launch_coef_to_c16(const float2 *C, uint32_t *C16, int *cexp, int nSC, int L, int A, hipStream_t st);
+hipError_t launch_coef_apply_c16(const uint32_t *C16, const int *cexp, const float2 *Y, float2 *O,
+                                 const float *noise_inv, void *llr, int nSC, int L, int S, int A, int nPamBits,
+                                 int llr_f16, hipStream_t st);
 // Streaming probes (HBM ceiling): read_only -> reads src, else copies src to dst.
 hipError_t launch_hbm_probe(const void *src, void *dst, size_t bytes, int read_only, int grid, hipStream_t st);
 // MMSE: FP32 pass (flags subcarriers for FP64) then the FP64 pass over flagged subcarriers.

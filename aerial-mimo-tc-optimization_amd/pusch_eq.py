@@ -45,6 +45,9 @@ SIGNATURES = {
     "pe_coef_apply_demap_f16": ([_P, _P, _P, _P, _I, _I, _I, _I, _I, _I, _P], _I),
     "pe_srs_filter": ([_P, _P, _P, _P, _I, _I, _I, _I, _I, _U, _P], _I),
     "pe_hbm_probe": ([_P, _P, ctypes.c_size_t, _I, _I, _P], _I),
+    "pe_coef_to_c16": ([_P, _P, _P, _I, _I, _I, _P], _I),
+    "pe_coef_apply_c16": ([_P, _P, _P, _P, _I, _I, _I, _I, _P], _I),
+    "pe_coef_apply_demap_c16": ([_P, _P, _P, _P, _P, _I, _I, _I, _I, _I, _U, _P], _I),
     "pe_slot_plan_create": ([_P, _I, _U, _P], _I),
     "pe_slot_plan_update": ([_P, _P, _I, _P], _I),
     "pe_slot_plan_launch": ([_P, _P], _I),
@@ -229,6 +232,46 @@ def srs_filter(SRS, W, FOCC, mfma=False, conj_w=False, out=None, stream=None):
     flags = (SRS_MFMA if mfma else 0) | (SRS_CONJ_W if conj_w else 0)
     _check(lib().pe_srs_filter(_ptr(out), _ptr(SRS), _ptr(W), _ptr(FOCC), nAnt, nBlocks, K, nPorts, foccLen, flags,
                                _stream(stream)))
+    return out
+
+
+LLR_F16 = 1
+
+
+def coef_to_c16(C, C16=None, c_exp=None, stream=None):
+    """complex64 coefficients [nSC, L, A] -> (C16 int32 [nSC, L, A] holding binary16 {re, im} pairs,
+    c_exp int32 [nSC]) with C = C16 * 2^c_exp (pe_coef_to_c16)."""
+    import torch
+    nSC, L, A = C.shape
+    if C16 is None:
+        C16 = torch.empty((nSC, L, A), dtype=torch.int32, device=C.device)
+    if c_exp is None:
+        c_exp = torch.empty((nSC,), dtype=torch.int32, device=C.device)
+    _check(lib().pe_coef_to_c16(_ptr(C), _ptr(C16), _ptr(c_exp), nSC, L, A, _stream(stream)))
+    return C16, c_exp
+
+
+def coef_apply_c16(C16, c_exp, Y, out=None, stream=None):
+    import torch
+    nSC, L, A = C16.shape
+    S = Y.shape[1]
+    if out is None:
+        out = torch.empty((nSC, L, S), dtype=torch.complex64, device=Y.device)
+    _check(lib().pe_coef_apply_c16(_ptr(C16), _ptr(c_exp), _ptr(Y), _ptr(out), nSC, L, S, A, _stream(stream)))
+    return out
+
+
+def coef_apply_demap_c16(C16, c_exp, Y, noise_inv, n_pam_bits, llr_f16=False, out=None, stream=None):
+    """Fused apply + demap from binary16 coefficients (pe_coef_apply_demap_c16): float32 LLRs, or binary16
+    (torch.float16) with llr_f16."""
+    import torch
+    nSC, L, A = C16.shape
+    S = Y.shape[1]
+    if out is None:
+        out = torch.empty((nSC, L, S, 2 * n_pam_bits), dtype=torch.float16 if llr_f16 else torch.float32,
+                          device=Y.device)
+    _check(lib().pe_coef_apply_demap_c16(_ptr(C16), _ptr(c_exp), _ptr(Y), _ptr(noise_inv), _ptr(out), nSC, L, S, A,
+                                         n_pam_bits, LLR_F16 if llr_f16 else 0, _stream(stream)))
     return out
 
 

@@ -578,7 +578,22 @@ def main():
                             bufs16, max(20, args.steps // 4), 5)
     per16 = el16 / max(20, args.steps // 4)
     bytes16 = nsc * (L * A * 8 + S * A * 8 + L * S * 2 * Q * 2 + L * 4)
-    del llr16, bufs16
+    # coefficients kept in HBM as binary16 pairs + exponent (pe_coef_to_c16 outside the timed region), FP32 and
+    # binary16 LLRs
+    c16s = [pe.coef_to_c16(s["C"]) for s in sets]
+    bufs_c16 = [(c16, ce, s["Y"], s["ni"], s["llr"]) for (c16, ce), s in zip(c16s, sets)]
+    elc, _ = time_launches(lambda c16, ce, Y, ni, o: pe.coef_apply_demap_c16(c16, ce, Y, ni, Q, out=o), bufs_c16,
+                           max(20, args.steps // 4), 5)
+    perc = elc / max(20, args.steps // 4)
+    bytes_c16 = nsc * (L * A * 4 + 4 + S * A * 8 + L * S * 2 * Q * 4 + L * 4)
+    llr16 = [torch.empty((nsc, L, S, 2 * Q), dtype=torch.float16, device=dev) for _ in sets]
+    bufs_c16w = [(c16, ce, s["Y"], s["ni"], o) for (c16, ce), s, o in zip(c16s, sets, llr16)]
+    elcw, _ = time_launches(lambda c16, ce, Y, ni, o: pe.coef_apply_demap_c16(c16, ce, Y, ni, Q, llr_f16=True,
+                                                                                out=o),
+                            bufs_c16w, max(20, args.steps // 4), 5)
+    percw = elcw / max(20, args.steps // 4)
+    bytes_c16w = nsc * (L * A * 4 + 4 + S * A * 8 + L * S * 2 * Q * 2 + L * 4)
+    del llr16, bufs16, bufs_c16, bufs_c16w, c16s
 
     # single-cell slot latency: MMSE + fused apply/demap for one cell, one stream
     cell = {k: (v[c1].contiguous() if torch.is_tensor(v) and v.shape[0] == nsc else v) for k, v in s0.items()}
@@ -652,6 +667,13 @@ def main():
         "llr_f16_wire": {"value": nsc / per16, "ms_per_step": per16 * 1e3, "hbm_gbs": bytes16 / per16 / 1e9,
                          "bytes_per_launch": bytes16,
                          "workload": "same launch, LLRs written as binary16 (pe_coef_apply_demap_f16)"},
+        "c16_coefficients": {"value": nsc / perc, "ms_per_step": perc * 1e3, "hbm_gbs": bytes_c16 / perc / 1e9,
+                             "bytes_per_launch": bytes_c16,
+                             "workload": "same batches, coefficients held as binary16 pairs + exponent "
+                                         "(pe_coef_apply_demap_c16), FP32 LLRs"},
+        "c16_coefficients_llr_f16": {"value": nsc / percw, "ms_per_step": percw * 1e3,
+                                     "hbm_gbs": bytes_c16w / percw / 1e9, "bytes_per_launch": bytes_c16w,
+                                     "workload": "binary16 coefficients and binary16 LLRs"},
         "slot_latency_ms": {"p50": pct(50), "p99": pct(99), "max": lat[-1] if lat else None, "iters": len(lat),
                             "workload": "one cell: pe_gram_mmse + fused apply/demap, events on one stream"},
         "multicell_slot": multicell,

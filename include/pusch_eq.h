@@ -157,6 +157,28 @@ pe_status pe_coef_apply_demap_f16(const pe_c64 *C, const pe_c64 *Y_rx, const flo
                                   int nSC, int nLayer, int nSym, int nAnt, int nPamBits, pe_prec prec,
                                   void *stream);
 
+/* ---- coefficients in the binary16 operand format ---------------------------------------------------- */
+
+/* No reference counterpart; the reference names this as future work ("FP16 native data path: eliminate the
+   FP32 -> FP16 conversion", README.md:201). The binary16 MFMA product rounds every coefficient to
+   binary16 anyway, so the coefficients can be kept in that form: C16[nSC][nLayer][nAnt] binary16 pairs {re, im}
+   and one exponent per subcarrier, C = C16 * 2^c_exp[sc]. c_exp is 0 when the subcarrier's largest |Re|, |Im|
+   lies in [2^-9, 2^14), else the frexp exponent of that maximum (the scaled block then peaks in [0.5, 1)).
+   The fused apply reads 4 bytes per coefficient instead of 8. Where samples and coefficients both lie in
+   [2^-9, 2^14) (neither operand needs range scaling), the results are bit-identical to pe_coef_apply /
+   pe_coef_apply_demap with PE_PREC_F16 on the complex64 coefficients; elsewhere they differ only by the
+   binary16 rounding of scaled versus unscaled values. Shapes as for PE_PREC_F16; C, C16 and Y_rx 16-byte
+   aligned. */
+#define PE_LLR_F16 1u  /* LLRs written as binary16 (the wire format of pe_coef_apply_demap_f16) */
+
+pe_status pe_coef_to_c16(const pe_c64 *C, pe_c16 *C16, int *c_exp, int nSC, int nLayer, int nAnt, void *stream);
+pe_status pe_coef_apply_c16(const pe_c16 *C16, const int *c_exp, const pe_c64 *Y_rx, pe_c64 *Y_eq, int nSC,
+                            int nLayer, int nSym, int nAnt, void *stream);
+/* llr: float [nSC][nLayer][nSym][2*nPamBits], or binary16 with PE_LLR_F16 */
+pe_status pe_coef_apply_demap_c16(const pe_c16 *C16, const int *c_exp, const pe_c64 *Y_rx, const float *noise_inv,
+                                  void *llr, int nSC, int nLayer, int nSym, int nAnt, int nPamBits,
+                                  unsigned flags, void *stream);
+
 /* ---- SRS filter multiply (channel estimation upstream of the equalizer) ---------------------------- */
 
 #define PE_SRS_MFMA 1u    /* MFMA with FP32 accumulation: one binary16 rounding per output (K % 4 == 0,

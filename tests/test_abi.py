@@ -19,7 +19,8 @@ def declared_functions():
 def test_header_declares_the_operator_surface():
     names = declared_functions()
     for required in ("pe_coef_apply", "pe_gram_mmse", "pe_llr_demap_f16", "pe_llr_demap", "pe_coef_apply_demap",
-                     "pe_coef_apply_demap_f16", "pe_srs_filter", "pe_gram", "pe_coef_apply_v1"):
+                     "pe_coef_apply_demap_f16", "pe_srs_filter", "pe_coef_to_c16", "pe_coef_apply_demap_c16",
+                     "pe_gram", "pe_coef_apply_v1"):
         assert required in names
 
 
@@ -93,6 +94,17 @@ def test_other_entry_points_validate(pe):
     assert h.pe_hbm_probe(n, n, 0, 0, 64, None) == 0                          # empty
     assert h.pe_hbm_probe(p, ctypes.c_void_p(256), 32, 0, 64, None) == 1      # unaligned src
     a16 = ctypes.c_void_p(256)
+    # binary16 coefficients: (C16, c_exp, Y, noise_inv, llr, nSC, L, S, A, q, flags, stream)
+    assert h.pe_coef_to_c16(n, n, n, 0, 8, 64, None) == 0
+    assert h.pe_coef_to_c16(a16, a16, n, 4, 8, 64, None) == 1              # null c_exp
+    assert h.pe_coef_to_c16(a16, p, a16, 4, 8, 64, None) == 1              # unaligned C16
+    assert h.pe_coef_to_c16(a16, a16, a16, 4, 8, 256, None) == 2           # nAnt > 128
+    assert h.pe_coef_apply_demap_c16(n, n, n, n, n, 0, 8, 11, 64, 4, 0, None) == 0
+    assert h.pe_coef_apply_demap_c16(a16, a16, a16, a16, a16, 4, 8, 11, 64, 4, 2, None) == 1  # flag
+    assert h.pe_coef_apply_demap_c16(a16, a16, a16, a16, a16, 4, 8, 11, 64, 5, 0, None) == 1  # q
+    assert h.pe_coef_apply_demap_c16(a16, a16, a16, n, a16, 4, 8, 11, 64, 4, 0, None) == 1    # noise_inv
+    assert h.pe_coef_apply_c16(a16, a16, a16, n, 4, 8, 11, 64, None) == 1                      # Y_eq
+    assert h.pe_coef_apply_c16(a16, a16, a16, a16, 4, 8, 11, 62, None) == 1                    # nAnt % 4
     assert h.pe_srs_filter(a16, a16, a16, a16, 64, 2, 10, 4, 8, 1, None) == 2  # MFMA needs K % 4 == 0
     assert h.pe_srs_filter(a16, p, a16, a16, 64, 2, 12, 4, 8, 1, None) == 1    # MFMA needs aligned SRS
 

@@ -266,3 +266,65 @@ def test_hbm_probe_copy_is_a_copy(pe, dev, nbytes, grid):
     pe.hbm_probe(a, sink, read_only=True, grid=grid)
     torch.cuda.synchronize()
     assert torch.equal(a, b)
+
+
+# ------------------------------------------------------------------------------------------------------
+# coefficients kept as binary16 pairs + per-subcarrier exponent
+# ------------------------------------------------------------------------------------------------------
+def c16_reference(C):
+    """numpy restatement of the format: e = 0 when max(|Re|, |Im|) of the subcarrier lies in [2^-9, 2^14),
+    else frexp's exponent of it; C16 = binary16(C * 2^-e) rounded to nearest even."""
+    re, im = C.real.astype(np.float32), C.imag.astype(np.float32)
+    m = np.maximum(np.abs(re), np.abs(im)).reshape(C.shape[0], -1).max(1)
+    e = np.where((m >= 2.0 ** -9) & (m < 2.0 ** 14), 0, np.frexp(m)[1]).astype(np.int32)
+    s = np.ldexp(np.float32(1), -e).astype(np.float32)[:, None, None]
+    pairs = np.stack([re * s, im * s], -1).astype(np.float16).view(np.uint16)
+    return pairs, e
+
+
+@pytest.mark.parametrize("L,A,nsc", [(8, 64, 300), (3, 16, 41), (4, 128, 17), (2, 24, 9)])
+def test_coef_to_c16_format(po, pe, dev, L, A, nsc):
+    rng = np.random.default_rng(L + A)
+    C = po.cn(rng, (nsc, L, A), 1.0 / A)
+    C[1] *= np.float32(1e-6)
+    C[2] *= np.float32(3e5)
+    C[3] = 0
+    C16, e = pe.coef_to_c16(dev_c64(C, dev))
+    pairs, e_ref = c16_reference(C)
+    assert np.array_equal(host(e), e_ref)
+    assert np.array_equal(host(C16).view(np.uint16).reshape(pairs.shape), pairs)
+
+
+@pytest.mark.parametrize("q", [1, 2, 4])
+@pytest.mark.parametrize("L,S,A,nsc", [(8, 11, 64, 500), (3, 9, 16, 41), (4, 14, 128, 33), (2, 5, 24, 19)])
+def test_c16_apply_equals_f16_path_in_range(po, pe, dev, q, L, S, A, nsc):
+    """In-range samples and coefficients: the binary16-coefficient path gives the same bits as the binary16
+    MFMA path on the complex64 coefficients (Y_eq, FP32 LLRs, binary16 LLRs)."""
+    import torch
+    rng = np.random.default_rng(3 * q + L)
+    H = po.rayleigh_h(rng, nsc, A, L)
+    C, _, _, _, _ = po.mmse_f64(H, np.full(nsc, 0.01, np.float32), unbiased=True)
+    C = C.astype(np.complex64)
+    Y = po.cn(rng, (nsc, S, A))
+    ni = rng.uniform(1, 100, (nsc, L)).astype(np.float32)
+    dC, dY, dni = dev_c64(C, dev), dev_c64(Y, dev), dev_f32(ni, dev)
+    C16, e = pe.coef_to_c16(dC)
+    assert int(host(e).max()) == 0  # in range: no coefficient scaling
+    assert np.array_equal(host(pe.coef_apply_c16(C16, e, dY)), host(pe.coef_apply(dC, dY, prec="f16")))
+    ref = host(pe.coef_apply_demap(dC, dY, dni, q, prec="f16"))
+    assert np.array_equal(host(pe.coef_apply_demap_c16(C16, e, dY, dni, q)), ref)
+    o16 = pe.coef_apply_demap_c16(C16, e, dY, dni, q, llr_f16=True)
+    assert np.array_equal(host(o16.view(torch.int16)).view(np.uint16), ref.astype(np.float16).view(np.uint16))
+
+
+def test_c16_apply_wide_range_keeps_accuracy(po, pe, dev):
+    """Coefficients and samples far outside the binary16 range: scaled blocks keep the binary16 EVM."""
+    rng = np.random.default_rng(19)
+    C = po.cn(rng, (64, 8, 64)) * np.float32(1e-6)
+    C[::3] *= np.float32(1e10)
+    Y = po.cn(rng, (64, 11, 64)) * np.float32(1e6)
+    ref = po.coef_apply_f64(C, Y)
+    C16, e = pe.coef_to_c16(dev_c64(C, dev))
+    out = host(pe.coef_apply_c16(C16, e, dev_c64(Y, dev)))
+    assert np.all(np.isfinite(out))
+    assert po.evm(ref, out)[1] <= MFMA_EVM_DB["f16"]
