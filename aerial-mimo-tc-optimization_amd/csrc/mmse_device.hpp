@@ -288,11 +288,11 @@ __device__ __forceinline__ float noise_inv_of(T x) {
 }
 
 // ---- coefficients: C[l][a] = s_l sum_j Ginv[l][j] conj(H[a][j]) for this lane's antennas --------------------
-// C[l][a] for all l from one antenna row of H
+// C[l][a] for all l from one antenna row of H, as the complex64 values stored
 template <typename T, int L>
-__device__ __forceinline__ void c_column(const T (&hr)[L], const T (&hi)[L], const T (&vr)[Dim<L>::NT],
-                                         const T (&vi)[Dim<L>::NT], const T (&scl)[L], int A, int a,
-                                         float2 *__restrict__ c_out, bool store) {
+__device__ __forceinline__ void c_values(const T (&hr)[L], const T (&hi)[L], const T (&vr)[Dim<L>::NT],
+                                         const T (&vi)[Dim<L>::NT], const T (&scl)[L], float (&cr)[L],
+                                         float (&ci)[L]) {
 #pragma unroll
     for (int l = 0; l < L; ++l) {
         T re = T(0), im = T(0);
@@ -304,8 +304,84 @@ __device__ __forceinline__ void c_column(const T (&hr)[L], const T (&hi)[L], con
             re = fma_(g_r, hr[j], fma_(g_i, hi[j], re));   // (g_r + i g_i)(h_r - i h_i)
             im = fma_(g_i, hr[j], fma_(-g_r, hi[j], im));
         }
-        if (store) c_out[(size_t)l * A + a] = make_float2((float)(re * scl[l]), (float)(im * scl[l]));
+        cr[l] = (float)(re * scl[l]);
+        ci[l] = (float)(im * scl[l]);
     }
+}
+
+template <typename T, int L>
+__device__ __forceinline__ void c_column(const T (&hr)[L], const T (&hi)[L], const T (&vr)[Dim<L>::NT],
+                                         const T (&vi)[Dim<L>::NT], const T (&scl)[L], int A, int a,
+                                         float2 *__restrict__ c_out, bool store) {
+    float cr[L], ci[L];
+    c_values<T, L>(hr, hi, vr, vi, scl, cr, ci);
+    if (store) {
+#pragma unroll
+        for (int l = 0; l < L; ++l) c_out[(size_t)l * A + a] = make_float2(cr[l], ci[l]);
+    }
+}
+
+// Binary16 coefficient output (C16 pairs and the subcarrier's exponent, rule in c16_exp) by a group of Q
+// lanes (lane q: antennas q and q + Q, A <= 2Q): the block's values stay in registers until the group's
+// maximum is known. Every lane of the wave calls it (the maximum is a butterfly over lane bits below Q).
+template <int L>
+__device__ __forceinline__ void write_c16_group(const float2 *__restrict__ h, int A, int q, int Q,
+                                                const float (&vr)[Dim<L>::NT], const float (&vi)[Dim<L>::NT],
+                                                const float (&scl)[L], uint32_t *__restrict__ c16,
+                                                int *__restrict__ cexp, bool store) {
+    float cr[2][L], ci[2][L];
+    float m = 0.f;
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+        const int a = q + u * Q;
+        if (a < A) {
+            float hr[L], hi[L];
+            load_row<float, L>(h + (size_t)a * L, hr, hi);
+            c_values<float, L>(hr, hi, vr, vi, scl, cr[u], ci[u]);
+#pragma unroll
+            for (int l = 0; l < L; ++l) m = fmaxf(m, fmaxf(abs_(cr[u][l]), abs_(ci[u][l])));
+        }
+    }
+    const int e = c16_exp(wave_max_xor(m, 1, Q));
+    const float s = ldexpf(1.f, -e);
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+        const int a = q + u * Q;
+        if (store && a < A) {
+#pragma unroll
+            for (int l = 0; l < L; ++l) c16[(size_t)l * A + a] = pack_half2(cr[u][l] * s, ci[u][l] * s);
+        }
+    }
+    if (store && q == 0) *cexp = e;
+}
+
+// The same for the FP64 pass's group of P lanes (lane p: antennas p, p + P, ...): two passes over the
+// antennas, the first for the maximum, the second recomputing the values to store them.
+template <int L>
+__device__ __forceinline__ void write_c16_twopass(const float2 *__restrict__ h, int A, int p, int P,
+                                                  const double (&vr)[Dim<L>::NT], const double (&vi)[Dim<L>::NT],
+                                                  const double (&scl)[L], uint32_t *__restrict__ c16,
+                                                  int *__restrict__ cexp) {
+    float m = 0.f;
+    for (int a = p; a < A; a += P) {
+        double hr[L], hi[L];
+        float cr[L], ci[L];
+        load_row<double, L>(h + (size_t)a * L, hr, hi);
+        c_values<double, L>(hr, hi, vr, vi, scl, cr, ci);
+#pragma unroll
+        for (int l = 0; l < L; ++l) m = fmaxf(m, fmaxf(abs_(cr[l]), abs_(ci[l])));
+    }
+    const int e = c16_exp(wave_max_xor(m, 1, P));
+    const float s = ldexpf(1.f, -e);
+    for (int a = p; a < A; a += P) {
+        double hr[L], hi[L];
+        float cr[L], ci[L];
+        load_row<double, L>(h + (size_t)a * L, hr, hi);
+        c_values<double, L>(hr, hi, vr, vi, scl, cr, ci);
+#pragma unroll
+        for (int l = 0; l < L; ++l) c16[(size_t)l * A + a] = pack_half2(cr[l] * s, ci[l] * s);
+    }
+    if (p == 0) *cexp = e;
 }
 
 template <typename T, int L>
@@ -361,12 +437,16 @@ __device__ __forceinline__ void tile_barrier(int W) {
 // rows in registers from the Gram to the coefficients instead of reading H a second time (the coefficient
 // stores then cover P consecutive antennas of a C row per lane group). It trades the second read of H for
 // registers (fewer waves per SIMD); which is faster depends on the shape.
-template <int L, int KR>
+// C16: coefficients go to c16 / cexp as binary16 pairs and exponents (write_c16_group; needs A <= 128 and
+// no kept rows) instead of complex64 to C.
+template <int L, int KR, bool C16 = false>
 __device__ __forceinline__ void tile_fp32(const float2 *__restrict__ H, const float *__restrict__ sigma2,
                                           float2 *__restrict__ C, float *__restrict__ noise_inv,
                                           uint8_t *__restrict__ flags, long sc0, int n, int A, int P,
                                           bool unbiased, float thr, float *__restrict__ lds, int tid, int W,
-                                          unsigned *flagged) {
+                                          unsigned *flagged, uint32_t *__restrict__ c16 = nullptr,
+                                          int *__restrict__ cexp = nullptr) {
+    static_assert(!C16 || KR == 0, "binary16 output reads H again for the coefficients");
     constexpr int NT = Dim<L>::NT;
     constexpr int SLOT = Dim<L>::SLOT;
     const int lane = tid & 63, wave = tid >> 6;
@@ -477,8 +557,12 @@ __device__ __forceinline__ void tile_fp32(const float2 *__restrict__ H, const fl
         }
 #pragma unroll
         for (int l = 0; l < L; ++l) scl[l] = in[2 * NT + l];
-        write_c<float, L>(H + (size_t)(sc0 + t) * A * L, A, q, 1 << lgQ, vr, vi, scl,
-                          C + (size_t)(sc0 + t) * L * A, store);
+        if constexpr (C16)
+            write_c16_group<L>(H + (size_t)(sc0 + t) * A * L, A, q, 1 << lgQ, vr, vi, scl,
+                               c16 + (size_t)(sc0 + t) * L * A, cexp + sc0 + t, store);
+        else
+            write_c<float, L>(H + (size_t)(sc0 + t) * A * L, A, q, 1 << lgQ, vr, vi, scl,
+                              C + (size_t)(sc0 + t) * L * A, store);
     }
     tile_barrier(W);  // slot reads done before the workgroup's next tile rewrites them
 }
@@ -486,10 +570,11 @@ __device__ __forceinline__ void tile_fp32(const float2 *__restrict__ H, const fl
 // ---- FP64 pass for one flagged subcarrier, by one group of P lanes -------------------------------------------
 // Writes C, noise_inv and the final flag (FP64, plus SINGULAR with zero C / noise_inv if the FP64
 // factorization breaks down too).
-template <int L>
+template <int L, bool C16 = false>
 __device__ __forceinline__ void group_fp64(const float2 *__restrict__ h, float s2f, float2 *__restrict__ c,
                                            float *__restrict__ ni, uint8_t *__restrict__ flag, int A, int p,
-                                           int P, bool unbiased) {
+                                           int P, bool unbiased, uint32_t *__restrict__ c16 = nullptr,
+                                           int *__restrict__ cexp = nullptr) {
     constexpr int NT = Dim<L>::NT;
     double gr[NT], gi[NT], vr[NT], vi[NT], scl[L], x[L];
     gram_group<double, L>(h, A, p, P, gr, gi);
@@ -499,10 +584,19 @@ __device__ __forceinline__ void group_fp64(const float2 *__restrict__ h, float s
 #pragma unroll
             for (int l = 0; l < L; ++l) ni[l] = noise_inv_of(x[l]);
         }
-        write_c<double, L>(h, A, p, P, vr, vi, scl, c);
+        if constexpr (C16)
+            write_c16_twopass<L>(h, A, p, P, vr, vi, scl, c16, cexp);
+        else
+            write_c<double, L>(h, A, p, P, vr, vi, scl, c);
     } else {
         for (int a = p; a < A; a += P)
-            for (int l = 0; l < L; ++l) c[(size_t)l * A + a] = make_float2(0.f, 0.f);
+            for (int l = 0; l < L; ++l) {
+                if constexpr (C16)
+                    c16[(size_t)l * A + a] = 0u;
+                else
+                    c[(size_t)l * A + a] = make_float2(0.f, 0.f);
+            }
+        if (C16 && p == 0) *cexp = 0;
         if (p == 0)
             for (int l = 0; l < L; ++l) ni[l] = 0.f;
     }
@@ -512,11 +606,12 @@ __device__ __forceinline__ void group_fp64(const float2 *__restrict__ h, float s
 // ---- FP64 pass over one tile: every flagged subcarrier of sc0 .. sc0 + n - 1 (n <= 64) --------------------
 // One flag load per lane; a wave whose tile holds no flagged subcarrier leaves at once. Flagged subcarriers
 // are taken 64 / P at a time, one per lane group.
-template <int L>
+template <int L, bool C16 = false>
 __device__ __forceinline__ void tile_fp64(const float2 *__restrict__ H, const float *__restrict__ sigma2,
                                           float2 *__restrict__ C, float *__restrict__ noise_inv,
                                           uint8_t *__restrict__ flags, long sc0, int n, int A, int P,
-                                          bool unbiased, int lane) {
+                                          bool unbiased, int lane, uint32_t *__restrict__ c16 = nullptr,
+                                          int *__restrict__ cexp = nullptr) {
     const bool pend = lane < n && (flags[sc0 + lane] & kPendingFp64);
     unsigned long long m = __ballot(pend);
     const int lgP = __builtin_ctz((unsigned)P);
@@ -531,8 +626,12 @@ __device__ __forceinline__ void tile_fp64(const float2 *__restrict__ H, const fl
         for (int k = 0; k < G && m; ++k) m &= m - 1;
         if (idx >= 0) {  // uniform per lane group
             const long sc = sc0 + idx;
-            group_fp64<L>(H + (size_t)sc * A * L, sigma2[sc], C + (size_t)sc * L * A, noise_inv + (size_t)sc * L,
-                          flags + sc, A, p, P, unbiased);
+            if constexpr (C16)
+                group_fp64<L, true>(H + (size_t)sc * A * L, sigma2[sc], nullptr, noise_inv + (size_t)sc * L,
+                                    flags + sc, A, p, P, unbiased, c16 + (size_t)sc * L * A, cexp + sc);
+            else
+                group_fp64<L>(H + (size_t)sc * A * L, sigma2[sc], C + (size_t)sc * L * A,
+                              noise_inv + (size_t)sc * L, flags + sc, A, p, P, unbiased);
         }
     }
 }

@@ -306,6 +306,23 @@ pe_status pe_gram_mmse(const pe_c64 *H, const float *sigma2, pe_c64 *C, float *n
     return from_hip(e, "gram_mmse launch");
 }
 
+pe_status pe_gram_mmse_c16(const pe_c64 *H, const float *sigma2, pe_c16 *C16, int *c_exp, float *noise_inv,
+                           uint8_t *sc_flags, int nSC, int nLayer, int nAnt, unsigned flags, float cond_threshold,
+                           void *stream) {
+    if (nSC < 0) return fail(PE_ERR_INVALID, "nSC = %d < 0", nSC);
+    if (nLayer < 1 || nLayer > 8) return fail(PE_ERR_INVALID, "nLayer = %d outside [1, 8]", nLayer);
+    if (nAnt < 4 || nAnt % 4 != 0) return fail(PE_ERR_INVALID, "nAnt = %d must be a positive multiple of 4", nAnt);
+    if (nAnt > 128) return fail(PE_ERR_UNSUPPORTED, "nAnt = %d > 128 for binary16 coefficients", nAnt);
+    if (flags & ~PE_MMSE_UNBIASED) return fail(PE_ERR_INVALID, "unknown flags 0x%x", flags);
+    if (nSC == 0) return ok();
+    if (!check_ptrs(H, sigma2, C16) || !c_exp || !noise_inv || !sc_flags) return fail(PE_ERR_INVALID, "null pointer");
+    const float thr = cond_threshold > 0.f ? cond_threshold : 1e4f;
+    hipError_t e = pe::launch_gram_mmse_c16(reinterpret_cast<const float2 *>(H), sigma2,
+                                            reinterpret_cast<uint32_t *>(C16), c_exp, noise_inv, sc_flags, nSC, nLayer,
+                                            nAnt, (flags & PE_MMSE_UNBIASED) != 0, thr, (hipStream_t)stream);
+    return from_hip(e, "gram_mmse_c16 launch");
+}
+
 pe_status pe_llr_demap_f16(const pe_c16 *softEst, const float *noiseInv, uint16_t *llrOut, int N, int nPamBits,
                            void *stream) {
     if (N < 0) return fail(PE_ERR_INVALID, "N = %d < 0", N);

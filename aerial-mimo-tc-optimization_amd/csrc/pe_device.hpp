@@ -113,6 +113,11 @@ __device__ __forceinline__ uint32_t float_to_half_bits(float f) {
     return (uint32_t)__builtin_bit_cast(uint16_t, (_Float16)f);
 }
 
+// two floats -> binary16 pair bits (a in the low half), round to nearest even
+__device__ __forceinline__ uint32_t pack_half2(float a, float b) {
+    return float_to_half_bits(a) | (float_to_half_bits(b) << 16);
+}
+
 // The 2*NPB LLRs of one symbol as binary16 (the reference's LLR wire format, aerial_llr_bench_v2.cu:221-227;
 // the values are already clamped to +-65504 by pam_llr_axis).
 template <int NPB>

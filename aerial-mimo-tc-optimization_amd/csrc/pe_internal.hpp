@@ -74,5 +74,9 @@ hipError_t launch_hbm_probe(const void *src, void *dst, size_t bytes, int read_o
 // MMSE: FP32 pass (flags subcarriers for FP64) then the FP64 pass over flagged subcarriers.
 hipError_t launch_gram_mmse(const float2 *H, const float *sigma2, float2 *C, float *noise_inv, uint8_t *flags,
                             int nSC, int L, int A, bool unbiased, float cond_threshold, hipStream_t st);
+// The same with the coefficients written in the binary16 operand format (A <= 128).
+hipError_t launch_gram_mmse_c16(const float2 *H, const float *sigma2, uint32_t *C16, int *cexp, float *noise_inv,
+                                uint8_t *flags, int nSC, int L, int A, bool unbiased, float cond_threshold,
+                                hipStream_t st);
 
 }  // namespace pe

@@ -46,6 +46,7 @@ SIGNATURES = {
     "pe_srs_filter": ([_P, _P, _P, _P, _I, _I, _I, _I, _I, _U, _P], _I),
     "pe_hbm_probe": ([_P, _P, ctypes.c_size_t, _I, _I, _P], _I),
     "pe_coef_to_c16": ([_P, _P, _P, _I, _I, _I, _P], _I),
+    "pe_gram_mmse_c16": ([_P, _P, _P, _P, _P, _P, _I, _I, _I, _U, _F, _P], _I),
     "pe_coef_apply_c16": ([_P, _P, _P, _P, _I, _I, _I, _I, _P], _I),
     "pe_coef_apply_demap_c16": ([_P, _P, _P, _P, _P, _I, _I, _I, _I, _I, _U, _P], _I),
     "pe_slot_plan_create": ([_P, _I, _U, _P], _I),
@@ -249,6 +250,26 @@ def coef_to_c16(C, C16=None, c_exp=None, stream=None):
         c_exp = torch.empty((nSC,), dtype=torch.int32, device=C.device)
     _check(lib().pe_coef_to_c16(_ptr(C), _ptr(C16), _ptr(c_exp), nSC, L, A, _stream(stream)))
     return C16, c_exp
+
+
+def gram_mmse_c16(H, sigma2, unbiased=True, cond_threshold=0.0, C16=None, c_exp=None, noise_inv=None, flags=None,
+                  stream=None):
+    """MMSE coefficients written as binary16 pairs + per-subcarrier exponent (pe_gram_mmse_c16). Returns
+    (C16 int32 [nSC, L, A], c_exp int32 [nSC], noise_inv [nSC, L], sc_flags uint8 [nSC])."""
+    import torch
+    nSC, A, L = H.shape
+    dev = H.device
+    if C16 is None:
+        C16 = torch.empty((nSC, L, A), dtype=torch.int32, device=dev)
+    if c_exp is None:
+        c_exp = torch.empty((nSC,), dtype=torch.int32, device=dev)
+    if noise_inv is None:
+        noise_inv = torch.empty((nSC, L), dtype=torch.float32, device=dev)
+    if flags is None:
+        flags = torch.empty((nSC,), dtype=torch.uint8, device=dev)
+    _check(lib().pe_gram_mmse_c16(_ptr(H), _ptr(sigma2), _ptr(C16), _ptr(c_exp), _ptr(noise_inv), _ptr(flags), nSC,
+                                  L, A, MMSE_UNBIASED if unbiased else 0, float(cond_threshold), _stream(stream)))
+    return C16, c_exp, noise_inv, flags
 
 
 def coef_apply_c16(C16, c_exp, Y, out=None, stream=None):

@@ -509,6 +509,13 @@ def main():
     for k in range(2):
         H, Y, s2 = make_cells(n, 1234 + k, dev)
         C, ni, fl = pe.gram_mmse(H, s2, unbiased=True)
+        if k == 0:  # the MMSE stage over the same batch, complex64 and binary16 coefficient output
+            c16o = pe.gram_mmse_c16(H, s2, unbiased=True)
+            el_m, _ = time_launches(lambda: pe.gram_mmse(H, s2, unbiased=True, C=C, noise_inv=ni, flags=fl), [()],
+                                    10, 2)
+            el_m16, _ = time_launches(lambda: pe.gram_mmse_c16(H, s2, True, 0.0, *c16o), [()], 10, 2)
+            mmse_ms = {"c64": el_m / 10 * 1e3, "c16": el_m16 / 10 * 1e3}
+            del c16o
         H0 = H[:NSC_CELL].clone() if k == 0 else None  # channels are not an input of the timed launch: keep one cell
         del H
         llr = torch.empty((nsc, L, S, 2 * Q), dtype=torch.float32, device=dev)
@@ -667,6 +674,11 @@ def main():
         "llr_f16_wire": {"value": nsc / per16, "ms_per_step": per16 * 1e3, "hbm_gbs": bytes16 / per16 / 1e9,
                          "bytes_per_launch": bytes16,
                          "workload": "same launch, LLRs written as binary16 (pe_coef_apply_demap_f16)"},
+        "mmse_stage": {"ms_per_launch": mmse_ms, "subcarriers": nsc,
+                       "hbm_gbs_c64": nsc * (A * L * 8 + 4 + L * A * 8 + L * 4 + 1) / (mmse_ms["c64"] * 1e-3) / 1e9,
+                       "hbm_gbs_c16": nsc * (A * L * 8 + 4 + L * A * 4 + 4 + L * 4 + 1) / (mmse_ms["c16"] * 1e-3)
+                       / 1e9,
+                       "workload": "pe_gram_mmse / pe_gram_mmse_c16 over the batch (H read, C written)"},
         "c16_coefficients": {"value": nsc / perc, "ms_per_step": perc * 1e3, "hbm_gbs": bytes_c16 / perc / 1e9,
                              "bytes_per_launch": bytes_c16,
                              "workload": "same batches, coefficients held as binary16 pairs + exponent "

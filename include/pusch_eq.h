@@ -172,6 +172,11 @@ pe_status pe_coef_apply_demap_f16(const pe_c64 *C, const pe_c64 *Y_rx, const flo
 #define PE_LLR_F16 1u  /* LLRs written as binary16 (the wire format of pe_coef_apply_demap_f16) */
 
 pe_status pe_coef_to_c16(const pe_c64 *C, pe_c16 *C16, int *c_exp, int nSC, int nLayer, int nAnt, void *stream);
+/* pe_gram_mmse writing the coefficients in this format (half the coefficient bytes written and read); the
+   values are those pe_coef_to_c16 makes of pe_gram_mmse's output, bit for bit. nAnt % 4 == 0, nAnt <= 128. */
+pe_status pe_gram_mmse_c16(const pe_c64 *H, const float *sigma2, pe_c16 *C16, int *c_exp, float *noise_inv,
+                           uint8_t *sc_flags, int nSC, int nLayer, int nAnt, unsigned flags, float cond_threshold,
+                           void *stream);
 pe_status pe_coef_apply_c16(const pe_c16 *C16, const int *c_exp, const pe_c64 *Y_rx, pe_c64 *Y_eq, int nSC,
                             int nLayer, int nSym, int nAnt, void *stream);
 /* llr: float [nSC][nLayer][nSym][2*nPamBits], or binary16 with PE_LLR_F16 */

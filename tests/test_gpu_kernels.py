@@ -328,3 +328,57 @@ def test_c16_apply_wide_range_keeps_accuracy(po, pe, dev):
     out = host(pe.coef_apply_c16(C16, e, dev_c64(Y, dev)))
     assert np.all(np.isfinite(out))
     assert po.evm(ref, out)[1] <= MFMA_EVM_DB["f16"]
+
+
+def _mmse_c16_case(po, kind, rng):
+    if kind == "ill":  # every other subcarrier goes through the FP64 pass
+        H = po.rayleigh_h(rng, 64, 32, 4)
+        H[::2, :, 3] = H[::2, :, 2] + 1e-4 * H[::2, :, 3]
+        return H, np.full(64, 1e-9, np.float32)
+    if kind == "singular":
+        H = po.rayleigh_h(rng, 16, 16, 4)
+        H[:, :, 2] = 0
+        return H, np.zeros(16, np.float32)
+    nsc, A, L = {"l8a64": (1584, 64, 8), "l2a128": (77, 128, 2), "l5a24": (45, 24, 5), "l8a128": (40, 128, 8)}[kind]
+    H = po.rayleigh_h(rng, nsc, A, L)
+    s2 = rng.uniform(1e-3, 0.3, nsc).astype(np.float32)
+    H[3] *= np.float32(1e-5)  # a subcarrier with coefficients above binary16 range: scaled block
+    return H, s2
+
+
+@pytest.mark.parametrize("kind", ["l8a64", "l2a128", "l5a24", "l8a128", "ill", "singular"])
+def test_gram_mmse_c16_equals_conversion_of_c64(po, pe, dev, kind):
+    """pe_gram_mmse_c16 writes exactly what pe_coef_to_c16 makes of pe_gram_mmse's complex64 output (both
+    passes, the FP64 pass and singular subcarriers included); noise_inv and flags unchanged."""
+    rng = np.random.default_rng(len(kind))
+    H, s2 = _mmse_c16_case(po, kind, rng)
+    dH, ds2 = dev_c64(H, dev), dev_f32(s2, dev)
+    C, ni, fl = pe.gram_mmse(dH, ds2, unbiased=True)
+    C16r, er = pe.coef_to_c16(C)
+    C16, e, ni16, fl16 = pe.gram_mmse_c16(dH, ds2, unbiased=True)
+    assert np.array_equal(host(fl16), host(fl)) and np.array_equal(host(ni16), host(ni))
+    assert np.array_equal(host(e), host(er))
+    assert np.array_equal(host(C16), host(C16r))
+    if kind == "ill":
+        assert (host(fl16) == pe.SCFLAG_FP64).sum() == 32
+    if kind.startswith("l"):
+        assert host(e)[3] != 0
+
+
+@pytest.mark.parametrize("snr", [10.0, 30.0])
+def test_c16_chain_accuracy(po, pe, dev, snr):
+    """MMSE with binary16 coefficients, then the fused 256-QAM demap from them: EVM vs the FP64 chain within
+    the binary16 bar and hard decisions equal except near zero."""
+    rng = np.random.default_rng(int(snr))
+    inp = po.chain_inputs(rng, 1584, 64, 8, 11, 4, snr)
+    dH, ds2, dY = dev_c64(inp["H"], dev), dev_f32(inp["sigma2"], dev), dev_c64(inp["Y"], dev)
+    C16, e, ni, fl = pe.gram_mmse_c16(dH, ds2, unbiased=True)
+    yeq = host(pe.coef_apply_c16(C16, e, dY))
+    Cd, nid, _, _, _ = po.mmse_f64(inp["H"], inp["sigma2"], unbiased=True)
+    yref = po.coef_apply_f64(Cd, inp["Y"])
+    assert po.evm(yref, yeq)[1] <= -60.0
+    llr = host(pe.coef_apply_demap_c16(C16, e, dY, ni, 4))
+    ref = po.llr_demap_f64(yref, nid, 4)
+    eps = 1e-2 * np.median(np.abs(ref))
+    sure = np.abs(ref) > eps
+    assert np.array_equal(np.sign(llr[sure]), np.sign(ref[sure]))
