@@ -10,8 +10,9 @@ namespace {
 using namespace mmse;
 
 // One workgroup of W waves per tile of T subcarriers (the last tile may be short); its LDS holds T slots.
-// C16: coefficients written as binary16 pairs to c16 and exponents to cexp (C unused).
-template <int L, int KR, bool C16>
+// C16: coefficients written as binary16 pairs to c16 and exponents to cexp (C unused); NA antennas per lane
+// (1: A <= 64, 2: A <= 128).
+template <int L, int KR, bool C16, int NA = 2>
 __global__ __launch_bounds__(512) void gram_mmse_f32_kernel(const float2 *__restrict__ H,
                                                             const float *__restrict__ sigma2,
                                                             float2 *__restrict__ C, float *__restrict__ noise_inv,
@@ -21,8 +22,8 @@ __global__ __launch_bounds__(512) void gram_mmse_f32_kernel(const float2 *__rest
     extern __shared__ __attribute__((aligned(16))) float lds[];
     const long sc0 = (long)blockIdx.x * T;
     const int n = (int)min((long)T, (long)nSC - sc0);
-    tile_fp32<L, KR, C16>(H, sigma2, C, noise_inv, flags, sc0, n, A, P, unbiased != 0, thr, lds, threadIdx.x,
-                          blockDim.x >> 6, nullptr, c16, cexp);
+    tile_fp32<L, KR, C16, NA>(H, sigma2, C, noise_inv, flags, sc0, n, A, P, unbiased != 0, thr, lds,
+                              threadIdx.x, blockDim.x >> 6, nullptr, c16, cexp);
 }
 
 // One wave per 64 subcarriers; leaves after one flag load unless one of them was flagged.
@@ -51,6 +52,9 @@ hipError_t launch_mmse_L(const float2 *H, const float *sigma2, float2 *C, float 
     const size_t lds = (size_t)tile_lds_bytes(L, T);
     if (keep)
         hipLaunchKernelGGL((gram_mmse_f32_kernel<L, 4, false>), g, b, lds, st, H, sigma2, C, ni, flags, nSC, A, P,
+                           T, unbiased ? 1 : 0, thr, c16, cexp);
+    else if (C16 && A <= 64)
+        hipLaunchKernelGGL((gram_mmse_f32_kernel<L, 0, C16, 1>), g, b, lds, st, H, sigma2, C, ni, flags, nSC, A, P,
                            T, unbiased ? 1 : 0, thr, c16, cexp);
     else
         hipLaunchKernelGGL((gram_mmse_f32_kernel<L, 0, C16>), g, b, lds, st, H, sigma2, C, ni, flags, nSC, A, P,

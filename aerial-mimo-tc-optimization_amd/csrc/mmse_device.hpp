@@ -322,17 +322,17 @@ __device__ __forceinline__ void c_column(const T (&hr)[L], const T (&hi)[L], con
 }
 
 // Binary16 coefficient output (C16 pairs and the subcarrier's exponent, rule in c16_exp) by a group of Q
-// lanes (lane q: antennas q and q + Q, A <= 2Q): the block's values stay in registers until the group's
+// lanes (lane q: antennas q, q + Q, ..., A <= NA Q): the block's values stay in registers until the group's
 // maximum is known. Every lane of the wave calls it (the maximum is a butterfly over lane bits below Q).
-template <int L>
+template <int L, int NA>
 __device__ __forceinline__ void write_c16_group(const float2 *__restrict__ h, int A, int q, int Q,
                                                 const float (&vr)[Dim<L>::NT], const float (&vi)[Dim<L>::NT],
                                                 const float (&scl)[L], uint32_t *__restrict__ c16,
                                                 int *__restrict__ cexp, bool store) {
-    float cr[2][L], ci[2][L];
+    float cr[NA][L], ci[NA][L];
     float m = 0.f;
 #pragma unroll
-    for (int u = 0; u < 2; ++u) {
+    for (int u = 0; u < NA; ++u) {
         const int a = q + u * Q;
         if (a < A) {
             float hr[L], hi[L];
@@ -345,7 +345,7 @@ __device__ __forceinline__ void write_c16_group(const float2 *__restrict__ h, in
     const int e = c16_exp(wave_max_xor(m, 1, Q));
     const float s = ldexpf(1.f, -e);
 #pragma unroll
-    for (int u = 0; u < 2; ++u) {
+    for (int u = 0; u < NA; ++u) {
         const int a = q + u * Q;
         if (store && a < A) {
 #pragma unroll
@@ -437,9 +437,9 @@ __device__ __forceinline__ void tile_barrier(int W) {
 // rows in registers from the Gram to the coefficients instead of reading H a second time (the coefficient
 // stores then cover P consecutive antennas of a C row per lane group). It trades the second read of H for
 // registers (fewer waves per SIMD); which is faster depends on the shape.
-// C16: coefficients go to c16 / cexp as binary16 pairs and exponents (write_c16_group; needs A <= 128 and
-// no kept rows) instead of complex64 to C.
-template <int L, int KR, bool C16 = false>
+// C16: coefficients go to c16 / cexp as binary16 pairs and exponents (write_c16_group with NA antennas per
+// lane: A <= 64 NA; no kept rows) instead of complex64 to C.
+template <int L, int KR, bool C16 = false, int NA = 2>
 __device__ __forceinline__ void tile_fp32(const float2 *__restrict__ H, const float *__restrict__ sigma2,
                                           float2 *__restrict__ C, float *__restrict__ noise_inv,
                                           uint8_t *__restrict__ flags, long sc0, int n, int A, int P,
@@ -558,7 +558,7 @@ __device__ __forceinline__ void tile_fp32(const float2 *__restrict__ H, const fl
 #pragma unroll
         for (int l = 0; l < L; ++l) scl[l] = in[2 * NT + l];
         if constexpr (C16)
-            write_c16_group<L>(H + (size_t)(sc0 + t) * A * L, A, q, 1 << lgQ, vr, vi, scl,
+            write_c16_group<L, NA>(H + (size_t)(sc0 + t) * A * L, A, q, 1 << lgQ, vr, vi, scl,
                                c16 + (size_t)(sc0 + t) * L * A, cexp + sc0 + t, store);
         else
             write_c<float, L>(H + (size_t)(sc0 + t) * A * L, A, q, 1 << lgQ, vr, vi, scl,

@@ -342,7 +342,7 @@ def _mmse_c16_case(po, kind, rng):
     nsc, A, L = {"l8a64": (1584, 64, 8), "l2a128": (77, 128, 2), "l5a24": (45, 24, 5), "l8a128": (40, 128, 8)}[kind]
     H = po.rayleigh_h(rng, nsc, A, L)
     s2 = rng.uniform(1e-3, 0.3, nsc).astype(np.float32)
-    H[3] *= np.float32(1e-5)  # a subcarrier with coefficients above binary16 range: scaled block
+    H[3] *= np.float32(1e5)  # a subcarrier whose coefficients (~1e-5) need the block scaling
     return H, s2
 
 
