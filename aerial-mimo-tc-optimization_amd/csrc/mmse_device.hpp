@@ -543,8 +543,7 @@ __device__ __forceinline__ void tile_fp32(const float2 *__restrict__ H, const fl
     }
     const int lgQ = A >= 64 ? 6 : A <= 1 ? 0 : 32 - __builtin_clz((unsigned)A - 1u);
     const int q = lane & ((1 << lgQ) - 1), g3 = lane >> lgQ, R3 = 64 >> lgQ;
-#pragma unroll 2
-    for (int t0 = wave * R3; !keep && t0 < n; t0 += W * R3) {
+    auto coef_round = [&](int t0) {
         const int t = min(t0 + g3, n - 1);
         float in[2 * NT + L + 1];
         slot_load<2 * NT + L + 1>(lds + t * SLOT, in);
@@ -559,10 +558,16 @@ __device__ __forceinline__ void tile_fp32(const float2 *__restrict__ H, const fl
         for (int l = 0; l < L; ++l) scl[l] = in[2 * NT + l];
         if constexpr (C16)
             write_c16_group<L, NA>(H + (size_t)(sc0 + t) * A * L, A, q, 1 << lgQ, vr, vi, scl,
-                               c16 + (size_t)(sc0 + t) * L * A, cexp + sc0 + t, store);
+                                   c16 + (size_t)(sc0 + t) * L * A, cexp + sc0 + t, store);
         else
             write_c<float, L>(H + (size_t)(sc0 + t) * A * L, A, q, 1 << lgQ, vr, vi, scl,
                               C + (size_t)(sc0 + t) * L * A, store);
+    };
+    if constexpr (C16) {  // the group maximum is a cross-lane butterfly, which the unroller does not duplicate
+        for (int t0 = wave * R3; t0 < n; t0 += W * R3) coef_round(t0);
+    } else {
+#pragma unroll 2
+        for (int t0 = wave * R3; !keep && t0 < n; t0 += W * R3) coef_round(t0);
     }
     tile_barrier(W);  // slot reads done before the workgroup's next tile rewrites them
 }

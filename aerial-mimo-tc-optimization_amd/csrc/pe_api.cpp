@@ -427,6 +427,22 @@ pe_status pe_coef_apply_demap_c16(const pe_c16 *C16, const int *c_exp, const pe_
                           nSym, nAnt, nPamBits, stream);
 }
 
+pe_status pe_bfp_decompress(const uint8_t *in, pe_c64 *Y_rx, int nPrb, int nSym, int nAnt, int iqWidth,
+                            size_t row_stride, float scale, void *stream) {
+    if (nPrb < 0) return fail(PE_ERR_INVALID, "nPrb = %d < 0", nPrb);
+    if (nSym < 1 || nSym > 65535) return fail(PE_ERR_INVALID, "nSym = %d outside [1, 65535]", nSym);
+    if (nAnt < 1) return fail(PE_ERR_INVALID, "nAnt = %d < 1", nAnt);
+    if (iqWidth < 1 || iqWidth > 16) return fail(PE_ERR_INVALID, "iqWidth = %d outside [1, 16]", iqWidth);
+    const size_t bpb = 1 + 3 * (size_t)iqWidth;
+    if (row_stride < (size_t)nPrb * bpb)
+        return fail(PE_ERR_INVALID, "row_stride = %zu < nPrb * (1 + 3 iqWidth) = %zu", row_stride, (size_t)nPrb * bpb);
+    if (nPrb == 0) return ok();
+    if (!in || !Y_rx) return fail(PE_ERR_INVALID, "null pointer");
+    return from_hip(pe::launch_bfp_decompress(in, reinterpret_cast<float2 *>(Y_rx), nPrb, nSym, nAnt, iqWidth,
+                                              row_stride, scale, (hipStream_t)stream),
+                    "bfp_decompress launch");
+}
+
 pe_status pe_srs_filter(pe_c16 *H_est, const pe_c16 *SRS, const pe_c16 *W, const pe_c16 *FOCC, int nAnt,
                         int nBlocks, int K, int nPorts, int foccLen, unsigned flags, void *stream) {
     if (nBlocks < 0) return fail(PE_ERR_INVALID, "nBlocks = %d < 0", nBlocks);

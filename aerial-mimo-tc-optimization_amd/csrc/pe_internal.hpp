@@ -69,6 +69,9 @@ hipError_t launch_coef_to_c16(const float2 *C, uint32_t *C16, int *cexp, int nSC
 hipError_t launch_coef_apply_c16(const uint32_t *C16, const int *cexp, const float2 *Y, float2 *O,
                                  const float *noise_inv, void *llr, int nSC, int L, int S, int A, int nPamBits,
                                  int llr_f16, hipStream_t st);
+// Fronthaul block floating point rows -> Y_rx [sc][sym][ant] (iqWidth 1..16).
+hipError_t launch_bfp_decompress(const uint8_t *in, float2 *Y, int nPrb, int nSym, int nAnt, int iqWidth,
+                                 size_t row_stride, float scale, hipStream_t st);
 // Streaming probes (HBM ceiling): read_only -> reads src, else copies src to dst.
 hipError_t launch_hbm_probe(const void *src, void *dst, size_t bytes, int read_only, int grid, hipStream_t st);
 // MMSE: FP32 pass (flags subcarriers for FP64) then the FP64 pass over flagged subcarriers.

@@ -45,6 +45,7 @@ SIGNATURES = {
     "pe_coef_apply_demap_f16": ([_P, _P, _P, _P, _I, _I, _I, _I, _I, _I, _P], _I),
     "pe_srs_filter": ([_P, _P, _P, _P, _I, _I, _I, _I, _I, _U, _P], _I),
     "pe_hbm_probe": ([_P, _P, ctypes.c_size_t, _I, _I, _P], _I),
+    "pe_bfp_decompress": ([_P, _P, _I, _I, _I, _I, ctypes.c_size_t, _F, _P], _I),
     "pe_coef_to_c16": ([_P, _P, _P, _I, _I, _I, _P], _I),
     "pe_gram_mmse_c16": ([_P, _P, _P, _P, _P, _P, _I, _I, _I, _U, _F, _P], _I),
     "pe_coef_apply_c16": ([_P, _P, _P, _P, _I, _I, _I, _I, _P], _I),
@@ -293,6 +294,17 @@ def coef_apply_demap_c16(C16, c_exp, Y, noise_inv, n_pam_bits, llr_f16=False, ou
                           device=Y.device)
     _check(lib().pe_coef_apply_demap_c16(_ptr(C16), _ptr(c_exp), _ptr(Y), _ptr(noise_inv), _ptr(out), nSC, L, S, A,
                                          n_pam_bits, LLR_F16 if llr_f16 else 0, _stream(stream)))
+    return out
+
+
+def bfp_decompress(packed, nPrb, nSym, nAnt, iq_width, scale=1.0, out=None, stream=None):
+    """Fronthaul block floating point rows (uint8 tensor [nSym*nAnt, row_stride]) -> complex64 Y_rx
+    [nPrb*12, nSym, nAnt] (pe_bfp_decompress)."""
+    import torch
+    if out is None:
+        out = torch.empty((nPrb * 12, nSym, nAnt), dtype=torch.complex64, device=packed.device)
+    _check(lib().pe_bfp_decompress(_ptr(packed), _ptr(out), nPrb, nSym, nAnt, iq_width, packed.shape[1],
+                                   float(scale), _stream(stream)))
     return out
 
 

@@ -327,6 +327,25 @@ def srs_bench(dev, iters=500, warmup=50):
     return rows
 
 
+def fronthaul_bench(dev, n_cells, steps=50):
+    """pe_bfp_decompress of the headline batch: n_cells x 132 PRB, 11 symbols, 64 antennas, 9-bit block
+    floating point (28 bytes per PRB) -> complex64 Y_rx; random bytes (every byte pattern is a valid block)."""
+    import torch
+    import pusch_eq as pe
+    nPrb = n_cells * NSC_CELL // 12
+    g = torch.Generator(device=dev)
+    g.manual_seed(7)
+    packed = torch.randint(0, 256, (S * A, nPrb * 28), dtype=torch.uint8, device=dev, generator=g)
+    out = torch.empty((nPrb * 12, S, A), dtype=torch.complex64, device=dev)
+    el, _ = time_launches(lambda: pe.bfp_decompress(packed, nPrb, S, A, 9, 2.0 ** -15, out=out), [()], steps, 5)
+    per = el / steps
+    nbytes = packed.numel() + out.numel() * 8
+    del packed, out
+    return {"workload": "%d PRB x 11 symbols x 64 antennas, 9-bit block floating point -> complex64 Y_rx" % nPrb,
+            "ms_per_launch": per * 1e3, "subcarriers_per_s": nPrb * 12 / per, "hbm_gbs": nbytes / per / 1e9,
+            "bytes_per_launch": nbytes}
+
+
 def serving_bench(dev, n_jobs):
     """64 cells; jobs (one slot of one cell: MMSE + fused apply/demap of 11 symbols) arrive as a Poisson
     process; capacity = throughput with every job queued at t = 0; then offered load 10-120% of capacity."""
@@ -624,6 +643,8 @@ def main():
 
     multicell = None if args.no_multicell else multicell_bench(dev, 50)
     srs = None if args.no_srs else srs_bench(dev)
+    fronthaul = fronthaul_bench(dev, n)
+    log("fronthaul decompress %.1f GB/s" % fronthaul["hbm_gbs"])
     serving = serving_bench(dev, args.serve_jobs) if args.serve_jobs > 0 else None
     sweep = None
     if not args.no_sweep:
@@ -691,6 +712,7 @@ def main():
         "multicell_slot": multicell,
         "serving": serving,
         "srs_filter": srs,
+        "fronthaul_bfp_decompress": fronthaul,
         "deployment_sweep": sweep,
         "wall_s_timed": wall,
     }

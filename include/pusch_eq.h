@@ -202,6 +202,18 @@ pe_status pe_coef_apply_demap_c16(const pe_c16 *C16, const int *c_exp, const pe_
 pe_status pe_srs_filter(pe_c16 *H_est, const pe_c16 *SRS, const pe_c16 *W, const pe_c16 *FOCC, int nAnt,
                         int nBlocks, int K, int nPorts, int foccLen, unsigned flags, void *stream);
 
+/* ---- fronthaul input: block floating point U-plane data -> Y_rx ----------------------------------- */
+
+/* No reference kernel: the reference cites the O-RAN WG4 CUS-plane specification (README.md:214) for the
+   receiver's input. Block floating point: in holds nSym * nAnt rows (row sym * nAnt + ant starts at
+   (sym * nAnt + ant) * row_stride bytes), each nPrb compressed PRBs of 1 + 3 * iqWidth bytes: a udCompParam byte
+   whose bits 3..0 are the exponent e, then 24 mantissas I0 Q0 I1 Q1 ... I11 Q11 of iqWidth bits, two's
+   complement, most significant bit first. Output Y_rx[nPrb * 12][nSym][nAnt] (the coefficient-apply layout,
+   subcarrier = 12 prb + re) = scale * mantissa * 2^e, one rounding (the multiplication by scale).
+   1 <= iqWidth <= 16, row_stride >= nPrb * (1 + 3 * iqWidth). */
+pe_status pe_bfp_decompress(const uint8_t *in, pe_c64 *Y_rx, int nPrb, int nSym, int nAnt, int iqWidth,
+                            size_t row_stride, float scale, void *stream);
+
 /* ---- HBM streaming probe ---------------------------------------------------------------------------- */
 
 /* No reference counterpart: measures the device's streaming ceiling, the yardstick for the bandwidth-bound

@@ -17,6 +17,7 @@
  *   src/llr/aerial_llr_v3.cu  cpu_ref :61-79, inputs :268-273
  *   src/srs/srs_wmma_bench.cu complex_conjmul :43-47, hcmadd :48-52, srs_filter_v1_scalar :57-90,
  *                             inputs :262-268
+ *   fronthaul block floating point: O-RAN WG4 CUS-plane format, cited only (README.md:214)
  */
 #include "pe_oracle.h"
 
@@ -728,4 +729,78 @@ void po_srs_bench_inputs(unsigned seed, int nAnt, int nBlocks, int K, int nPorts
     for (int t = 0; t < 3; t++)
         for (size_t i = 0; i < 2 * n[t]; i++)
             dst[t][i] = po_f32_to_f16(((float)rand() / RAND_MAX - 0.5f) * 1.4f);
+}
+
+/* ------------------------------------------------------------------------------------------------------
+ * Fronthaul block floating point (O-RAN WG4 CUS-plane U-plane compression, cited by README.md:214; no code
+ * in the reference). A compressed PRB is one udCompParam byte (exponent in bits 3..0) followed by 24
+ * mantissas I0 Q0 I1 Q1 ... I11 Q11, each an iqWidth-bit two's complement integer, packed most significant
+ * bit first; sample = mantissa * 2^exponent. Blocks are stored per (symbol, antenna) row, PRBs consecutive:
+ * in[(sym * nAnt + ant) * row_stride + prb * (1 + 3 * iqWidth)].
+ * ------------------------------------------------------------------------------------------------------ */
+static int bfp_get(const uint8_t *blk, int bitoff, int w) {
+    uint32_t v = 0;
+    for (int b = 0; b < w; b++) {
+        const int bit = bitoff + b;
+        v = (v << 1) | ((blk[bit >> 3] >> (7 - (bit & 7))) & 1u);
+    }
+    return (v & (1u << (w - 1))) ? (int)v - (1 << w) : (int)v; /* sign extension */
+}
+
+static void bfp_put(uint8_t *blk, int bitoff, int w, int m) {
+    const uint32_t v = (uint32_t)m & ((1u << w) - 1u);
+    for (int b = 0; b < w; b++) {
+        const int bit = bitoff + b;
+        const uint8_t one = (uint8_t)(1u << (7 - (bit & 7)));
+        if ((v >> (w - 1 - b)) & 1u) blk[bit >> 3] |= one;
+        else blk[bit >> 3] &= (uint8_t)~one;
+    }
+}
+
+/* Y[(sc * nSym + sym) * nAnt + ant] (complex float, sc = prb * 12 + re) = scale * mantissa * 2^exponent */
+void po_bfp_decompress(const uint8_t *in, float *Y, int nPrb, int nSym, int nAnt, int iqWidth, size_t row_stride,
+                       float scale) {
+    const int bpb = 1 + 3 * iqWidth;
+    for (int sym = 0; sym < nSym; sym++)
+        for (int ant = 0; ant < nAnt; ant++)
+            for (int prb = 0; prb < nPrb; prb++) {
+                const uint8_t *blk = in + ((size_t)sym * nAnt + ant) * row_stride + (size_t)prb * bpb;
+                const int e = blk[0] & 0xf;
+                for (int re = 0; re < 12; re++) {
+                    const int mi = bfp_get(blk, 8 + (2 * re) * iqWidth, iqWidth);
+                    const int mq = bfp_get(blk, 8 + (2 * re + 1) * iqWidth, iqWidth);
+                    const size_t o = (((size_t)(prb * 12 + re) * nSym + sym) * nAnt + ant) * 2;
+                    Y[o] = (float)ldexp((double)mi, e) * scale;
+                    Y[o + 1] = (float)ldexp((double)mq, e) * scale;
+                }
+            }
+}
+
+/* Test-vector encoder: the smallest exponent e for which every value of the PRB, shifted right by e
+   (arithmetic shift, i.e. rounded toward minus infinity), fits iqWidth-bit two's complement. Samples are
+   int32 [sym][ant][sc][2] (sc = prb * 12 + re). */
+void po_bfp_compress(const int32_t *x, uint8_t *out, int nPrb, int nSym, int nAnt, int iqWidth, size_t row_stride) {
+    const int bpb = 1 + 3 * iqWidth;
+    const int32_t hi = (1 << (iqWidth - 1)) - 1, lo = -(1 << (iqWidth - 1));
+    for (int sym = 0; sym < nSym; sym++)
+        for (int ant = 0; ant < nAnt; ant++)
+            for (int prb = 0; prb < nPrb; prb++) {
+                const int32_t *v = x + (((size_t)sym * nAnt + ant) * nPrb * 12 + (size_t)prb * 12) * 2;
+                int e = 0;
+                for (;; e++) {
+                    int fits = 1;
+                    for (int k = 0; k < 24; k++) {
+                        const int32_t m = v[k] >> e;
+                        if (m > hi || m < lo) fits = 0;
+                    }
+                    if (fits || e == 15) break;
+                }
+                uint8_t *blk = out + ((size_t)sym * nAnt + ant) * row_stride + (size_t)prb * bpb;
+                blk[0] = (uint8_t)e;
+                for (int k = 0; k < 24; k++) {
+                    int32_t m = v[k] >> e;
+                    m = m > hi ? hi : (m < lo ? lo : m);
+                    bfp_put(blk, 8 + k * iqWidth, iqWidth, m);
+                }
+            }
 }

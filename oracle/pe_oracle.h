@@ -70,4 +70,8 @@ void po_srs_filter_f64(double *H_est, const uint16_t *SRS, const uint16_t *W, co
                        int nBlocks, int K, int nPorts, int foccLen, int conj_w, int prod_f16);
 void po_srs_bench_inputs(unsigned seed, int nAnt, int nBlocks, int K, int nPorts, int foccLen, uint16_t *SRS,
                          uint16_t *W, uint16_t *FOCC);
+/* -- fronthaul block floating point (O-RAN U-plane) -- */
+void po_bfp_decompress(const uint8_t *in, float *Y, int nPrb, int nSym, int nAnt, int iqWidth, size_t row_stride,
+                       float scale);
+void po_bfp_compress(const int32_t *x, uint8_t *out, int nPrb, int nSym, int nAnt, int iqWidth, size_t row_stride);
 #endif

@@ -53,6 +53,8 @@ def lib():
             "po_srs_filter_v1": ([P, P, P, P, I, I, I, I, I, I], None),
             "po_srs_filter_f64": ([P, P, P, P, I, I, I, I, I, I, I], None),
             "po_srs_bench_inputs": ([U, I, I, I, I, I, P, P, P], None),
+            "po_bfp_decompress": ([P, P, I, I, I, I, Z, F], None),
+            "po_bfp_compress": ([P, P, I, I, I, I, Z], None),
         }
         for name, (args, res) in sig.items():
             fn = getattr(h, name)
@@ -390,3 +392,29 @@ def f16_complex(bits):
     """[..., 2] binary16 bit patterns -> complex128."""
     v = np.ascontiguousarray(bits, dtype=np.uint16).view(np.float16).astype(np.float64)
     return v[..., 0] + 1j * v[..., 1]
+
+
+# ---------------------------------------------------------------------------------------------------------
+# fronthaul block floating point (O-RAN U-plane compression format)
+# ---------------------------------------------------------------------------------------------------------
+def bfp_block_bytes(iq_width):
+    return 1 + 3 * iq_width
+
+
+def bfp_compress(x, iq_width, row_stride=None):
+    """int32 samples [nSym, nAnt, nPrb*12, 2] -> packed rows uint8 [nSym*nAnt, row_stride]."""
+    x = np.ascontiguousarray(x, dtype=np.int32)
+    nSym, nAnt, nSC, _ = x.shape
+    nPrb = nSC // 12
+    rs = row_stride or nPrb * bfp_block_bytes(iq_width)
+    out = np.zeros((nSym * nAnt, rs), dtype=np.uint8)
+    lib().po_bfp_compress(_p(x), _p(out), nPrb, nSym, nAnt, iq_width, rs)
+    return out
+
+
+def bfp_decompress(packed, nPrb, nSym, nAnt, iq_width, scale=1.0):
+    """packed rows [nSym*nAnt, row_stride] -> complex64 Y_rx [nPrb*12, nSym, nAnt]."""
+    packed = np.ascontiguousarray(packed, dtype=np.uint8)
+    Y = np.empty((nPrb * 12, nSym, nAnt, 2), dtype=np.float32)
+    lib().po_bfp_decompress(_p(packed), _p(Y), nPrb, nSym, nAnt, iq_width, packed.shape[1], float(scale))
+    return Y.view(np.complex64)[..., 0]

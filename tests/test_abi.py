@@ -20,6 +20,7 @@ def test_header_declares_the_operator_surface():
     names = declared_functions()
     for required in ("pe_coef_apply", "pe_gram_mmse", "pe_llr_demap_f16", "pe_llr_demap", "pe_coef_apply_demap",
                      "pe_coef_apply_demap_f16", "pe_srs_filter", "pe_coef_to_c16", "pe_coef_apply_demap_c16",
+                     "pe_bfp_decompress", "pe_gram_mmse_c16",
                      "pe_gram", "pe_coef_apply_v1"):
         assert required in names
 
@@ -94,6 +95,11 @@ def test_other_entry_points_validate(pe):
     assert h.pe_hbm_probe(n, n, 0, 0, 64, None) == 0                          # empty
     assert h.pe_hbm_probe(p, ctypes.c_void_p(256), 32, 0, 64, None) == 1      # unaligned src
     a16 = ctypes.c_void_p(256)
+    # fronthaul: (in, Y_rx, nPrb, nSym, nAnt, iqWidth, row_stride, scale, stream)
+    assert h.pe_bfp_decompress(n, n, 0, 11, 64, 9, 0, 1.0, None) == 0
+    assert h.pe_bfp_decompress(p, p, 4, 11, 64, 17, 4 * 52, 1.0, None) == 1   # iqWidth
+    assert h.pe_bfp_decompress(p, p, 4, 11, 64, 9, 4 * 28 - 1, 1.0, None) == 1  # row too short
+    assert h.pe_bfp_decompress(p, n, 4, 11, 64, 9, 4 * 28, 1.0, None) == 1     # null output
     # binary16 coefficients: (C16, c_exp, Y, noise_inv, llr, nSC, L, S, A, q, flags, stream)
     assert h.pe_coef_to_c16(n, n, n, 0, 8, 64, None) == 0
     assert h.pe_coef_to_c16(a16, a16, n, 4, 8, 64, None) == 1              # null c_exp

@@ -382,3 +382,39 @@ def test_c16_chain_accuracy(po, pe, dev, snr):
     eps = 1e-2 * np.median(np.abs(ref))
     sure = np.abs(ref) > eps
     assert np.array_equal(np.sign(llr[sure]), np.sign(ref[sure]))
+
+
+# ------------------------------------------------------------------------------------------------------
+# fronthaul block floating point -> Y_rx
+# ------------------------------------------------------------------------------------------------------
+@pytest.mark.parametrize("w,nPrb,nSym,nAnt,pad", [(9, 132, 11, 64, 0), (14, 25, 3, 16, 7), (16, 7, 2, 130, 1),
+                                                  (1, 3, 1, 5, 0), (8, 273, 2, 4, 3), (12, 1, 14, 65, 0)])
+def test_bfp_decompress_bitexact(po, pe, dev, w, nPrb, nSym, nAnt, pad):
+    """Arbitrary bytes are valid blocks: the GPU decoder equals the oracle's bit for bit (scale 2^-15 and an
+    arbitrary scale)."""
+    import torch
+    rng = np.random.default_rng(w * 100 + nPrb)
+    stride = nPrb * po.bfp_block_bytes(w) + pad
+    packed = rng.integers(0, 256, (nSym * nAnt, stride), dtype=np.uint8)
+    dp = torch.from_numpy(packed).to(dev)
+    for scale in (2.0 ** -15, 0.37):
+        ref = po.bfp_decompress(packed, nPrb, nSym, nAnt, w, scale)
+        out = host(pe.bfp_decompress(dp, nPrb, nSym, nAnt, w, scale))
+        assert np.array_equal(out.view(np.uint32), ref.view(np.uint32))
+
+
+def test_bfp_to_fused_apply(po, pe, dev):
+    """Compressed fronthaul samples decoded straight into the fused apply + demap equal the same samples
+    given as complex64."""
+    import torch
+    rng = np.random.default_rng(5)
+    nPrb, S, A, L, q = 20, 11, 32, 4, 4
+    x = rng.integers(-20000, 20000, (S, A, nPrb * 12, 2)).astype(np.int32)
+    packed = po.bfp_compress(x, 9)
+    Y_ref = po.bfp_decompress(packed, nPrb, S, A, 9, 2.0 ** -15)
+    Y = pe.bfp_decompress(torch.from_numpy(packed).to(dev), nPrb, S, A, 9, 2.0 ** -15)
+    H = po.rayleigh_h(rng, nPrb * 12, A, L)
+    C, ni, _ = pe.gram_mmse(dev_c64(H, dev), dev_f32(np.full(nPrb * 12, 0.01, np.float32), dev))
+    a = host(pe.coef_apply_demap(C, Y, ni, q))
+    b = host(pe.coef_apply_demap(C, dev_c64(Y_ref, dev), ni, q))
+    assert np.array_equal(a, b)

@@ -383,3 +383,61 @@ def test_srs_bench_inputs_follow_the_stream(po):
     n1, n2 = S.size, S.size + W.size
     assert np.array_equal(S.ravel(), bits[:n1]) and np.array_equal(W.ravel(), bits[n1:n2])
     assert np.array_equal(F.ravel(), bits[n2:])
+
+
+# ------------------------------------------------------------------------------------------------------
+# fronthaul block floating point
+# ------------------------------------------------------------------------------------------------------
+def _bfp_block(exponent, mantissas, w):
+    """One compressed PRB built from a bit string written out here (independent of the oracle's packer)."""
+    bits = format(exponent & 0xF, "08b") + "".join(format(m & ((1 << w) - 1), "0%db" % w) for m in mantissas)
+    bits += "0" * (-len(bits) % 8)
+    return np.array([int(bits[i:i + 8], 2) for i in range(0, len(bits), 8)], dtype=np.uint8)
+
+
+@pytest.mark.parametrize("w", [1, 8, 9, 14, 16])
+def test_bfp_decompress_known_blocks(po, w):
+    """Hand-packed PRBs: two's complement extremes, sign extension, exponent, odd block lengths and padded
+    rows; subcarrier = 12 prb + re, I then Q."""
+    rng = np.random.default_rng(w)
+    lo, hi = -(1 << (w - 1)), (1 << (w - 1)) - 1
+    nPrb, nSym, nAnt = 3, 2, 2
+    bpb = po.bfp_block_bytes(w)
+    assert all(len(_bfp_block(0, [0] * 24, w)) == bpb for _ in range(1))
+    stride = nPrb * bpb + 5  # padded rows
+    packed = np.full((nSym * nAnt, stride), 0xAB, dtype=np.uint8)
+    expect = np.zeros((nPrb * 12, nSym, nAnt), dtype=np.complex128)
+    for sym in range(nSym):
+        for ant in range(nAnt):
+            for prb in range(nPrb):
+                e = int(rng.integers(0, 16))
+                m = rng.integers(lo, hi + 1, 24)
+                m[:4] = [lo, hi, -1 if w > 1 else lo, 0]
+                packed[sym * nAnt + ant, prb * bpb:(prb + 1) * bpb] = _bfp_block(e | 0xA0, m, w)  # reserved bits set
+                expect[prb * 12:(prb + 1) * 12, sym, ant] = (m[0::2] + 1j * m[1::2]) * 2.0 ** e
+    Y = po.bfp_decompress(packed, nPrb, nSym, nAnt, w, scale=2.0 ** -15)
+    assert np.array_equal(Y.astype(np.complex128), expect * 2.0 ** -15)
+
+
+@pytest.mark.parametrize("w", [9, 14])
+def test_bfp_round_trip(po, w):
+    """The oracle encoder picks the smallest exponent that fits; decoding returns the samples rounded toward
+    minus infinity at that exponent, exactly when it is 0."""
+    rng = np.random.default_rng(40 + w)
+    nPrb, nSym, nAnt = 5, 3, 4
+    x = rng.integers(-(1 << 15), 1 << 15, (nSym, nAnt, nPrb * 12, 2)).astype(np.int32)
+    x[0, 0, :12] //= 1 << 10  # a PRB small enough for exponent 0 at w = 9
+    packed = po.bfp_compress(x, w)
+    Y = po.bfp_decompress(packed, nPrb, nSym, nAnt, w)
+    got = np.stack([Y.real, Y.imag], -1).transpose(1, 2, 0, 3)  # [sym, ant, sc, 2]
+    e = packed[:, ::po.bfp_block_bytes(w)][:, :nPrb].reshape(nSym, nAnt, nPrb) & 0xF
+    step = np.repeat(2.0 ** e, 12, axis=2)[..., None]
+    assert np.all(got <= x) and np.all(x - got < step)
+    assert np.all((x // step) * step == got)
+    assert e[0, 0, 0] == 0 and np.array_equal(got[0, 0, :12], x[0, 0, :12])
+    # minimal: at exponent e - 1 some value of the PRB would not fit
+    lo, hi = -(1 << (w - 1)), (1 << (w - 1)) - 1
+    xb = x.reshape(nSym, nAnt, nPrb, 24)
+    for idx in zip(*np.nonzero(e)):
+        m = xb[idx] >> (int(e[idx]) - 1)
+        assert m.max() > hi or m.min() < lo
