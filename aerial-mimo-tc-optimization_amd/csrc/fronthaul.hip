@@ -5,10 +5,11 @@
 // iqWidth bits, two's complement, packed most significant bit first (1 + 3 iqWidth bytes). A sample is
 // mantissa * 2^exponent, times the caller's scale in Y_rx.
 //
-// One 64-lane workgroup per (PRB, symbol); lane = antenna (looping past 64). A lane loads its PRB's bytes
-// into registers (byte loads: blocks of odd length are not word-aligned), extracts the 24 mantissas at
-// compile-time bit offsets (W is a template parameter), and stores the 12 complex samples; for each
-// subcarrier the wave's stores cover consecutive antennas, i.e. contiguous 8-byte values. Every product
+// One 256-lane workgroup per (symbol, 8 PRBs, 64 antennas). The input is read row by row (each antenna row's
+// 8 PRBs are contiguous bytes: coalesced loads, 4-byte words when rows and blocks are word-aligned, bytes
+// otherwise) into LDS; then each lane decodes one PRB of one antenna from LDS (W, the mantissa width, is a
+// template parameter, so every bit offset is a constant) and stores its 12 complex samples. For each
+// subcarrier a wave's stores cover 64 consecutive antennas: contiguous 512-byte rows of Y_rx. Every product
 // mantissa * 2^exponent is exact in float (at most 16 significant bits, |value| <= 2^30), so the only rounding
 // is the multiplication by scale.
 #include "pe_internal.hpp"
@@ -17,14 +18,40 @@ namespace pe {
 
 namespace {
 
-template <int W>
-__global__ __launch_bounds__(64) void bfp_decompress_kernel(const uint8_t *__restrict__ in, float2 *__restrict__ Y,
-                                                            int nPrb, int nSym, int nAnt, size_t row_stride,
-                                                            float scale) {
+constexpr int kPrbPerWg = 8;
+
+template <int W, bool WORDS>
+__global__ __launch_bounds__(256) void bfp_decompress_kernel(const uint8_t *__restrict__ in, float2 *__restrict__ Y,
+                                                             int nPrb, int nSym, int nAnt, size_t row_stride,
+                                                             float scale) {
     constexpr int BPB = 1 + 3 * W;
-    const int prb = blockIdx.x, sym = blockIdx.y;
-    for (int ant = threadIdx.x; ant < nAnt; ant += 64) {
-        const uint8_t *blk = in + ((size_t)sym * nAnt + ant) * row_stride + (size_t)prb * BPB;
+    constexpr int SPAN = kPrbPerWg * BPB;              // bytes of one antenna row in this workgroup
+    constexpr int PITCH = ((SPAN + 3) / 4) | 1;        // LDS row pitch in words, odd: lanes on distinct banks
+    __shared__ uint32_t rows[64 * PITCH];
+    const int prb0 = blockIdx.x * kPrbPerWg, sym = blockIdx.y, ant0 = blockIdx.z * 64;
+    const int np = min(kPrbPerWg, nPrb - prb0), na = min(64, nAnt - ant0);
+    const int span = np * BPB;
+    const uint8_t *base = in + ((size_t)sym * nAnt + ant0) * row_stride + (size_t)prb0 * BPB;
+    if constexpr (WORDS) {  // rows and spans are whole words
+        const int nw = span >> 2;
+        for (int i = threadIdx.x; i < na * nw; i += 256) {
+            const int a = i / nw, k = i - a * nw;
+            rows[a * PITCH + k] = reinterpret_cast<const uint32_t *>(base + (size_t)a * row_stride)[k];
+        }
+    } else {
+        uint8_t *rb = reinterpret_cast<uint8_t *>(rows);
+        for (int i = threadIdx.x; i < na * span; i += 256) {
+            const int a = i / span, k = i - a * span;
+            rb[a * PITCH * 4 + k] = base[(size_t)a * row_stride + k];
+        }
+    }
+    __syncthreads();
+
+    const int a = threadIdx.x & 63;
+    if (a >= na) return;  // no barrier follows
+    const uint8_t *rb = reinterpret_cast<const uint8_t *>(rows) + a * PITCH * 4;
+    for (int p = threadIdx.x >> 6; p < np; p += 4) {
+        const uint8_t *blk = rb + p * BPB;
         uint32_t b[BPB + 2];
 #pragma unroll
         for (int i = 0; i < BPB; ++i) b[i] = blk[i];
@@ -41,17 +68,27 @@ __global__ __launch_bounds__(64) void bfp_decompress_kernel(const uint8_t *__res
             const int32_t m = raw >= (1 << (W - 1)) ? raw - (1 << W) : raw;
             v[k] = (float)m * s;
         }
-        float2 *out = Y + ((size_t)prb * 12 * nSym + sym) * nAnt + ant;
+        typedef float f2v __attribute__((ext_vector_type(2)));
+        f2v *out = reinterpret_cast<f2v *>(Y + ((size_t)(prb0 + p) * 12 * nSym + sym) * nAnt + ant0 + a);
 #pragma unroll
-        for (int re = 0; re < 12; ++re) out[(size_t)re * nSym * nAnt] = make_float2(v[2 * re], v[2 * re + 1]);
+        for (int re = 0; re < 12; ++re) {
+            const f2v val = {v[2 * re], v[2 * re + 1]};
+            __builtin_nontemporal_store(val, out + (size_t)re * nSym * nAnt);  // written once, streamed out
+        }
     }
 }
 
 template <int W>
 hipError_t launch_w(const uint8_t *in, float2 *Y, int nPrb, int nSym, int nAnt, size_t rs, float scale,
                     hipStream_t st) {
-    hipLaunchKernelGGL((bfp_decompress_kernel<W>), dim3((unsigned)nPrb, (unsigned)nSym), dim3(64), 0, st, in, Y,
-                       nPrb, nSym, nAnt, rs, scale);
+    const dim3 g((unsigned)((nPrb + kPrbPerWg - 1) / kPrbPerWg), (unsigned)nSym, (unsigned)((nAnt + 63) / 64));
+    const bool words = (1 + 3 * W) % 4 == 0 && rs % 4 == 0 && (reinterpret_cast<uintptr_t>(in) & 3u) == 0;
+    if (words)
+        hipLaunchKernelGGL((bfp_decompress_kernel<W, true>), g, dim3(256), 0, st, in, Y, nPrb, nSym, nAnt, rs,
+                           scale);
+    else
+        hipLaunchKernelGGL((bfp_decompress_kernel<W, false>), g, dim3(256), 0, st, in, Y, nPrb, nSym, nAnt, rs,
+                           scale);
     return hipGetLastError();
 }
 

@@ -431,7 +431,7 @@ pe_status pe_bfp_decompress(const uint8_t *in, pe_c64 *Y_rx, int nPrb, int nSym,
                             size_t row_stride, float scale, void *stream) {
     if (nPrb < 0) return fail(PE_ERR_INVALID, "nPrb = %d < 0", nPrb);
     if (nSym < 1 || nSym > 65535) return fail(PE_ERR_INVALID, "nSym = %d outside [1, 65535]", nSym);
-    if (nAnt < 1) return fail(PE_ERR_INVALID, "nAnt = %d < 1", nAnt);
+    if (nAnt < 1 || nAnt > 4096) return fail(PE_ERR_INVALID, "nAnt = %d outside [1, 4096]", nAnt);
     if (iqWidth < 1 || iqWidth > 16) return fail(PE_ERR_INVALID, "iqWidth = %d outside [1, 16]", iqWidth);
     const size_t bpb = 1 + 3 * (size_t)iqWidth;
     if (row_stride < (size_t)nPrb * bpb)

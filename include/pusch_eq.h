@@ -210,7 +210,7 @@ pe_status pe_srs_filter(pe_c16 *H_est, const pe_c16 *SRS, const pe_c16 *W, const
    whose bits 3..0 are the exponent e, then 24 mantissas I0 Q0 I1 Q1 ... I11 Q11 of iqWidth bits, two's
    complement, most significant bit first. Output Y_rx[nPrb * 12][nSym][nAnt] (the coefficient-apply layout,
    subcarrier = 12 prb + re) = scale * mantissa * 2^e, one rounding (the multiplication by scale).
-   1 <= iqWidth <= 16, row_stride >= nPrb * (1 + 3 * iqWidth). */
+   1 <= iqWidth <= 16, 1 <= nAnt <= 4096, row_stride >= nPrb * (1 + 3 * iqWidth). */
 pe_status pe_bfp_decompress(const uint8_t *in, pe_c64 *Y_rx, int nPrb, int nSym, int nAnt, int iqWidth,
                             size_t row_stride, float scale, void *stream);
 
