@@ -317,7 +317,11 @@ __device__ __forceinline__ void c_column(const T (&hr)[L], const T (&hi)[L], con
     c_values<T, L>(hr, hi, vr, vi, scl, cr, ci);
     if (store) {
 #pragma unroll
-        for (int l = 0; l < L; ++l) c_out[(size_t)l * A + a] = make_float2(cr[l], ci[l]);
+        for (int l = 0; l < L; ++l) {
+            typedef float f2v __attribute__((ext_vector_type(2)));
+            const f2v v = {cr[l], ci[l]};
+            __builtin_nontemporal_store(v, reinterpret_cast<f2v *>(c_out) + (size_t)l * A + a);
+        }
     }
 }
 
@@ -349,7 +353,8 @@ __device__ __forceinline__ void write_c16_group(const float2 *__restrict__ h, in
         const int a = q + u * Q;
         if (store && a < A) {
 #pragma unroll
-            for (int l = 0; l < L; ++l) c16[(size_t)l * A + a] = pack_half2(cr[u][l] * s, ci[u][l] * s);
+            for (int l = 0; l < L; ++l)
+                __builtin_nontemporal_store(pack_half2(cr[u][l] * s, ci[u][l] * s), c16 + (size_t)l * A + a);
         }
     }
     if (store && q == 0) *cexp = e;
