@@ -64,7 +64,8 @@ def test_serve_deadline_aware_admission_under_overload(po, pe, dev):
     import torch
     cells, refs = population(po, pe, dev, 23)
     descs = [c[0] for c in cells]
-    n_jobs = 400
+    n_jobs = 1200  # at 3x capacity the serve-everything queue grows to several deadlines
+    pe.serve_run(descs, 200, 0.0, hbm_budget=256 << 20, max_batch=16, max_inflight=2, use_graph=False, seed=4)  # warm-up
     cap, _, _ = pe.serve_run(descs, n_jobs, 0.0, hbm_budget=256 << 20, max_batch=16, max_inflight=2,
                              use_graph=False, seed=5)
     rate = 3.0 * cap["jobs_per_s"]
@@ -78,7 +79,7 @@ def test_serve_deadline_aware_admission_under_overload(po, pe, dev):
     print("capacity", cap["jobs_per_s"], "serve-all", base, "deadline-aware", rep)
     assert rep["iso_job_s"] > 0 and rep["contention_index"] > 0
     assert rep["n_done"] + rep["n_shed"] == n_jobs and rep["n_shed"] > 0 and rep["n_done"] > 0
-    assert base["n_shed"] == 0 and base["p99_s"] > 2 * deadline
+    assert base["n_shed"] == 0 and base["p99_s"] > 4 * deadline
     assert rep["p99_s"] < base["p99_s"]
     assert rep["p50_s"] <= 1.5 * deadline
     for j in np.nonzero(lat > 0)[0]:
