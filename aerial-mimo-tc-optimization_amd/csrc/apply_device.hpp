@@ -403,6 +403,22 @@ __device__ __attribute__((noinline)) void apply_one_sc(const float2 *__restrict_
     process_sc<KS, PREC, NPB>(ra, (size_t)sc, lds, O, noise_inv, llr, L, S, A, lane, nC, nY, rc, ry, npb_rt, f16);
 }
 
+// The same from binary16 coefficients (C16 pairs, exponents in cexp; binary16 product only).
+template <int KS, int NPB>
+__device__ __attribute__((noinline)) void apply_one_sc_c16(const uint32_t *__restrict__ C16,
+                                                          const int *__restrict__ cexp,
+                                                          const float2 *__restrict__ Y, float2 *__restrict__ O,
+                                                          const float *__restrict__ noise_inv,
+                                                          float *__restrict__ llr, long sc, int L, int S, int A,
+                                                          unsigned char *lds, int lane, int npb_rt) {
+    const int nC = L * (A >> 2), nY = S * (A >> 1);
+    const int rc = (nC + 63) >> 6, ry = (nY + 63) >> 6;
+    RawSC<KS> ra;
+    load_raw_c16<KS>(ra, C16, Y, (size_t)sc, L, S, A, lane, nC, nY, rc, ry);
+    process_sc<KS, 1, NPB, true>(ra, (size_t)sc, lds, O, noise_inv, llr, L, S, A, lane, nC, nY, rc, ry, npb_rt, 0,
+                                 cexp[sc]);
+}
+
 // Subcarriers [first, last) of one cell by one wave (a one-wave workgroup). PIPE: the next subcarrier's raw
 // data is loaded into a second register set while the current one is processed (one subcarrier of loads
 // always in flight, at about 3x the registers); without it the wave relies on other resident waves to

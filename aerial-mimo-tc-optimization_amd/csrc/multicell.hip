@@ -58,23 +58,39 @@ __host__ __device__ __forceinline__ int ks_class_dev(int A) { return A <= 16 ? 1
 
 // One subcarrier of a cell through the noinline per-subcarrier function (used inside loops, where inlining
 // would make the compiler hoist per-lane address arithmetic and double the registers).
+// PREC 5 (PE_PREC_F16_C16): binary16 product on binary16 coefficients (C holds the C16 block, cexp the
+// exponents), the code of the single-cell pe_coef_apply_demap_c16.
 template <int KS, int PREC>
 __device__ __forceinline__ void apply_one_sc_cell(const CellDev &cd, long sc, unsigned char *lds, int lane) {
-    if (cd.A < 16 * KS) mfma::zero_operands<KS, PREC>(lds, lane);
-    mfma::apply_one_sc<KS, PREC, -1>(cd.C, cd.Y, nullptr, cd.noise_inv, cd.llr, sc, cd.L, cd.S, cd.A, lds, lane,
-                                     cd.npb);
+    constexpr int P = PREC == 5 ? 1 : PREC;
+    if (cd.A < 16 * KS) mfma::zero_operands<KS, P>(lds, lane);
+    if constexpr (PREC == 5)
+        mfma::apply_one_sc_c16<KS, -1>(reinterpret_cast<const uint32_t *>(cd.C), cd.cexp, cd.Y, nullptr,
+                                       cd.noise_inv, cd.llr, sc, cd.L, cd.S, cd.A, lds, lane, cd.npb);
+    else
+        mfma::apply_one_sc<KS, PREC, -1>(cd.C, cd.Y, nullptr, cd.noise_inv, cd.llr, sc, cd.L, cd.S, cd.A, lds, lane,
+                                         cd.npb);
 }
 
 template <int KS, int PREC>
 __device__ __forceinline__ void apply_sc_class(const CellDev &cd, long sc, unsigned char *lds, int lane) {
     using namespace mfma;
-    if (cd.A < 16 * KS) zero_operands<KS, PREC>(lds, lane);
-    const int nC = cd.L * (cd.A >> 1), nY = cd.S * (cd.A >> 1);
-    const int rc = (nC + 63) >> 6, ry = (nY + 63) >> 6;
+    constexpr int P = PREC == 5 ? 1 : PREC;
+    if (cd.A < 16 * KS) zero_operands<KS, P>(lds, lane);
+    const int nY = cd.S * (cd.A >> 1), ry = (nY + 63) >> 6;
     RawSC<KS> ra;
-    load_raw<KS>(ra, cd.C, cd.Y, (size_t)sc, cd.L, cd.S, cd.A, lane, nC, nY, rc, ry);
-    process_sc<KS, PREC, -1>(ra, (size_t)sc, lds, nullptr, cd.noise_inv, cd.llr, cd.L, cd.S, cd.A, lane, nC, nY, rc,
-                             ry, cd.npb);
+    if constexpr (PREC == 5) {
+        const int nC = cd.L * (cd.A >> 2), rc = (nC + 63) >> 6;
+        load_raw_c16<KS>(ra, reinterpret_cast<const uint32_t *>(cd.C), cd.Y, (size_t)sc, cd.L, cd.S, cd.A, lane, nC,
+                         nY, rc, ry);
+        process_sc<KS, 1, -1, true>(ra, (size_t)sc, lds, nullptr, cd.noise_inv, cd.llr, cd.L, cd.S, cd.A, lane, nC,
+                                    nY, rc, ry, cd.npb, 0, cd.cexp[sc]);
+    } else {
+        const int nC = cd.L * (cd.A >> 1), rc = (nC + 63) >> 6;
+        load_raw<KS>(ra, cd.C, cd.Y, (size_t)sc, cd.L, cd.S, cd.A, lane, nC, nY, rc, ry);
+        process_sc<KS, PREC, -1>(ra, (size_t)sc, lds, nullptr, cd.noise_inv, cd.llr, cd.L, cd.S, cd.A, lane, nC, nY,
+                                 rc, ry, cd.npb);
+    }
 }
 
 template <int KSMAX, int PREC, bool LOOP>
@@ -119,19 +135,22 @@ __global__ __launch_bounds__(64 * kApplyWaves) void slot_apply_kernel(const unsi
 // tile (tile solve). FP64 pass: a one-wave workgroup revisits the tile's flagged subcarriers, 64 at a time.
 // Lanes per subcarrier are the cell shape's choice, the same as in the single-cell kernels, so the
 // coefficients are bit-identical to the cell solved alone.
-template <bool FP64, int KR>
+// C16: coefficients written in the binary16 operand format (cells of precision PE_PREC_F16_C16).
+template <bool FP64, int KR, bool C16 = false>
 __device__ __forceinline__ void mmse_tile(const CellDev &cd, const Tile &tl, float *lds, int tid, int W,
                                           int unbiased, float thr, Counters *q) {
     using namespace mmse;
+    uint32_t *c16 = reinterpret_cast<uint32_t *>(cd.C);
 #define PE_MMSE_L(LL)                                                                                            \
     case LL:                                                                                                     \
         if constexpr (FP64) {                                                                                    \
             for (int c = 0; c < tl.n; c += 64)                                                                   \
-                tile_fp64<LL>(cd.H, cd.sigma2, cd.C, cd.noise_inv, cd.flags, tl.sc0 + c, min(64, tl.n - c),      \
-                              cd.A, cd.mmse_p, unbiased != 0, tid);                                              \
+                tile_fp64<LL, C16>(cd.H, cd.sigma2, cd.C, cd.noise_inv, cd.flags, tl.sc0 + c, min(64, tl.n - c), \
+                                   cd.A, cd.mmse_p, unbiased != 0, tid, c16, cd.cexp);                           \
         } else {                                                                                                 \
-            tile_fp32<LL, KR>(cd.H, cd.sigma2, cd.C, cd.noise_inv, cd.flags, tl.sc0, tl.n, cd.A, cd.mmse_p,      \
-                              unbiased != 0, thr, lds, tid, W, &q->flagged);                                     \
+            tile_fp32<LL, C16 ? 0 : KR, C16, 2>(cd.H, cd.sigma2, cd.C, cd.noise_inv, cd.flags, tl.sc0, tl.n,     \
+                                                cd.A, cd.mmse_p, unbiased != 0, thr, lds, tid, W, &q->flagged,   \
+                                                c16, cd.cexp);                                                   \
         }                                                                                                        \
         break;
     switch (cd.L) {
@@ -145,7 +164,7 @@ __device__ __forceinline__ void mmse_tile(const CellDev &cd, const Tile &tl, flo
 // All layer counts in one kernel (the tile's cell selects the unrolled solve). KR = 4: antenna rows kept in
 // registers where a tile allows it (see tile_fp32). LOOP as for the apply kernel: one tile per workgroup, or
 // workgroups striding over the tiles.
-template <int KR, bool LOOP>
+template <int KR, bool LOOP, bool C16 = false>
 __global__ __launch_bounds__(512) void slot_mmse_fp32_kernel(const unsigned char *__restrict__ block, int li,
                                                              int off_cells, int off_tiles, Counters *q,
                                                              int unbiased, float thr) {
@@ -156,17 +175,18 @@ __global__ __launch_bounds__(512) void slot_mmse_fp32_kernel(const unsigned char
         if ((int)blockIdx.x >= rg.ntiles) return;  // uniform per workgroup (tile_fp32 has barriers)
         const Tile tl = v.tiles[rg.tile0 + blockIdx.x];
         const CellDev cd = v.cells[tl.cell];
-        mmse_tile<false, KR>(cd, tl, lds, threadIdx.x, blockDim.x >> 6, unbiased, thr, q);
+        mmse_tile<false, KR, C16>(cd, tl, lds, threadIdx.x, blockDim.x >> 6, unbiased, thr, q);
     } else {
         for (int t = blockIdx.x; t < rg.ntiles; t += gridDim.x) {  // uniform per workgroup
             const Tile tl = v.tiles[rg.tile0 + t];
             const CellDev cd = v.cells[tl.cell];
-            mmse_tile<false, KR>(cd, tl, lds, threadIdx.x, blockDim.x >> 6, unbiased, thr, q);
+            mmse_tile<false, KR, C16>(cd, tl, lds, threadIdx.x, blockDim.x >> 6, unbiased, thr, q);
         }
     }
 }
 
 // Same tiles, one wave each; leaves at once unless the FP32 pass (counter slot li - 1) flagged a subcarrier.
+template <bool C16 = false>
 __global__ __launch_bounds__(64) void slot_mmse_fp64_kernel(const unsigned char *__restrict__ block, int li,
                                                             int off_cells, int off_tiles, Counters *q,
                                                             int unbiased) {
@@ -176,7 +196,7 @@ __global__ __launch_bounds__(64) void slot_mmse_fp64_kernel(const unsigned char 
     for (int t = blockIdx.x; t < rg.ntiles; t += gridDim.x) {
         const Tile tl = v.tiles[rg.tile0 + t];
         const CellDev cd = v.cells[tl.cell];
-        mmse_tile<true, 0>(cd, tl, nullptr, threadIdx.x, 1, unbiased, 0.f, q);
+        mmse_tile<true, 0, C16>(cd, tl, nullptr, threadIdx.x, 1, unbiased, 0.f, q);
     }
 }
 
@@ -214,6 +234,14 @@ pe_status SlotExec::init(const pe_cell_desc *pop, int npop, int max_cells, unsig
     flags_ = flags;
     max_cells_ = max_cells;
     fixed_ = fixed;
+    c16_ = false;
+    for (int i = 0; i < npop; ++i) c16_ = c16_ || pop[i].prec == PE_PREC_F16_C16;
+    for (int i = 0; i < npop; ++i)
+        if (c16_ && pop[i].H && pop[i].prec != PE_PREC_F16_C16) {
+            err = "cell " + std::to_string(i) + ": a plan whose cells compute coefficients uses one coefficient "
+                  "format; mix PE_PREC_F16_C16 only with cells that bring their coefficients (H == NULL)";
+            return PE_ERR_UNSUPPORTED;
+        }
     int max_sc = 1;
     std::vector<std::pair<int, int>> apply_cls;  // (precision, largest antenna class)
     for (int i = 0; i < npop; ++i) {
@@ -334,6 +362,10 @@ pe_status SlotExec::stage(const pe_cell_desc *cells, int n, std::string &err) {
         d.Y = reinterpret_cast<const float2 *>(c.Y_rx);
         d.llr = c.llr;
         d.flags = d_flags_ + flags_per_cell_ * (size_t)i;
+        d.cexp = c.prec == PE_PREC_F16_C16
+                     ? reinterpret_cast<int *>(reinterpret_cast<unsigned char *>(c.C) +
+                                               (size_t)c.nSC * c.nLayer * c.nAnt * 4)
+                     : nullptr;
         d.nSC = c.nSC;
         d.L = c.nLayer;
         d.S = c.nSym;
@@ -400,7 +432,13 @@ hipError_t SlotExec::launch_one(int i, hipStream_t st, bool exact) const {
     dim3 g(exact ? std::max(1, std::min(l.grid, staged_ntiles_[i])) : l.grid), b(l.block);
     if (l.kind == 0) {
         const size_t lds = (size_t)mmse_lds_;
-        if (mmse_keep_ && fixed_)
+        if (c16_ && fixed_)
+            hipLaunchKernelGGL((slot_mmse_fp32_kernel<0, false, true>), g, b, lds, st, d_block_, i, oc, ot, q, unb,
+                               thr);
+        else if (c16_)
+            hipLaunchKernelGGL((slot_mmse_fp32_kernel<0, true, true>), g, b, lds, st, d_block_, i, oc, ot, q, unb,
+                               thr);
+        else if (mmse_keep_ && fixed_)
             hipLaunchKernelGGL((slot_mmse_fp32_kernel<4, false>), g, b, lds, st, d_block_, i, oc, ot, q, unb, thr);
         else if (mmse_keep_)
             hipLaunchKernelGGL((slot_mmse_fp32_kernel<4, true>), g, b, lds, st, d_block_, i, oc, ot, q, unb, thr);
@@ -411,7 +449,10 @@ hipError_t SlotExec::launch_one(int i, hipStream_t st, bool exact) const {
         return hipGetLastError();
     }
     if (l.kind == 1) {
-        hipLaunchKernelGGL(slot_mmse_fp64_kernel, g, b, 0, st, d_block_, i, oc, ot, q, unb);
+        if (c16_)
+            hipLaunchKernelGGL(slot_mmse_fp64_kernel<true>, g, b, 0, st, d_block_, i, oc, ot, q, unb);
+        else
+            hipLaunchKernelGGL(slot_mmse_fp64_kernel<false>, g, b, 0, st, d_block_, i, oc, ot, q, unb);
         return hipGetLastError();
     }
     // the first apply kernel zeroes the MMSE FP32 pass's counter (launch 0) once both MMSE passes are done
@@ -426,6 +467,7 @@ hipError_t SlotExec::launch_one(int i, hipStream_t st, bool exact) const {
     }
     PE_APPLY_CASE(4, 1) PE_APPLY_CASE(4, 2) PE_APPLY_CASE(4, 3) PE_APPLY_CASE(4, 4)
     PE_APPLY_CASE(8, 1) PE_APPLY_CASE(8, 2) PE_APPLY_CASE(8, 3) PE_APPLY_CASE(8, 4)
+    PE_APPLY_CASE(4, 5) PE_APPLY_CASE(8, 5)
 #undef PE_APPLY_CASE
     return hipErrorInvalidValue;
 }
@@ -486,13 +528,18 @@ pe_status check_cell_desc(const pe_cell_desc &c, int i, std::string &err) {
         err = b;
         return PE_ERR_INVALID;
     }
-    if (c.prec < PE_PREC_F16 || c.prec > PE_PREC_BF16_SPLIT) {
+    if (c.prec < PE_PREC_F16 || c.prec > PE_PREC_F16_C16) {
         snprintf(b, sizeof(b), "cell %d: precision %d not available in a slot plan (MFMA precisions only)", i, c.prec);
         err = b;
         return PE_ERR_UNSUPPORTED;
     }
     if (!c.C || !c.noise_inv || !c.Y_rx || !c.llr || (c.H && !c.sigma2)) {
         snprintf(b, sizeof(b), "cell %d: null pointer", i);
+        err = b;
+        return PE_ERR_INVALID;
+    }
+    if (c.prec == PE_PREC_F16_C16 && ((reinterpret_cast<uintptr_t>(c.C) | reinterpret_cast<uintptr_t>(c.Y_rx)) & 15u)) {
+        snprintf(b, sizeof(b), "cell %d: C and Y_rx must be 16-byte aligned for binary16 coefficients", i);
         err = b;
         return PE_ERR_INVALID;
     }

@@ -121,9 +121,16 @@ struct LatencyModel {
     }
 };
 
+// Coefficient bytes of a job: complex64, or binary16 pairs plus one exponent per subcarrier.
+size_t coef_bytes(const pe_cell_desc &c) {
+    const size_t nsc = (size_t)c.nSC;
+    const size_t b = c.prec == PE_PREC_F16_C16 ? nsc * c.nLayer * c.nAnt * 4 + nsc * 4 : nsc * c.nLayer * c.nAnt * 8;
+    return (b + 255) & ~size_t(255);
+}
+
 size_t job_bytes(const pe_cell_desc &c, bool llr_in_arena) {
     const size_t nsc = (size_t)c.nSC;
-    size_t b = ((nsc * c.nLayer * c.nAnt * 8 + 255) & ~size_t(255)) + ((nsc * c.nLayer * 4 + 255) & ~size_t(255));
+    size_t b = coef_bytes(c) + ((nsc * c.nLayer * 4 + 255) & ~size_t(255));
     if (llr_in_arena) b += (nsc * c.nLayer * c.nSym * 2 * c.nPamBits * 4 + 255) & ~size_t(255);
     return b;
 }
@@ -217,7 +224,7 @@ extern "C" pe_status pe_serve_run(const pe_serve_config *cfg, pe_serve_report *r
         const size_t nsc = (size_t)d.nSC;
         unsigned char *p = arena_dev + b.off;
         d.C = reinterpret_cast<pe_c64 *>(p);
-        p += (nsc * d.nLayer * d.nAnt * 8 + 255) & ~size_t(255);
+        p += coef_bytes(d);
         d.noise_inv = reinterpret_cast<float *>(p);
         p += (nsc * d.nLayer * 4 + 255) & ~size_t(255);
         d.llr = llr_in_arena ? reinterpret_cast<float *>(p) : cfg->job_llr[j];

@@ -21,6 +21,7 @@ struct CellDev {
     const float2 *Y;
     float *llr;
     uint8_t *flags;  // stage-1 per-subcarrier flags (executor-owned)
+    int *cexp;       // binary16 coefficient cells (PE_PREC_F16_C16): exponents after the C16 block, else null
     int nSC, L, S, A, npb, prec;
     int mmse_p;      // stage-1 lanes per subcarrier (the shape's choice, as in the single-cell path)
 };
@@ -81,6 +82,7 @@ private:
 
     unsigned flags_ = 0;
     bool fixed_ = false;  // every batch is the population (grids = tile counts, one tile per workgroup)
+    bool c16_ = false;    // the cells' coefficients are binary16 (PE_PREC_F16_C16): stage 1 writes that format
     int max_cells_ = 0, max_tiles_ = 0;
     int mmse_tile_ = 16, apply_tile_ = kApplyTileDefault;  // subcarriers per tile
     int mmse_waves_ = 1;                                     // waves per stage-1 workgroup

@@ -16,9 +16,9 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libpusch_eq.so")
 
 PE_OK, PE_ERR_INVALID, PE_ERR_UNSUPPORTED, PE_ERR_DEVICE, PE_ERR_NOMEM, PE_ERR_CAPACITY = range(6)
-PREC_F32, PREC_F16, PREC_BF16, PREC_F16_SPLIT, PREC_BF16_SPLIT = range(5)
+PREC_F32, PREC_F16, PREC_BF16, PREC_F16_SPLIT, PREC_BF16_SPLIT, PREC_F16_C16 = range(6)
 PREC_NAMES = {"f32": PREC_F32, "f16": PREC_F16, "bf16": PREC_BF16, "f16_split": PREC_F16_SPLIT,
-              "bf16_split": PREC_BF16_SPLIT}
+              "bf16_split": PREC_BF16_SPLIT, "f16_c16": PREC_F16_C16}
 MMSE_UNBIASED = 1
 SCFLAG_FP64 = 1
 SCFLAG_SINGULAR = 2
@@ -315,11 +315,20 @@ def hbm_probe(src, dst, read_only=False, grid=4096, nbytes=None, stream=None):
 
 
 def cell_desc(C, noise_inv, Y, llr, n_pam_bits, prec="f16", H=None, sigma2=None):
-    """Builds a pe_cell_desc from torch tensors (H/sigma2 None: coefficients are inputs)."""
+    """Builds a pe_cell_desc from torch tensors (H/sigma2 None: coefficients are inputs). With prec "f16_c16"
+    the complex64 C tensor's storage holds the binary16 coefficients and exponents (see c16_view)."""
     nSC, L, A = C.shape
     return CellDesc(H.data_ptr() if H is not None else None, sigma2.data_ptr() if sigma2 is not None else None,
                     C.data_ptr(), noise_inv.data_ptr(), Y.data_ptr(), llr.data_ptr(), nSC, L, Y.shape[1], A,
                     n_pam_bits, _prec(prec))
+
+
+def c16_view(C):
+    """(C16 int32 [nSC, L, A], c_exp int32 [nSC]) views of a complex64 coefficient buffer [nSC, L, A] used by a
+    PE_PREC_F16_C16 cell."""
+    nSC, L, A = C.shape
+    w = C.view(dtype=__import__("torch").int32).reshape(-1)
+    return w[: nSC * L * A].view(nSC, L, A), w[nSC * L * A: nSC * L * A + nSC]
 
 
 def job_desc(H, sigma2, Y, n_pam_bits, prec="f16"):

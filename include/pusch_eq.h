@@ -59,7 +59,11 @@ typedef enum {
     PE_PREC_F16 = 1,         /* MFMA fp16 x fp16 -> fp32, per-subcarrier power-of-two scaling */
     PE_PREC_BF16 = 2,        /* MFMA bf16 x bf16 -> fp32 */
     PE_PREC_F16_SPLIT = 3,   /* MFMA fp16 hi+lo compensated product (3 MFMAs per k-step) */
-    PE_PREC_BF16_SPLIT = 4   /* MFMA bf16 hi+lo compensated product */
+    PE_PREC_BF16_SPLIT = 4,  /* MFMA bf16 hi+lo compensated product */
+    PE_PREC_F16_C16 = 5      /* slot plans and the serving loop only: PE_PREC_F16 with the cell's coefficients
+                                kept in the binary16 operand format (pe_gram_mmse_c16); the cell's C buffer holds
+                                C16[nSC][nLayer][nAnt] (4 bytes each) followed by the exponents int32[nSC],
+                                nSC * (4 nLayer nAnt + 4) bytes, 16-byte aligned */
 } pe_prec;
 
 /* ---- library ------------------------------------------------------------------------------------- */

@@ -93,3 +93,41 @@ def test_plan_with_given_coefficients(po, pe, dev):
     for c, r in zip(cells, ref):
         assert np.array_equal(c["llr"].cpu().numpy(), r)
     plan.close()
+
+
+C16_SHAPES = [(2, 16, 2, 2, "f16_c16"), (4, 32, 4, 3, "f16_c16"), (9, 64, 8, 4, "f16_c16"),
+              (1, 128, 8, 4, "f16_c16"), (3, 24, 5, 4, "f16_c16"), (2, 128, 2, 1, "f16_c16")]
+
+
+def alone_c16(pe, c):
+    C16, e, ni, _ = pe.gram_mmse_c16(c["H"], c["s2"], unbiased=True)
+    return pe.coef_apply_demap_c16(C16, e, c["Y"], ni, c["q"]).cpu().numpy(), C16.cpu().numpy(), e.cpu().numpy()
+
+
+@pytest.mark.parametrize("graph", [False, True])
+def test_plan_binary16_coefficients_match_cells_alone(po, pe, dev, graph):
+    """Cells of precision f16_c16: the plan's MMSE writes binary16 coefficients and exponents into each cell's C
+    buffer and its apply reads them; coefficients and LLRs equal pe_gram_mmse_c16 + pe_coef_apply_demap_c16 of
+    the cell alone, bit for bit."""
+    import torch
+    cells = make_cells(po, dev, 8, C16_SHAPES)
+    ref = [alone_c16(pe, c) for c in cells]
+    descs = [pe.cell_desc(c["C"], c["ni"], c["Y"], c["llr"], c["q"], c["prec"], c["H"], c["s2"]) for c in cells]
+    plan = pe.SlotPlan(descs)
+    for _ in range(2):
+        for c in cells:
+            c["llr"].zero_()
+        plan.graph_launch() if graph else plan.launch()
+        torch.cuda.synchronize()
+        for c, (llr, C16, e) in zip(cells, ref):
+            c16, ce = pe.c16_view(c["C"])
+            assert np.array_equal(c16.cpu().numpy(), C16) and np.array_equal(ce.cpu().numpy(), e)
+            assert np.array_equal(c["llr"].cpu().numpy(), llr)
+    plan.close()
+
+
+def test_plan_rejects_mixed_coefficient_formats(po, pe, dev):
+    cells = make_cells(po, dev, 9, [(2, 16, 2, 2, "f16_c16"), (2, 16, 2, 2, "f16")])
+    descs = [pe.cell_desc(c["C"], c["ni"], c["Y"], c["llr"], c["q"], c["prec"], c["H"], c["s2"]) for c in cells]
+    with pytest.raises(pe.PeError):
+        pe.SlotPlan(descs)

@@ -86,3 +86,30 @@ def test_serve_deadline_aware_admission_under_overload(po, pe, dev):
         r = refs[cell[j]]
         got = outs[j][: r.size].cpu().numpy().reshape(r.shape)
         assert np.array_equal(got, r), "job %d (cell %d)" % (j, cell[j])
+
+
+def test_serve_binary16_coefficient_cells(po, pe, dev):
+    """Cells of precision f16_c16 in the serving loop (their jobs' coefficients take half the arena bytes):
+    every job's LLRs equal pe_gram_mmse_c16 + pe_coef_apply_demap_c16 of its cell alone."""
+    import torch
+    rng = np.random.default_rng(31)
+    descs, refs = [], []
+    keep = []
+    for (prb, A, L, q, _) in SHAPES:
+        nsc = 12 * prb
+        inp = po.chain_inputs(rng, nsc, A, L, 11, q, float(rng.uniform(5, 25)))
+        t = lambda a, dt=np.complex64: torch.from_numpy(np.ascontiguousarray(a, dtype=dt)).to(dev)
+        H, s2, Y = t(inp["H"]), t(inp["sigma2"], np.float32), t(inp["Y"])
+        C16, e, ni, _ = pe.gram_mmse_c16(H, s2, unbiased=True)
+        refs.append(pe.coef_apply_demap_c16(C16, e, Y, ni, q).cpu().numpy())
+        descs.append(pe.job_desc(H, s2, Y, q, "f16_c16"))
+        keep.append((H, s2, Y))
+    n_jobs = 40
+    size = max(int(np.prod(r.shape)) for r in refs)
+    outs = [torch.full((size,), -7.0, dtype=torch.float32, device=dev) for _ in range(n_jobs)]
+    rep, lat, cell = pe.serve_run(descs, n_jobs, 30000.0, hbm_budget=64 << 20, max_batch=8, max_inflight=2,
+                                  use_graph=False, seed=8, job_llr=outs)
+    assert rep["n_done"] == n_jobs and np.all(lat > 0)
+    for j in range(n_jobs):
+        r = refs[cell[j]]
+        assert np.array_equal(outs[j][: r.size].cpu().numpy().reshape(r.shape), r), "job %d" % j
