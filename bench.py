@@ -259,7 +259,25 @@ def multicell_bench(dev, steps):
     tiles, wg = plans[0].info()
     for p in plans:
         p.close()
-    return {"workload": "24 cells per slot, PRB {24,52,106,273}, antennas {16,32,64}, layers {2,4,8}, QAM "
+    # the same cells with binary16 coefficients (PE_PREC_F16_C16): eager back-to-back slots
+    for cs in sets:
+        for c in cs:
+            c["desc"] = pe.cell_desc(c["C"], c["ni"], c["Y"], c["llr"], c["q"], "f16_c16", c["H"], c["s2"])
+    plans16 = [pe.SlotPlan([c["desc"] for c in cs]) for cs in sets]
+    for i in range(6):
+        plans16[i % 2].launch()
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for i in range(steps):
+        plans16[i % 2].launch()
+    e1.record()
+    torch.cuda.synchronize()
+    t16 = e0.elapsed_time(e1) / 1e3 / steps
+    for p in plans16:
+        p.close()
+    return {"binary16_coefficients": {"slot_ms_eager": t16 * 1e3, "subcarriers_per_s": nsc / t16,
+                                      "workload": "the same plans with cells of precision f16_c16"},"workload": "24 cells per slot, PRB {24,52,106,273}, antennas {16,32,64}, layers {2,4,8}, QAM "
                         "{16,64,256}, f16, cell c of a plan drawn from seed (42 or 43) + 1000 c; two plans "
                         "alternate; one slot = MMSE + fused apply/demap, back-to-back slots",
             "subcarriers_per_plan": nscs, "bytes_per_plan": bys,
@@ -358,6 +376,8 @@ def serving_bench(dev, n_jobs):
     pe.serve_run(descs, min(n_jobs, 2000), 0.0, seed=1, **kw)  # warm-up run (GPU clocks, first use)
     cap, _, _ = pe.serve_run(descs, n_jobs, 0.0, seed=1, **kw)
     capacity = cap["jobs_per_s"]
+    descs16 = [pe.job_desc(c["H"], c["s2"], c["Y"], c["q"], "f16_c16") for c in cells]
+    cap16, _, _ = pe.serve_run(descs16, n_jobs, 0.0, seed=1, **kw)
     sweep = []
     for f in (0.1, 0.3, 0.5, 0.7, 0.9, 1.0, 1.1, 1.2):
         rep, _, _ = pe.serve_run(descs, n_jobs, f * capacity, seed=2, **kw)
@@ -388,6 +408,7 @@ def serving_bench(dev, n_jobs):
             "capacity_subcarriers_per_s": cap["subcarriers_per_s"], "capacity_p99_ms": cap["p99_s"] * 1e3,
             "load_sweep": sweep, "deadline_aware_admission": admission,
             "iso_job_ms": cap["iso_job_s"] * 1e3, "contention_index_first5": cap["contention_index"],
+            "capacity_jobs_per_s_binary16_coefficients": cap16["jobs_per_s"],
             "plain_f32_sequential_jobs_per_s": f32_jobs_per_s}
 
 
