@@ -85,38 +85,39 @@ struct JobBuf {
 
 // Online batch-latency model for deadline-aware admission. The latencies (launch to observed completion) of
 // the last 16 completed batches, which already include the sharing of the GPU with the batches in flight at
-// the time, are fitted by least squares as latency = a + b * subcarriers; a batch of n subcarriers is
-// predicted to take a + b n plus three residual standard deviations (a margin for the spread). Until two
-// batches have completed nothing is predicted late.
+// the time, are fitted as latency = a + b * subcarriers by a Theil-Sen line (b: median of the pairwise
+// slopes, a: median of latency - b * subcarriers), robust to the occasional batch delayed by the host; a
+// batch of n subcarriers is predicted to take a + b n plus a margin of four median absolute residuals. Until
+// two batches have completed nothing is predicted late.
 struct LatencyModel {
     static constexpr size_t kWindow = 16;
     double iso = 0.0;  // one-job batch latency at warm-up (seconds), reported
     std::deque<std::pair<double, double>> recent;  // (subcarriers, latency)
     double a = 0.0, b = 0.0, margin = 0.0;
     bool ready = false;
+    double fit(double n_sc) const { return ready ? a + b * n_sc : 0.0; }
     double predict(double n_sc) const { return ready ? a + b * n_sc + margin : 0.0; }
+    static double median(std::vector<double> &v) {
+        std::nth_element(v.begin(), v.begin() + v.size() / 2, v.end());
+        return v[v.size() / 2];
+    }
     void observe(double latency, double n_sc) {
         if (n_sc <= 0) return;
         recent.emplace_back(n_sc, latency);
         if (recent.size() > kWindow) recent.pop_front();
-        const double n = (double)recent.size();
-        if (n < 2) return;
-        double sx = 0, sy = 0, sxx = 0, sxy = 0;
-        for (const auto &r : recent) {
-            sx += r.first;
-            sy += r.second;
-            sxx += r.first * r.first;
-            sxy += r.first * r.second;
-        }
-        const double var = sxx - sx * sx / n;
-        b = var > 0 ? std::max(0.0, (sxy - sx * sy / n) / var) : 0.0;
-        a = std::max(0.0, (sy - b * sx) / n);
-        double ss = 0;
-        for (const auto &r : recent) {
-            const double e = r.second - (a + b * r.first);
-            ss += e * e;
-        }
-        margin = 3.0 * std::sqrt(ss / n);
+        if (recent.size() < 2) return;
+        std::vector<double> v;
+        for (size_t i = 0; i < recent.size(); ++i)
+            for (size_t j = i + 1; j < recent.size(); ++j)
+                if (recent[j].first != recent[i].first)
+                    v.push_back((recent[j].second - recent[i].second) / (recent[j].first - recent[i].first));
+        b = v.empty() ? 0.0 : std::max(0.0, median(v));
+        v.clear();
+        for (const auto &r : recent) v.push_back(r.second - b * r.first);
+        a = std::max(0.0, median(v));
+        v.clear();
+        for (const auto &r : recent) v.push_back(std::fabs(r.second - (a + b * r.first)));
+        margin = 4.0 * median(v);
         ready = true;
     }
 };
@@ -340,7 +341,11 @@ extern "C" pe_status pe_serve_run(const pe_serve_config *cfg, pe_serve_report *r
                 int m = 0;
                 for (auto it = ready.begin(); it != ready.end() && m < cfg->max_batch_jobs; ++it, ++m)
                     sc_full += pop[cellof[*it]].nSC;
-                pred = model.predict(sc_full);
+                // with no batch on the GPU the launch runs alone: the fit without the margin (this also lets an
+                // overly pessimistic model see new batches)
+                bool any_busy = false;
+                for (int kk = 0; kk < K; ++kk) any_busy = any_busy || busy[kk];
+                pred = any_busy ? model.predict(sc_full) : model.fit(sc_full);
             }
             while (!ready.empty() && (int)cells.size() < cfg->max_batch_jobs) {
                 const int j = ready.front();
