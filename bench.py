@@ -512,6 +512,10 @@ def main():
     ap.add_argument("--no-sweep", action="store_true", help="skip the deployment sweep")
     ap.add_argument("--no-srs", action="store_true", help="skip the SRS filter section")
     ap.add_argument("--srs-only", action="store_true", help="only the SRS filter section, printed as JSON")
+    ap.add_argument("--mmse-only", action="store_true",
+                    help="only the MMSE stage over the 32-cell batch (10 launches each output format), for profiling")
+    ap.add_argument("--fronthaul-only", action="store_true",
+                    help="only the fronthaul decode of the 32-cell batch, for profiling")
     ap.add_argument("--sweep-out", default=None, help="write the deployment sweep table (JSON) here")
     ap.add_argument("--multicell-only", action="store_true",
                     help="only the 24-cell slot section (graph and eager slots), printed as JSON (for profiling)")
@@ -541,6 +545,19 @@ def main():
         return
     if args.srs_only:
         print(json.dumps(srs_bench(dev)), flush=True)
+        return
+    if args.fronthaul_only:
+        print(json.dumps(fronthaul_bench(dev, args.cells, 10)), flush=True)
+        return
+    if args.mmse_only:
+        H, Y, s2 = make_cells(args.cells, 1234, dev)
+        del Y
+        C, ni, fl = pe.gram_mmse(H, s2, unbiased=True)
+        c16o = pe.gram_mmse_c16(H, s2, unbiased=True)
+        el_m, _ = time_launches(lambda: pe.gram_mmse(H, s2, unbiased=True, C=C, noise_inv=ni, flags=fl), [()], 10, 2)
+        el_m16, _ = time_launches(lambda: pe.gram_mmse_c16(H, s2, True, 0.0, *c16o), [()], 10, 2)
+        print(json.dumps({"subcarriers": args.cells * NSC_CELL, "c64_ms": el_m / 10 * 1e3,
+                          "c16_ms": el_m16 / 10 * 1e3}), flush=True)
         return
     n = args.cells
     nsc = n * NSC_CELL
