@@ -18,7 +18,7 @@ namespace pe {
 
 namespace {
 
-constexpr int kPrbPerWg = 8;
+constexpr int kPrbPerWg = 8;  // measured on the 32-cell batch: 4 -> 68.5 us, 8 -> 64.5 us, 16 -> 74.4 us
 
 template <int W, bool WORDS>
 __global__ __launch_bounds__(256) void bfp_decompress_kernel(const uint8_t *__restrict__ in, float2 *__restrict__ Y,
