@@ -145,6 +145,27 @@ __device__ __forceinline__ void load_raw(RawSC<KS> &R, const float2 *__restrict_
         }
 }
 
+// The Y block alone (coefficients supplied another way).
+template <int KS>
+__device__ __forceinline__ void load_raw_y(RawSC<KS> &R, const float2 *__restrict__ Y, size_t sc, int S, int A,
+                                           int lane, int nY, int ry) {
+    const float4 *yb = reinterpret_cast<const float4 *>(Y + sc * S * A);
+#pragma unroll
+    for (int j = 0; j < RawSC<KS>::RY; ++j)
+        if (j < ry) {
+            const int c = lane + 64 * j;
+            R.y[j] = ld_stream(yb + (c < nY ? c : 0));
+        }
+}
+
+// Coefficients of one subcarrier as columns in registers (computed in the same kernel instead of loaded):
+// lane q holds C[l][a] for antennas a = q + 64 u (u < 2, A <= 128) and every layer l < 8; entries for
+// l >= L or a >= A are zero.
+struct CCols {
+    float r[2][8];
+    float i[2][8];
+};
+
 // Binary16 coefficient blocks (C16 [nSC][L][A] pairs, c16_exp scaling): the C chunks hold 4 pairs each,
 // L*A/4 of them per subcarrier.
 template <int KS>
@@ -225,12 +246,17 @@ __device__ __forceinline__ void demap_store(float2 s0, float2 s1, size_t sc, int
 // NPB: 0 -> store Y_eq; 1..4 -> fused demap of that QAM order; -1 -> fused demap of order npb_rt.
 // C16IN: R.c holds binary16 coefficient chunks (load_raw_c16; nC, rc count those) scaled by 2^ec_in; only the
 // samples are range-checked here, and only PREC 1 applies.
-template <int KS, int PREC, int NPB, bool C16IN = false>
+// CCOLS: the coefficients come as columns in *cc instead of R.c (nC, rc unused). The values, the scale
+// decision and every operand written to LDS are those of the chunk path on the same complex64 coefficients.
+template <int KS, int PREC, int NPB, bool C16IN = false, bool CCOLS = false>
 __device__ __forceinline__ void process_sc(RawSC<KS> &R, size_t sc, unsigned char *lds, float2 *__restrict__ O,
                                            const float *__restrict__ noise_inv, float *__restrict__ llr, int L,
                                            int S, int A, int lane, int nC, int nY, int rc, int ry,
-                                           int npb_rt = 0, int f16 = 0, int ec_in = 0) {
+                                           int npb_rt = 0, int f16 = 0, int ec_in = 0,
+                                           const CCols *cc = nullptr) {
     static_assert(!C16IN || PREC == 1, "binary16 coefficient input is for the binary16 product");
+    static_assert(!(C16IN && CCOLS), "one coefficient source");
+    constexpr int NU = KS > 4 ? 2 : 1;  // antennas per lane in the column form
     constexpr bool BF = (PREC == 2 || PREC == 4);
     constexpr bool SPLIT = (PREC == 3 || PREC == 4);
     constexpr int AG = 4 * KS;
@@ -247,9 +273,16 @@ __device__ __forceinline__ void process_sc(RawSC<KS> &R, size_t sc, unsigned cha
 
     // per-subcarrier power-of-two scales of C and Y
     float mc = 0.f, my = 0.f;
+    if constexpr (CCOLS) {
 #pragma unroll
-    for (int j = 0; j < RawSC<KS>::RC; ++j)
-        if (!C16IN && j < rc) mc = max4abs(mc, R.c[j]);
+        for (int u = 0; u < NU; ++u)
+#pragma unroll
+            for (int l = 0; l < 8; ++l) mc = max3abs(mc, cc->r[u][l], cc->i[u][l]);
+    } else {
+#pragma unroll
+        for (int j = 0; j < RawSC<KS>::RC; ++j)
+            if (!C16IN && j < rc) mc = max4abs(mc, R.c[j]);
+    }
 #pragma unroll
     for (int j = 0; j < RawSC<KS>::RY; ++j)
         if (j < ry) my = max4abs(my, R.y[j]);
@@ -290,10 +323,31 @@ __device__ __forceinline__ void process_sc(RawSC<KS> &R, size_t sc, unsigned cha
             }
         }
     }
+    if constexpr (CCOLS) {  // pair (Re, Im) of C[l][a] at the same place as in the chunk path
+#pragma unroll
+        for (int u = 0; u < NU; ++u) {
+            const int a = lane + 64 * u;
+            if (a < A) {
+#pragma unroll
+                for (int l = 0; l < 8; ++l)
+                    if (l < L) {
+                        float re = cc->r[u][l], im = cc->i[u][l];
+                        if (scaled) { re *= sc_c; im *= sc_c; }
+                        const uint32_t p = pack2<BF>(re, im);
+                        *reinterpret_cast<uint32_t *>(lds + l * RB + a * 4) = p;
+                        if constexpr (SPLIT) {
+                            const float2 h = unpack2<BF>(p);
+                            *reinterpret_cast<uint32_t *>(lds + HI + l * RB + a * 4) =
+                                pack2<BF>(re - h.x, im - h.y);
+                        }
+                    }
+            }
+        }
+    }
 #pragma unroll
     for (int j = 0; j < RawSC<KS>::RC; ++j) {
         const int c = lane + 64 * j;
-        if (!C16IN && j < rc && c < nC) {
+        if (!C16IN && !CCOLS && j < rc && c < nC) {
             float4 v = R.c[j];
             if (scaled) { v.x *= sc_c; v.y *= sc_c; v.z *= sc_c; v.w *= sc_c; }
             const int row = (int)(((unsigned)c * cpr_mag) >> 20), part = c - row * cpr;

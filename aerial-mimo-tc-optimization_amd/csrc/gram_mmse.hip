@@ -89,6 +89,31 @@ hipError_t launch_gram_mmse(const float2 *H, const float *sigma2, float2 *C, flo
     return launch_mmse<false>(H, sigma2, C, noise_inv, flags, nSC, L, A, unbiased, thr, nullptr, nullptr, st);
 }
 
+hipError_t launch_gram_mmse_fp64_pass(const float2 *H, const float *sigma2, float2 *C, float *noise_inv,
+                                      uint8_t *flags, int nSC, int L, int A, bool unbiased, hipStream_t st) {
+    const int P = mmse_lanes_for(L, A);
+    const dim3 g((unsigned)((nSC + 63) / 64)), b(64);
+    const int u = unbiased ? 1 : 0;
+    switch (L) {
+#define PE_F64_L(LL)                                                                                              \
+    case LL:                                                                                                      \
+        hipLaunchKernelGGL((gram_mmse_f64_kernel<LL, false>), g, b, 0, st, H, sigma2, C, noise_inv, flags, nSC, A, P, \
+                           u, nullptr, nullptr);                                                                  \
+        break;
+        PE_F64_L(1)
+        PE_F64_L(2)
+        PE_F64_L(3)
+        PE_F64_L(4)
+        PE_F64_L(5)
+        PE_F64_L(6)
+        PE_F64_L(7)
+        PE_F64_L(8)
+#undef PE_F64_L
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
 hipError_t launch_gram_mmse_c16(const float2 *H, const float *sigma2, uint32_t *C16, int *cexp, float *noise_inv,
                                 uint8_t *flags, int nSC, int L, int A, bool unbiased, float thr, hipStream_t st) {
     return launch_mmse<true>(H, sigma2, nullptr, noise_inv, flags, nSC, L, A, unbiased, thr, C16, cexp, st);

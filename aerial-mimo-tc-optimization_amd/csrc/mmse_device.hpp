@@ -433,33 +433,20 @@ __device__ __forceinline__ void tile_barrier(int W) {
 }
 
 // ---- FP32 pass over one tile: subcarriers sc0 .. sc0 + n - 1 of one cell, by a workgroup of W waves -------
-// n <= 64 W; lds: the workgroup's region of at least n * Dim<L>::SLOT floats; tid = thread in the workgroup.
-// Gram and coefficient rounds are spread over the W waves; the algebra runs on threads 0 .. n - 1, i.e. on as
-// few waves as possible, while the others wait at the barrier and leave their issue slots to other
-// workgroups. flags[sc] = 0 or kPendingFp64 (C and noise_inv of a flagged subcarrier are left to the FP64
-// pass); `flagged` (optional) counts flagged subcarriers.
-// KR > 0: when every wave has a single Gram round (n <= 64 W / P) and A <= KR P, each lane keeps its KR antenna
-// rows in registers from the Gram to the coefficients instead of reading H a second time (the coefficient
-// stores then cover P consecutive antennas of a C row per lane group). It trades the second read of H for
-// registers (fewer waves per SIMD); which is faster depends on the shape.
-// C16: coefficients go to c16 / cexp as binary16 pairs and exponents (write_c16_group with NA antennas per
-// lane: A <= 64 NA; no kept rows) instead of complex64 to C.
-template <int L, int KR, bool C16 = false, int NA = 2>
-__device__ __forceinline__ void tile_fp32(const float2 *__restrict__ H, const float *__restrict__ sigma2,
-                                          float2 *__restrict__ C, float *__restrict__ noise_inv,
-                                          uint8_t *__restrict__ flags, long sc0, int n, int A, int P,
-                                          bool unbiased, float thr, float *__restrict__ lds, int tid, int W,
-                                          unsigned *flagged, uint32_t *__restrict__ c16 = nullptr,
-                                          int *__restrict__ cexp = nullptr) {
-    static_assert(!C16 || KR == 0, "binary16 output reads H again for the coefficients");
+// Phases 1 and 2 of a tile (Gram, algebra), ending at a barrier with Ginv, the scales and the ok flag of
+// every subcarrier in its LDS slot, noise_inv and flags written. keep: the lane's KR1 antenna rows stay in
+// kr / ki (single Gram round only).
+template <int L, int KR1>
+__device__ __forceinline__ void tile_solve(const float2 *__restrict__ H, const float *__restrict__ sigma2,
+                                           float *__restrict__ noise_inv, uint8_t *__restrict__ flags, long sc0,
+                                           int n, int A, int P, bool unbiased, float thr, float *__restrict__ lds,
+                                           int tid, int W, unsigned *flagged, bool keep, float (&kr)[KR1][L],
+                                           float (&ki)[KR1][L]) {
     constexpr int NT = Dim<L>::NT;
     constexpr int SLOT = Dim<L>::SLOT;
     const int lane = tid & 63, wave = tid >> 6;
     const int lgP = __builtin_ctz((unsigned)P);
     const int p = lane & (P - 1), g = lane >> lgP, G = 64 >> lgP;
-    const bool keep = KR > 0 && A <= KR * P && n <= W * G;
-    constexpr int KR1 = KR > 0 ? KR : 1;
-    float kr[KR1][L], ki[KR1][L];
 
     if (keep) {
         const int t = min(wave * G + g, n - 1);
@@ -522,6 +509,36 @@ __device__ __forceinline__ void tile_fp32(const float2 *__restrict__ H, const fl
         if (!ok && flagged) atomicAdd(flagged, 1u);
     }
     tile_barrier(W);
+}
+
+// n <= 64 W; lds: the workgroup's region of at least n * Dim<L>::SLOT floats; tid = thread in the workgroup.
+// Gram and coefficient rounds are spread over the W waves; the algebra runs on threads 0 .. n - 1, i.e. on as
+// few waves as possible, while the others wait at the barrier and leave their issue slots to other
+// workgroups. flags[sc] = 0 or kPendingFp64 (C and noise_inv of a flagged subcarrier are left to the FP64
+// pass); `flagged` (optional) counts flagged subcarriers.
+// KR > 0: when every wave has a single Gram round (n <= 64 W / P) and A <= KR P, each lane keeps its KR antenna
+// rows in registers from the Gram to the coefficients instead of reading H a second time (the coefficient
+// stores then cover P consecutive antennas of a C row per lane group). It trades the second read of H for
+// registers (fewer waves per SIMD); which is faster depends on the shape.
+// C16: coefficients go to c16 / cexp as binary16 pairs and exponents (write_c16_group with NA antennas per
+// lane: A <= 64 NA; no kept rows) instead of complex64 to C.
+template <int L, int KR, bool C16 = false, int NA = 2>
+__device__ __forceinline__ void tile_fp32(const float2 *__restrict__ H, const float *__restrict__ sigma2,
+                                          float2 *__restrict__ C, float *__restrict__ noise_inv,
+                                          uint8_t *__restrict__ flags, long sc0, int n, int A, int P,
+                                          bool unbiased, float thr, float *__restrict__ lds, int tid, int W,
+                                          unsigned *flagged, uint32_t *__restrict__ c16 = nullptr,
+                                          int *__restrict__ cexp = nullptr) {
+    static_assert(!C16 || KR == 0, "binary16 output reads H again for the coefficients");
+    constexpr int NT = Dim<L>::NT;
+    constexpr int SLOT = Dim<L>::SLOT;
+    const int lane = tid & 63, wave = tid >> 6;
+    const int lgP = __builtin_ctz((unsigned)P);
+    const int p = lane & (P - 1), g = lane >> lgP, G = 64 >> lgP;
+    const bool keep = KR > 0 && A <= KR * P && n <= W * G;
+    constexpr int KR1 = KR > 0 ? KR : 1;
+    float kr[KR1][L], ki[KR1][L];
+    tile_solve<L, KR1>(H, sigma2, noise_inv, flags, sc0, n, A, P, unbiased, thr, lds, tid, W, flagged, keep, kr, ki);
 
     // coefficients: Q = min(64, 2^ceil(log2 A)) lanes per subcarrier, lane q writing antennas q, q + Q, ..., so
     // a store instruction covers up to 512 contiguous bytes of a C row whatever P is (C[l][a] is computed by

@@ -427,6 +427,30 @@ pe_status pe_coef_apply_demap_c16(const pe_c16 *C16, const int *c_exp, const pe_
                           nSym, nAnt, nPamBits, stream);
 }
 
+pe_status pe_mmse_apply_demap(const pe_c64 *H, const float *sigma2, const pe_c64 *Y_rx, pe_c64 *C, float *noise_inv,
+                              uint8_t *sc_flags, void *llr, int nSC, int nLayer, int nSym, int nAnt, int nPamBits,
+                              pe_prec prec, unsigned flags, float cond_threshold, void *stream) {
+    pe_status s = check_coef_shape(nSC, nLayer, nSym, nAnt, (int)prec);
+    if (s != PE_OK) return s;
+    if (prec == PE_PREC_F32)
+        return fail(PE_ERR_UNSUPPORTED, "precision %d: the fused MMSE + apply runs the MFMA product", (int)prec);
+    if (nPamBits < 1 || nPamBits > 4) return fail(PE_ERR_INVALID, "nPamBits = %d outside [1, 4]", nPamBits);
+    if (flags & ~(PE_MMSE_UNBIASED | PE_EQ_WRITE_C | PE_EQ_LLR_F16))
+        return fail(PE_ERR_INVALID, "unknown flag bits 0x%x", flags);
+    if (nSC == 0) return ok();
+    if (!check_ptrs(H, sigma2, Y_rx) || !C || !noise_inv || !sc_flags || !llr)
+        return fail(PE_ERR_INVALID, "null pointer");
+    if (!aligned16(H) || !aligned16(Y_rx) || !aligned16(llr))
+        return fail(PE_ERR_INVALID, "H, Y_rx and llr must be 16-byte aligned");
+    if (reinterpret_cast<uintptr_t>(C) & 7u) return fail(PE_ERR_INVALID, "C must be 8-byte aligned");
+    const float thr = cond_threshold > 0.f ? cond_threshold : 1e4f;
+    hipError_t e = pe::launch_mmse_apply_demap(
+        reinterpret_cast<const float2 *>(H), sigma2, reinterpret_cast<const float2 *>(Y_rx), reinterpret_cast<float2 *>(C),
+        noise_inv, sc_flags, llr, nSC, nLayer, nSym, nAnt, nPamBits, (int)prec, (flags & PE_MMSE_UNBIASED) != 0, thr,
+        (flags & PE_EQ_LLR_F16) ? 1 : 0, (flags & PE_EQ_WRITE_C) ? 1 : 0, (hipStream_t)stream);
+    return from_hip(e, "mmse_apply_demap launch");
+}
+
 pe_status pe_bfp_decompress(const uint8_t *in, pe_c64 *Y_rx, int nPrb, int nSym, int nAnt, int iqWidth,
                             size_t row_stride, float scale, void *stream) {
     if (nPrb < 0) return fail(PE_ERR_INVALID, "nPrb = %d < 0", nPrb);

@@ -81,5 +81,14 @@ hipError_t launch_gram_mmse(const float2 *H, const float *sigma2, float2 *C, flo
 hipError_t launch_gram_mmse_c16(const float2 *H, const float *sigma2, uint32_t *C16, int *cexp, float *noise_inv,
                                 uint8_t *flags, int nSC, int L, int A, bool unbiased, float cond_threshold,
                                 hipStream_t st);
+// The FP64 pass alone (subcarriers the FP32 pass flagged): C, noise_inv and the final flags.
+hipError_t launch_gram_mmse_fp64_pass(const float2 *H, const float *sigma2, float2 *C, float *noise_inv,
+                                      uint8_t *flags, int nSC, int L, int A, bool unbiased, hipStream_t st);
+// MMSE + fused MFMA apply + demap without the coefficients in HBM (prec 1..4; C written for the FP64 pass's
+// subcarriers, or for all with write_c; llr_f16 -> binary16 LLRs).
+hipError_t launch_mmse_apply_demap(const float2 *H, const float *sigma2, const float2 *Y, float2 *C,
+                                   float *noise_inv, uint8_t *flags, void *llr, int nSC, int L, int S, int A,
+                                   int nPamBits, int prec, bool unbiased, float thr, int llr_f16, int write_c,
+                                   hipStream_t st);
 
 }  // namespace pe

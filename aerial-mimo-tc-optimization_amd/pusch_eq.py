@@ -50,6 +50,7 @@ SIGNATURES = {
     "pe_gram_mmse_c16": ([_P, _P, _P, _P, _P, _P, _I, _I, _I, _U, _F, _P], _I),
     "pe_coef_apply_c16": ([_P, _P, _P, _P, _I, _I, _I, _I, _P], _I),
     "pe_coef_apply_demap_c16": ([_P, _P, _P, _P, _P, _I, _I, _I, _I, _I, _U, _P], _I),
+    "pe_mmse_apply_demap": ([_P, _P, _P, _P, _P, _P, _P, _I, _I, _I, _I, _I, _I, _U, _F, _P], _I),
     "pe_slot_plan_create": ([_P, _I, _U, _P], _I),
     "pe_slot_plan_update": ([_P, _P, _I, _P], _I),
     "pe_slot_plan_launch": ([_P, _P], _I),
@@ -216,6 +217,35 @@ def coef_apply_demap_f16(C, Y, noise_inv, n_pam_bits, prec="f16", out=None, stre
     _check(lib().pe_coef_apply_demap_f16(_ptr(C), _ptr(Y), _ptr(noise_inv), _ptr(out), nSC, L, S, A,
                                          n_pam_bits, _prec(prec), _stream(stream)))
     return out
+
+
+EQ_WRITE_C = 2
+EQ_LLR_F16 = 4
+
+
+def mmse_apply_demap(H, sigma2, Y, n_pam_bits, prec="f16", unbiased=True, cond_threshold=0.0, write_c=False,
+                     llr_f16=False, C=None, noise_inv=None, flags=None, out=None, stream=None):
+    """MMSE coefficients and the fused apply + demap in one pass (pe_mmse_apply_demap): H [nSC, nAnt, nLayer],
+    Y [nSC, nSym, nAnt] -> LLRs [nSC, nLayer, nSym, 2*n_pam_bits] (float32, or float16 with llr_f16). C is the
+    workspace for subcarriers solved in FP64 (every row with write_c). Returns (llr, C, noise_inv, flags)."""
+    import torch
+    nSC, A, L = H.shape
+    S = Y.shape[1]
+    dev = H.device
+    if C is None:
+        C = torch.empty((nSC, L, A), dtype=torch.complex64, device=dev)
+    if noise_inv is None:
+        noise_inv = torch.empty((nSC, L), dtype=torch.float32, device=dev)
+    if flags is None:
+        flags = torch.empty((nSC,), dtype=torch.uint8, device=dev)
+    if out is None:
+        out = torch.empty((nSC, L, S, 2 * n_pam_bits), dtype=torch.float16 if llr_f16 else torch.float32,
+                          device=dev)
+    f = (MMSE_UNBIASED if unbiased else 0) | (EQ_WRITE_C if write_c else 0) | (EQ_LLR_F16 if llr_f16 else 0)
+    _check(lib().pe_mmse_apply_demap(_ptr(H), _ptr(sigma2), _ptr(Y), _ptr(C), _ptr(noise_inv), _ptr(flags),
+                                     _ptr(out), nSC, L, S, A, n_pam_bits, _prec(prec), f, float(cond_threshold),
+                                     _stream(stream)))
+    return out, C, noise_inv, flags
 
 
 SRS_MFMA = 1

@@ -188,6 +188,25 @@ pe_status pe_coef_apply_demap_c16(const pe_c16 *C16, const int *c_exp, const pe_
                                   void *llr, int nSC, int nLayer, int nSym, int nAnt, int nPamBits,
                                   unsigned flags, void *stream);
 
+/* ---- MMSE coefficients and the fused apply + demap in one pass ----------------------------------------- */
+
+/* No single reference kernel: the reference's chain is the coefficient compute it names
+   (eqMmseCoefCompHighMimoKernel, aerial_bottleneck.cu:51-53), then coefficient apply and demap as separate
+   launches with the coefficients and Y_eq in HBM between them (pipeline_overlap.cu:293-301). Here the
+   coefficients of each subcarrier go from the MMSE solve straight into the MFMA operands on chip and are not
+   written to HBM. The LLRs (and noise_inv, sc_flags) are bit-identical to pe_gram_mmse(H, sigma2, C, noise_inv,
+   sc_flags, ..., flags & PE_MMSE_UNBIASED, cond_threshold) followed by pe_coef_apply_demap (or, with
+   PE_EQ_LLR_F16, pe_coef_apply_demap_f16) at the same precision.
+   C [nSC][nLayer][nAnt] is a workspace: only the rows of subcarriers solved in FP64 are written (the FP64 pass
+   writes them there and a third kernel applies them), or every row with PE_EQ_WRITE_C (then C equals
+   pe_gram_mmse's output). Shapes as pe_coef_apply_demap with prec PE_PREC_F16, PE_PREC_BF16 or their split
+   forms (MFMA only); H, Y_rx and llr 16-byte aligned, C 8-byte aligned. flags: PE_MMSE_UNBIASED, and: */
+#define PE_EQ_WRITE_C 2u  /* write every coefficient to C as well */
+#define PE_EQ_LLR_F16 4u  /* LLRs in binary16 (the wire format of pe_coef_apply_demap_f16) instead of FP32 */
+pe_status pe_mmse_apply_demap(const pe_c64 *H, const float *sigma2, const pe_c64 *Y_rx, pe_c64 *C, float *noise_inv,
+                              uint8_t *sc_flags, void *llr, int nSC, int nLayer, int nSym, int nAnt, int nPamBits,
+                              pe_prec prec, unsigned flags, float cond_threshold, void *stream);
+
 /* ---- SRS filter multiply (channel estimation upstream of the equalizer) ---------------------------- */
 
 #define PE_SRS_MFMA 1u    /* MFMA with FP32 accumulation: one binary16 rounding per output (K % 4 == 0,

@@ -20,7 +20,7 @@ def test_header_declares_the_operator_surface():
     names = declared_functions()
     for required in ("pe_coef_apply", "pe_gram_mmse", "pe_llr_demap_f16", "pe_llr_demap", "pe_coef_apply_demap",
                      "pe_coef_apply_demap_f16", "pe_srs_filter", "pe_coef_to_c16", "pe_coef_apply_demap_c16",
-                     "pe_bfp_decompress", "pe_gram_mmse_c16",
+                     "pe_bfp_decompress", "pe_gram_mmse_c16", "pe_mmse_apply_demap",
                      "pe_gram", "pe_coef_apply_v1"):
         assert required in names
 
@@ -111,6 +111,16 @@ def test_other_entry_points_validate(pe):
     assert h.pe_coef_apply_demap_c16(a16, a16, a16, n, a16, 4, 8, 11, 64, 4, 0, None) == 1    # noise_inv
     assert h.pe_coef_apply_c16(a16, a16, a16, n, 4, 8, 11, 64, None) == 1                      # Y_eq
     assert h.pe_coef_apply_c16(a16, a16, a16, a16, 4, 8, 11, 62, None) == 1                    # nAnt % 4
+    # one-pass MMSE + apply: (H, sigma2, Y, C, noise_inv, flags, llr, nSC, L, S, A, q, prec, flags, thr, stream)
+    ma = h.pe_mmse_apply_demap
+    assert ma(n, n, n, n, n, n, n, 0, 8, 11, 64, 4, 1, 1, 0.0, None) == 0                  # empty
+    assert ma(a16, a16, a16, a16, a16, a16, a16, 4, 8, 11, 64, 4, 0, 1, 0.0, None) == 2    # FP32: no MFMA
+    assert ma(a16, a16, a16, a16, a16, a16, a16, 4, 8, 11, 64, 4, 1, 8, 0.0, None) == 1    # unknown flag
+    assert ma(a16, a16, a16, a16, a16, a16, a16, 4, 8, 11, 64, 5, 1, 1, 0.0, None) == 1    # q
+    assert ma(a16, a16, a16, a16, a16, a16, a16, 4, 8, 11, 132, 4, 1, 1, 0.0, None) == 2   # nAnt > 128
+    assert ma(a16, a16, a16, n, a16, a16, a16, 4, 8, 11, 64, 4, 1, 1, 0.0, None) == 1      # C workspace
+    assert ma(a16, a16, a16, a16, a16, n, a16, 4, 8, 11, 64, 4, 1, 1, 0.0, None) == 1      # sc_flags
+    assert ma(a16, a16, p, a16, a16, a16, a16, 4, 8, 11, 64, 4, 1, 1, 0.0, None) == 1      # unaligned Y
     assert h.pe_srs_filter(a16, a16, a16, a16, 64, 2, 10, 4, 8, 1, None) == 2  # MFMA needs K % 4 == 0
     assert h.pe_srs_filter(a16, p, a16, a16, 64, 2, 12, 4, 8, 1, None) == 1    # MFMA needs aligned SRS
 
