@@ -1,19 +1,22 @@
-// mmse_apply.hip -- channel estimates to LLRs in one pass: MMSE coefficients and the fused MFMA apply + demap
-// in one kernel, so the coefficients of a subcarrier go from registers into the MFMA operand layout in LDS
-// and never travel through HBM (per subcarrier at 8 layers x 64 antennas x 11 symbols: 8 KB of coefficient
-// write + read saved out of 20.6 KB).
+// mmse_apply.hip -- channel estimates to LLRs without the coefficient matrix in HBM: the MMSE stage writes
+// its compact solve per subcarrier (Ginv as a lower triangle plus the L output scales: L (L + 2) floats,
+// 320 B at 8 layers, instead of the 4 KB coefficient block) and the fused apply forms the coefficients from
+// it and the antenna rows of H while the Y block is in flight.
 //
-// Per tile of T subcarriers (one workgroup of W waves; the MMSE tile of the same shape):
-//   1-2. Gram and the L x L algebra exactly as the MMSE kernel (tile_solve): Ginv, scales and ok flag per
-//        subcarrier in LDS slots, noise_inv and flags written;
-//   3.   per subcarrier, one wave: lane q computes the coefficient column C[.][q] (and C[.][q + 64]) with the
-//        MMSE kernel's c_values, while the Y block of the subcarrier is in flight; the columns go straight
-//        into the operand layout of the fused apply (process_sc, column form), which runs the MFMA product and
-//        the demapper as the standalone apply kernel does.
-// Subcarriers the FP32 pass hands to FP64 are skipped here; the MMSE FP64 pass writes their coefficients to C
-// and a third kernel applies them (it leaves after one flag load per lane when nothing was flagged).
-// The coefficient values, the apply and the LLRs are therefore bit-identical to pe_gram_mmse followed by
-// pe_coef_apply_demap at the same precision.
+//   1. mmse_solve_kernel: the MMSE kernel's Gram and L x L algebra (tile_solve) per tile, then the solve
+//      record of every subcarrier is copied from its LDS slot to the start of that subcarrier's block of the
+//      caller's C workspace; noise_inv and flags are written as by pe_gram_mmse.
+//   2. the MMSE FP64 pass over the flagged subcarriers (their coefficient blocks are written to C).
+//   3. apply_solve_kernel, one subcarrier per wave: lane a reads antenna row a of H and forms the column
+//      C[.][a] with the operation sequence of the MMSE kernel's c_values (Ginv entries expanded once into a
+//      full matrix in LDS and read there by broadcast); the column goes straight into the operand layout of
+//      the fused apply (process_sc, column form), which runs the MFMA product and the demapper as the
+//      standalone apply kernel does. Subcarriers solved in FP64 take the coefficient block from C instead.
+// Per subcarrier at 8 layers x 64 antennas x 11 symbols this moves 4.4 + 12.7 KB instead of 8.2 + 12.6 KB
+// (H is read by both kernels, the coefficient block neither written nor read). The coefficient values, the
+// apply and the LLRs are bit-identical to pe_gram_mmse followed by pe_coef_apply_demap at the same precision.
+// Shapes whose record does not fit the coefficient block (L + 2 > 2 A) or with more than 64 antennas (one
+// column per lane) run pe_gram_mmse's and the apply's kernels with C in HBM: the same kernels, same results.
 #include "apply_device.hpp"
 #include "mmse_device.hpp"
 #include "pe_internal.hpp"
@@ -25,121 +28,114 @@ namespace {
 using namespace mmse;
 using namespace mfma;
 
-constexpr int kNiBytes = 32;           // per-wave LDS copy of the subcarrier's noise_inv (8 floats)
-constexpr size_t kMaxLds = 64 * 1024;  // LDS per workgroup kept within 64 KB
+constexpr int kWpb = 4;                // apply: waves (subcarriers) per workgroup
+constexpr int kGinvBytes = 8 * 8 * 8;  // apply: per-wave full Ginv, complex [8][8]
 
 template <int KS, int PREC>
 __host__ __device__ constexpr int wave_lds_bytes() {
-    return LdsLayout<KS>::bytes(PREC) + kNiBytes;
+    return LdsLayout<KS>::bytes(PREC) + kGinvBytes;
 }
 
-// Phase 3 for one subcarrier by one wave (slot: its LDS slot; ops, ni: the wave's operand region and
-// noise_inv copy). A separate function: inlined into the loop over the tile's subcarriers, the compiler keeps
-// per-lane address arithmetic of the apply alive across the MMSE phases and nearly doubles the registers.
-template <int L, int KS, int PREC>
-__device__ __attribute__((noinline)) void fused_one_sc(const float2 *__restrict__ H, const float *__restrict__ sigma2,
-                                                      const float2 *__restrict__ Y, float2 *__restrict__ C,
-                                                      float *__restrict__ llr, const float *__restrict__ slot,
-                                                      long sc, int S, int A, unsigned char *ops, float *ni, int lane,
-                                                      int npb, int f16, int write_c) {
-    constexpr int NT = Dim<L>::NT;
-    constexpr int NU = KS > 4 ? 2 : 1;
-    float vr[NT], vi[NT], scl[L];
-    {
-        float in[2 * NT + L];
-        slot_load<2 * NT + L>(slot, in);
-#pragma unroll
-        for (int k = 0; k < NT; ++k) {
-            vr[k] = in[k];
-            vi[k] = in[NT + k];
-        }
-#pragma unroll
-        for (int l = 0; l < L; ++l) scl[l] = in[2 * NT + l];
-    }
-    CCols cc;
-#pragma unroll
-    for (int u = 0; u < NU; ++u) {
-        const int a = lane + 64 * u;
-        float cr[L], ci[L];
-#pragma unroll
-        for (int l = 0; l < L; ++l) cr[l] = ci[l] = 0.f;
-        if (a < A) {
-            float hr[L], hi[L];
-            load_row<float, L>(H + ((size_t)sc * A + a) * L, hr, hi);
-            c_values<float, L>(hr, hi, vr, vi, scl, cr, ci);
-            if (write_c) {
-                typedef float f2v __attribute__((ext_vector_type(2)));
-                f2v *cb = reinterpret_cast<f2v *>(C + (size_t)sc * L * A) + a;
-#pragma unroll
-                for (int l = 0; l < L; ++l) {
-                    const f2v v = {cr[l], ci[l]};
-                    __builtin_nontemporal_store(v, cb + (size_t)l * A);
-                }
-            }
-        }
-#pragma unroll
-        for (int l = 0; l < 8; ++l) {
-            cc.r[u][l] = l < L ? cr[l < L ? l : 0] : 0.f;
-            cc.i[u][l] = l < L ? ci[l < L ? l : 0] : 0.f;
-        }
-    }
-    const int nY = S * (A >> 1), ry = (nY + 63) >> 6;
-    RawSC<KS> ra;
-    load_raw_y<KS>(ra, Y, (size_t)sc, S, A, lane, nY, ry);
-    lds_order_wave();  // the previous subcarrier's noise_inv reads are done
-    // noise_inv of the layers as the algebra computed them (x_l = sigma2 Ginv_ll)
-    if (lane < L) ni[lane] = noise_inv_of(sigma2[sc] * slot[tri(lane, lane)]);
-    const size_t off = (size_t)sc * L * S * 2 * npb;
-    float *llr_sc = f16 ? reinterpret_cast<float *>(reinterpret_cast<uint16_t *>(llr) + off) : llr + off;
-    process_sc<KS, PREC, -1, false, true>(ra, 0, ops, nullptr, ni, llr_sc, L, S, A, lane, 0, nY, 0, ry, npb, f16, 0,
-                                          &cc);
-}
+// Solve record of a subcarrier: Ginv lower triangle re (NT), im (NT), then the L scales.
+__host__ __device__ constexpr int record_floats(int L) { return L * (L + 2); }
 
-template <int L, int KS, int PREC>
-__global__ __launch_bounds__(512) void mmse_apply_kernel(const float2 *__restrict__ H, const float *__restrict__ sigma2,
-                                                         const float2 *__restrict__ Y, float2 *__restrict__ C,
-                                                         float *__restrict__ noise_inv, uint8_t *__restrict__ flags,
-                                                         float *__restrict__ llr, int nSC, int S, int A, int P, int T,
-                                                         int unbiased, float thr, int npb, int f16, int write_c) {
-    constexpr int NT = Dim<L>::NT;
+template <int L>
+__global__ __launch_bounds__(512) void mmse_solve_kernel(const float2 *__restrict__ H, const float *__restrict__ sigma2,
+                                                         float2 *__restrict__ C, float *__restrict__ noise_inv,
+                                                         uint8_t *__restrict__ flags, int nSC, int A, int P, int T,
+                                                         int unbiased, float thr) {
     constexpr int SLOT = Dim<L>::SLOT;
+    constexpr int REC = record_floats(L);
     extern __shared__ __attribute__((aligned(16))) float lds_f[];
-    const int W = blockDim.x >> 6, wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int W = blockDim.x >> 6;
     const long sc0 = (long)blockIdx.x * T;
     const int n = (int)min((long)T, (long)nSC - sc0);
     float kr[1][L], ki[1][L];
     tile_solve<L, 1>(H, sigma2, noise_inv, flags, sc0, n, A, P, unbiased != 0, thr, lds_f, threadIdx.x, W, nullptr,
                      false, kr, ki);
-
-    unsigned char *ops = reinterpret_cast<unsigned char *>(lds_f + T * SLOT) + wave * wave_lds_bytes<KS, PREC>();
-    float *ni = reinterpret_cast<float *>(ops + LdsLayout<KS>::bytes(PREC));
-    if (A < 16 * KS) zero_operands<KS, PREC>(ops, lane);
-    for (int t = wave; t < n; t += W) {  // t is uniform per wave
-        if (lds_f[t * SLOT + 2 * NT + L] == 0.f) continue;  // flagged: the FP64 pass and the flagged apply take it
-        fused_one_sc<L, KS, PREC>(H, sigma2, Y, C, llr, lds_f + t * SLOT, sc0 + t, S, A, ops, ni, lane, npb, f16,
-                                  write_c);
+    // records of the tile (the slots hold Ginv, the scales, then the ok flag); flagged subcarriers' blocks are
+    // rewritten with coefficients by the FP64 pass
+    for (int i = threadIdx.x; i < n * REC; i += blockDim.x) {
+        const int t = i / REC, k = i - t * REC;
+        reinterpret_cast<float *>(C + (size_t)(sc0 + t) * L * A)[k] = lds_f[t * SLOT + k];
     }
 }
 
-// The subcarriers the FP64 pass solved (flag PE_SCFLAG_FP64): their coefficients are in C; one wave per 64
-// subcarriers, one flag load per lane, the flagged ones applied one after another.
-template <int KS, int PREC>
-__global__ __launch_bounds__(64) void apply_fp64_kernel(const float2 *__restrict__ C, const float2 *__restrict__ Y,
-                                                        const float *__restrict__ noise_inv,
-                                                        const uint8_t *__restrict__ flags, float *__restrict__ llr,
-                                                        int nSC, int L, int S, int A, int npb, int f16) {
-    __shared__ __attribute__((aligned(16))) unsigned char lds[LdsLayout<KS>::bytes(PREC)];
-    const long sc0 = (long)blockIdx.x * 64;
-    const int lane = threadIdx.x;
-    const bool mine = sc0 + lane < nSC && (flags[sc0 + lane] & PE_SCFLAG_FP64);
-    unsigned long long m = __ballot(mine);
-    if (!m) return;  // uniform
-    if (A < 16 * KS) zero_operands<KS, PREC>(lds, lane);
-    while (m) {
-        const int idx = __builtin_ctzll(m);
-        m &= m - 1;
-        apply_one_sc<KS, PREC, -1>(C, Y, nullptr, noise_inv, llr, sc0 + idx, L, S, A, lds, lane, npb, f16);
+template <int L, int KS, int PREC>
+__global__ __launch_bounds__(64 * kWpb) void apply_solve_kernel(
+    const float2 *__restrict__ H, const float2 *__restrict__ Y, float2 *C, const float *__restrict__ noise_inv,
+    const uint8_t *__restrict__ flags, float *__restrict__ llr, int nSC, int S, int A, int npb, int f16,
+    int write_c) {
+#pragma clang fp contract(off)
+    static_assert(KS <= 4, "one antenna per lane");
+    constexpr int NT = Dim<L>::NT;
+    __shared__ __attribute__((aligned(16))) unsigned char lds_all[kWpb * wave_lds_bytes<KS, PREC>()];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    unsigned char *ops = lds_all + wave * wave_lds_bytes<KS, PREC>();
+    float2 *gfull = reinterpret_cast<float2 *>(ops + LdsLayout<KS>::bytes(PREC));
+    const long sc = (long)blockIdx.x * kWpb + wave;
+    if (sc >= nSC) return;  // uniform per wave; no workgroup barrier follows
+    if (A < 16 * KS) zero_operands<KS, PREC>(ops, lane);
+    const int nY = S * (A >> 1), ry = (nY + 63) >> 6;
+    RawSC<KS> ra;
+    if (flags[sc] & PE_SCFLAG_FP64) {  // the FP64 pass wrote the coefficient block
+        const int nC = L * (A >> 1), rc = (nC + 63) >> 6;
+        load_raw<KS>(ra, C, Y, (size_t)sc, L, S, A, lane, nC, nY, rc, ry);
+        process_sc<KS, PREC, -1>(ra, (size_t)sc, ops, nullptr, noise_inv, llr, L, S, A, lane, nC, nY, rc, ry, npb,
+                                 f16);
+        return;
     }
+    load_raw_y<KS>(ra, Y, (size_t)sc, S, A, lane, nY, ry);  // in flight during the coefficients
+    float hr[L], hi[L];
+    if (lane < A) {
+        load_row<float, L>(H + ((size_t)sc * A + lane) * L, hr, hi);
+    } else {
+#pragma unroll
+        for (int l = 0; l < L; ++l) hr[l] = hi[l] = 0.f;
+    }
+    const float *rec = reinterpret_cast<const float *>(C + (size_t)sc * L * A);
+    {
+        const int l = lane >> 3, j = lane & 7;
+        if (l < L && j < L) {  // Ginv[l][j]: stored lower; for j > l it is conj(V[j][l])
+            const int k = j <= l ? tri(l, j) : tri(j, l);
+            const float im = rec[NT + k];
+            gfull[lane] = make_float2(rec[k], j <= l ? im : -im);
+        }
+    }
+    float scl[L];
+#pragma unroll
+    for (int l = 0; l < L; ++l) scl[l] = rec[2 * NT + l];
+    lds_order_wave();
+    // C[l][lane] = s_l sum_j Ginv[l][j] conj(H[lane][j]); zero on lanes past A (they then change neither the
+    // block maximum nor the operands)
+    CCols cc;
+#pragma unroll
+    for (int l = 0; l < L; ++l) {
+        float re = 0.f, im = 0.f;
+#pragma unroll
+        for (int j = 0; j < L; ++j) {
+            const float2 g = gfull[l * 8 + j];
+            re = fma_(g.x, hr[j], fma_(g.y, hi[j], re));  // (g_r + i g_i)(h_r - i h_i)
+            im = fma_(g.y, hr[j], fma_(-g.x, hi[j], im));
+        }
+        cc.r[0][l] = re * scl[l];
+        cc.i[0][l] = im * scl[l];
+    }
+#pragma unroll
+    for (int l = L; l < 8; ++l) cc.r[0][l] = cc.i[0][l] = 0.f;
+    if (write_c) {  // over the record, whose reads the values above depend on
+        if (lane < A) {
+            typedef float f2v __attribute__((ext_vector_type(2)));
+            f2v *cb = reinterpret_cast<f2v *>(C + (size_t)sc * L * A) + lane;
+#pragma unroll
+            for (int l = 0; l < L; ++l) {
+                const f2v v = {cc.r[0][l], cc.i[0][l]};
+                __builtin_nontemporal_store(v, cb + (size_t)l * A);
+            }
+        }
+    }
+    process_sc<KS, PREC, -1, false, true>(ra, (size_t)sc, ops, nullptr, noise_inv, llr, L, S, A, lane, 0, nY, 0, ry,
+                                          npb, f16, 0, &cc);
 }
 
 template <int L, int KS, int PREC>
@@ -148,21 +144,15 @@ hipError_t launch_l(const float2 *H, const float *s2, const float2 *Y, float2 *C
                     hipStream_t st) {
     const int P = mmse_lanes_for(L, A);
     const MmseTile mt = mmse_tile_for(L, A, nSC);
-    int T = mt.tile, W = mt.waves;
-    auto lds_of = [&](int t, int w) { return (size_t)tile_lds_bytes(L, t) + (size_t)w * wave_lds_bytes<KS, PREC>(); };
-    while (W > 1 && lds_of(T, W) > kMaxLds) {  // fewer waves (and at most 64 per wave subcarriers) per tile
-        W >>= 1;
-        if (T > 64 * W) T = 64 * W;
-    }
-    const size_t lds = lds_of(T, W);
-    hipLaunchKernelGGL((mmse_apply_kernel<L, KS, PREC>), dim3((unsigned)((nSC + T - 1) / T)), dim3(64 * W), lds, st,
-                       H, s2, Y, C, ni, flags, llr, nSC, S, A, P, T, unbiased ? 1 : 0, thr, npb, f16, write_c);
+    const int T = mt.tile, W = mt.waves;
+    hipLaunchKernelGGL((mmse_solve_kernel<L>), dim3((unsigned)((nSC + T - 1) / T)), dim3(64 * W),
+                       (size_t)tile_lds_bytes(L, T), st, H, s2, C, ni, flags, nSC, A, P, T, unbiased ? 1 : 0, thr);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     e = launch_gram_mmse_fp64_pass(H, s2, C, ni, flags, nSC, L, A, unbiased, st);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL((apply_fp64_kernel<KS, PREC>), dim3((unsigned)((nSC + 63) / 64)), dim3(64), 0, st, C, Y, ni,
-                       flags, llr, nSC, L, S, A, npb, f16);
+    hipLaunchKernelGGL((apply_solve_kernel<L, KS, PREC>), dim3((unsigned)((nSC + kWpb - 1) / kWpb)), dim3(64 * kWpb),
+                       0, st, H, Y, C, ni, flags, llr, nSC, S, A, npb, f16, write_c);
     return hipGetLastError();
 }
 
@@ -207,15 +197,18 @@ hipError_t launch_mmse_apply_demap(const float2 *H, const float *sigma2, const f
                                    int nPamBits, int prec, bool unbiased, float thr, int llr_f16, int write_c,
                                    hipStream_t st) {
     float *l = reinterpret_cast<float *>(llr);
+    if (A > 128) return hipErrorInvalidValue;
+    if (A > 64 || L + 2 > 2 * A) {  // the two kernels with C in HBM
+        hipError_t e = launch_gram_mmse(H, sigma2, C, noise_inv, flags, nSC, L, A, unbiased, thr, st);
+        if (e != hipSuccess) return e;
+        return launch_coef_apply_mfma(C, Y, nullptr, noise_inv, l, nSC, L, S, A, nPamBits, prec, st, llr_f16);
+    }
     if (A <= 16) return launch_k<1>(H, sigma2, Y, C, noise_inv, flags, l, nSC, L, S, A, nPamBits, prec, unbiased, thr,
                                     llr_f16, write_c, st);
     if (A <= 32) return launch_k<2>(H, sigma2, Y, C, noise_inv, flags, l, nSC, L, S, A, nPamBits, prec, unbiased, thr,
                                     llr_f16, write_c, st);
-    if (A <= 64) return launch_k<4>(H, sigma2, Y, C, noise_inv, flags, l, nSC, L, S, A, nPamBits, prec, unbiased, thr,
-                                    llr_f16, write_c, st);
-    if (A <= 128) return launch_k<8>(H, sigma2, Y, C, noise_inv, flags, l, nSC, L, S, A, nPamBits, prec, unbiased,
-                                     thr, llr_f16, write_c, st);
-    return hipErrorInvalidValue;
+    return launch_k<4>(H, sigma2, Y, C, noise_inv, flags, l, nSC, L, S, A, nPamBits, prec, unbiased, thr, llr_f16,
+                       write_c, st);
 }
 
 }  // namespace pe
