@@ -17,9 +17,8 @@
 // apply and the LLRs are bit-identical to pe_gram_mmse followed by pe_coef_apply_demap at the same precision.
 // Shapes whose record does not fit the coefficient block (L + 2 > 2 A) or with more than 64 antennas (one
 // column per lane) run pe_gram_mmse's and the apply's kernels with C in HBM: the same kernels, same results.
-#include "apply_device.hpp"
-#include "mmse_device.hpp"
 #include "pe_internal.hpp"
+#include "solve_record.hpp"
 
 namespace pe {
 
@@ -28,24 +27,18 @@ namespace {
 using namespace mmse;
 using namespace mfma;
 
-constexpr int kWpb = 4;                // apply: waves (subcarriers) per workgroup
-constexpr int kGinvBytes = 8 * 8 * 8;  // apply: per-wave full Ginv, complex [8][8]
+constexpr int kWpb = 4;  // apply: waves (subcarriers) per workgroup
 
 template <int KS, int PREC>
 __host__ __device__ constexpr int wave_lds_bytes() {
-    return LdsLayout<KS>::bytes(PREC) + kGinvBytes;
+    return LdsLayout<KS>::bytes(PREC) + rec::kGinvBytes;
 }
-
-// Solve record of a subcarrier: Ginv lower triangle re (NT), im (NT), then the L scales.
-__host__ __device__ constexpr int record_floats(int L) { return L * (L + 2); }
 
 template <int L>
 __global__ __launch_bounds__(512) void mmse_solve_kernel(const float2 *__restrict__ H, const float *__restrict__ sigma2,
                                                          float2 *__restrict__ C, float *__restrict__ noise_inv,
                                                          uint8_t *__restrict__ flags, int nSC, int A, int P, int T,
                                                          int unbiased, float thr) {
-    constexpr int SLOT = Dim<L>::SLOT;
-    constexpr int REC = record_floats(L);
     extern __shared__ __attribute__((aligned(16))) float lds_f[];
     const int W = blockDim.x >> 6;
     const long sc0 = (long)blockIdx.x * T;
@@ -53,12 +46,8 @@ __global__ __launch_bounds__(512) void mmse_solve_kernel(const float2 *__restric
     float kr[1][L], ki[1][L];
     tile_solve<L, 1>(H, sigma2, noise_inv, flags, sc0, n, A, P, unbiased != 0, thr, lds_f, threadIdx.x, W, nullptr,
                      false, kr, ki);
-    // records of the tile (the slots hold Ginv, the scales, then the ok flag); flagged subcarriers' blocks are
-    // rewritten with coefficients by the FP64 pass
-    for (int i = threadIdx.x; i < n * REC; i += blockDim.x) {
-        const int t = i / REC, k = i - t * REC;
-        reinterpret_cast<float *>(C + (size_t)(sc0 + t) * L * A)[k] = lds_f[t * SLOT + k];
-    }
+    // flagged subcarriers' blocks are rewritten with coefficients by the FP64 pass
+    rec::write_records<L>(C, sc0, n, A, lds_f, threadIdx.x, blockDim.x);
 }
 
 template <int L, int KS, int PREC>
@@ -66,9 +55,6 @@ __global__ __launch_bounds__(64 * kWpb) void apply_solve_kernel(
     const float2 *__restrict__ H, const float2 *__restrict__ Y, float2 *C, const float *__restrict__ noise_inv,
     const uint8_t *__restrict__ flags, float *__restrict__ llr, int nSC, int S, int A, int npb, int f16,
     int write_c) {
-#pragma clang fp contract(off)
-    static_assert(KS <= 4, "one antenna per lane");
-    constexpr int NT = Dim<L>::NT;
     __shared__ __attribute__((aligned(16))) unsigned char lds_all[kWpb * wave_lds_bytes<KS, PREC>()];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     unsigned char *ops = lds_all + wave * wave_lds_bytes<KS, PREC>();
@@ -76,66 +62,15 @@ __global__ __launch_bounds__(64 * kWpb) void apply_solve_kernel(
     const long sc = (long)blockIdx.x * kWpb + wave;
     if (sc >= nSC) return;  // uniform per wave; no workgroup barrier follows
     if (A < 16 * KS) zero_operands<KS, PREC>(ops, lane);
-    const int nY = S * (A >> 1), ry = (nY + 63) >> 6;
-    RawSC<KS> ra;
     if (flags[sc] & PE_SCFLAG_FP64) {  // the FP64 pass wrote the coefficient block
-        const int nC = L * (A >> 1), rc = (nC + 63) >> 6;
+        const int nC = L * (A >> 1), rc = (nC + 63) >> 6, nY = S * (A >> 1), ry = (nY + 63) >> 6;
+        RawSC<KS> ra;
         load_raw<KS>(ra, C, Y, (size_t)sc, L, S, A, lane, nC, nY, rc, ry);
         process_sc<KS, PREC, -1>(ra, (size_t)sc, ops, nullptr, noise_inv, llr, L, S, A, lane, nC, nY, rc, ry, npb,
                                  f16);
         return;
     }
-    load_raw_y<KS>(ra, Y, (size_t)sc, S, A, lane, nY, ry);  // in flight during the coefficients
-    float hr[L], hi[L];
-    if (lane < A) {
-        load_row<float, L>(H + ((size_t)sc * A + lane) * L, hr, hi);
-    } else {
-#pragma unroll
-        for (int l = 0; l < L; ++l) hr[l] = hi[l] = 0.f;
-    }
-    const float *rec = reinterpret_cast<const float *>(C + (size_t)sc * L * A);
-    {
-        const int l = lane >> 3, j = lane & 7;
-        if (l < L && j < L) {  // Ginv[l][j]: stored lower; for j > l it is conj(V[j][l])
-            const int k = j <= l ? tri(l, j) : tri(j, l);
-            const float im = rec[NT + k];
-            gfull[lane] = make_float2(rec[k], j <= l ? im : -im);
-        }
-    }
-    float scl[L];
-#pragma unroll
-    for (int l = 0; l < L; ++l) scl[l] = rec[2 * NT + l];
-    lds_order_wave();
-    // C[l][lane] = s_l sum_j Ginv[l][j] conj(H[lane][j]); zero on lanes past A (they then change neither the
-    // block maximum nor the operands)
-    CCols cc;
-#pragma unroll
-    for (int l = 0; l < L; ++l) {
-        float re = 0.f, im = 0.f;
-#pragma unroll
-        for (int j = 0; j < L; ++j) {
-            const float2 g = gfull[l * 8 + j];
-            re = fma_(g.x, hr[j], fma_(g.y, hi[j], re));  // (g_r + i g_i)(h_r - i h_i)
-            im = fma_(g.y, hr[j], fma_(-g.x, hi[j], im));
-        }
-        cc.r[0][l] = re * scl[l];
-        cc.i[0][l] = im * scl[l];
-    }
-#pragma unroll
-    for (int l = L; l < 8; ++l) cc.r[0][l] = cc.i[0][l] = 0.f;
-    if (write_c) {  // over the record, whose reads the values above depend on
-        if (lane < A) {
-            typedef float f2v __attribute__((ext_vector_type(2)));
-            f2v *cb = reinterpret_cast<f2v *>(C + (size_t)sc * L * A) + lane;
-#pragma unroll
-            for (int l = 0; l < L; ++l) {
-                const f2v v = {cc.r[0][l], cc.i[0][l]};
-                __builtin_nontemporal_store(v, cb + (size_t)l * A);
-            }
-        }
-    }
-    process_sc<KS, PREC, -1, false, true>(ra, (size_t)sc, ops, nullptr, noise_inv, llr, L, S, A, lane, 0, nY, 0, ry,
-                                          npb, f16, 0, &cc);
+    rec::apply_from_record<L, KS, PREC>(L, H, Y, C, noise_inv, llr, sc, S, A, npb, f16, write_c, ops, gfull, lane);
 }
 
 template <int L, int KS, int PREC>
@@ -198,7 +133,7 @@ hipError_t launch_mmse_apply_demap(const float2 *H, const float *sigma2, const f
                                    hipStream_t st) {
     float *l = reinterpret_cast<float *>(llr);
     if (A > 128) return hipErrorInvalidValue;
-    if (A > 64 || L + 2 > 2 * A) {  // the two kernels with C in HBM
+    if (!rec::record_shape(L, A)) {  // the two kernels with C in HBM
         hipError_t e = launch_gram_mmse(H, sigma2, C, noise_inv, flags, nSC, L, A, unbiased, thr, st);
         if (e != hipSuccess) return e;
         return launch_coef_apply_mfma(C, Y, nullptr, noise_inv, l, nSC, L, S, A, nPamBits, prec, st, llr_f16);

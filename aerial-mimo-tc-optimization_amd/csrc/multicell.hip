@@ -19,11 +19,10 @@
 #include <string>
 #include <vector>
 
-#include "apply_device.hpp"
-#include "mmse_device.hpp"
 #include "pe_internal.hpp"
 #include "pusch_eq.h"
 #include "slot_exec.hpp"
+#include "solve_record.hpp"
 
 namespace pe {
 
@@ -60,6 +59,8 @@ __host__ __device__ __forceinline__ int ks_class_dev(int A) { return A <= 16 ? 1
 // would make the compiler hoist per-lane address arithmetic and double the registers).
 // PREC 5 (PE_PREC_F16_C16): binary16 product on binary16 coefficients (C holds the C16 block, cexp the
 // exponents), the code of the single-cell pe_coef_apply_demap_c16.
+// Cells with solve records (rec): stage 2 forms the coefficients from H and the record, except on subcarriers
+// the FP64 pass solved (their coefficient block is in C).
 template <int KS, int PREC>
 __device__ __forceinline__ void apply_one_sc_cell(const CellDev &cd, long sc, unsigned char *lds, int lane) {
     constexpr int P = PREC == 5 ? 1 : PREC;
@@ -131,6 +132,76 @@ __global__ __launch_bounds__(64 * kApplyWaves) void slot_apply_kernel(const unsi
     }
 }
 
+// Cells with solve records (SlotExec::uses_records): their own apply launch per precision, so that the other
+// cells' kernel keeps its registers. Per subcarrier, the coefficients are formed on chip from H and the record,
+// or read from C where the FP64 pass ran; the code is inlined (a called function is scheduled without the
+// kernel's occupancy target and takes twice the registers).
+template <int KS, int PREC>
+__device__ __forceinline__ void apply_record_class(const CellDev &cd, long sc, unsigned char *lds, float2 *gfull,
+                                                   int lane) {
+    using namespace mfma;
+    if (cd.A < 16 * KS) zero_operands<KS, PREC>(lds, lane);
+    if (cd.flags[sc] & PE_SCFLAG_FP64) {  // the FP64 pass wrote the coefficient block
+        const int nC = cd.L * (cd.A >> 1), rc = (nC + 63) >> 6, nY = cd.S * (cd.A >> 1), ry = (nY + 63) >> 6;
+        RawSC<KS> ra;
+        load_raw<KS>(ra, cd.C, cd.Y, (size_t)sc, cd.L, cd.S, cd.A, lane, nC, nY, rc, ry);
+        process_sc<KS, PREC, -1>(ra, (size_t)sc, lds, nullptr, cd.noise_inv, cd.llr, cd.L, cd.S, cd.A, lane, nC, nY,
+                                 rc, ry, cd.npb);
+        return;
+    }
+    rec::apply_from_record<0, KS, PREC>(cd.L, cd.H, cd.Y, cd.C, cd.noise_inv, cd.llr, sc, cd.S, cd.A, cd.npb, 0, 0,
+                                        lds, gfull, lane);
+}
+
+template <int PREC, bool LOOP>
+__device__ __forceinline__ void slot_apply_rec(const unsigned char *__restrict__ block, int li, int off_cells,
+                                               int off_tiles, Counters *reset) {
+    using namespace mfma;
+    __shared__ __attribute__((aligned(16))) unsigned char lds[kApplyWaves][LdsLayout<4>::bytes(PREC)];
+    __shared__ __attribute__((aligned(16))) float2 gfull_all[kApplyWaves][rec::kGinvBytes / sizeof(float2)];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (reset && blockIdx.x == 0 && threadIdx.x == 0) reset->flagged = 0u;
+    const BlockView v = view(block, off_cells, off_tiles);
+    const LaunchRange rg = v.ranges[li];
+    auto one = [&](const CellDev &cd, long sc) {
+        const int ks = ks_class_dev(cd.A);
+        if (ks == 1) apply_record_class<1, PREC>(cd, sc, lds[w], gfull_all[w], lane);
+        else if (ks == 2) apply_record_class<2, PREC>(cd, sc, lds[w], gfull_all[w], lane);
+        else apply_record_class<4, PREC>(cd, sc, lds[w], gfull_all[w], lane);
+    };
+    if constexpr (!LOOP) {
+        if ((int)blockIdx.x >= rg.ntiles) return;
+        const Tile tl = v.tiles[rg.tile0 + blockIdx.x];
+        const long sc = tl.sc0 + w;
+        if (sc >= (long)tl.sc0 + tl.n) return;  // uniform per wave; no workgroup barrier follows
+        one(v.cells[tl.cell], sc);
+    } else {
+        for (int t = blockIdx.x; t < rg.ntiles; t += gridDim.x) {
+            const Tile tl = v.tiles[rg.tile0 + t];
+            const CellDev cd = v.cells[tl.cell];
+            for (long sc = tl.sc0 + w; sc < (long)tl.sc0 + tl.n; sc += kApplyWaves) one(cd, sc);
+            lds_order_wave();  // LDS of this tile consumed before the next tile may zero it
+        }
+    }
+}
+
+// One tile per workgroup. Five waves per SIMD asked for: without the hint the compiler schedules the inlined
+// layer and antenna variants for 128-148 registers; with it, 93 for the binary16 and bfloat16 products, with no
+// spills (measured on the 24-cell plans: 66.0 us against 70.9 us per eager slot).
+template <int PREC>
+__global__ __launch_bounds__(64 * kApplyWaves) __attribute__((amdgpu_waves_per_eu(5, 8))) void slot_apply_rec_kernel(
+    const unsigned char *__restrict__ block, int li, int off_cells, int off_tiles, Counters *reset) {
+    slot_apply_rec<PREC, false>(block, li, off_cells, off_tiles, reset);
+}
+
+// Workgroups striding over the tiles (graph replays of varying batches); the hint would make this one spill.
+template <int PREC>
+__global__ __launch_bounds__(64 * kApplyWaves) void slot_apply_rec_loop_kernel(const unsigned char *__restrict__ block,
+                                                                              int li, int off_cells, int off_tiles,
+                                                                              Counters *reset) {
+    slot_apply_rec<PREC, true>(block, li, off_cells, off_tiles, reset);
+}
+
 // Stage 1: a tile is up to mmse_tile subcarriers of one cell. FP32 pass: one workgroup of W waves solves the
 // tile (tile solve). FP64 pass: a one-wave workgroup revisits the tile's flagged subcarriers, 64 at a time.
 // Lanes per subcarrier are the cell shape's choice, the same as in the single-cell kernels, so the
@@ -147,6 +218,12 @@ __device__ __forceinline__ void mmse_tile(const CellDev &cd, const Tile &tl, flo
             for (int c = 0; c < tl.n; c += 64)                                                                   \
                 tile_fp64<LL, C16>(cd.H, cd.sigma2, cd.C, cd.noise_inv, cd.flags, tl.sc0 + c, min(64, tl.n - c), \
                                    cd.A, cd.mmse_p, unbiased != 0, tid, c16, cd.cexp);                           \
+        } else if (!C16 && cd.rec) {                                                                             \
+            float kr[1][LL], ki[1][LL];                                                                          \
+            tile_solve<LL, 1>(cd.H, cd.sigma2, cd.noise_inv, cd.flags, tl.sc0, tl.n, cd.A, cd.mmse_p,            \
+                              unbiased != 0, thr, lds, tid, W, &q->flagged, false, kr, ki);                      \
+            rec::write_records<LL>(cd.C, tl.sc0, tl.n, cd.A, lds, tid, 64 * W);                                  \
+            tile_barrier(W); /* records copied before the workgroup's next tile rewrites the slots */            \
         } else {                                                                                                 \
             tile_fp32<LL, C16 ? 0 : KR, C16, 2>(cd.H, cd.sigma2, cd.C, cd.noise_inv, cd.flags, tl.sc0, tl.n,     \
                                                 cd.A, cd.mmse_p, unbiased != 0, thr, lds, tid, W, &q->flagged,   \
@@ -223,9 +300,13 @@ int SlotExec::max_grid() const {
     return g;
 }
 
+bool SlotExec::uses_records(const pe_cell_desc &c) const {
+    return (flags_ & PE_SLOT_SOLVE_RECORDS) && c.H && c.prec != PE_PREC_F16_C16 && rec::record_shape(c.nLayer, c.nAnt);
+}
+
 int SlotExec::launch_index(int kind, int prec) const {
     for (size_t i = 0; i < launches_.size(); ++i)
-        if (launches_[i].kind == kind && (kind != 2 || launches_[i].prec == prec)) return (int)i;
+        if (launches_[i].kind == kind && (kind < 2 || launches_[i].prec == prec)) return (int)i;
     return -1;
 }
 
@@ -243,23 +324,30 @@ pe_status SlotExec::init(const pe_cell_desc *pop, int npop, int max_cells, unsig
             return PE_ERR_UNSUPPORTED;
         }
     int max_sc = 1;
-    std::vector<std::pair<int, int>> apply_cls;  // (precision, largest antenna class)
+    struct ApplyClass {
+        int kind, prec, ks;  // launch kind 2 (coefficients in C) or 3 (solve records), largest antenna class
+    };
+    std::vector<ApplyClass> apply_cls;
     for (int i = 0; i < npop; ++i) {
         const pe_cell_desc &c = pop[i];
         max_sc = std::max(max_sc, c.nSC);
-        auto it = std::find_if(apply_cls.begin(), apply_cls.end(), [&](const std::pair<int, int> &k) {
-            return k.first == c.prec;
-        });
+        const int kind = uses_records(c) ? 3 : 2;
+        auto it = std::find_if(apply_cls.begin(), apply_cls.end(),
+                               [&](const ApplyClass &k) { return k.kind == kind && k.prec == c.prec; });
         const int ks = ks_class(c.nAnt) <= 4 ? 4 : 8;
         if (it == apply_cls.end())
-            apply_cls.emplace_back(c.prec, ks);
+            apply_cls.push_back({kind, c.prec, ks});
         else
-            it->second = std::max(it->second, ks);
+            it->ks = std::max(it->ks, ks);
     }
-    std::sort(apply_cls.begin(), apply_cls.end(), std::greater<std::pair<int, int>>());
+    std::sort(apply_cls.begin(), apply_cls.end(), [](const ApplyClass &a, const ApplyClass &b) {
+        return a.kind != b.kind ? a.kind > b.kind : a.prec != b.prec ? a.prec > b.prec : a.ks > b.ks;
+    });
     const SlotTile tt = slot_tile();
-    // a fixed batch runs one subcarrier per wave (the loop-free kernel); otherwise the table's tile
-    apply_tile_ = fixed ? kApplyWaves : tt.apply_tile > 0 ? tt.apply_tile : kApplyTileDefault;
+    // one subcarrier per wave of a 4-wave workgroup: the loop-free kernels (fixed plans, eager launches) take one
+    // tile per workgroup, the striding kernels of graph replays one tile per iteration
+    static_assert(kApplyTile == kApplyWaves, "one subcarrier per wave");
+    apply_tile_ = kApplyTile;
     if (tt.mmse_tile > 0) {
         mmse_tile_ = tt.mmse_tile;
         mmse_waves_ = tt.mmse_waves;
@@ -310,9 +398,11 @@ pe_status SlotExec::init(const pe_cell_desc *pop, int npop, int max_cells, unsig
         launches_.push_back({1, 0, 0, std::min(cap, 64), 64, cap});
         max_tiles_ += cap;
     }
-    for (const auto &k : apply_cls) {  // one fused apply/demap launch per precision, every antenna class
-        const int cap = class_tiles([&](const pe_cell_desc &c) { return c.prec == k.first; }, apply_tile_);
-        launches_.push_back({2, k.second, k.first, grid_for(cap, kApplyWaves, 32), 64 * kApplyWaves, cap});
+    for (const auto &k : apply_cls) {  // one fused apply/demap launch per (kind, precision), every antenna class
+        const int cap = class_tiles(
+            [&](const pe_cell_desc &c) { return c.prec == k.prec && (uses_records(c) ? 3 : 2) == k.kind; },
+            apply_tile_);
+        launches_.push_back({k.kind, k.ks, k.prec, grid_for(cap, kApplyWaves, 32), 64 * kApplyWaves, cap});
         max_tiles_ += cap;
     }
     off_cells_ = align_up(sizeof(LaunchRange) * launches_.size(), 64);
@@ -348,8 +438,8 @@ pe_status SlotExec::stage(const pe_cell_desc *cells, int n, std::string &err) {
             err = "cell " + std::to_string(i) + " has more subcarriers than the executor was sized for";
             return PE_ERR_CAPACITY;
         }
-        if ((c.H && launch_index(0, 0) < 0) || launch_index(2, c.prec) < 0 ||
-            (ks_class(c.nAnt) == 8 && launches_[launch_index(2, c.prec)].cls < 8) ||
+        const int ai = launch_index(uses_records(c) ? 3 : 2, c.prec);
+        if ((c.H && launch_index(0, 0) < 0) || ai < 0 || (ks_class(c.nAnt) == 8 && launches_[ai].cls < 8) ||
             c.nLayer > max_layers_) {
             err = "cell " + std::to_string(i) + " has a shape class the executor was not built for";
             return PE_ERR_INVALID;
@@ -373,6 +463,7 @@ pe_status SlotExec::stage(const pe_cell_desc *cells, int n, std::string &err) {
         d.npb = c.nPamBits;
         d.prec = c.prec;
         d.mmse_p = mmse_lanes_for(c.nLayer, c.nAnt);
+        d.rec = uses_records(c) ? 1 : 0;
     }
     int nt = 0;
     std::vector<int> order;
@@ -387,7 +478,8 @@ pe_status SlotExec::stage(const pe_cell_desc *cells, int n, std::string &err) {
         order.clear();
         for (int i = 0; i < n; ++i) {
             const pe_cell_desc &c = cells[i];
-            if (l.kind == 0 ? c.H != nullptr : c.prec == l.prec) order.push_back(i);
+            if (l.kind == 0 ? c.H != nullptr : c.prec == l.prec && (uses_records(c) ? 3 : 2) == l.kind)
+                order.push_back(i);
         }
         if (l.kind == 0)
             std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
@@ -429,20 +521,24 @@ hipError_t SlotExec::launch_one(int i, hipStream_t st, bool exact) const {
     const float thr = 1e4f;
     const int oc = (int)off_cells_, ot = (int)off_tiles_;
     Counters *q = d_counters_ + i;
-    dim3 g(exact ? std::max(1, std::min(l.grid, staged_ntiles_[i])) : l.grid), b(l.block);
+    // loop-free kernels (one tile per workgroup, fewer registers) for fixed plans and for eager launches, whose
+    // grid is the staged tile count; the striding kernels for graph replays of varying batches
+    const bool lf = fixed_ || exact;
+    const int st_tiles = std::max(1, staged_ntiles_[i]);
+    dim3 g(l.kind == 1 ? (exact ? std::min(l.grid, st_tiles) : l.grid) : (exact ? st_tiles : l.grid)), b(l.block);
     if (l.kind == 0) {
         const size_t lds = (size_t)mmse_lds_;
-        if (c16_ && fixed_)
+        if (c16_ && lf)
             hipLaunchKernelGGL((slot_mmse_fp32_kernel<0, false, true>), g, b, lds, st, d_block_, i, oc, ot, q, unb,
                                thr);
         else if (c16_)
             hipLaunchKernelGGL((slot_mmse_fp32_kernel<0, true, true>), g, b, lds, st, d_block_, i, oc, ot, q, unb,
                                thr);
-        else if (mmse_keep_ && fixed_)
+        else if (mmse_keep_ && lf)
             hipLaunchKernelGGL((slot_mmse_fp32_kernel<4, false>), g, b, lds, st, d_block_, i, oc, ot, q, unb, thr);
         else if (mmse_keep_)
             hipLaunchKernelGGL((slot_mmse_fp32_kernel<4, true>), g, b, lds, st, d_block_, i, oc, ot, q, unb, thr);
-        else if (fixed_)
+        else if (lf)
             hipLaunchKernelGGL((slot_mmse_fp32_kernel<0, false>), g, b, lds, st, d_block_, i, oc, ot, q, unb, thr);
         else
             hipLaunchKernelGGL((slot_mmse_fp32_kernel<0, true>), g, b, lds, st, d_block_, i, oc, ot, q, unb, thr);
@@ -456,10 +552,23 @@ hipError_t SlotExec::launch_one(int i, hipStream_t st, bool exact) const {
         return hipGetLastError();
     }
     // the first apply kernel zeroes the MMSE FP32 pass's counter (launch 0) once both MMSE passes are done
-    Counters *rs = (i > 0 && launches_[i - 1].kind != 2 && launches_[0].kind == 0) ? d_counters_ : nullptr;
+    Counters *rs = (i > 0 && launches_[i - 1].kind < 2 && launches_[0].kind == 0) ? d_counters_ : nullptr;
+    if (l.kind == 3) {  // cells with solve records
+#define PE_REC_CASE(PP)                                                                                          \
+    if (l.prec == PP) {                                                                                          \
+        if (lf)                                                                                                  \
+            hipLaunchKernelGGL((slot_apply_rec_kernel<PP>), g, b, 0, st, d_block_, i, oc, ot, rs);               \
+        else                                                                                                     \
+            hipLaunchKernelGGL((slot_apply_rec_loop_kernel<PP>), g, b, 0, st, d_block_, i, oc, ot, rs);          \
+        return hipGetLastError();                                                                                \
+    }
+        PE_REC_CASE(1) PE_REC_CASE(2) PE_REC_CASE(3) PE_REC_CASE(4)
+#undef PE_REC_CASE
+        return hipErrorInvalidValue;
+    }
 #define PE_APPLY_CASE(KK, PP)                                                                                    \
     if (l.cls == KK && l.prec == PP) {                                                                           \
-        if (fixed_)                                                                                              \
+        if (lf)                                                                                                  \
             hipLaunchKernelGGL((slot_apply_kernel<KK, PP, false>), g, b, 0, st, d_block_, i, oc, ot, rs);        \
         else                                                                                                     \
             hipLaunchKernelGGL((slot_apply_kernel<KK, PP, true>), g, b, 0, st, d_block_, i, oc, ot, rs);         \
@@ -571,7 +680,7 @@ pe_status pe_slot_plan_create(const pe_cell_desc *cells, int nCells, unsigned fl
     if (!out) return plan_fail(PE_ERR_INVALID, "plan output pointer is null");
     *out = nullptr;
     if (!cells || nCells < 1) return plan_fail(PE_ERR_INVALID, "no cells");
-    if (flags & ~PE_MMSE_UNBIASED) return plan_fail(PE_ERR_INVALID, "unknown plan flags");
+    if (flags & ~(PE_MMSE_UNBIASED | PE_SLOT_SOLVE_RECORDS)) return plan_fail(PE_ERR_INVALID, "unknown plan flags");
     std::string err;
     for (int i = 0; i < nCells; ++i) {
         pe_status s = pe::check_cell_desc(cells[i], i, err);

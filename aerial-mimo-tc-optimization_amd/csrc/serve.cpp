@@ -210,7 +210,10 @@ extern "C" pe_status pe_serve_run(const pe_serve_config *cfg, pe_serve_report *r
     };
     for (int k = 0; k < K; ++k) {
         ex[k].reset(new pe::SlotExec());
-        pe_status s = ex[k]->init(pop.data(), (int)pop.size(), cfg->max_batch_jobs, PE_MMSE_UNBIASED, false, err);
+        // eager batches form the coefficients on chip from solve records (per-job C is arena memory nobody else
+        // reads); graph replays keep the coefficient path, whose striding apply kernel needs fewer registers
+        const unsigned xf = PE_MMSE_UNBIASED | (cfg->use_graph ? 0u : PE_SLOT_SOLVE_RECORDS);
+        pe_status s = ex[k]->init(pop.data(), (int)pop.size(), cfg->max_batch_jobs, xf, false, err);
         if (s == PE_OK && hipStreamCreateWithFlags(&st[k], hipStreamNonBlocking) != hipSuccess) s = PE_ERR_DEVICE;
         if (s == PE_OK && hipEventCreateWithFlags(&ev[k], hipEventDisableTiming) != hipSuccess) s = PE_ERR_DEVICE;
         if (s != PE_OK) {

@@ -24,6 +24,7 @@ struct CellDev {
     int *cexp;       // binary16 coefficient cells (PE_PREC_F16_C16): exponents after the C16 block, else null
     int nSC, L, S, A, npb, prec;
     int mmse_p;      // stage-1 lanes per subcarrier (the shape's choice, as in the single-cell path)
+    int rec;         // stage 1 keeps solve records in C, stage 2 forms the coefficients on chip
 };
 
 struct Tile {
@@ -42,10 +43,10 @@ struct LaunchRange {
 
 class SlotExec {
 public:
-    static constexpr int kApplyTileDefault = 8;  // subcarriers per stage-2 tile (one wave)
+    static constexpr int kApplyTile = 4;  // subcarriers per stage-2 tile (one per wave of a 4-wave workgroup)
 
     struct Launch {
-        int kind;  // 0 mmse fp32, 1 mmse fp64, 2 apply
+        int kind;  // 0 mmse fp32, 1 mmse fp64, 2 apply, 3 apply forming coefficients from solve records
         int cls;   // apply: largest antenna class KS of the launch (4 or 8)
         int prec;
         int grid, block;  // workgroups (each takes tiles g, g + grid, ...) and threads per workgroup
@@ -78,13 +79,15 @@ public:
 
 private:
     int launch_index(int kind, int prec) const;
+    // the cell's coefficients go through solve records (PE_SLOT_SOLVE_RECORDS and a shape that allows it)
+    bool uses_records(const pe_cell_desc &c) const;
     hipError_t launch_one(int i, hipStream_t st, bool exact) const;
 
     unsigned flags_ = 0;
     bool fixed_ = false;  // every batch is the population (grids = tile counts, one tile per workgroup)
     bool c16_ = false;    // the cells' coefficients are binary16 (PE_PREC_F16_C16): stage 1 writes that format
     int max_cells_ = 0, max_tiles_ = 0;
-    int mmse_tile_ = 16, apply_tile_ = kApplyTileDefault;  // subcarriers per tile
+    int mmse_tile_ = 16, apply_tile_ = kApplyTile;  // subcarriers per tile
     int mmse_waves_ = 1;                                     // waves per stage-1 workgroup
     int mmse_keep_ = 0;                                      // stage 1 keeps antenna rows in registers
     int mmse_lds_ = 0;                                       // LDS bytes of a stage-1 wave

@@ -221,6 +221,7 @@ def coef_apply_demap_f16(C, Y, noise_inv, n_pam_bits, prec="f16", out=None, stre
 
 EQ_WRITE_C = 2
 EQ_LLR_F16 = 4
+SLOT_SOLVE_RECORDS = 2
 
 
 def mmse_apply_demap(H, sigma2, Y, n_pam_bits, prec="f16", unbiased=True, cond_threshold=0.0, write_c=False,
@@ -371,10 +372,11 @@ def job_desc(H, sigma2, Y, n_pam_bits, prec="f16"):
 class SlotPlan:
     """Owner of a pe_slot_plan: one slot of many cells as three tile kernels."""
 
-    def __init__(self, cells, unbiased=True):
+    def __init__(self, cells, unbiased=True, solve_records=False):
         arr = (CellDesc * len(cells))(*cells)
         h = ctypes.c_void_p()
-        _check(lib().pe_slot_plan_create(arr, len(cells), MMSE_UNBIASED if unbiased else 0, ctypes.byref(h)))
+        f = (MMSE_UNBIASED if unbiased else 0) | (SLOT_SOLVE_RECORDS if solve_records else 0)
+        _check(lib().pe_slot_plan_create(arr, len(cells), f, ctypes.byref(h)))
         self._h = h
         self.n_cells = len(cells)
 

@@ -221,6 +221,44 @@ def slot_bytes(cells):
     return b
 
 
+def slot_bytes_records(cells):
+    """The same slot with solve records (PE_SLOT_SOLVE_RECORDS; every cell of the draw qualifies): MMSE (H, sigma2
+    in; record, noise_inv, flags out) + apply (H, record, flags, Y_rx, noise_inv in; LLR out)."""
+    b = 0
+    for c in cells:
+        n, a, l, q = c["nsc"], c["A"], c["L"], c["q"]
+        rec = l * (l + 2) * 4
+        b += n * (a * l * 8 + 4 + rec + l * 4 + 1)
+        b += n * (a * l * 8 + rec + 1 + S * a * 8 + l * 4 + l * S * 2 * q * 4)
+    return b
+
+
+def time_slots(plans, mode, steps):
+    """Back-to-back slots alternating over the plans (seconds per slot), and the median latency of one slot on
+    an idle GPU (ms; events around the call, so host submission is included)."""
+    import torch
+    for i in range(6):
+        getattr(plans[i % 2], mode)()
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for i in range(steps):
+        getattr(plans[i % 2], mode)()
+    e1.record()
+    torch.cuda.synchronize()
+    per = e0.elapsed_time(e1) / 1e3 / steps
+    lat = []
+    for i in range(40):
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record()
+        getattr(plans[i % 2], mode)()
+        b.record()
+        b.synchronize()
+        lat.append(a.elapsed_time(b))
+    lat.sort()
+    return per, lat[len(lat) // 2]
+
+
 def multicell_bench(dev, steps):
     """24 uneven cells; one slot = the slot plan's kernel sequence (MMSE FP32 and FP64 passes, one fused
     apply/demap launch) as one graph replay, and the same launched eagerly; two slot buffer sets alternate.
@@ -231,26 +269,7 @@ def multicell_bench(dev, steps):
     plans = [pe.SlotPlan([c["desc"] for c in cs]) for cs in sets]
     res = {}
     for mode in ("graph_launch", "launch"):
-        for i in range(6):
-            getattr(plans[i % 2], mode)()
-        torch.cuda.synchronize()
-        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        e0.record()
-        for i in range(steps):
-            getattr(plans[i % 2], mode)()
-        e1.record()
-        torch.cuda.synchronize()
-        res[mode] = e0.elapsed_time(e1) / 1e3 / steps
-        lat = []
-        for i in range(40):
-            a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-            a.record()
-            getattr(plans[i % 2], mode)()
-            b.record()
-            b.synchronize()
-            lat.append(a.elapsed_time(b))
-        lat.sort()
-        res[mode + "_latency"] = lat[len(lat) // 2]
+        res[mode], res[mode + "_latency"] = time_slots(plans, mode, steps)
     t = res["graph_launch"]
     # the two plans alternate: per-slot figures are their means
     nscs = [sum(c["nsc"] for c in cs) for cs in sets]
@@ -259,6 +278,21 @@ def multicell_bench(dev, steps):
     tiles, wg = plans[0].info()
     for p in plans:
         p.close()
+    # the same plans with solve records: coefficients formed on chip by the apply
+    plans_r = [pe.SlotPlan([c["desc"] for c in cs], solve_records=True) for cs in sets]
+    rec = {}
+    for mode in ("graph_launch", "launch"):
+        rec[mode], rec[mode + "_latency"] = time_slots(plans_r, mode, steps)
+    by_r = sum(slot_bytes_records(cs) for cs in sets) / 2
+    for p in plans_r:
+        p.close()
+    solve_records = {"slot_ms": rec["graph_launch"] * 1e3, "slot_ms_eager": rec["launch"] * 1e3,
+                     "subcarriers_per_s": nsc / rec["graph_launch"],
+                     "subcarriers_per_s_eager": nsc / rec["launch"], "bytes_per_slot": by_r,
+                     "hbm_gbs_eager": by_r / rec["launch"] / 1e9,
+                     "idle_slot_latency_ms": {"graph": rec["graph_launch_latency"], "eager": rec["launch_latency"]},
+                     "workload": "the same plans with PE_SLOT_SOLVE_RECORDS (stage 1 writes solve records, stage 2 "
+                                 "forms the coefficients on chip)"}
     # the same cells with binary16 coefficients (PE_PREC_F16_C16): eager back-to-back slots
     for cs in sets:
         for c in cs:
@@ -277,7 +311,9 @@ def multicell_bench(dev, steps):
     for p in plans16:
         p.close()
     return {"binary16_coefficients": {"slot_ms_eager": t16 * 1e3, "subcarriers_per_s": nsc / t16,
-                                      "workload": "the same plans with cells of precision f16_c16"},"workload": "24 cells per slot, PRB {24,52,106,273}, antennas {16,32,64}, layers {2,4,8}, QAM "
+                                      "workload": "the same plans with cells of precision f16_c16"},
+            "solve_records": solve_records,
+            "workload": "24 cells per slot, PRB {24,52,106,273}, antennas {16,32,64}, layers {2,4,8}, QAM "
                         "{16,64,256}, f16, cell c of a plan drawn from seed (42 or 43) + 1000 c; two plans "
                         "alternate; one slot = MMSE + fused apply/demap, back-to-back slots",
             "subcarriers_per_plan": nscs, "bytes_per_plan": bys,
@@ -659,6 +695,41 @@ def main():
     bytes_c16w = nsc * (L * A * 4 + 4 + S * A * 8 + L * S * 2 * Q * 2 + L * 4)
     del llr16, bufs16, bufs_c16, bufs_c16w, c16s
 
+    # the whole chain from channel estimates: two launches with C in HBM, and the one-pass entry point
+    # (pe_mmse_apply_demap: the MMSE writes a compact solve record, the apply forms the coefficients on chip)
+    def chain2(Hf, s2, C, ni, fl, Y, llr):
+        pe.gram_mmse(Hf, s2, unbiased=True, C=C, noise_inv=ni, flags=fl)
+        pe.coef_apply_demap(C, Y, ni, Q, prec=args.prec, out=llr)
+
+    def chain1(Hf, s2, C, ni, fl, Y, llr):
+        pe.mmse_apply_demap(Hf, s2, Y, Q, prec=args.prec, C=C, noise_inv=ni, flags=fl, out=llr)
+
+    hfull = [make_cells(n, 1234 + k, dev)[0] for k in range(2)]  # the channels of the two sets again
+    cbufs = [(h, s["s2"], s["C"], s["ni"], s["fl"], s["Y"], s["llr"]) for h, s in zip(hfull, sets)]
+    nch = max(20, args.steps // 4)
+    el_c2, _ = time_launches(chain2, cbufs, nch, 5)
+    el_c1, _ = time_launches(chain1, cbufs, nch, 5)
+    ref_llr = sets[0]["llr"].clone()
+    chain2(*cbufs[0])
+    ref_llr.copy_(sets[0]["llr"])
+    chain1(*cbufs[0])
+    one_pass_identical = bool(torch.equal(ref_llr, sets[0]["llr"]))
+    del ref_llr
+    chain2(*cbufs[0])  # set 0's coefficients again (the one-pass call keeps its solve records in C)
+    rec = L * (L + 2) * 4
+    bytes_one_pass = nsc * (2 * A * L * 8 + 2 * rec + S * A * 8 + 2 * L * 4 + L * S * 2 * Q * 4 + 4 + 2)
+    bytes_two = nsc * (2 * A * L * 8 + A * L * 8 + S * A * 8 + 2 * L * 4 + L * S * 2 * Q * 4 + 4 + 1)
+    one_pass = {"ms_per_chain_two_launches": el_c2 / nch * 1e3, "ms_per_chain_one_pass": el_c1 / nch * 1e3,
+                "subcarriers_per_s_one_pass": nsc / (el_c1 / nch),
+                "speedup": el_c2 / el_c1, "llrs_identical": one_pass_identical,
+                "bytes_per_chain_one_pass": bytes_one_pass, "bytes_per_chain_two_launches": bytes_two,
+                "hbm_gbs_one_pass": bytes_one_pass / (el_c1 / nch) / 1e9,
+                "workload": "H -> LLR for the whole batch: pe_gram_mmse + pe_coef_apply_demap against "
+                            "pe_mmse_apply_demap (%s, FP32 LLRs)" % args.prec}
+    log("chain from H: two launches %.1f us, one pass %.1f us (identical LLRs: %s)" % (
+        el_c2 / nch * 1e6, el_c1 / nch * 1e6, one_pass_identical))
+    del cbufs, hfull
+
     # single-cell slot latency: MMSE + fused apply/demap for one cell, one stream
     cell = {k: (v[c1].contiguous() if torch.is_tensor(v) and v.shape[0] == nsc else v) for k, v in s0.items()}
     lat = []
@@ -673,6 +744,19 @@ def main():
             lat.append(a.elapsed_time(b))
     lat.sort()
     pct = lambda p: lat[min(len(lat) - 1, max(0, int(-(-p * len(lat) // 100)) - 1))] if lat else None
+    lat1 = []
+    for i in range(args.latency_iters + 20):
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record()
+        pe.mmse_apply_demap(cell["H"], cell["s2"], cell["Y"], Q, prec=args.prec, C=cell["C"], noise_inv=cell["ni"],
+                            flags=cell["fl"], out=cell["llr"])
+        b.record()
+        b.synchronize()
+        if i >= 20:
+            lat1.append(a.elapsed_time(b))
+    pe.gram_mmse(cell["H"], cell["s2"], unbiased=True, C=cell["C"], noise_inv=cell["ni"], flags=cell["fl"])
+    lat1.sort()
+    pct1 = lambda p: lat1[min(len(lat1) - 1, max(0, int(-(-p * len(lat1) // 100)) - 1))] if lat1 else None
 
     cpu = None
     if not args.no_cpu:
@@ -746,7 +830,10 @@ def main():
                                      "hbm_gbs": bytes_c16w / percw / 1e9, "bytes_per_launch": bytes_c16w,
                                      "workload": "binary16 coefficients and binary16 LLRs"},
         "slot_latency_ms": {"p50": pct(50), "p99": pct(99), "max": lat[-1] if lat else None, "iters": len(lat),
-                            "workload": "one cell: pe_gram_mmse + fused apply/demap, events on one stream"},
+                            "workload": "one cell: pe_gram_mmse + fused apply/demap, events on one stream",
+                            "one_pass": {"p50": pct1(50), "p99": pct1(99), "max": lat1[-1] if lat1 else None,
+                                         "workload": "one cell: pe_mmse_apply_demap"}},
+        "chain_from_channel_estimates": one_pass,
         "multicell_slot": multicell,
         "serving": serving,
         "srs_filter": srs,

@@ -265,10 +265,16 @@ typedef struct pe_slot_plan pe_slot_plan;
 /* Builds the tile lists of a slot (cells with H for the MMSE stage, cells of each precision for the fused
    apply/demap, ordered by per-subcarrier cost and cut into subcarrier tiles) and uploads descriptors and
    tiles. Allocates its device memory (descriptors, tiles, the MMSE flag counter, per-subcarrier MMSE flags)
-   once, here. flags: PE_MMSE_UNBIASED for the MMSE stage.
+   once, here. flags: PE_MMSE_UNBIASED for the MMSE stage, PE_SLOT_SOLVE_RECORDS (below).
    Shapes per cell: nSC >= 1, 1 <= nLayer <= 8, 1 <= nSym <= 16, 4 <= nAnt <= 128 with nAnt % 4 == 0.
    Replaces the reference's one-launch-per-cell-per-stream protocol (multicell_contention.cu:194-283,
    multicell_evidence.cu:497-566). */
+#define PE_SLOT_SOLVE_RECORDS 2u /* stage 1 keeps each subcarrier's compact solve (Ginv and the output scales,
+                                   nLayer (nLayer + 2) floats) at the start of its block of C instead of the
+                                   coefficients, and stage 2 forms the coefficients on chip from H and that record,
+                                   as pe_mmse_apply_demap does. Applies to cells with H, an MFMA precision other
+                                   than PE_PREC_F16_C16, nAnt <= 64 and nLayer + 2 <= 2 nAnt; LLRs and noise_inv
+                                   are unchanged bit for bit, C holds coefficients only where the FP64 pass ran. */
 pe_status pe_slot_plan_create(const pe_cell_desc *cells, int nCells, unsigned flags, pe_slot_plan **plan);
 /* Rewrites the per-cell pointers of an existing plan (same shapes) for the next slot, ordered on `stream`. */
 pe_status pe_slot_plan_update(pe_slot_plan *plan, const pe_cell_desc *cells, int nCells, void *stream);
@@ -300,7 +306,9 @@ typedef struct {
     int shed_late;             /* 0: serve every job; 1: drop jobs whose deadline passed before launch; 2:
                                   deadline-aware admission, drop jobs whose predicted completion (age + predicted
                                   batch latency, online model) is past the deadline. Dropped jobs count as misses. */
-    int use_graph;             /* nonzero: each stream replays one captured slot graph */
+    int use_graph;             /* nonzero: each stream replays one captured slot graph; zero: eager launches,
+                                  with the coefficients formed on chip from solve records (as with
+                                  PE_SLOT_SOLVE_RECORDS) where the cell shape allows it */
     unsigned seed;             /* arrival process and cell draws */
     float *const *job_llr;     /* optional [n_jobs]: LLR destination of each job (sized for its cell) */
 } pe_serve_config;

@@ -131,3 +131,44 @@ def test_plan_rejects_mixed_coefficient_formats(po, pe, dev):
     descs = [pe.cell_desc(c["C"], c["ni"], c["Y"], c["llr"], c["q"], c["prec"], c["H"], c["s2"]) for c in cells]
     with pytest.raises(pe.PeError):
         pe.SlotPlan(descs)
+
+
+@pytest.mark.parametrize("graph", [False, True])
+def test_plan_solve_records_match_cells_alone(po, pe, dev, graph):
+    """PE_SLOT_SOLVE_RECORDS: stage 1 keeps compact solve records in C and stage 2 forms the coefficients on chip.
+    LLRs and noise_inv equal the cells alone bit for bit, including subcarriers the FP64 pass solves (every third
+    subcarrier of one cell has two nearly parallel layers) and cells the flag leaves on the coefficient path
+    (128 antennas; 8 layers on 4 antennas, whose record does not fit the block), over repeated launches and a
+    pointer update."""
+    import torch
+    shapes = SHAPES + [(3, 4, 8, 2, "f16"), (4, 32, 8, 4, "bf16")]
+    cells = make_cells(po, dev, 15, shapes)
+    bad = cells[2]
+    bad["H"][::3, :, 1] = bad["H"][::3, :, 0] * (1 + 1e-4)
+    bad["s2"].fill_(1e-6)
+    ref, ref_ni = [], []
+    for c in cells:
+        C, ni, fl = pe.gram_mmse(c["H"], c["s2"], unbiased=True)
+        ref.append(pe.coef_apply_demap(C, c["Y"], ni, c["q"], prec=c["prec"]).cpu().numpy())
+        ref_ni.append(ni.cpu().numpy())
+        if c is bad:
+            assert int((fl != 0).sum()) > 0
+    descs = [pe.cell_desc(c["C"], c["ni"], c["Y"], c["llr"], c["q"], c["prec"], c["H"], c["s2"]) for c in cells]
+    plan = pe.SlotPlan(descs, solve_records=True)
+    for rep in range(3):
+        for c in cells:
+            c["llr"].fill_(float(rep + 1))
+            c["ni"].zero_()
+        plan.graph_launch() if graph else plan.launch()
+        torch.cuda.synchronize()
+        for i, c in enumerate(cells):
+            assert np.array_equal(c["llr"].cpu().numpy(), ref[i]), "cell %d rep %d" % (i, rep)
+            assert np.array_equal(c["ni"].cpu().numpy(), ref_ni[i]), "cell %d rep %d" % (i, rep)
+    cells2 = make_cells(po, dev, 16, shapes)
+    ref2 = [alone(pe, c) for c in cells2]
+    plan.update([pe.cell_desc(c["C"], c["ni"], c["Y"], c["llr"], c["q"], c["prec"], c["H"], c["s2"]) for c in cells2])
+    plan.graph_launch() if graph else plan.launch()
+    torch.cuda.synchronize()
+    for c, r in zip(cells2, ref2):
+        assert np.array_equal(c["llr"].cpu().numpy(), r)
+    plan.close()
