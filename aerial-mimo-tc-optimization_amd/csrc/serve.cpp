@@ -87,7 +87,7 @@ struct JobBuf {
 // the last 16 completed batches, which already include the sharing of the GPU with the batches in flight at
 // the time, are fitted as latency = a + b * subcarriers by a Theil-Sen line (b: median of the pairwise
 // slopes, a: median of latency - b * subcarriers), robust to the occasional batch delayed by the host; a
-// batch of n subcarriers is predicted to take a + b n plus a margin of four median absolute residuals. Until
+// batch of n subcarriers is predicted to take a + b n plus a margin of six median absolute residuals. Until
 // two batches have completed nothing is predicted late.
 struct LatencyModel {
     static constexpr size_t kWindow = 16;
@@ -117,7 +117,7 @@ struct LatencyModel {
         a = std::max(0.0, median(v));
         v.clear();
         for (const auto &r : recent) v.push_back(std::fabs(r.second - (a + b * r.first)));
-        margin = 4.0 * median(v);
+        margin = 6.0 * median(v);
         ready = true;
     }
 };
@@ -334,6 +334,8 @@ extern "C" pe_status pe_serve_run(const pe_serve_config *cfg, pe_serve_report *r
         // 3. admission and launch on free streams
         for (int k = 0; k < K && !ready.empty(); ++k) {
             if (busy[k]) continue;
+            // job ages at this admission (a batch submitted on a previous stream in this pass took host time)
+            const double t_adm = now_s();
             cells.clear();
             double sc_batch = 0.0;
             // deadline-aware admission predicts with the batch this launch will have if nothing is shed: the
@@ -353,8 +355,8 @@ extern "C" pe_status pe_serve_run(const pe_serve_config *cfg, pe_serve_report *r
             while (!ready.empty() && (int)cells.size() < cfg->max_batch_jobs) {
                 const int j = ready.front();
                 const double nsc_j = pop[cellof[j]].nSC;
-                const bool late = cfg->shed_late == 1 && now - arrival[j] > cfg->deadline_s;
-                const bool predicted_late = cfg->shed_late == 2 && now - arrival[j] + pred > cfg->deadline_s;
+                const bool late = cfg->shed_late == 1 && t_adm - arrival[j] > cfg->deadline_s;
+                const bool predicted_late = cfg->shed_late == 2 && t_adm - arrival[j] + pred > cfg->deadline_s;
                 if (late || predicted_late) {
                     ready.pop_front();
                     ++shed;
