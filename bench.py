@@ -550,6 +550,9 @@ def main():
     ap.add_argument("--srs-only", action="store_true", help="only the SRS filter section, printed as JSON")
     ap.add_argument("--mmse-only", action="store_true",
                     help="only the MMSE stage over the 32-cell batch (10 launches each output format), for profiling")
+    ap.add_argument("--chain-only", action="store_true",
+                    help="only the chain from channel estimates over the 32-cell batch (10 two-launch chains, then "
+                         "10 pe_mmse_apply_demap calls), for profiling")
     ap.add_argument("--fronthaul-only", action="store_true",
                     help="only the fronthaul decode of the 32-cell batch, for profiling")
     ap.add_argument("--sweep-out", default=None, help="write the deployment sweep table (JSON) here")
@@ -584,6 +587,26 @@ def main():
         return
     if args.fronthaul_only:
         print(json.dumps(fronthaul_bench(dev, args.cells, 10)), flush=True)
+        return
+    if args.chain_only:
+        H, Y, s2 = make_cells(args.cells, 1234, dev)
+        nsc = args.cells * NSC_CELL
+        C = torch.empty((nsc, L, A), dtype=torch.complex64, device=dev)
+        ni = torch.empty((nsc, L), dtype=torch.float32, device=dev)
+        fl = torch.empty((nsc,), dtype=torch.uint8, device=dev)
+        llr = torch.empty((nsc, L, S, 2 * Q), dtype=torch.float32, device=dev)
+
+        def two():
+            pe.gram_mmse(H, s2, unbiased=True, C=C, noise_inv=ni, flags=fl)
+            pe.coef_apply_demap(C, Y, ni, Q, prec=args.prec, out=llr)
+
+        def one():
+            pe.mmse_apply_demap(H, s2, Y, Q, prec=args.prec, C=C, noise_inv=ni, flags=fl, out=llr)
+
+        el2, _ = time_launches(two, [()], 10, 2)
+        el1, _ = time_launches(one, [()], 10, 2)
+        print(json.dumps({"subcarriers": nsc, "two_launches_ms": el2 / 10 * 1e3, "one_pass_ms": el1 / 10 * 1e3}),
+              flush=True)
         return
     if args.mmse_only:
         H, Y, s2 = make_cells(args.cells, 1234, dev)
