@@ -15,8 +15,9 @@
 // Per subcarrier at 8 layers x 64 antennas x 11 symbols this moves 4.4 + 12.7 KB instead of 8.2 + 12.6 KB
 // (H is read by both kernels, the coefficient block neither written nor read). The coefficient values, the
 // apply and the LLRs are bit-identical to pe_gram_mmse followed by pe_coef_apply_demap at the same precision.
-// Shapes whose record does not fit the coefficient block (L + 2 > 2 A) or with more than 64 antennas (one
-// column per lane) run pe_gram_mmse's and the apply's kernels with C in HBM: the same kernels, same results.
+// Shapes for which this does not pay (fewer than 32 antennas or 4 layers: measured slower, single cells being
+// launch- and latency-bound) or cannot be done (more than 64 antennas: one column per lane) run pe_gram_mmse's
+// and the apply's kernels with C in HBM: the same kernels, hence the same results.
 #include "pe_internal.hpp"
 #include "solve_record.hpp"
 
@@ -98,10 +99,7 @@ hipError_t launch_kp(const float2 *H, const float *s2, const float2 *Y, float2 *
 #define PE_MA_L(LL)                                                                                               \
     case LL:                                                                                                      \
         return launch_l<LL, KS, PREC>(H, s2, Y, C, ni, flags, llr, nSC, S, A, npb, unbiased, thr, f16, write_c, st);
-    switch (L) {
-        PE_MA_L(1)
-        PE_MA_L(2)
-        PE_MA_L(3)
+    switch (L) {  // 4 <= L (record_pays)
         PE_MA_L(4)
         PE_MA_L(5)
         PE_MA_L(6)
@@ -133,13 +131,11 @@ hipError_t launch_mmse_apply_demap(const float2 *H, const float *sigma2, const f
                                    hipStream_t st) {
     float *l = reinterpret_cast<float *>(llr);
     if (A > 128) return hipErrorInvalidValue;
-    if (!rec::record_shape(L, A)) {  // the two kernels with C in HBM
+    if (!rec::record_pays(L, A)) {  // the two kernels with C in HBM
         hipError_t e = launch_gram_mmse(H, sigma2, C, noise_inv, flags, nSC, L, A, unbiased, thr, st);
         if (e != hipSuccess) return e;
         return launch_coef_apply_mfma(C, Y, nullptr, noise_inv, l, nSC, L, S, A, nPamBits, prec, st, llr_f16);
     }
-    if (A <= 16) return launch_k<1>(H, sigma2, Y, C, noise_inv, flags, l, nSC, L, S, A, nPamBits, prec, unbiased, thr,
-                                    llr_f16, write_c, st);
     if (A <= 32) return launch_k<2>(H, sigma2, Y, C, noise_inv, flags, l, nSC, L, S, A, nPamBits, prec, unbiased, thr,
                                     llr_f16, write_c, st);
     return launch_k<4>(H, sigma2, Y, C, noise_inv, flags, l, nSC, L, S, A, nPamBits, prec, unbiased, thr, llr_f16,

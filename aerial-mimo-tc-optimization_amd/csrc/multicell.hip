@@ -163,10 +163,8 @@ __device__ __forceinline__ void slot_apply_rec(const unsigned char *__restrict__
     if (reset && blockIdx.x == 0 && threadIdx.x == 0) reset->flagged = 0u;
     const BlockView v = view(block, off_cells, off_tiles);
     const LaunchRange rg = v.ranges[li];
-    auto one = [&](const CellDev &cd, long sc) {
-        const int ks = ks_class_dev(cd.A);
-        if (ks == 1) apply_record_class<1, PREC>(cd, sc, lds[w], gfull_all[w], lane);
-        else if (ks == 2) apply_record_class<2, PREC>(cd, sc, lds[w], gfull_all[w], lane);
+    auto one = [&](const CellDev &cd, long sc) {  // 32 <= A <= 64 (record_pays)
+        if (cd.A <= 32) apply_record_class<2, PREC>(cd, sc, lds[w], gfull_all[w], lane);
         else apply_record_class<4, PREC>(cd, sc, lds[w], gfull_all[w], lane);
     };
     if constexpr (!LOOP) {
@@ -301,7 +299,7 @@ int SlotExec::max_grid() const {
 }
 
 bool SlotExec::uses_records(const pe_cell_desc &c) const {
-    return (flags_ & PE_SLOT_SOLVE_RECORDS) && c.H && c.prec != PE_PREC_F16_C16 && rec::record_shape(c.nLayer, c.nAnt);
+    return (flags_ & PE_SLOT_SOLVE_RECORDS) && c.H && c.prec != PE_PREC_F16_C16 && rec::record_pays(c.nLayer, c.nAnt);
 }
 
 int SlotExec::launch_index(int kind, int prec) const {

@@ -17,6 +17,10 @@ namespace rec {
 __host__ __device__ constexpr int record_floats(int L) { return L * (L + 2); }
 // shapes whose coefficients can be formed on chip: one antenna per lane, and the record fits the block
 __host__ __device__ constexpr bool record_shape(int L, int A) { return A <= 64 && L + 2 <= 2 * A; }
+// shapes for which it pays: measured over the deployment sweep (one cell per launch, f16), the one-pass chain ran
+// at 0.93-0.99x of the two launches below 32 antennas or 4 layers, 0.99-1.06x at 32 antennas and 1.04-1.14x at
+// 64 antennas with 4-8 layers (1.2x on 32-cell batches of 8 layers x 64 antennas)
+__host__ __device__ constexpr bool record_pays(int L, int A) { return record_shape(L, A) && A >= 32 && L >= 4; }
 constexpr int kGinvBytes = 8 * 8 * 8;  // per-wave full Ginv, complex [8][8], in LDS
 
 // Records of a solved tile (subcarriers sc0 .. sc0 + n - 1, slots in lds as tile_solve leaves them) into the

@@ -457,7 +457,9 @@ def deployment_sweep(dev, seed=5, reps=10):
     configuration: apply EVM of each MFMA precision vs the FP64 product of the same coefficients, and the
     device time per launch of the fused apply + demap (median of `reps` single launches, each queued behind a
     short spin so that host launch overhead is excluded). The bf16 path at the -60 dB bar is plain bf16 where
-    that meets it, else bf16_split; it is compared with the plain FP32 path."""
+    that meets it, else bf16_split; it is compared with the plain FP32 path. Also the chain from channel
+    estimates at f16 (pe_gram_mmse + pe_coef_apply_demap against pe_mmse_apply_demap), timed the same way, and
+    whether the two give identical LLRs."""
     import numpy as np
     import torch
     import pusch_eq as pe
@@ -499,6 +501,30 @@ def deployment_sweep(dev, seed=5, reps=10):
                             ts.append(a.elapsed_time(b) * 1e3)
                     ts.sort()
                     row["us_" + prec] = ts[len(ts) // 2]
+                # the chain from channel estimates, f16: two launches against the one pass (same device-time method)
+                def dev_us(fn):
+                    ts = []
+                    for i in range(reps + 2):
+                        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                        torch.cuda._sleep(100000)
+                        a.record()
+                        fn()
+                        b.record()
+                        b.synchronize()
+                        if i >= 2:
+                            ts.append(a.elapsed_time(b) * 1e3)
+                    ts.sort()
+                    return ts[len(ts) // 2]
+
+                C2, ni2, fl2 = torch.empty_like(C), torch.empty_like(ni), torch.empty_like(fl)
+                llr1, llr2 = torch.empty_like(llr), torch.empty_like(llr)
+                row["us_chain_two_launches"] = dev_us(lambda: (
+                    pe.gram_mmse(H, sigma2, unbiased=True, C=C2, noise_inv=ni2, flags=fl2),
+                    pe.coef_apply_demap(C2, Y, ni2, q, prec="f16", out=llr1)))
+                row["us_chain_one_pass"] = dev_us(lambda: pe.mmse_apply_demap(H, sigma2, Y, q, prec="f16", C=C2,
+                                                                              noise_inv=ni2, flags=fl2, out=llr2))
+                row["one_pass_identical"] = bool(torch.equal(llr1, llr2))
+                row["speedup_one_pass"] = row["us_chain_two_launches"] / row["us_chain_one_pass"]
                 bf = "bf16" if row["evm_db_bf16"] <= -60.0 else "bf16_split"
                 row["bf16_at_bar"] = bf
                 row["speedup_bf16_at_bar_vs_f32"] = row["us_f32"] / row["us_" + bf]
@@ -516,6 +542,12 @@ def deployment_sweep(dev, seed=5, reps=10):
     summ["bf16_at_bar_speedup_vs_f32_median"] = statistics.median(r["speedup_bf16_at_bar_vs_f32"] for r in rows)
     summ["bf16_at_bar_speedup_vs_f32_min"] = min(r["speedup_bf16_at_bar_vs_f32"] for r in rows)
     summ["f16_speedup_vs_f32_median"] = statistics.median(r["speedup_f16_vs_f32"] for r in rows)
+    sp = [r["speedup_one_pass"] for r in rows]
+    summ["one_pass_chain_speedup_median"] = statistics.median(sp)
+    summ["one_pass_chain_speedup_min_max"] = [min(sp), max(sp)]
+    summ["one_pass_chain_speedup_median_64_antennas"] = statistics.median(
+        r["speedup_one_pass"] for r in rows if r["A"] == 64)
+    summ["one_pass_llrs_identical_all"] = all(r["one_pass_identical"] for r in rows)
     big = [r for r in rows if r["nsc"] == 3276]
     summ["largest_cells_bf16_at_bar_speedup_vs_f32_median"] = statistics.median(
         r["speedup_bf16_at_bar_vs_f32"] for r in big)

@@ -197,10 +197,11 @@ pe_status pe_coef_apply_demap_c16(const pe_c16 *C16, const int *c_exp, const pe_
    written to HBM. The LLRs (and noise_inv, sc_flags) are bit-identical to pe_gram_mmse(H, sigma2, C, noise_inv,
    sc_flags, ..., flags & PE_MMSE_UNBIASED, cond_threshold) followed by pe_coef_apply_demap (or, with
    PE_EQ_LLR_F16, pe_coef_apply_demap_f16) at the same precision.
-   C [nSC][nLayer][nAnt] is a workspace: only the rows of subcarriers solved in FP64 are written (the FP64 pass
-   writes them there and a third kernel applies them), or every row with PE_EQ_WRITE_C (then C equals
-   pe_gram_mmse's output). Shapes as pe_coef_apply_demap with prec PE_PREC_F16, PE_PREC_BF16 or their split
-   forms (MFMA only); H, Y_rx and llr 16-byte aligned, C 8-byte aligned. flags: PE_MMSE_UNBIASED, and: */
+   C [nSC][nLayer][nAnt] is a workspace whose contents afterwards are unspecified, except with PE_EQ_WRITE_C
+   (then C equals pe_gram_mmse's output). The on-chip path serves 32 <= nAnt <= 64 with nLayer >= 4, where it
+   measured faster; other shapes run the two kernels with C in HBM (same results). Shapes as
+   pe_coef_apply_demap with prec PE_PREC_F16, PE_PREC_BF16 or their split forms (MFMA only); H, Y_rx and llr
+   16-byte aligned, C 8-byte aligned. flags: PE_MMSE_UNBIASED, and: */
 #define PE_EQ_WRITE_C 2u  /* write every coefficient to C as well */
 #define PE_EQ_LLR_F16 4u  /* LLRs in binary16 (the wire format of pe_coef_apply_demap_f16) instead of FP32 */
 pe_status pe_mmse_apply_demap(const pe_c64 *H, const float *sigma2, const pe_c64 *Y_rx, pe_c64 *C, float *noise_inv,
@@ -273,8 +274,9 @@ typedef struct pe_slot_plan pe_slot_plan;
                                    nLayer (nLayer + 2) floats) at the start of its block of C instead of the
                                    coefficients, and stage 2 forms the coefficients on chip from H and that record,
                                    as pe_mmse_apply_demap does. Applies to cells with H, an MFMA precision other
-                                   than PE_PREC_F16_C16, nAnt <= 64 and nLayer + 2 <= 2 nAnt; LLRs and noise_inv
-                                   are unchanged bit for bit, C holds coefficients only where the FP64 pass ran. */
+                                   than PE_PREC_F16_C16, 32 <= nAnt <= 64 and nLayer >= 4 (where it measured
+                                   faster); LLRs and noise_inv are unchanged bit for bit, C holds coefficients only
+                                   where the FP64 pass ran. */
 pe_status pe_slot_plan_create(const pe_cell_desc *cells, int nCells, unsigned flags, pe_slot_plan **plan);
 /* Rewrites the per-cell pointers of an existing plan (same shapes) for the next slot, ordered on `stream`. */
 pe_status pe_slot_plan_update(pe_slot_plan *plan, const pe_cell_desc *cells, int nCells, void *stream);
