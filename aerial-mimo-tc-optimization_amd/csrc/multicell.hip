@@ -153,9 +153,13 @@ __device__ __forceinline__ void apply_record_class(const CellDev &cd, long sc, u
                                         lds, gfull, lane);
 }
 
-template <int PREC, bool LOOP>
-__device__ __forceinline__ void slot_apply_rec(const unsigned char *__restrict__ block, int li, int off_cells,
-                                               int off_tiles, Counters *reset) {
+// One tile (4 subcarriers, one per wave) per workgroup: plans and eager batches (graph replays of varying batches
+// keep the coefficient path). Five waves per SIMD asked for: without the hint the compiler schedules the inlined
+// layer and antenna variants for 128-148 registers; with it, 93 for the binary16 and bfloat16 products, with no
+// spills.
+template <int PREC>
+__global__ __launch_bounds__(64 * kApplyWaves) __attribute__((amdgpu_waves_per_eu(5, 8))) void slot_apply_rec_kernel(
+    const unsigned char *__restrict__ block, int li, int off_cells, int off_tiles, Counters *reset) {
     using namespace mfma;
     __shared__ __attribute__((aligned(16))) unsigned char lds[kApplyWaves][LdsLayout<4>::bytes(PREC)];
     __shared__ __attribute__((aligned(16))) float2 gfull_all[kApplyWaves][rec::kGinvBytes / sizeof(float2)];
@@ -167,37 +171,11 @@ __device__ __forceinline__ void slot_apply_rec(const unsigned char *__restrict__
         if (cd.A <= 32) apply_record_class<2, PREC>(cd, sc, lds[w], gfull_all[w], lane);
         else apply_record_class<4, PREC>(cd, sc, lds[w], gfull_all[w], lane);
     };
-    if constexpr (!LOOP) {
-        if ((int)blockIdx.x >= rg.ntiles) return;
-        const Tile tl = v.tiles[rg.tile0 + blockIdx.x];
-        const long sc = tl.sc0 + w;
-        if (sc >= (long)tl.sc0 + tl.n) return;  // uniform per wave; no workgroup barrier follows
-        one(v.cells[tl.cell], sc);
-    } else {
-        for (int t = blockIdx.x; t < rg.ntiles; t += gridDim.x) {
-            const Tile tl = v.tiles[rg.tile0 + t];
-            const CellDev cd = v.cells[tl.cell];
-            for (long sc = tl.sc0 + w; sc < (long)tl.sc0 + tl.n; sc += kApplyWaves) one(cd, sc);
-            lds_order_wave();  // LDS of this tile consumed before the next tile may zero it
-        }
-    }
-}
-
-// One tile per workgroup. Five waves per SIMD asked for: without the hint the compiler schedules the inlined
-// layer and antenna variants for 128-148 registers; with it, 93 for the binary16 and bfloat16 products, with no
-// spills (measured on the 24-cell plans: 66.0 us against 70.9 us per eager slot).
-template <int PREC>
-__global__ __launch_bounds__(64 * kApplyWaves) __attribute__((amdgpu_waves_per_eu(5, 8))) void slot_apply_rec_kernel(
-    const unsigned char *__restrict__ block, int li, int off_cells, int off_tiles, Counters *reset) {
-    slot_apply_rec<PREC, false>(block, li, off_cells, off_tiles, reset);
-}
-
-// Workgroups striding over the tiles (graph replays of varying batches); the hint would make this one spill.
-template <int PREC>
-__global__ __launch_bounds__(64 * kApplyWaves) void slot_apply_rec_loop_kernel(const unsigned char *__restrict__ block,
-                                                                              int li, int off_cells, int off_tiles,
-                                                                              Counters *reset) {
-    slot_apply_rec<PREC, true>(block, li, off_cells, off_tiles, reset);
+    if ((int)blockIdx.x >= rg.ntiles) return;
+    const Tile tl = v.tiles[rg.tile0 + blockIdx.x];
+    const long sc = tl.sc0 + w;
+    if (sc >= (long)tl.sc0 + tl.n) return;  // uniform per wave; no workgroup barrier follows
+    one(v.cells[tl.cell], sc);
 }
 
 // Stage 1: a tile is up to mmse_tile subcarriers of one cell. FP32 pass: one workgroup of W waves solves the
@@ -551,13 +529,11 @@ hipError_t SlotExec::launch_one(int i, hipStream_t st, bool exact) const {
     }
     // the first apply kernel zeroes the MMSE FP32 pass's counter (launch 0) once both MMSE passes are done
     Counters *rs = (i > 0 && launches_[i - 1].kind < 2 && launches_[0].kind == 0) ? d_counters_ : nullptr;
-    if (l.kind == 3) {  // cells with solve records
+    if (l.kind == 3) {  // cells with solve records: plans and eager batches only (one tile per workgroup)
+        if (!lf) return hipErrorInvalidValue;
 #define PE_REC_CASE(PP)                                                                                          \
     if (l.prec == PP) {                                                                                          \
-        if (lf)                                                                                                  \
-            hipLaunchKernelGGL((slot_apply_rec_kernel<PP>), g, b, 0, st, d_block_, i, oc, ot, rs);               \
-        else                                                                                                     \
-            hipLaunchKernelGGL((slot_apply_rec_loop_kernel<PP>), g, b, 0, st, d_block_, i, oc, ot, rs);          \
+        hipLaunchKernelGGL((slot_apply_rec_kernel<PP>), g, b, 0, st, d_block_, i, oc, ot, rs);                   \
         return hipGetLastError();                                                                                \
     }
         PE_REC_CASE(1) PE_REC_CASE(2) PE_REC_CASE(3) PE_REC_CASE(4)
