@@ -9,6 +9,8 @@
 // 128-bit loads (each 128-bit load is two complex samples), so every HBM byte is read once and each staged
 // sample is reused by 4 tiles from LDS instead of being re-fetched through the vector cache. The staging
 // addresses of a lane are computed once per workgroup (no divisions in the chunk loop).
+#include <algorithm>
+
 #include "pe_device.hpp"
 #include "pe_internal.hpp"
 
@@ -152,22 +154,48 @@ __global__ __launch_bounds__(kF32Threads) void coef_apply_f32_kernel(
     }
 }
 
-// Gram product, one lane per (sc, r, c); M rows and H columns are read through the vector cache (every
-// element of M[sc] is used by L lanes of the same wave, every element of H[sc] by L lanes).
+// Gram product, one lane per (sc, r, c) with the antenna-serial cma chain of gram_scalar (bit-exact order). A
+// workgroup takes spw subcarriers (spw L^2 lanes); M rows and H rows of the workgroup's subcarriers are staged
+// through LDS in chunks of kGramKC antennas with coalesced loads (each element read from HBM once), and every
+// lane reads its M row and H column from there.
+constexpr int kGramKC = 32;
+constexpr int kGramLds = 20 * 1024;
+
 __global__ __launch_bounds__(256) void gram_f32_kernel(const float2 *__restrict__ M, const float2 *__restrict__ H,
-                                                       float2 *__restrict__ G, int nSC, int L, int A) {
-    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+                                                       float2 *__restrict__ G, int nSC, int L, int A, int spw) {
+    extern __shared__ float2 g_lds[];
+    // M rows padded by one value: the lanes of a wave read L different rows at the same k, which would otherwise
+    // all fall in the same LDS bank
+    constexpr int MS = kGramKC + 1;
+    float2 *lm = g_lds;                        // [spw][L][MS]
+    float2 *lh = g_lds + (size_t)spw * L * MS;  // [spw][kGramKC][L]
     const int per = L * L;
-    const long sc = t / per;
-    if (sc >= nSC) return;
-    const int rc = (int)(t - sc * per);
-    const int r = rc / L;
-    const int c = rc - r * L;
-    const float2 *m = M + ((size_t)sc * L + r) * A;
-    const float2 *h = H + (size_t)sc * A * L + c;
+    const int s = threadIdx.x / per, rc = threadIdx.x - s * per;
+    const int r = rc / L, c = rc - r * L;
+    const long sc0 = (long)blockIdx.x * spw;
+    const int nv = (int)min((long)spw, (long)nSC - sc0);  // subcarriers of this workgroup
+    const bool mine = s < nv;
     float2 acc = make_float2(0.f, 0.f);
-    for (int k = 0; k < A; ++k) acc = cma(m[k], h[(size_t)k * L], acc);
-    G[((size_t)sc * L + r) * L + c] = acc;
+    for (int k0 = 0; k0 < A; k0 += kGramKC) {
+        const int kc = min(kGramKC, A - k0);
+        // M[sc][row][k0 .. k0 + kc): row sr = s * L + row, column k (kGramKC is a power of two: shifts, no division)
+        for (int i = threadIdx.x; i < nv * L * kGramKC; i += blockDim.x) {
+            const int sr = i / kGramKC, k = i % kGramKC;
+            if (k < kc) lm[sr * MS + k] = M[((size_t)sc0 * L + sr) * A + k0 + k];
+        }
+        // H[sc][k0 .. k0 + kc)[col]: kc L contiguous values per subcarrier
+        for (int ss = 0; ss < nv; ++ss)
+            for (int i = threadIdx.x; i < kc * L; i += blockDim.x)
+                lh[ss * kGramKC * L + i] = H[((size_t)(sc0 + ss) * A + k0) * L + i];
+        __syncthreads();
+        if (mine) {
+            const float2 *mr = lm + (s * L + r) * MS;
+            const float2 *hc = lh + s * kGramKC * L + c;
+            for (int k = 0; k < kc; ++k) acc = cma(mr[k], hc[k * L], acc);
+        }
+        __syncthreads();
+    }
+    if (mine) G[((size_t)(sc0 + s) * L + r) * L + c] = acc;
 }
 
 template <int KC, int NPB>
@@ -216,9 +244,14 @@ hipError_t launch_coef_apply_demap_f32(const float2 *C, const float2 *Y, const f
 }
 
 hipError_t launch_gram_f32(const float2 *M, const float2 *H, float2 *G, int nSC, int L, int A, hipStream_t st) {
-    const long total = (long)nSC * L * L;
-    const int grid = (int)((total + 255) / 256);
-    hipLaunchKernelGGL(gram_f32_kernel, dim3(grid), dim3(256), 0, st, M, H, G, nSC, L, A);
+    // subcarriers per workgroup: at most 256 lanes and kGramLds bytes of staged M and H chunks
+    const int per = L * L;
+    const int per_sc = L * (2 * kGramKC + 1) * (int)sizeof(float2);  // staged bytes of one subcarrier
+    const int spw = std::max(1, std::min(256 / per, kGramLds / per_sc));
+    const int threads = (spw * per + 63) / 64 * 64;
+    const size_t lds = (size_t)spw * per_sc;
+    const long grid = ((long)nSC + spw - 1) / spw;
+    hipLaunchKernelGGL(gram_f32_kernel, dim3((unsigned)grid), dim3(threads), lds, st, M, H, G, nSC, L, A, spw);
     return hipGetLastError();
 }
 

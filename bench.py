@@ -381,6 +381,61 @@ def srs_bench(dev, iters=500, warmup=50):
     return rows
 
 
+def operators_bench(dev, n_cells, steps=50):
+    """The standalone drop-ins of the reference's other kernels over the same batch size (n_cells x 1584
+    subcarriers, 8 layers, 11 symbols, 64 antennas, 256-QAM): pe_gram (gram_scalar's M @ H, FP32, bit-exact
+    order), pe_llr_demap_f16 (llr_v1_baseline's binary16 format, one noise value per symbol) and pe_llr_demap
+    (FP32 on the equalizer layout). Mean of `steps` back-to-back launches rotating over buffer sets that together
+    exceed the 256 MB Infinity Cache (2 sets for the Gram, 4 for the demappers), so the bytes come from HBM."""
+    import torch
+    import pusch_eq as pe
+    nsc = n_cells * NSC_CELL
+    g = torch.Generator(device=dev)
+    g.manual_seed(77)
+    out = {}
+
+    def timed(fn, bufs):
+        el, _ = time_launches(fn, bufs, steps, 5)
+        return el / steps
+
+    # Gram M @ H: M [nsc, L, A], H [nsc, A, L] -> G [nsc, L, L]
+    gb = []
+    for k in range(2):
+        M = torch.randn((nsc, L, A), dtype=torch.complex64, device=dev, generator=g)
+        Hh = torch.randn((nsc, A, L), dtype=torch.complex64, device=dev, generator=g)
+        gb.append((M, Hh, torch.empty((nsc, L, L), dtype=torch.complex64, device=dev)))
+    t = timed(lambda M, Hh, G: pe.gram(M, Hh, out=G), gb)
+    by = nsc * (L * A * 8 + A * L * 8 + L * L * 8)
+    out["gram_f32"] = {"ms_per_launch": t * 1e3, "subcarriers_per_s": nsc / t, "bytes_per_launch": by,
+                       "hbm_gbs": by / t / 1e9}
+    del gb
+    # binary16 demapper, the reference's format: N symbols, noise per symbol
+    N = nsc * L * S
+    db = []
+    for k in range(4):
+        soft = (torch.randn((N, 2), device=dev, generator=g) * 0.7).to(torch.float16)
+        nis = torch.rand((N,), device=dev, generator=g) * 10 + 1
+        db.append((soft, nis, torch.empty((N, 2 * Q), dtype=torch.float16, device=dev)))
+    t = timed(lambda soft, nis, o: pe.llr_demap_f16(soft, nis, Q, out=o), db)
+    by = N * (4 + 4 + 2 * Q * 2)
+    out["llr_demap_f16"] = {"ms_per_launch": t * 1e3, "symbols_per_s": N / t, "bytes_per_launch": by,
+                            "hbm_gbs": by / t / 1e9, "symbols": N}
+    del db
+    # FP32 demapper on Y_eq [nsc, L, S], noise_inv [nsc, L]
+    fb = []
+    for k in range(4):
+        Yeq = torch.randn((nsc, L, S), dtype=torch.complex64, device=dev, generator=g) * 0.7
+        ni = torch.rand((nsc, L), device=dev, generator=g) * 10 + 1
+        fb.append((Yeq, ni, torch.empty((nsc, L, S, 2 * Q), dtype=torch.float32, device=dev)))
+    t = timed(lambda Yeq, ni, o: pe.llr_demap(Yeq, ni, Q, out=o), fb)
+    by = nsc * (L * S * 8 + L * 4 + L * S * 2 * Q * 4)
+    out["llr_demap_f32"] = {"ms_per_launch": t * 1e3, "symbols_per_s": N / t, "bytes_per_launch": by,
+                            "hbm_gbs": by / t / 1e9}
+    out["workload"] = ("%d x 1584 subcarriers, 8 layers, 11 symbols, 64 antennas, 256-QAM; random inputs; "
+                       "buffer sets rotate beyond the Infinity Cache" % n_cells)
+    return out
+
+
 def fronthaul_bench(dev, n_cells, steps=50):
     """pe_bfp_decompress of the headline batch: n_cells x 132 PRB, 11 symbols, 64 antennas, 9-bit block
     floating point (28 bytes per PRB) -> complex64 Y_rx; random bytes (every byte pattern is a valid block)."""
@@ -821,6 +876,9 @@ def main():
     multicell = None if args.no_multicell else multicell_bench(dev, 50)
     srs = None if args.no_srs else srs_bench(dev)
     fronthaul = fronthaul_bench(dev, n)
+    operators = operators_bench(dev, n)
+    log("standalone operators: gram %.1f GB/s, demap f16 %.1f GB/s, demap f32 %.1f GB/s" % (
+        operators["gram_f32"]["hbm_gbs"], operators["llr_demap_f16"]["hbm_gbs"], operators["llr_demap_f32"]["hbm_gbs"]))
     log("fronthaul decompress %.1f GB/s" % fronthaul["hbm_gbs"])
     serving = serving_bench(dev, args.serve_jobs) if args.serve_jobs > 0 else None
     sweep = None
@@ -893,6 +951,7 @@ def main():
         "serving": serving,
         "srs_filter": srs,
         "fronthaul_bfp_decompress": fronthaul,
+        "standalone_operators": operators,
         "deployment_sweep": sweep,
         "wall_s_timed": wall,
     }

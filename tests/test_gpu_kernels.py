@@ -196,7 +196,8 @@ def test_fused_llr_wire_format_f16(po, pe, dev, prec, q, L, S, A, nsc, ni_hi):
 # ------------------------------------------------------------------------------------------------------
 # Gram product and MMSE
 # ------------------------------------------------------------------------------------------------------
-@pytest.mark.parametrize("L,A,nsc", [(8, 64, 1584), (4, 32, 55), (1, 4, 9), (6, 100, 21)])
+@pytest.mark.parametrize("L,A,nsc", [(8, 64, 1584), (4, 32, 55), (1, 4, 9), (6, 100, 21), (3, 200, 37),
+                                     (8, 1024, 5), (2, 65, 70), (7, 1, 3), (5, 128, 13)])
 def test_gram_bitexact(po, pe, dev, L, A, nsc):
     rng = np.random.default_rng(L + A)
     M = po.cn(rng, (nsc, L, A))
