@@ -488,6 +488,28 @@ pe_status pe_srs_filter(pe_c16 *H_est, const pe_c16 *SRS, const pe_c16 *W, const
     return from_hip(e, "srs_filter launch");
 }
 
+pe_status pe_freq_interp(const pe_c64 *W, const pe_c64 *X, pe_c64 *H_f, int nBatch, int nOut, int nIn, int nCol,
+                         pe_prec prec, void *stream) {
+    if (nBatch < 0) return fail(PE_ERR_INVALID, "nBatch = %d < 0", nBatch);
+    if (nOut < 1 || nOut > 256) return fail(PE_ERR_INVALID, "nOut = %d outside [1, 256]", nOut);
+    if (nIn < 1 || nIn > 128) return fail(PE_ERR_INVALID, "nIn = %d outside [1, 128]", nIn);
+    if (nCol < 1 || nCol > 16) return fail(PE_ERR_INVALID, "nCol = %d outside [1, 16]", nCol);
+    if (!valid_prec(prec)) return fail(PE_ERR_INVALID, "unknown precision %d", prec);
+    if (prec != PE_PREC_F32 && prec != PE_PREC_BF16 && prec != PE_PREC_BF16_SPLIT)
+        return fail(PE_ERR_UNSUPPORTED, "precision %d: frequency interpolation runs F32, BF16 or BF16_SPLIT", prec);
+    if (prec == PE_PREC_F32 && pe::freq_interp_f32_lds_bytes(nOut, nIn, nCol) > 64 * 1024)
+        return fail(PE_ERR_UNSUPPORTED, "PE_PREC_F32 keeps W and one X block in 64 KB of LDS: nOut %d x nIn %d too large",
+                    nOut, nIn);
+    if (nBatch == 0) return ok();
+    if (!check_ptrs(W, X, H_f)) return fail(PE_ERR_INVALID, "null pointer");
+    if ((reinterpret_cast<uintptr_t>(W) | reinterpret_cast<uintptr_t>(X) | reinterpret_cast<uintptr_t>(H_f)) & 7u)
+        return fail(PE_ERR_INVALID, "W, X and H_f must be 8-byte aligned");
+    hipError_t e = pe::launch_freq_interp(reinterpret_cast<const float2 *>(W), reinterpret_cast<const float2 *>(X),
+                                          reinterpret_cast<float2 *>(H_f), nBatch, nOut, nIn, nCol, (int)prec,
+                                          (hipStream_t)stream);
+    return from_hip(e, "freq_interp launch");
+}
+
 pe_status pe_hbm_probe(const void *src, void *dst, size_t bytes, int read_only, int grid, void *stream) {
     if (bytes % 16 != 0) return fail(PE_ERR_INVALID, "bytes = %zu is not a multiple of 16", bytes);
     if (grid < 1 || grid > (1 << 20)) return fail(PE_ERR_INVALID, "grid = %d outside [1, 2^20]", grid);

@@ -69,6 +69,11 @@ hipError_t launch_coef_to_c16(const float2 *C, uint32_t *C16, int *cexp, int nSC
 hipError_t launch_coef_apply_c16(const uint32_t *C16, const int *cexp, const float2 *Y, float2 *O,
                                  const float *noise_inv, void *llr, int nSC, int L, int S, int A, int nPamBits,
                                  int llr_f16, hipStream_t st);
+// DMRS frequency interpolation H_f[b][m][n] = sum_k W[m][k] X[b][k][n]; prec 0 (FP32 chain), 2 (bf16 MFMA),
+// 4 (bf16 split MFMA). 1 <= K <= 128, 1 <= NC <= 16; FP32 needs freq_interp_f32_lds_bytes() <= 64 KB.
+size_t freq_interp_f32_lds_bytes(int M, int K, int NC);
+hipError_t launch_freq_interp(const float2 *W, const float2 *X, float2 *Hf, int nBatch, int M, int K, int NC,
+                              int prec, hipStream_t st);
 // Fronthaul block floating point rows -> Y_rx [sc][sym][ant] (iqWidth 1..16).
 hipError_t launch_bfp_decompress(const uint8_t *in, float2 *Y, int nPrb, int nSym, int nAnt, int iqWidth,
                                  size_t row_stride, float scale, hipStream_t st);

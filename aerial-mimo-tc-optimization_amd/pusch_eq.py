@@ -46,6 +46,7 @@ SIGNATURES = {
     "pe_srs_filter": ([_P, _P, _P, _P, _I, _I, _I, _I, _I, _U, _P], _I),
     "pe_hbm_probe": ([_P, _P, ctypes.c_size_t, _I, _I, _P], _I),
     "pe_bfp_decompress": ([_P, _P, _I, _I, _I, _I, ctypes.c_size_t, _F, _P], _I),
+    "pe_freq_interp": ([_P, _P, _P, _I, _I, _I, _I, _I, _P], _I),
     "pe_coef_to_c16": ([_P, _P, _P, _I, _I, _I, _P], _I),
     "pe_gram_mmse_c16": ([_P, _P, _P, _P, _P, _P, _I, _I, _I, _U, _F, _P], _I),
     "pe_coef_apply_c16": ([_P, _P, _P, _P, _I, _I, _I, _I, _P], _I),
@@ -336,6 +337,20 @@ def bfp_decompress(packed, nPrb, nSym, nAnt, iq_width, scale=1.0, out=None, stre
         out = torch.empty((nPrb * 12, nSym, nAnt), dtype=torch.complex64, device=packed.device)
     _check(lib().pe_bfp_decompress(_ptr(packed), _ptr(out), nPrb, nSym, nAnt, iq_width, packed.shape[1],
                                    float(scale), _stream(stream)))
+    return out
+
+
+def freq_interp(W, X, prec="f32", out=None, stream=None):
+    """DMRS frequency interpolation (pe_freq_interp): W [nOut, nIn], X [nBatch, nIn, nCol] complex64 ->
+    H_f [nBatch, nOut, nCol], H_f[b] = W @ X[b]. prec "f32" (bit-exact chain), "bf16" or "bf16_split" (MFMA)."""
+    import torch
+    nOut, nIn = W.shape
+    nBatch, nIn2, nCol = X.shape
+    if nIn2 != nIn:
+        raise ValueError("W %s and X %s disagree on nIn" % (tuple(W.shape), tuple(X.shape)))
+    if out is None:
+        out = torch.empty((nBatch, nOut, nCol), dtype=torch.complex64, device=X.device)
+    _check(lib().pe_freq_interp(_ptr(W), _ptr(X), _ptr(out), nBatch, nOut, nIn, nCol, _prec(prec), _stream(stream)))
     return out
 
 

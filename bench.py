@@ -455,6 +455,54 @@ def fronthaul_bench(dev, n_cells, steps=50):
             "bytes_per_launch": nbytes}
 
 
+FI_M, FI_K, FI_NC = 64, 96, 4           # the frequency-interpolation GEMM the reference names: [64 x 96] @ [96 x 4]
+FI_BATCH = 14 * 64 * 64                  # 14 symbols x 64 antennas x 64 UE allocations
+
+
+def freq_interp_bench(dev, steps=50, cpu_seconds=2.0, precs=("bf16", "bf16_split", "f32")):
+    """pe_freq_interp over FI_BATCH items of [96 pilots x 4 ports] through one [64 x 96] filter, per precision:
+    mean of `steps` launches rotating over 2 buffer sets (586 MB together, beyond the Infinity Cache). Bytes per
+    item: X 3,072 + H_f 2,048 (W, 48 KB, once per launch). The oracle's FP32 chain (OpenMP over items) on a
+    sample of items is the CPU figure."""
+    import torch
+    import pusch_eq as pe
+    g = torch.Generator(device=dev)
+    g.manual_seed(96)
+    W = torch.randn((FI_M, FI_K), dtype=torch.complex64, device=dev, generator=g)
+    sets = []
+    for k in range(2):
+        X = torch.randn((FI_BATCH, FI_K, FI_NC), dtype=torch.complex64, device=dev, generator=g)
+        sets.append((X, torch.empty((FI_BATCH, FI_M, FI_NC), dtype=torch.complex64, device=dev)))
+    item_bytes = FI_K * FI_NC * 8 + FI_M * FI_NC * 8
+    nbytes = FI_BATCH * item_bytes + FI_M * FI_K * 8
+    out = {"workload": "%d items (14 symbols x 64 antennas x 64 UE allocations) of [%d pilots x %d ports] through "
+                       "one [%d x %d] filter, complex64 in and out" % (FI_BATCH, FI_K, FI_NC, FI_M, FI_K),
+           "bytes_per_launch": nbytes, "flops_per_launch": 8 * FI_M * FI_K * FI_NC * FI_BATCH}
+    for p in precs:
+        el, _ = time_launches(lambda X, H: pe.freq_interp(W, X, prec=p, out=H), sets, steps, 5)
+        per = el / steps
+        out[p] = {"ms_per_launch": per * 1e3, "items_per_s": FI_BATCH / per, "hbm_gbs": nbytes / per / 1e9,
+                  "tflops": out["flops_per_launch"] / per / 1e12}
+    if cpu_seconds > 0:  # CPU baseline leg: the oracle, timed
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import numpy as np
+        from pe_oracle import freq_interp_f32_omp
+        Wh = W.cpu().numpy()
+        Xh = sets[0][0][:4096].cpu().numpy()
+        Hh = np.empty((Xh.shape[0], FI_M, FI_NC), dtype=np.complex64)
+        freq_interp_f32_omp(Wh, Xh, Hh)  # warm-up
+        reps, t0 = 0, time.perf_counter()
+        while time.perf_counter() - t0 < cpu_seconds:
+            _, used = freq_interp_f32_omp(Wh, Xh, Hh)
+            reps += 1
+        el = time.perf_counter() - t0
+        out["cpu_baseline"] = {"items_per_s": reps * Xh.shape[0] / el, "cores": used, "kind": "port",
+                               "sample": "%d repetitions of %d items, oracle FP32 chain, OpenMP over items, %.1f s"
+                                         % (reps, Xh.shape[0], el)}
+    del sets
+    return out
+
+
 def serving_bench(dev, n_jobs):
     """64 cells; jobs (one slot of one cell: MMSE + fused apply/demap of 11 symbols) arrive as a Poisson
     process; capacity = throughput with every job queued at t = 0; then offered load 10-120% of capacity."""
@@ -642,6 +690,8 @@ def main():
                          "10 pe_mmse_apply_demap calls), for profiling")
     ap.add_argument("--fronthaul-only", action="store_true",
                     help="only the fronthaul decode of the 32-cell batch, for profiling")
+    ap.add_argument("--freq-interp-only", action="store_true",
+                    help="only the DMRS frequency interpolation (10 launches per precision), for profiling")
     ap.add_argument("--sweep-out", default=None, help="write the deployment sweep table (JSON) here")
     ap.add_argument("--multicell-only", action="store_true",
                     help="only the 24-cell slot section (graph and eager slots), printed as JSON (for profiling)")
@@ -674,6 +724,9 @@ def main():
         return
     if args.fronthaul_only:
         print(json.dumps(fronthaul_bench(dev, args.cells, 10)), flush=True)
+        return
+    if args.freq_interp_only:
+        print(json.dumps(freq_interp_bench(dev, 10, 0.0)), flush=True)
         return
     if args.chain_only:
         H, Y, s2 = make_cells(args.cells, 1234, dev)
@@ -880,6 +933,10 @@ def main():
     log("standalone operators: gram %.1f GB/s, demap f16 %.1f GB/s, demap f32 %.1f GB/s" % (
         operators["gram_f32"]["hbm_gbs"], operators["llr_demap_f16"]["hbm_gbs"], operators["llr_demap_f32"]["hbm_gbs"]))
     log("fronthaul decompress %.1f GB/s" % fronthaul["hbm_gbs"])
+    freq_interp = freq_interp_bench(dev)
+    log("frequency interpolation: bf16 %.1f GB/s, bf16 split %.1f GB/s, f32 %.1f GB/s (%.1f TFLOP/s)" % (
+        freq_interp["bf16"]["hbm_gbs"], freq_interp["bf16_split"]["hbm_gbs"], freq_interp["f32"]["hbm_gbs"],
+        freq_interp["f32"]["tflops"]))
     serving = serving_bench(dev, args.serve_jobs) if args.serve_jobs > 0 else None
     sweep = None
     if not args.no_sweep:
@@ -951,6 +1008,7 @@ def main():
         "serving": serving,
         "srs_filter": srs,
         "fronthaul_bfp_decompress": fronthaul,
+        "freq_interp": freq_interp,
         "standalone_operators": operators,
         "deployment_sweep": sweep,
         "wall_s_timed": wall,

@@ -226,6 +226,20 @@ pe_status pe_mmse_apply_demap(const pe_c64 *H, const float *sigma2, const pe_c64
 pe_status pe_srs_filter(pe_c16 *H_est, const pe_c16 *SRS, const pe_c16 *W, const pe_c16 *FOCC, int nAnt,
                         int nBlocks, int K, int nPorts, int foccLen, unsigned flags, void *stream);
 
+/* ---- DMRS frequency interpolation (channel estimation upstream of the equalizer) ------------------- */
+
+/* No reference kernel: the reference names the GEMM [64 x 96] @ [96 x 4] of the channel-estimation frequency
+   interpolation, batched over OFDM symbols and UEs (README.md:200), without code. One filter for the batch:
+     H_f[b][m][n] = sum_{k < nIn} W[m][k] * X[b][k][n]
+   W [nOut][nIn], X [nBatch][nIn][nCol] (estimates on the pilot tones), H_f [nBatch][nOut][nCol], complex64.
+   prec PE_PREC_F32: per output one fused complex multiply-add chain in k order (cma() of
+   aerial_coef_wmma.cu:60-64), bit-identical to the oracle; needs (nOut * (nIn | 1) + nIn * nCol) * 8 <= 64 KB.
+   PE_PREC_BF16: MFMA on bf16-rounded W and X with FP32 accumulation; PE_PREC_BF16_SPLIT: hi + lo bf16 operands
+   (Ahi*Bhi + Ahi*Blo + Alo*Bhi). Other precisions: PE_ERR_UNSUPPORTED. 1 <= nOut <= 256, 1 <= nIn <= 128,
+   1 <= nCol <= 16; W, X, H_f 8-byte aligned. */
+pe_status pe_freq_interp(const pe_c64 *W, const pe_c64 *X, pe_c64 *H_f, int nBatch, int nOut, int nIn, int nCol,
+                         pe_prec prec, void *stream);
+
 /* ---- fronthaul input: block floating point U-plane data -> Y_rx ----------------------------------- */
 
 /* No reference kernel: the reference cites the O-RAN WG4 CUS-plane specification (README.md:214) for the

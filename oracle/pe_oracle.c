@@ -18,6 +18,7 @@
  *   src/srs/srs_wmma_bench.cu complex_conjmul :43-47, hcmadd :48-52, srs_filter_v1_scalar :57-90,
  *                             inputs :262-268
  *   fronthaul block floating point: O-RAN WG4 CUS-plane format, cited only (README.md:214)
+ *   DMRS frequency interpolation GEMM: named only (README.md:200); cma() order of aerial_coef_wmma.cu:60-64
  */
 #include "pe_oracle.h"
 
@@ -803,4 +804,69 @@ void po_bfp_compress(const int32_t *x, uint8_t *out, int nPrb, int nSym, int nAn
                     bfp_put(blk, 8 + k * iqWidth, iqWidth, m);
                 }
             }
+}
+
+/* ------------------------------------------------------------------------------------------------------
+ * DMRS frequency interpolation: H_f[b][m][n] = sum_k W[m][k] X[b][k][n]
+ * The reference names this GEMM ([64 x 96] @ [96 x 4], batched over OFDM symbols and UEs, README.md:200)
+ * without code; the chain below uses the complex multiply-accumulate of cma() (aerial_coef_wmma.cu:60-64) in
+ * its fused form, k ascending, the order every FP32 kernel of the reference uses for its K loop.
+ * ------------------------------------------------------------------------------------------------------ */
+static void freq_interp_item_f32(const float *W, const float *X, float *Hf, int M, int K, int NC) {
+    for (int m = 0; m < M; m++)
+        for (int n = 0; n < NC; n++) {
+            float re = 0.f, im = 0.f;
+            for (int k = 0; k < K; k++) {
+                const float ar = W[2 * ((size_t)m * K + k)], ai = W[2 * ((size_t)m * K + k) + 1];
+                const float br = X[2 * ((size_t)k * NC + n)], bi = X[2 * ((size_t)k * NC + n) + 1];
+                re = fmaf(ar, br, re);
+                re = fmaf(-ai, bi, re);
+                im = fmaf(ar, bi, im);
+                im = fmaf(ai, br, im);
+            }
+            Hf[2 * ((size_t)m * NC + n)] = re;
+            Hf[2 * ((size_t)m * NC + n) + 1] = im;
+        }
+}
+
+void po_freq_interp_f32(const float *W, const float *X, float *Hf, int nBatch, int M, int K, int NC) {
+    for (int b = 0; b < nBatch; b++)
+        freq_interp_item_f32(W, X + 2 * (size_t)b * K * NC, Hf + 2 * (size_t)b * M * NC, M, K, NC);
+}
+
+/* The same chain, OpenMP over the batch (the timed CPU baseline); returns the thread count used. */
+int po_freq_interp_f32_omp(const float *W, const float *X, float *Hf, int nBatch, int M, int K, int NC,
+                           int nthreads) {
+    if (nthreads > 0) omp_set_num_threads(nthreads);
+    int used = 1;
+#pragma omp parallel
+    {
+#pragma omp single
+        used = omp_get_num_threads();
+#pragma omp for schedule(static)
+        for (int b = 0; b < nBatch; b++)
+            freq_interp_item_f32(W, X + 2 * (size_t)b * K * NC, Hf + 2 * (size_t)b * M * NC, M, K, NC);
+    }
+    return used;
+}
+
+/* FP64 sum of the exact products of the given operands (double W, so that operands rounded to a narrower
+ * format by the caller are taken as they are). */
+void po_freq_interp_f64(const double *W, const double *X, double *Hf, int nBatch, int M, int K, int NC) {
+    for (int b = 0; b < nBatch; b++) {
+        const double *x = X + 2 * (size_t)b * K * NC;
+        double *h = Hf + 2 * (size_t)b * M * NC;
+        for (int m = 0; m < M; m++)
+            for (int n = 0; n < NC; n++) {
+                double re = 0.0, im = 0.0;
+                for (int k = 0; k < K; k++) {
+                    const double ar = W[2 * ((size_t)m * K + k)], ai = W[2 * ((size_t)m * K + k) + 1];
+                    const double br = x[2 * ((size_t)k * NC + n)], bi = x[2 * ((size_t)k * NC + n) + 1];
+                    re += ar * br - ai * bi;
+                    im += ar * bi + ai * br;
+                }
+                h[2 * ((size_t)m * NC + n)] = re;
+                h[2 * ((size_t)m * NC + n) + 1] = im;
+            }
+    }
 }

@@ -74,4 +74,9 @@ void po_srs_bench_inputs(unsigned seed, int nAnt, int nBlocks, int K, int nPorts
 void po_bfp_decompress(const uint8_t *in, float *Y, int nPrb, int nSym, int nAnt, int iqWidth, size_t row_stride,
                        float scale);
 void po_bfp_compress(const int32_t *x, uint8_t *out, int nPrb, int nSym, int nAnt, int iqWidth, size_t row_stride);
+/* -- DMRS frequency interpolation (README.md:200, named without code) -- */
+void po_freq_interp_f32(const float *W, const float *X, float *Hf, int nBatch, int M, int K, int NC);
+int po_freq_interp_f32_omp(const float *W, const float *X, float *Hf, int nBatch, int M, int K, int NC,
+                           int nthreads);
+void po_freq_interp_f64(const double *W, const double *X, double *Hf, int nBatch, int M, int K, int NC);
 #endif

@@ -55,6 +55,9 @@ def lib():
             "po_srs_bench_inputs": ([U, I, I, I, I, I, P, P, P], None),
             "po_bfp_decompress": ([P, P, I, I, I, I, Z, F], None),
             "po_bfp_compress": ([P, P, I, I, I, I, Z], None),
+            "po_freq_interp_f32": ([P, P, P, I, I, I, I], None),
+            "po_freq_interp_f32_omp": ([P, P, P, I, I, I, I, I], I),
+            "po_freq_interp_f64": ([P, P, P, I, I, I, I], None),
         }
         for name, (args, res) in sig.items():
             fn = getattr(h, name)
@@ -418,3 +421,55 @@ def bfp_decompress(packed, nPrb, nSym, nAnt, iq_width, scale=1.0):
     Y = np.empty((nPrb * 12, nSym, nAnt, 2), dtype=np.float32)
     lib().po_bfp_decompress(_p(packed), _p(Y), nPrb, nSym, nAnt, iq_width, packed.shape[1], float(scale))
     return Y.view(np.complex64)[..., 0]
+
+
+# ---------------------------------------------------------------------------------------------------------
+# DMRS frequency interpolation: H_f[b][m][n] = sum_k W[m][k] X[b][k][n] (README.md:200 names the GEMM)
+# ---------------------------------------------------------------------------------------------------------
+def _fi_dims(W, X):
+    M, K = W.shape
+    nBatch, K2, NC = X.shape
+    assert K2 == K, (W.shape, X.shape)
+    return nBatch, M, K, NC
+
+
+def freq_interp_f32(W, X):
+    """FP32 chain, k ascending, fused cma() per k: W [M, K], X [nBatch, K, NC] complex64 -> [nBatch, M, NC]."""
+    W, X = _c64(W), _c64(X)
+    nBatch, M, K, NC = _fi_dims(W, X)
+    out = np.empty((nBatch, M, NC), dtype=np.complex64)
+    lib().po_freq_interp_f32(_p(W), _p(X), _p(out), nBatch, M, K, NC)
+    return out
+
+
+def freq_interp_f32_omp(W, X, out=None, nthreads=0):
+    """The FP32 chain with OpenMP over the batch; returns (out, threads used)."""
+    W, X = _c64(W), _c64(X)
+    nBatch, M, K, NC = _fi_dims(W, X)
+    if out is None:
+        out = np.empty((nBatch, M, NC), dtype=np.complex64)
+    used = lib().po_freq_interp_f32_omp(_p(W), _p(X), _p(out), nBatch, M, K, NC, int(nthreads))
+    return out, used
+
+
+def freq_interp_f64(W, X):
+    """FP64 sum of exact products of the operands as given (complex128 accepted): -> complex128."""
+    W = np.ascontiguousarray(W, dtype=np.complex128)
+    X = np.ascontiguousarray(X, dtype=np.complex128)
+    nBatch, M, K, NC = _fi_dims(W, X)
+    out = np.empty((nBatch, M, NC), dtype=np.complex128)
+    lib().po_freq_interp_f64(_p(W), _p(X), _p(out), nBatch, M, K, NC)
+    return out
+
+
+def round_bf16(a):
+    """float32 values rounded to bfloat16 (nearest even), returned as float32."""
+    u = np.ascontiguousarray(a, dtype=np.float32).view(np.uint32).astype(np.uint64)
+    u = (u + 0x7FFF + ((u >> 16) & 1)) & 0xFFFF0000
+    return u.astype(np.uint32).view(np.float32)
+
+
+def round_bf16_complex(z):
+    """complex64 -> complex64 with real and imaginary parts rounded to bfloat16."""
+    z = np.ascontiguousarray(z, dtype=np.complex64)
+    return (round_bf16(z.real) + 1j * round_bf16(z.imag)).astype(np.complex64)

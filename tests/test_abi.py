@@ -20,7 +20,7 @@ def test_header_declares_the_operator_surface():
     names = declared_functions()
     for required in ("pe_coef_apply", "pe_gram_mmse", "pe_llr_demap_f16", "pe_llr_demap", "pe_coef_apply_demap",
                      "pe_coef_apply_demap_f16", "pe_srs_filter", "pe_coef_to_c16", "pe_coef_apply_demap_c16",
-                     "pe_bfp_decompress", "pe_gram_mmse_c16", "pe_mmse_apply_demap",
+                     "pe_bfp_decompress", "pe_gram_mmse_c16", "pe_mmse_apply_demap", "pe_freq_interp",
                      "pe_gram", "pe_coef_apply_v1"):
         assert required in names
 
@@ -123,6 +123,17 @@ def test_other_entry_points_validate(pe):
     assert ma(a16, a16, p, a16, a16, a16, a16, 4, 8, 11, 64, 4, 1, 1, 0.0, None) == 1      # unaligned Y
     assert h.pe_srs_filter(a16, a16, a16, a16, 64, 2, 10, 4, 8, 1, None) == 2  # MFMA needs K % 4 == 0
     assert h.pe_srs_filter(a16, p, a16, a16, 64, 2, 12, 4, 8, 1, None) == 1    # MFMA needs aligned SRS
+    # frequency interpolation: (W, X, H_f, nBatch, nOut, nIn, nCol, prec, stream)
+    fi = h.pe_freq_interp
+    assert fi(n, n, n, 0, 64, 96, 4, 0, None) == 0                   # empty batch
+    assert fi(a16, a16, a16, 4, 0, 96, 4, 0, None) == 1              # nOut
+    assert fi(a16, a16, a16, 4, 64, 129, 4, 2, None) == 1            # nIn > 128
+    assert fi(a16, a16, a16, 4, 64, 96, 17, 2, None) == 1            # nCol > 16
+    assert fi(a16, a16, a16, 4, 64, 96, 4, 1, None) == 2             # f16: not offered
+    assert fi(a16, a16, a16, 4, 256, 128, 4, 0, None) == 2           # FP32: W does not fit LDS
+    assert fi(a16, a16, a16, 4, 64, 96, 4, 9, None) == 1             # unknown precision
+    assert fi(a16, n, a16, 4, 64, 96, 4, 2, None) == 1               # null X
+    assert fi(a16, p, a16, 4, 64, 96, 4, 2, None) == 1               # unaligned X
 
 
 def test_slot_plan_rejects_bad_cells_without_a_device(pe):

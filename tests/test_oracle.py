@@ -441,3 +441,64 @@ def test_bfp_round_trip(po, w):
     for idx in zip(*np.nonzero(e)):
         m = xb[idx] >> (int(e[idx]) - 1)
         assert m.max() > hi or m.min() < lo
+
+
+# ------------------------------------------------------------------------------------------------------
+# DMRS frequency interpolation (README.md:200 names the GEMM; no reference code)
+# ------------------------------------------------------------------------------------------------------
+def _fi_case(rng, nB, M, K, NC):
+    c = lambda *s: (rng.standard_normal(s) + 1j * rng.standard_normal(s)).astype(np.complex64)
+    return c(M, K), c(nB, K, NC)
+
+
+@pytest.mark.parametrize("nB,M,K,NC", [(3, 64, 96, 4), (2, 5, 7, 3), (1, 1, 1, 1), (4, 17, 128, 16)])
+def test_freq_interp_f64_matches_numpy_matmul(po, nB, M, K, NC):
+    """The FP64 oracle against numpy's complex matrix product (a library routine, independent of the loop)."""
+    rng = np.random.default_rng(M * 1000 + K)
+    W, X = _fi_case(rng, nB, M, K, NC)
+    got = po.freq_interp_f64(W, X)
+    ref = np.matmul(W.astype(np.complex128)[None], X.astype(np.complex128))
+    assert np.allclose(got, ref, rtol=0, atol=1e-12 * K)
+
+
+@pytest.mark.parametrize("nB,M,K,NC", [(3, 64, 96, 4), (2, 5, 7, 3), (4, 17, 128, 16)])
+def test_freq_interp_f32_chain_within_fp32_bound(po, nB, M, K, NC):
+    """The FP32 chain is within the forward error bound of 2K fused steps of the exact product, element by
+    element; a dropped, conjugated or transposed term would be O(|W| |X|) off."""
+    rng = np.random.default_rng(K)
+    W, X = _fi_case(rng, nB, M, K, NC)
+    got = po.freq_interp_f32(W, X).astype(np.complex128)
+    ref = po.freq_interp_f64(W, X)
+    mag = np.matmul(np.abs(W.astype(np.complex128))[None], np.abs(X.astype(np.complex128)))
+    bound = 2 * K * 2.0 ** -24 * mag * 2
+    assert np.all(np.abs(got.real - ref.real) <= bound) and np.all(np.abs(got.imag - ref.imag) <= bound)
+    assert np.abs(got - ref).max() > 0  # FP32, not a copy of the FP64 sum
+
+
+def test_freq_interp_selection_filters_are_exact(po):
+    """W = unit-phase selection (W[m][k] = u_m where k = s(m), 0 elsewhere, u_m in {1, i, -1, -i}) picks rows of
+    X rotated exactly: catches a wrong sign, a swapped real/imaginary part or a transposed operand."""
+    rng = np.random.default_rng(5)
+    M, K, NC, nB = 40, 48, 4, 3
+    s = rng.integers(0, K, M)
+    u = np.array([1, 1j, -1, -1j])[rng.integers(0, 4, M)]
+    W = np.zeros((M, K), dtype=np.complex64)
+    W[np.arange(M), s] = u
+    X = _fi_case(rng, nB, M, K, NC)[1]
+    got = po.freq_interp_f32(W, X)
+    assert np.array_equal(got, (u[None, :, None] * X[:, s, :]).astype(np.complex64))
+    # the omp variant computes the same chain
+    omp, used = po.freq_interp_f32_omp(W, X, nthreads=2)
+    assert used >= 1 and np.array_equal(omp, got)
+
+
+def test_round_bf16_matches_torch():
+    """The test helper's bfloat16 rounding against torch's (round to nearest even), ties included."""
+    import torch
+    from pe_oracle import round_bf16
+    rng = np.random.default_rng(9)
+    v = rng.standard_normal(4096).astype(np.float32)
+    ties = (np.arange(64, dtype=np.uint32) << 16 | 0x8000).view(np.float32)  # exactly half-way
+    v = np.concatenate([v, ties, -ties, np.float32([0.0, -0.0, 1e-40, 3e38])])
+    ref = torch.from_numpy(v).to(torch.bfloat16).to(torch.float32).numpy()
+    assert np.array_equal(round_bf16(v).view(np.uint32), ref.view(np.uint32))
