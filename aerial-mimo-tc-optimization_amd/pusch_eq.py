@@ -348,6 +348,8 @@ def freq_interp(W, X, prec="f32", out=None, stream=None):
     nBatch, nIn2, nCol = X.shape
     if nIn2 != nIn:
         raise ValueError("W %s and X %s disagree on nIn" % (tuple(W.shape), tuple(X.shape)))
+    if W.dtype != torch.complex64 or X.dtype != torch.complex64 or not (W.is_contiguous() and X.is_contiguous()):
+        raise TypeError("W and X must be contiguous complex64 tensors")
     if out is None:
         out = torch.empty((nBatch, nOut, nCol), dtype=torch.complex64, device=X.device)
     _check(lib().pe_freq_interp(_ptr(W), _ptr(X), _ptr(out), nBatch, nOut, nIn, nCol, _prec(prec), _stream(stream)))
