@@ -134,3 +134,27 @@ def test_output_untouched_outside_batch(pe):
         pe.freq_interp(W, X, prec=prec, out=buf[:nB])
         torch.cuda.synchronize()
         assert bool((buf[nB:] == complex(7.0, -7.0)).all())
+
+
+@pytest.mark.parametrize("prec", ["f32", "bf16", "bf16_split"])
+def test_unaligned_operands(pe, po, prec):
+    """W, X and H_f 8-byte but not 16-byte aligned (views one complex value into larger buffers): at 4 columns
+    the MFMA kernel then stores element by element; results equal those of aligned operands."""
+    import torch
+    rng = np.random.default_rng(31)
+    nB, M, K, NC = 45, 64, 96, 4
+    W, X = _case(rng, nB, M, K, NC)
+    Wb = torch.zeros(M * K + 1, dtype=torch.complex64, device="cuda")
+    Xb = torch.zeros(nB * K * NC + 1, dtype=torch.complex64, device="cuda")
+    Hb = torch.full((nB * M * NC + 2,), complex(5.0, 5.0), dtype=torch.complex64, device="cuda")
+    Wu = Wb[1:].view(M, K)
+    Xu = Xb[1:].view(nB, K, NC)
+    Hu = Hb[1:1 + nB * M * NC].view(nB, M, NC)
+    assert Wu.data_ptr() % 16 == 8 and Xu.data_ptr() % 16 == 8 and Hu.data_ptr() % 16 == 8
+    Wu.copy_(torch.from_numpy(W))
+    Xu.copy_(torch.from_numpy(X))
+    pe.freq_interp(Wu, Xu, prec=prec, out=Hu)
+    ref = _run(pe, W, X, prec)
+    torch.cuda.synchronize()
+    assert np.array_equal(Hu.cpu().numpy().view(np.uint32), ref.view(np.uint32))
+    assert complex(Hb[0]) == complex(5.0, 5.0) and complex(Hb[-1]) == complex(5.0, 5.0)
