@@ -3,12 +3,17 @@ independent numpy/scipy routines, and against brute force on small inputs. No GP
 import numpy as np
 import pytest
 
-# Published outputs of the reference (A100 runs of the reference programs):
-#   results/coef_rigor_results.txt:5  EVM(V2 vs V1) = 0.0307% (-70.3 dB) on srand(7), U(+-0.7), 1584x8x11x64
-#   README.md:152-154                 cos 0.99999997, max abs 0.001289 on srand(42), U(+-0.5), same shape
-RIGOR_EVM_DB = -70.3
-README_COS = 0.99999997
-README_MAXABS = 0.001289
+# Published outputs of the reference (A100 runs of the reference programs), kept as a fixture written by
+# tests/golden/make_reference_published.py from results/coef_rigor_results.txt:5 (EVM(V2 vs V1) on srand(7),
+# U(+-0.7), 1584x8x11x64) and README.md:153-154 (cosine and max abs error on srand(42), U(+-0.5), same shape).
+import json
+import os
+
+PUBLISHED = json.load(open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden",
+                                        "reference_published.json")))
+RIGOR_EVM_DB = PUBLISHED["coef_rigor_evm_db"]
+README_COS = PUBLISHED["readme_cosine"]
+README_MAXABS = PUBLISHED["readme_max_abs"]
 NSC, L, S, A = 1584, 8, 11, 64
 
 
@@ -502,3 +507,9 @@ def test_round_bf16_matches_torch():
     v = np.concatenate([v, ties, -ties, np.float32([0.0, -0.0, 1e-40, 3e38])])
     ref = torch.from_numpy(v).to(torch.bfloat16).to(torch.float32).numpy()
     assert np.array_equal(round_bf16(v).view(np.uint32), ref.view(np.uint32))
+
+
+def test_published_fixture_is_consistent():
+    """The fixture's EVM in percent and in dB describe the same ratio (20 log10), and its shape is the headline's."""
+    assert abs(20 * np.log10(PUBLISHED["coef_rigor_evm_pct"] / 100) - RIGOR_EVM_DB) < 0.05
+    assert PUBLISHED["coef_rigor_shape"] == [NSC, L, S, A]
