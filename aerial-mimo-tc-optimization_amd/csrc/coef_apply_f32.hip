@@ -160,6 +160,7 @@ __global__ __launch_bounds__(kF32Threads) void coef_apply_f32_kernel(
 // lane reads its M row and H column from there.
 constexpr int kGramKC = 32;
 constexpr int kGramLds = 20 * 1024;
+constexpr int kGramBatch = 8;
 
 __global__ __launch_bounds__(256) void gram_f32_kernel(const float2 *__restrict__ M, const float2 *__restrict__ H,
                                                        float2 *__restrict__ G, int nSC, int L, int A, int spw) {
@@ -178,15 +179,34 @@ __global__ __launch_bounds__(256) void gram_f32_kernel(const float2 *__restrict_
     float2 acc = make_float2(0.f, 0.f);
     for (int k0 = 0; k0 < A; k0 += kGramKC) {
         const int kc = min(kGramKC, A - k0);
-        // M[sc][row][k0 .. k0 + kc): row sr = s * L + row, column k (kGramKC is a power of two: shifts, no division)
-        for (int i = threadIdx.x; i < nv * L * kGramKC; i += blockDim.x) {
-            const int sr = i / kGramKC, k = i % kGramKC;
-            if (k < kc) lm[sr * MS + k] = M[((size_t)sc0 * L + sr) * A + k0 + k];
+        // Staging, kGramBatch loads per thread in flight before their LDS writes (a load-then-write loop would
+        // wait for each load in turn). Element i < nM: M[sc][row][k0 + k], row sr = s L + row, column k (kGramKC
+        // is a power of two); then H[sc][k0 .. k0 + kc)[col], kc L contiguous values per subcarrier. Addresses are
+        // clamped into the operands so every load is issued unconditionally; only the LDS writes are selective.
+        const int nM = nv * L * kGramKC, hs = kc * L, nT = nM + nv * hs;
+        for (int b0 = 0; b0 < nT; b0 += kGramBatch * (int)blockDim.x) {
+            float2 v[kGramBatch];
+            int dst[kGramBatch];
+#pragma unroll
+            for (int b = 0; b < kGramBatch; ++b) {
+                const int i = b0 + b * (int)blockDim.x + (int)threadIdx.x;
+                const float2 *src = M + (size_t)sc0 * L * A;
+                dst[b] = -1;
+                if (i < nM) {
+                    const int sr = i / kGramKC, k = i % kGramKC;
+                    src = M + ((size_t)sc0 * L + sr) * A + k0 + min(k, kc - 1);
+                    if (k < kc) dst[b] = sr * MS + k;
+                } else if (i < nT) {
+                    const int j = i - nM, ss = j / hs, e = j - ss * hs;
+                    src = H + ((size_t)(sc0 + ss) * A + k0) * L + e;
+                    dst[b] = (int)(lh - g_lds) + ss * kGramKC * L + e;
+                }
+                v[b] = *src;
+            }
+#pragma unroll
+            for (int b = 0; b < kGramBatch; ++b)
+                if (dst[b] >= 0) g_lds[dst[b]] = v[b];
         }
-        // H[sc][k0 .. k0 + kc)[col]: kc L contiguous values per subcarrier
-        for (int ss = 0; ss < nv; ++ss)
-            for (int i = threadIdx.x; i < kc * L; i += blockDim.x)
-                lh[ss * kGramKC * L + i] = H[((size_t)(sc0 + ss) * A + k0) * L + i];
         __syncthreads();
         if (mine) {
             const float2 *mr = lm + (s * L + r) * MS;
