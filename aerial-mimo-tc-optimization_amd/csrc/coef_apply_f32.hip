@@ -79,15 +79,24 @@ __global__ __launch_bounds__(kF32Threads) void coef_apply_f32_kernel(
 
     for (int k0 = 0; k0 < A; k0 += KC) {
         const int kn = min(KC, A - k0);  // multiple of 4 (A % 4 == 0)
+        // all of the lane's slot loads are issued before the first LDS write (a slot whose load sat under its
+        // own branch would wait for every earlier load in turn); unused slots read the workgroup's first value
+        float4 sv[NQ];
 #pragma unroll
         for (int i = 0; i < NQ; ++i) {
             const int q = tid + i * kF32Threads;
             const int p2 = 2 * (q & (F4_PER_ROW - 1));
             const unsigned o = soff[i];
             const float2 *src = (o & 0x80000000u) ? Yw + (o & 0x7fffffffu) : Cw + o;
-            if (q < nf4)
-                *reinterpret_cast<float4 *>(lds + (q >> LG_F4) * RS + p2) =
-                    p2 < kn ? *reinterpret_cast<const float4 *>(src + k0 + p2) : make_float4(0.f, 0.f, 0.f, 0.f);
+            const bool use = q < nf4 && p2 < kn;
+            sv[i] = *reinterpret_cast<const float4 *>(use ? src + k0 + p2 : Cw);
+            if (!use) sv[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+        }
+#pragma unroll
+        for (int i = 0; i < NQ; ++i) {
+            const int q = tid + i * kF32Threads;
+            const int p2 = 2 * (q & (F4_PER_ROW - 1));
+            if (q < nf4) *reinterpret_cast<float4 *>(lds + (q >> LG_F4) * RS + p2) = sv[i];
         }
         __syncthreads();
         if (active) {
