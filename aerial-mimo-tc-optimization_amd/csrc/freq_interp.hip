@@ -56,7 +56,15 @@ __global__ __launch_bounds__(256) void fi_f32_kernel(const float2 *__restrict__ 
         const int ni = (int)min((long)IPW, (long)nBatch - b0);
         __syncthreads();  // W staged; every thread is done with the previous items' x
         const float2 *xs = X + (size_t)b0 * KN;
-        for (int i = tid; i < ni * KN; i += 256) x[i] = xs[i];
+        const int nx = ni * KN;
+        for (int i0 = 0; i0 < nx; i0 += 8 * 256) {  // 8 loads in flight per thread before their LDS writes
+            float2 v[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) v[u] = xs[min(i0 + u * 256 + tid, nx - 1)];
+#pragma unroll
+            for (int u = 0; u < 8; ++u)
+                if (i0 + u * 256 + tid < nx) x[i0 + u * 256 + tid] = v[u];
+        }
         __syncthreads();
         if (slot < ni) {
             const float2 *wr = w + m * WS;
