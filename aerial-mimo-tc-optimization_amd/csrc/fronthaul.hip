@@ -33,10 +33,20 @@ __global__ __launch_bounds__(256) void bfp_decompress_kernel(const uint8_t *__re
     const int span = np * BPB;
     const uint8_t *base = in + ((size_t)sym * nAnt + ant0) * row_stride + (size_t)prb0 * BPB;
     if constexpr (WORDS) {  // rows and spans are whole words
-        const int nw = span >> 2;
-        for (int i = threadIdx.x; i < na * nw; i += 256) {
-            const int a = i / nw, k = i - a * nw;
-            rows[a * PITCH + k] = reinterpret_cast<const uint32_t *>(base + (size_t)a * row_stride)[k];
+        const int nw = span >> 2, tot = na * nw;
+        for (int i0 = 0; i0 < tot; i0 += 8 * 256) {  // 8 loads in flight per thread before their LDS writes
+            uint32_t v[8];
+            int dst[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                const int i = i0 + u * 256 + (int)threadIdx.x, ic = min(i, tot - 1);
+                const int a = ic / nw, k = ic - a * nw;
+                v[u] = reinterpret_cast<const uint32_t *>(base + (size_t)a * row_stride)[k];
+                dst[u] = i < tot ? a * PITCH + k : -1;
+            }
+#pragma unroll
+            for (int u = 0; u < 8; ++u)
+                if (dst[u] >= 0) rows[dst[u]] = v[u];
         }
     } else {
         uint8_t *rb = reinterpret_cast<uint8_t *>(rows);
