@@ -621,11 +621,25 @@ pe_status check_cell_desc(const pe_cell_desc &c, int i, std::string &err) {
         err = b;
         return PE_ERR_INVALID;
     }
-    if (c.prec == PE_PREC_F16_C16 && ((reinterpret_cast<uintptr_t>(c.C) | reinterpret_cast<uintptr_t>(c.Y_rx)) & 15u)) {
-        snprintf(b, sizeof(b), "cell %d: C and Y_rx must be 16-byte aligned for binary16 coefficients", i);
+    if (c.prec == PE_PREC_F16_C16 && !is_aligned(c.C, 16)) {
+        snprintf(b, sizeof(b), "cell %d: C must be 16-byte aligned for binary16 coefficients", i);
         err = b;
         return PE_ERR_INVALID;
     }
+    // the widest accesses of the tile kernels: 16-byte loads of Y_rx, C and H rows (even layer counts),
+    // LLR stores of 2 nPamBits floats
+    const struct {
+        const void *p;
+        size_t a;
+        const char *name;
+    } req[] = {{c.Y_rx, 16, "Y_rx"}, {c.C, 16, "C"}, {c.H, h_align(c.nLayer), "H"}, {c.sigma2, 4, "sigma2"},
+               {c.noise_inv, 4, "noise_inv"}, {c.llr, llr_align(c.nPamBits, false), "llr"}};
+    for (const auto &r : req)
+        if (r.p && !is_aligned(r.p, r.a)) {
+            snprintf(b, sizeof(b), "cell %d: %s must be %zu-byte aligned", i, r.name, r.a);
+            err = b;
+            return PE_ERR_INVALID;
+        }
     return PE_OK;
 }
 

@@ -10,6 +10,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <initializer_list>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -122,6 +123,20 @@ void load_default_table() {
 }
 
 bool check_ptrs(const void *a, const void *b, const void *c) { return a && b && c; }
+
+// Alignment contract of an entry point: each buffer aligned to the widest access the kernels make to it
+// (pe::is_aligned, pe::llr_align, pe::h_align). Misaligned pointers are rejected before anything is launched.
+struct AlignReq {
+    const void *p;
+    size_t bytes;
+    const char *name;
+};
+pe_status check_align(std::initializer_list<AlignReq> reqs) {
+    for (const AlignReq &r : reqs)
+        if (r.p && !pe::is_aligned(r.p, r.bytes))
+            return fail(PE_ERR_INVALID, "%s must be %zu-byte aligned", r.name, r.bytes);
+    return PE_OK;
+}
 
 }  // namespace
 
@@ -253,6 +268,8 @@ pe_status pe_coef_apply(const pe_c64 *C, const pe_c64 *Y_rx, pe_c64 *Y_eq, int n
     if (s != PE_OK) return s;
     if (nSC == 0) return ok();
     if (!check_ptrs(C, Y_rx, Y_eq)) return fail(PE_ERR_INVALID, "null pointer");
+    s = check_align({{C, 16, "C"}, {Y_rx, 16, "Y_rx"}, {Y_eq, 8, "Y_eq"}});
+    if (s != PE_OK) return s;
     hipStream_t st = (hipStream_t)stream;
     const float2 *c = reinterpret_cast<const float2 *>(C);
     const float2 *y = reinterpret_cast<const float2 *>(Y_rx);
@@ -286,6 +303,8 @@ pe_status pe_gram(const pe_c64 *M, const pe_c64 *H, pe_c64 *G, int nSC, int nLay
     if (nAnt < 1 || nAnt > 1024) return fail(PE_ERR_INVALID, "nAnt = %d outside [1, 1024]", nAnt);
     if (nSC == 0) return ok();
     if (!check_ptrs(M, H, G)) return fail(PE_ERR_INVALID, "null pointer");
+    pe_status s = check_align({{M, 8, "M"}, {H, 8, "H"}, {G, 8, "G"}});
+    if (s != PE_OK) return s;
     hipError_t e = pe::launch_gram_f32(reinterpret_cast<const float2 *>(M), reinterpret_cast<const float2 *>(H),
                                        reinterpret_cast<float2 *>(G), nSC, nLayer, nAnt, (hipStream_t)stream);
     return from_hip(e, "gram launch");
@@ -299,6 +318,9 @@ pe_status pe_gram_mmse(const pe_c64 *H, const float *sigma2, pe_c64 *C, float *n
     if (flags & ~PE_MMSE_UNBIASED) return fail(PE_ERR_INVALID, "unknown flags 0x%x", flags);
     if (nSC == 0) return ok();
     if (!check_ptrs(H, sigma2, C) || !noise_inv || !sc_flags) return fail(PE_ERR_INVALID, "null pointer");
+    pe_status s = check_align({{H, pe::h_align(nLayer), "H"}, {sigma2, 4, "sigma2"}, {C, 8, "C"},
+                               {noise_inv, 4, "noise_inv"}});
+    if (s != PE_OK) return s;
     const float thr = cond_threshold > 0.f ? cond_threshold : 1e4f;
     hipError_t e = pe::launch_gram_mmse(reinterpret_cast<const float2 *>(H), sigma2, reinterpret_cast<float2 *>(C),
                                         noise_inv, sc_flags, nSC, nLayer, nAnt, (flags & PE_MMSE_UNBIASED) != 0,
@@ -316,6 +338,9 @@ pe_status pe_gram_mmse_c16(const pe_c64 *H, const float *sigma2, pe_c16 *C16, in
     if (flags & ~PE_MMSE_UNBIASED) return fail(PE_ERR_INVALID, "unknown flags 0x%x", flags);
     if (nSC == 0) return ok();
     if (!check_ptrs(H, sigma2, C16) || !c_exp || !noise_inv || !sc_flags) return fail(PE_ERR_INVALID, "null pointer");
+    pe_status s = check_align({{H, pe::h_align(nLayer), "H"}, {sigma2, 4, "sigma2"}, {C16, 4, "C16"}, {c_exp, 4, "c_exp"},
+                               {noise_inv, 4, "noise_inv"}});
+    if (s != PE_OK) return s;
     const float thr = cond_threshold > 0.f ? cond_threshold : 1e4f;
     hipError_t e = pe::launch_gram_mmse_c16(reinterpret_cast<const float2 *>(H), sigma2,
                                             reinterpret_cast<uint32_t *>(C16), c_exp, noise_inv, sc_flags, nSC, nLayer,
@@ -329,6 +354,9 @@ pe_status pe_llr_demap_f16(const pe_c16 *softEst, const float *noiseInv, uint16_
     if (nPamBits < 1 || nPamBits > 4) return fail(PE_ERR_INVALID, "nPamBits = %d outside [1, 4]", nPamBits);
     if (N == 0) return ok();
     if (!check_ptrs(softEst, noiseInv, llrOut)) return fail(PE_ERR_INVALID, "null pointer");
+    pe_status s = check_align({{softEst, 4, "softEst"}, {noiseInv, 4, "noiseInv"},
+                               {llrOut, pe::llr_align(nPamBits, true), "llrOut"}});
+    if (s != PE_OK) return s;
     hipError_t e = pe::launch_llr_f16(reinterpret_cast<const uint32_t *>(softEst), noiseInv, llrOut, N, nPamBits,
                                       (hipStream_t)stream);
     return from_hip(e, "llr_demap_f16 launch");
@@ -342,6 +370,8 @@ pe_status pe_llr_demap(const pe_c64 *Y_eq, const float *noise_inv, float *llr, i
     if (nPamBits < 1 || nPamBits > 4) return fail(PE_ERR_INVALID, "nPamBits = %d outside [1, 4]", nPamBits);
     if (nSC == 0) return ok();
     if (!check_ptrs(Y_eq, noise_inv, llr)) return fail(PE_ERR_INVALID, "null pointer");
+    pe_status s = check_align({{Y_eq, 8, "Y_eq"}, {noise_inv, 4, "noise_inv"}, {llr, pe::llr_align(nPamBits, false), "llr"}});
+    if (s != PE_OK) return s;
     hipError_t e = pe::launch_llr_f32(reinterpret_cast<const float2 *>(Y_eq), noise_inv, llr, nSC, nLayer, nSym,
                                       nPamBits, (hipStream_t)stream);
     return from_hip(e, "llr_demap launch");
@@ -354,6 +384,9 @@ pe_status pe_coef_apply_demap(const pe_c64 *C, const pe_c64 *Y_rx, const float *
     if (nPamBits < 1 || nPamBits > 4) return fail(PE_ERR_INVALID, "nPamBits = %d outside [1, 4]", nPamBits);
     if (nSC == 0) return ok();
     if (!check_ptrs(C, Y_rx, noise_inv) || !llr) return fail(PE_ERR_INVALID, "null pointer");
+    s = check_align({{C, 16, "C"}, {Y_rx, 16, "Y_rx"}, {noise_inv, 4, "noise_inv"},
+                     {llr, pe::llr_align(nPamBits, false), "llr"}});
+    if (s != PE_OK) return s;
     hipStream_t st = (hipStream_t)stream;
     const float2 *c = reinterpret_cast<const float2 *>(C);
     const float2 *y = reinterpret_cast<const float2 *>(Y_rx);
@@ -372,6 +405,9 @@ pe_status pe_coef_apply_demap_f16(const pe_c64 *C, const pe_c64 *Y_rx, const flo
     if (nPamBits < 1 || nPamBits > 4) return fail(PE_ERR_INVALID, "nPamBits = %d outside [1, 4]", nPamBits);
     if (nSC == 0) return ok();
     if (!check_ptrs(C, Y_rx, noise_inv) || !llr) return fail(PE_ERR_INVALID, "null pointer");
+    s = check_align({{C, 16, "C"}, {Y_rx, 16, "Y_rx"}, {noise_inv, 4, "noise_inv"},
+                     {llr, pe::llr_align(nPamBits, true), "llr"}});
+    if (s != PE_OK) return s;
     hipStream_t st = (hipStream_t)stream;
     const float2 *c = reinterpret_cast<const float2 *>(C);
     const float2 *y = reinterpret_cast<const float2 *>(Y_rx);
@@ -406,6 +442,9 @@ static pe_status coef_apply_c16(const pe_c16 *C16, const int *c_exp, const pe_c6
     if (nSC == 0) return ok();
     if (!check_ptrs(C16, c_exp, Y_rx) || (llr ? !noise_inv : !Y_eq)) return fail(PE_ERR_INVALID, "null pointer");
     if (!aligned16(C16) || !aligned16(Y_rx)) return fail(PE_ERR_INVALID, "C16 and Y_rx must be 16-byte aligned");
+    s = check_align({{c_exp, 4, "c_exp"}, {Y_eq, 8, "Y_eq"}, {noise_inv, 4, "noise_inv"},
+                     {llr, pe::llr_align(nPamBits, llr_f16 != 0), "llr"}});
+    if (s != PE_OK) return s;
     hipError_t e = pe::launch_coef_apply_c16(reinterpret_cast<const uint32_t *>(C16), c_exp,
                                              reinterpret_cast<const float2 *>(Y_rx), reinterpret_cast<float2 *>(Y_eq),
                                              noise_inv, llr, nSC, nLayer, nSym, nAnt, llr ? nPamBits : 0, llr_f16,
@@ -427,21 +466,32 @@ pe_status pe_coef_apply_demap_c16(const pe_c16 *C16, const int *c_exp, const pe_
                           nSym, nAnt, nPamBits, stream);
 }
 
-pe_status pe_mmse_apply_demap(const pe_c64 *H, const float *sigma2, const pe_c64 *Y_rx, pe_c64 *C, float *noise_inv,
-                              uint8_t *sc_flags, void *llr, int nSC, int nLayer, int nSym, int nAnt, int nPamBits,
-                              pe_prec prec, unsigned flags, float cond_threshold, void *stream) {
+// Arguments of the one-pass chain other than its workspace C (shared by pe_mmse_apply_demap and the variant
+// without allocation).
+static pe_status check_mmse_apply(const pe_c64 *H, const float *sigma2, const pe_c64 *Y_rx, const float *noise_inv,
+                                  const uint8_t *sc_flags, const void *llr, int nSC, int nLayer, int nSym, int nAnt,
+                                  int nPamBits, pe_prec prec, unsigned flags, unsigned allowed_flags) {
     pe_status s = check_coef_shape(nSC, nLayer, nSym, nAnt, (int)prec);
     if (s != PE_OK) return s;
     if (prec == PE_PREC_F32)
         return fail(PE_ERR_UNSUPPORTED, "precision %d: the fused MMSE + apply runs the MFMA product", (int)prec);
     if (nPamBits < 1 || nPamBits > 4) return fail(PE_ERR_INVALID, "nPamBits = %d outside [1, 4]", nPamBits);
-    if (flags & ~(PE_MMSE_UNBIASED | PE_EQ_WRITE_C | PE_EQ_LLR_F16))
-        return fail(PE_ERR_INVALID, "unknown flag bits 0x%x", flags);
-    if (nSC == 0) return ok();
-    if (!check_ptrs(H, sigma2, Y_rx) || !C || !noise_inv || !sc_flags || !llr)
-        return fail(PE_ERR_INVALID, "null pointer");
+    if (flags & ~allowed_flags) return fail(PE_ERR_INVALID, "unknown flag bits 0x%x", flags & ~allowed_flags);
+    if (nSC == 0) return PE_OK;
+    if (!check_ptrs(H, sigma2, Y_rx) || !noise_inv || !sc_flags || !llr) return fail(PE_ERR_INVALID, "null pointer");
     if (!aligned16(H) || !aligned16(Y_rx) || !aligned16(llr))
         return fail(PE_ERR_INVALID, "H, Y_rx and llr must be 16-byte aligned");
+    return check_align({{sigma2, 4, "sigma2"}, {noise_inv, 4, "noise_inv"}});
+}
+
+pe_status pe_mmse_apply_demap(const pe_c64 *H, const float *sigma2, const pe_c64 *Y_rx, pe_c64 *C, float *noise_inv,
+                              uint8_t *sc_flags, void *llr, int nSC, int nLayer, int nSym, int nAnt, int nPamBits,
+                              pe_prec prec, unsigned flags, float cond_threshold, void *stream) {
+    pe_status s = check_mmse_apply(H, sigma2, Y_rx, noise_inv, sc_flags, llr, nSC, nLayer, nSym, nAnt, nPamBits, prec,
+                                   flags, PE_MMSE_UNBIASED | PE_EQ_WRITE_C | PE_EQ_LLR_F16);
+    if (s != PE_OK) return s;
+    if (nSC == 0) return ok();
+    if (!C) return fail(PE_ERR_INVALID, "null pointer");
     if (reinterpret_cast<uintptr_t>(C) & 7u) return fail(PE_ERR_INVALID, "C must be 8-byte aligned");
     const float thr = cond_threshold > 0.f ? cond_threshold : 1e4f;
     hipError_t e = pe::launch_mmse_apply_demap(

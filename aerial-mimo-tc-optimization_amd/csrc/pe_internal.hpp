@@ -36,6 +36,15 @@ struct SlotTile {
 };
 SlotTile slot_tile();
 
+// Alignment the kernels' widest accesses need (bytes). C, Y_rx, C16 and H with an even layer count are read
+// 16 bytes at a time; an LLR buffer is written 2 nPamBits values per symbol in the widest stores that fit.
+inline bool is_aligned(const void *p, size_t a) { return (reinterpret_cast<uintptr_t>(p) & (a - 1)) == 0; }
+inline size_t llr_align(int nPamBits, bool f16) {
+    if (f16) return nPamBits == 4 ? 16 : nPamBits == 2 ? 8 : 4;
+    return (nPamBits == 2 || nPamBits == 4) ? 16 : 8;
+}
+inline size_t h_align(int nLayer) { return nLayer % 2 == 0 ? 16 : 8; }
+
 // Compute units of the current device (cached per process).
 int device_cus();
 // Sets the thread-local message returned by pe_last_error().

@@ -174,9 +174,14 @@ extern "C" pe_status pe_serve_run(const pe_serve_config *cfg, pe_serve_report *r
     if (largest > cfg->hbm_budget_bytes)
         return serve_fail(PE_ERR_CAPACITY, "hbm_budget_bytes is smaller than one job of the largest cell (" +
                                                std::to_string(largest) + " bytes)");
+    const int nj = cfg->n_jobs;
+    for (int j = 0; cfg->job_cells && j < nj; ++j)
+        if (cfg->job_cells[j] < 0 || cfg->job_cells[j] >= cfg->n_cells)
+            return serve_fail(PE_ERR_INVALID, "job " + std::to_string(j) + ": cell index out of range");
+    if ((cfg->job_H || cfg->job_sigma2 || cfg->job_Y) && !cfg->job_cells)
+        return serve_fail(PE_ERR_INVALID, "per-job inputs need job_cells (their shapes follow the job's cell)");
 
     // ---- arrival process (virtual times from the start of the run) ------------------------------------
-    const int nj = cfg->n_jobs;
     std::vector<double> arrival(nj);
     std::vector<int> cellof(nj);
     {
@@ -187,8 +192,27 @@ extern "C" pe_status pe_serve_run(const pe_serve_config *cfg, pe_serve_report *r
         for (int j = 0; j < nj; ++j) {
             if (cfg->arrival_rate_hz > 0) t += ex(rng);
             arrival[j] = cfg->arrival_rate_hz > 0 ? t : 0.0;
-            cellof[j] = pick(rng);
+            const int drawn = pick(rng);  // drawn either way: the arrival times do not depend on job_cells
+            cellof[j] = cfg->job_cells ? cfg->job_cells[j] : drawn;
         }
+    }
+    // per-job inputs and destinations: present and aligned as the tile kernels need
+    for (int j = 0; j < nj; ++j) {
+        const pe_cell_desc &c = pop[cellof[j]];
+        const struct {
+            const void *p;
+            bool given;
+            size_t a;
+            const char *name;
+        } req[] = {{cfg->job_llr ? cfg->job_llr[j] : nullptr, cfg->job_llr != nullptr,
+                    pe::llr_align(c.nPamBits, false), "job_llr"},
+                   {cfg->job_H ? cfg->job_H[j] : nullptr, cfg->job_H != nullptr, pe::h_align(c.nLayer), "job_H"},
+                   {cfg->job_sigma2 ? cfg->job_sigma2[j] : nullptr, cfg->job_sigma2 != nullptr, 4, "job_sigma2"},
+                   {cfg->job_Y ? cfg->job_Y[j] : nullptr, cfg->job_Y != nullptr, 16, "job_Y"}};
+        for (const auto &r : req)
+            if (r.given && (!r.p || !pe::is_aligned(r.p, r.a)))
+                return serve_fail(PE_ERR_INVALID, "job " + std::to_string(j) + ": " + r.name + " is null or not " +
+                                                      std::to_string(r.a) + "-byte aligned");
     }
 
     // ---- device state ------------------------------------------------------------------------------------
@@ -223,15 +247,19 @@ extern "C" pe_status pe_serve_run(const pe_serve_config *cfg, pe_serve_report *r
     }
     Arena arena(cfg->hbm_budget_bytes);
     std::vector<JobBuf> buf(nj);
-    auto make_desc = [&](int j, const JobBuf &b) {
+    // job j's descriptor; its C / noise_inv (and LLRs when in_arena) in the arena block b
+    auto make_desc = [&](int j, const JobBuf &b, bool in_arena) {
         pe_cell_desc d = pop[cellof[j]];
+        if (cfg->job_H) d.H = cfg->job_H[j];
+        if (cfg->job_sigma2) d.sigma2 = cfg->job_sigma2[j];
+        if (cfg->job_Y) d.Y_rx = cfg->job_Y[j];
         const size_t nsc = (size_t)d.nSC;
         unsigned char *p = arena_dev + b.off;
         d.C = reinterpret_cast<pe_c64 *>(p);
         p += coef_bytes(d);
         d.noise_inv = reinterpret_cast<float *>(p);
         p += (nsc * d.nLayer * 4 + 255) & ~size_t(255);
-        d.llr = llr_in_arena ? reinterpret_cast<float *>(p) : cfg->job_llr[j];
+        d.llr = in_arena ? reinterpret_cast<float *>(p) : cfg->job_llr[j];
         return d;
     };
     // graph replay (capacity grids) for batches of at least a quarter of max_batch_jobs; smaller batches are
@@ -253,13 +281,14 @@ extern "C" pe_status pe_serve_run(const pe_serve_config *cfg, pe_serve_report *r
     // of one job (eager, idle GPU; median of 5) seeds the latency model.
     LatencyModel model;
     {
+        // the warm-up job's LLRs stay in the arena: no caller buffer is written for a job that may be shed
         JobBuf b;
-        if (!arena.alloc(job_bytes(pop[cellof[0]], llr_in_arena), b.off)) {
+        b.bytes = job_bytes(pop[cellof[0]], true);
+        if (!arena.alloc(b.bytes, b.off)) {
             cleanup();
             return serve_fail(PE_ERR_CAPACITY, "arena cannot hold one job");
         }
-        b.bytes = job_bytes(pop[cellof[0]], llr_in_arena);
-        pe_cell_desc d = make_desc(0, b);  // LLRs go to job 0's destination, rewritten by the measured run
+        pe_cell_desc d = make_desc(0, b, true);
         std::vector<pe_cell_desc> one(1, d);
         for (int k = 0; k < K; ++k) {
             pe_status s = submit(k, one, true);
@@ -367,7 +396,7 @@ extern "C" pe_status pe_serve_run(const pe_serve_config *cfg, pe_serve_report *r
                 if (!arena.alloc(b.bytes, b.off)) break;  // budget exhausted: wait for completions (FIFO order)
                 buf[j] = b;
                 ready.pop_front();
-                cells.push_back(make_desc(j, b));
+                cells.push_back(make_desc(j, b, llr_in_arena));
                 batch_jobs[k].push_back(j);
                 sc_batch += nsc_j;
             }

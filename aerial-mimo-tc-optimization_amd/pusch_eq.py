@@ -105,6 +105,46 @@ def _ptr(t):
     return ctypes.c_void_p(t.data_ptr()) if t is not None else ctypes.c_void_p(0)
 
 
+def _need(t, name, dtype, shape=None, min_numel=None):
+    """Checks a tensor handed to the library before any launch: its dtype (a torch dtype, or an element size in
+    bytes for the binary16-pair buffers), contiguity, and shape (None entries match any size) or minimum number of
+    elements. Every wrapper then checks with _same_device that all its tensors are on one GPU, so dtype and shape
+    mistakes are reported whatever the device. Raises TypeError / ValueError; returns t."""
+    import torch
+    if not isinstance(t, torch.Tensor):
+        raise TypeError("%s must be a torch tensor, got %s" % (name, type(t).__name__))
+    if isinstance(dtype, int):
+        if t.element_size() != dtype:
+            raise TypeError("%s: elements of %d bytes expected, got %s" % (name, dtype, t.dtype))
+    elif t.dtype != dtype:
+        raise TypeError("%s: dtype %s expected, got %s" % (name, dtype, t.dtype))
+    if not t.is_contiguous():
+        raise ValueError("%s must be contiguous" % name)
+    if shape is not None and (t.dim() != len(shape) or
+                              any(w is not None and int(w) != int(g) for w, g in zip(shape, t.shape))):
+        raise ValueError("%s: shape %s expected, got %s" % (name, tuple(shape), tuple(t.shape)))
+    if min_numel is not None and t.numel() < min_numel:
+        raise ValueError("%s: at least %d elements expected, got %d" % (name, min_numel, t.numel()))
+    return t
+
+
+def _same_device(*ts):
+    """All given tensors (None entries skipped) on one GPU."""
+    ds = {t.device for t in ts if t is not None}
+    if len(ds) > 1:
+        raise ValueError("tensors on different devices: %s" % sorted(str(d) for d in ds))
+    for d in ds:
+        if d.type != "cuda":
+            raise ValueError("tensors must be GPU tensors, got device %s" % d)
+
+
+def _q(n_pam_bits):
+    q = int(n_pam_bits)
+    if not 1 <= q <= 4:
+        raise ValueError("n_pam_bits = %d outside [1, 4]" % q)
+    return q
+
+
 def _stream(stream):
     import torch
     s = stream if stream is not None else torch.cuda.current_stream()
@@ -126,25 +166,39 @@ def version():
 def coef_apply(C, Y, out=None, prec="f32", stream=None):
     """Y_eq[sc, l, s] = sum_a C[sc, l, a] * Y[sc, s, a]."""
     import torch
-    nSC, L, A = C.shape
-    S = Y.shape[1]
-    assert C.dtype == torch.complex64 and Y.dtype == torch.complex64 and Y.shape == (nSC, S, A)
-    assert C.is_contiguous() and Y.is_contiguous()
+    c64 = torch.complex64
+    nSC, L, A = _need(C, "C", c64, (None, None, None)).shape
+    S = _need(Y, "Y", c64, (nSC, None, A)).shape[1]
     if out is None:
-        out = torch.empty((nSC, L, S), dtype=torch.complex64, device=C.device)
+        out = torch.empty((nSC, L, S), dtype=c64, device=C.device)
+    _need(out, "out", c64, (nSC, L, S))
+    _same_device(C, Y, out)
     _check(lib().pe_coef_apply(_ptr(C), _ptr(Y), _ptr(out), nSC, L, S, A, _prec(prec), _stream(stream)))
     return out
 
 
 def coef_apply_v1(C, Y, out, nSC, nSym, stream=None):
     """The reference's exact signature (8 layers, 64 antennas)."""
+    import torch
+    c64 = torch.complex64
+    _need(C, "C", c64, min_numel=nSC * 8 * 64)
+    _need(Y, "Y", c64, min_numel=nSC * nSym * 64)
+    _need(out, "out", c64, min_numel=nSC * 8 * nSym)
+    _same_device(C, Y, out)
     _check(lib().pe_coef_apply_v1(_ptr(C), _ptr(Y), _ptr(out), nSC, nSym, _stream(stream)))
     return out
 
 
 def coef_apply_range(C, Y, out, sc_offset, n_sc, prec="f32", stream=None):
-    _, L, A = C.shape
-    S = Y.shape[1]
+    """Subcarriers [sc_offset, sc_offset + n_sc) of full-size tensors C [N, L, A], Y [N, S, A], out [N, L, S]."""
+    import torch
+    c64 = torch.complex64
+    N, L, A = _need(C, "C", c64, (None, None, None)).shape
+    S = _need(Y, "Y", c64, (N, None, A)).shape[1]
+    _need(out, "out", c64, (N, L, S))
+    if sc_offset < 0 or n_sc < 0 or sc_offset + n_sc > N:
+        raise ValueError("range [%d, %d) outside the %d subcarriers" % (sc_offset, sc_offset + n_sc, N))
+    _same_device(C, Y, out)
     _check(lib().pe_coef_apply_range(_ptr(C), _ptr(Y), _ptr(out), n_sc, sc_offset, L, S, A, _prec(prec),
                                      _stream(stream)))
     return out
@@ -153,10 +207,13 @@ def coef_apply_range(C, Y, out, sc_offset, n_sc, prec="f32", stream=None):
 def gram(M, H, out=None, stream=None):
     """G[sc] = M[sc] @ H[sc] (plain product)."""
     import torch
-    nSC, L, A = M.shape
-    assert H.shape == (nSC, A, L)
+    c64 = torch.complex64
+    nSC, L, A = _need(M, "M", c64, (None, None, None)).shape
+    _need(H, "H", c64, (nSC, A, L))
     if out is None:
-        out = torch.empty((nSC, L, L), dtype=torch.complex64, device=M.device)
+        out = torch.empty((nSC, L, L), dtype=c64, device=M.device)
+    _need(out, "out", c64, (nSC, L, L))
+    _same_device(M, H, out)
     _check(lib().pe_gram(_ptr(M), _ptr(H), _ptr(out), nSC, L, A, _stream(stream)))
     return out
 
@@ -164,7 +221,8 @@ def gram(M, H, out=None, stream=None):
 def gram_mmse(H, sigma2, unbiased=True, cond_threshold=0.0, C=None, noise_inv=None, flags=None, stream=None):
     """C = (H^H H + sigma2 I)^-1 H^H per subcarrier; returns (C, noise_inv, flags)."""
     import torch
-    nSC, A, L = H.shape
+    nSC, A, L = _need(H, "H", torch.complex64, (None, None, None)).shape
+    _need(sigma2, "sigma2", torch.float32, (nSC,))
     dev = H.device
     if C is None:
         C = torch.empty((nSC, L, A), dtype=torch.complex64, device=dev)
@@ -172,36 +230,52 @@ def gram_mmse(H, sigma2, unbiased=True, cond_threshold=0.0, C=None, noise_inv=No
         noise_inv = torch.empty((nSC, L), dtype=torch.float32, device=dev)
     if flags is None:
         flags = torch.empty((nSC,), dtype=torch.uint8, device=dev)
+    _need(C, "C", torch.complex64, (nSC, L, A))
+    _need(noise_inv, "noise_inv", torch.float32, (nSC, L))
+    _need(flags, "flags", torch.uint8, (nSC,))
+    _same_device(H, sigma2, C, noise_inv, flags)
     _check(lib().pe_gram_mmse(_ptr(H), _ptr(sigma2), _ptr(C), _ptr(noise_inv), _ptr(flags), nSC, L, A,
                               MMSE_UNBIASED if unbiased else 0, float(cond_threshold), _stream(stream)))
     return C, noise_inv, flags
 
 
 def llr_demap_f16(soft_half, noise_inv, n_pam_bits, out=None, stream=None):
-    """soft_half: [N, 2] binary16 (re, im); returns [N, 2*q] binary16 LLRs."""
+    """soft_half: [N, 2] binary16 bits (re, im) in any 2-byte dtype; returns [N, 2*q] binary16 LLRs."""
     import torch
-    N = soft_half.shape[0]
+    q = _q(n_pam_bits)
+    N = _need(soft_half, "soft_half", 2, (None, 2)).shape[0]
+    _need(noise_inv, "noise_inv", torch.float32, (N,))
     if out is None:
-        out = torch.empty((N, 2 * n_pam_bits), dtype=torch.float16, device=soft_half.device)
+        out = torch.empty((N, 2 * q), dtype=torch.float16, device=soft_half.device)
+    _need(out, "out", 2, (N, 2 * q))
+    _same_device(soft_half, noise_inv, out)
     _check(lib().pe_llr_demap_f16(_ptr(soft_half), _ptr(noise_inv), _ptr(out), N, n_pam_bits, _stream(stream)))
     return out
 
 
 def llr_demap(Yeq, noise_inv, n_pam_bits, out=None, stream=None):
     import torch
-    nSC, L, S = Yeq.shape
+    q = _q(n_pam_bits)
+    nSC, L, S = _need(Yeq, "Yeq", torch.complex64, (None, None, None)).shape
+    _need(noise_inv, "noise_inv", torch.float32, (nSC, L))
     if out is None:
-        out = torch.empty((nSC, L, S, 2 * n_pam_bits), dtype=torch.float32, device=Yeq.device)
+        out = torch.empty((nSC, L, S, 2 * q), dtype=torch.float32, device=Yeq.device)
+    _need(out, "out", torch.float32, (nSC, L, S, 2 * q))
+    _same_device(Yeq, noise_inv, out)
     _check(lib().pe_llr_demap(_ptr(Yeq), _ptr(noise_inv), _ptr(out), nSC, L, S, n_pam_bits, _stream(stream)))
     return out
 
 
 def coef_apply_demap(C, Y, noise_inv, n_pam_bits, prec="f16", out=None, stream=None):
     import torch
-    nSC, L, A = C.shape
-    S = Y.shape[1]
+    q = _q(n_pam_bits)
+    nSC, L, A = _need(C, "C", torch.complex64, (None, None, None)).shape
+    S = _need(Y, "Y", torch.complex64, (nSC, None, A)).shape[1]
+    _need(noise_inv, "noise_inv", torch.float32, (nSC, L))
     if out is None:
-        out = torch.empty((nSC, L, S, 2 * n_pam_bits), dtype=torch.float32, device=C.device)
+        out = torch.empty((nSC, L, S, 2 * q), dtype=torch.float32, device=C.device)
+    _need(out, "out", torch.float32, (nSC, L, S, 2 * q))
+    _same_device(C, Y, noise_inv, out)
     _check(lib().pe_coef_apply_demap(_ptr(C), _ptr(Y), _ptr(noise_inv), _ptr(out), nSC, L, S, A, n_pam_bits,
                                      _prec(prec), _stream(stream)))
     return out
@@ -211,10 +285,14 @@ def coef_apply_demap_f16(C, Y, noise_inv, n_pam_bits, prec="f16", out=None, stre
     """Fused apply + demap writing binary16 LLRs (the decoder wire format); returns a torch.float16 tensor
     [nSC, nLayer, nSym, 2*n_pam_bits] (pe_coef_apply_demap_f16)."""
     import torch
-    nSC, L, A = C.shape
-    S = Y.shape[1]
+    q = _q(n_pam_bits)
+    nSC, L, A = _need(C, "C", torch.complex64, (None, None, None)).shape
+    S = _need(Y, "Y", torch.complex64, (nSC, None, A)).shape[1]
+    _need(noise_inv, "noise_inv", torch.float32, (nSC, L))
     if out is None:
-        out = torch.empty((nSC, L, S, 2 * n_pam_bits), dtype=torch.float16, device=C.device)
+        out = torch.empty((nSC, L, S, 2 * q), dtype=torch.float16, device=C.device)
+    _need(out, "out", 2, (nSC, L, S, 2 * q))
+    _same_device(C, Y, noise_inv, out)
     _check(lib().pe_coef_apply_demap_f16(_ptr(C), _ptr(Y), _ptr(noise_inv), _ptr(out), nSC, L, S, A,
                                          n_pam_bits, _prec(prec), _stream(stream)))
     return out
@@ -231,9 +309,12 @@ def mmse_apply_demap(H, sigma2, Y, n_pam_bits, prec="f16", unbiased=True, cond_t
     Y [nSC, nSym, nAnt] -> LLRs [nSC, nLayer, nSym, 2*n_pam_bits] (float32, or float16 with llr_f16). C is the
     workspace for subcarriers solved in FP64 (every row with write_c). Returns (llr, C, noise_inv, flags)."""
     import torch
-    nSC, A, L = H.shape
-    S = Y.shape[1]
+    q = _q(n_pam_bits)
+    nSC, A, L = _need(H, "H", torch.complex64, (None, None, None)).shape
+    _need(sigma2, "sigma2", torch.float32, (nSC,))
+    S = _need(Y, "Y", torch.complex64, (nSC, None, A)).shape[1]
     dev = H.device
+    ldt = torch.float16 if llr_f16 else torch.float32
     if C is None:
         C = torch.empty((nSC, L, A), dtype=torch.complex64, device=dev)
     if noise_inv is None:
@@ -241,8 +322,12 @@ def mmse_apply_demap(H, sigma2, Y, n_pam_bits, prec="f16", unbiased=True, cond_t
     if flags is None:
         flags = torch.empty((nSC,), dtype=torch.uint8, device=dev)
     if out is None:
-        out = torch.empty((nSC, L, S, 2 * n_pam_bits), dtype=torch.float16 if llr_f16 else torch.float32,
-                          device=dev)
+        out = torch.empty((nSC, L, S, 2 * q), dtype=ldt, device=dev)
+    _need(C, "C", torch.complex64, (nSC, L, A))
+    _need(noise_inv, "noise_inv", torch.float32, (nSC, L))
+    _need(flags, "flags", torch.uint8, (nSC,))
+    _need(out, "out", ldt, (nSC, L, S, 2 * q))
+    _same_device(H, sigma2, Y, C, noise_inv, flags, out)
     f = (MMSE_UNBIASED if unbiased else 0) | (EQ_WRITE_C if write_c else 0) | (EQ_LLR_F16 if llr_f16 else 0)
     _check(lib().pe_mmse_apply_demap(_ptr(H), _ptr(sigma2), _ptr(Y), _ptr(C), _ptr(noise_inv), _ptr(flags),
                                      _ptr(out), nSC, L, S, A, n_pam_bits, _prec(prec), f, float(cond_threshold),
@@ -259,10 +344,13 @@ def srs_filter(SRS, W, FOCC, mfma=False, conj_w=False, out=None, stream=None):
     binary16 complex pairs: torch int32 tensors whose 32 bits hold {re, im} binary16 (or any 4-byte dtype
     viewed so). Returns H_est [nBlocks, nAnt, K, nPorts] of the same dtype."""
     import torch
-    nBlocks, nAnt, K = SRS.shape
-    nPorts, foccLen = FOCC.shape
+    nBlocks, nAnt, K = _need(SRS, "SRS", 4, (None, None, None)).shape
+    _need(W, "W", 4, (K, K))
+    nPorts, foccLen = _need(FOCC, "FOCC", 4, (None, None)).shape
     if out is None:
         out = torch.empty((nBlocks, nAnt, K, nPorts), dtype=SRS.dtype, device=SRS.device)
+    _need(out, "out", 4, (nBlocks, nAnt, K, nPorts))
+    _same_device(SRS, W, FOCC, out)
     flags = (SRS_MFMA if mfma else 0) | (SRS_CONJ_W if conj_w else 0)
     _check(lib().pe_srs_filter(_ptr(out), _ptr(SRS), _ptr(W), _ptr(FOCC), nAnt, nBlocks, K, nPorts, foccLen, flags,
                                _stream(stream)))
@@ -276,11 +364,14 @@ def coef_to_c16(C, C16=None, c_exp=None, stream=None):
     """complex64 coefficients [nSC, L, A] -> (C16 int32 [nSC, L, A] holding binary16 {re, im} pairs,
     c_exp int32 [nSC]) with C = C16 * 2^c_exp (pe_coef_to_c16)."""
     import torch
-    nSC, L, A = C.shape
+    nSC, L, A = _need(C, "C", torch.complex64, (None, None, None)).shape
     if C16 is None:
         C16 = torch.empty((nSC, L, A), dtype=torch.int32, device=C.device)
     if c_exp is None:
         c_exp = torch.empty((nSC,), dtype=torch.int32, device=C.device)
+    _need(C16, "C16", 4, (nSC, L, A))
+    _need(c_exp, "c_exp", torch.int32, (nSC,))
+    _same_device(C, C16, c_exp)
     _check(lib().pe_coef_to_c16(_ptr(C), _ptr(C16), _ptr(c_exp), nSC, L, A, _stream(stream)))
     return C16, c_exp
 
@@ -290,7 +381,8 @@ def gram_mmse_c16(H, sigma2, unbiased=True, cond_threshold=0.0, C16=None, c_exp=
     """MMSE coefficients written as binary16 pairs + per-subcarrier exponent (pe_gram_mmse_c16). Returns
     (C16 int32 [nSC, L, A], c_exp int32 [nSC], noise_inv [nSC, L], sc_flags uint8 [nSC])."""
     import torch
-    nSC, A, L = H.shape
+    nSC, A, L = _need(H, "H", torch.complex64, (None, None, None)).shape
+    _need(sigma2, "sigma2", torch.float32, (nSC,))
     dev = H.device
     if C16 is None:
         C16 = torch.empty((nSC, L, A), dtype=torch.int32, device=dev)
@@ -300,6 +392,11 @@ def gram_mmse_c16(H, sigma2, unbiased=True, cond_threshold=0.0, C16=None, c_exp=
         noise_inv = torch.empty((nSC, L), dtype=torch.float32, device=dev)
     if flags is None:
         flags = torch.empty((nSC,), dtype=torch.uint8, device=dev)
+    _need(C16, "C16", 4, (nSC, L, A))
+    _need(c_exp, "c_exp", torch.int32, (nSC,))
+    _need(noise_inv, "noise_inv", torch.float32, (nSC, L))
+    _need(flags, "flags", torch.uint8, (nSC,))
+    _same_device(H, sigma2, C16, c_exp, noise_inv, flags)
     _check(lib().pe_gram_mmse_c16(_ptr(H), _ptr(sigma2), _ptr(C16), _ptr(c_exp), _ptr(noise_inv), _ptr(flags), nSC,
                                   L, A, MMSE_UNBIASED if unbiased else 0, float(cond_threshold), _stream(stream)))
     return C16, c_exp, noise_inv, flags
@@ -307,10 +404,13 @@ def gram_mmse_c16(H, sigma2, unbiased=True, cond_threshold=0.0, C16=None, c_exp=
 
 def coef_apply_c16(C16, c_exp, Y, out=None, stream=None):
     import torch
-    nSC, L, A = C16.shape
-    S = Y.shape[1]
+    nSC, L, A = _need(C16, "C16", 4, (None, None, None)).shape
+    _need(c_exp, "c_exp", torch.int32, (nSC,))
+    S = _need(Y, "Y", torch.complex64, (nSC, None, A)).shape[1]
     if out is None:
         out = torch.empty((nSC, L, S), dtype=torch.complex64, device=Y.device)
+    _need(out, "out", torch.complex64, (nSC, L, S))
+    _same_device(C16, c_exp, Y, out)
     _check(lib().pe_coef_apply_c16(_ptr(C16), _ptr(c_exp), _ptr(Y), _ptr(out), nSC, L, S, A, _stream(stream)))
     return out
 
@@ -319,11 +419,16 @@ def coef_apply_demap_c16(C16, c_exp, Y, noise_inv, n_pam_bits, llr_f16=False, ou
     """Fused apply + demap from binary16 coefficients (pe_coef_apply_demap_c16): float32 LLRs, or binary16
     (torch.float16) with llr_f16."""
     import torch
-    nSC, L, A = C16.shape
-    S = Y.shape[1]
+    q = _q(n_pam_bits)
+    nSC, L, A = _need(C16, "C16", 4, (None, None, None)).shape
+    _need(c_exp, "c_exp", torch.int32, (nSC,))
+    S = _need(Y, "Y", torch.complex64, (nSC, None, A)).shape[1]
+    _need(noise_inv, "noise_inv", torch.float32, (nSC, L))
+    ldt = torch.float16 if llr_f16 else torch.float32
     if out is None:
-        out = torch.empty((nSC, L, S, 2 * n_pam_bits), dtype=torch.float16 if llr_f16 else torch.float32,
-                          device=Y.device)
+        out = torch.empty((nSC, L, S, 2 * q), dtype=ldt, device=Y.device)
+    _need(out, "out", ldt, (nSC, L, S, 2 * q))
+    _same_device(C16, c_exp, Y, noise_inv, out)
     _check(lib().pe_coef_apply_demap_c16(_ptr(C16), _ptr(c_exp), _ptr(Y), _ptr(noise_inv), _ptr(out), nSC, L, S, A,
                                          n_pam_bits, LLR_F16 if llr_f16 else 0, _stream(stream)))
     return out
@@ -333,8 +438,11 @@ def bfp_decompress(packed, nPrb, nSym, nAnt, iq_width, scale=1.0, out=None, stre
     """Fronthaul block floating point rows (uint8 tensor [nSym*nAnt, row_stride]) -> complex64 Y_rx
     [nPrb*12, nSym, nAnt] (pe_bfp_decompress)."""
     import torch
+    _need(packed, "packed", torch.uint8, (nSym * nAnt, None))
     if out is None:
         out = torch.empty((nPrb * 12, nSym, nAnt), dtype=torch.complex64, device=packed.device)
+    _need(out, "out", torch.complex64, (nPrb * 12, nSym, nAnt))
+    _same_device(packed, out)
     _check(lib().pe_bfp_decompress(_ptr(packed), _ptr(out), nPrb, nSym, nAnt, iq_width, packed.shape[1],
                                    float(scale), _stream(stream)))
     return out
@@ -344,14 +452,12 @@ def freq_interp(W, X, prec="f32", out=None, stream=None):
     """DMRS frequency interpolation (pe_freq_interp): W [nOut, nIn], X [nBatch, nIn, nCol] complex64 ->
     H_f [nBatch, nOut, nCol], H_f[b] = W @ X[b]. prec "f32" (bit-exact chain), "bf16" or "bf16_split" (MFMA)."""
     import torch
-    nOut, nIn = W.shape
-    nBatch, nIn2, nCol = X.shape
-    if nIn2 != nIn:
-        raise ValueError("W %s and X %s disagree on nIn" % (tuple(W.shape), tuple(X.shape)))
-    if W.dtype != torch.complex64 or X.dtype != torch.complex64 or not (W.is_contiguous() and X.is_contiguous()):
-        raise TypeError("W and X must be contiguous complex64 tensors")
+    nOut, nIn = _need(W, "W", torch.complex64, (None, None)).shape
+    nBatch, _, nCol = _need(X, "X", torch.complex64, (None, nIn, None)).shape
     if out is None:
         out = torch.empty((nBatch, nOut, nCol), dtype=torch.complex64, device=X.device)
+    _need(out, "out", torch.complex64, (nBatch, nOut, nCol))
+    _same_device(W, X, out)
     _check(lib().pe_freq_interp(_ptr(W), _ptr(X), _ptr(out), nBatch, nOut, nIn, nCol, _prec(prec), _stream(stream)))
     return out
 
@@ -359,16 +465,32 @@ def freq_interp(W, X, prec="f32", out=None, stream=None):
 def hbm_probe(src, dst, read_only=False, grid=4096, nbytes=None, stream=None):
     """Streaming probe (pe_hbm_probe): copy src -> dst, or read src (dst: at least `grid` words)."""
     n = src.numel() * src.element_size() if nbytes is None else nbytes
+    _need(src, "src", src.element_size(), min_numel=-(-n // src.element_size()))
+    if read_only:
+        _need(dst, "dst", dst.element_size(), min_numel=-(-4 * grid // dst.element_size()))
+    else:
+        _need(dst, "dst", dst.element_size(), min_numel=-(-n // dst.element_size()))
+    _same_device(src, dst)
     _check(lib().pe_hbm_probe(_ptr(src), _ptr(dst), n, 1 if read_only else 0, grid, _stream(stream)))
 
 
 def cell_desc(C, noise_inv, Y, llr, n_pam_bits, prec="f16", H=None, sigma2=None):
     """Builds a pe_cell_desc from torch tensors (H/sigma2 None: coefficients are inputs). With prec "f16_c16"
     the complex64 C tensor's storage holds the binary16 coefficients and exponents (see c16_view)."""
-    nSC, L, A = C.shape
+    import torch
+    q = _q(n_pam_bits)
+    nSC, L, A = _need(C, "C", torch.complex64, (None, None, None)).shape
+    S = _need(Y, "Y", torch.complex64, (nSC, None, A)).shape[1]
+    _need(noise_inv, "noise_inv", torch.float32, (nSC, L))
+    _need(llr, "llr", torch.float32, (nSC, L, S, 2 * q))
+    if (H is None) != (sigma2 is None):
+        raise ValueError("H and sigma2 go together")
+    if H is not None:
+        _need(H, "H", torch.complex64, (nSC, A, L))
+        _need(sigma2, "sigma2", torch.float32, (nSC,))
+    _same_device(C, Y, noise_inv, llr, H, sigma2)
     return CellDesc(H.data_ptr() if H is not None else None, sigma2.data_ptr() if sigma2 is not None else None,
-                    C.data_ptr(), noise_inv.data_ptr(), Y.data_ptr(), llr.data_ptr(), nSC, L, Y.shape[1], A,
-                    n_pam_bits, _prec(prec))
+                    C.data_ptr(), noise_inv.data_ptr(), Y.data_ptr(), llr.data_ptr(), nSC, L, S, A, q, _prec(prec))
 
 
 def c16_view(C):
@@ -381,9 +503,13 @@ def c16_view(C):
 
 def job_desc(H, sigma2, Y, n_pam_bits, prec="f16"):
     """pe_cell_desc of a serving-loop cell: inputs only (each job's C / noise_inv / LLR come from the arena)."""
-    nSC, A, L = H.shape
-    return CellDesc(H.data_ptr(), sigma2.data_ptr(), None, None, Y.data_ptr(), None, nSC, L, Y.shape[1], A,
-                    n_pam_bits, _prec(prec))
+    import torch
+    q = _q(n_pam_bits)
+    nSC, A, L = _need(H, "H", torch.complex64, (None, None, None)).shape
+    _need(sigma2, "sigma2", torch.float32, (nSC,))
+    S = _need(Y, "Y", torch.complex64, (nSC, None, A)).shape[1]
+    _same_device(H, sigma2, Y)
+    return CellDesc(H.data_ptr(), sigma2.data_ptr(), None, None, Y.data_ptr(), None, nSC, L, S, A, q, _prec(prec))
 
 
 class SlotPlan:
@@ -428,7 +554,8 @@ class ServeConfig(ctypes.Structure):
     """pe_serve_config of include/pusch_eq.h."""
     _fields_ = [("cells", _P), ("n_cells", _I), ("n_jobs", _I), ("arrival_rate_hz", ctypes.c_double),
                 ("deadline_s", ctypes.c_double), ("hbm_budget_bytes", ctypes.c_size_t), ("max_batch_jobs", _I),
-                ("max_inflight", _I), ("shed_late", _I), ("use_graph", _I), ("seed", _U), ("job_llr", _P)]
+                ("max_inflight", _I), ("shed_late", _I), ("use_graph", _I), ("seed", _U), ("job_llr", _P),
+                ("job_cells", _P), ("job_H", _P), ("job_sigma2", _P), ("job_Y", _P)]
 
 
 class ServeReport(ctypes.Structure):
@@ -443,19 +570,52 @@ class ServeReport(ctypes.Structure):
 
 
 def serve_run(cells, n_jobs, rate_hz, deadline_s=0.5e-3, hbm_budget=1 << 30, max_batch=64, max_inflight=2,
-              shed_late=False, use_graph=True, seed=0, job_llr=None):
+              shed_late=False, use_graph=True, seed=0, job_llr=None, job_cells=None, job_H=None, job_sigma2=None,
+              job_Y=None):
     """Runs the slot serving loop (pe_serve_run). cells: list of CellDesc with H, sigma2, Y_rx set.
-    job_llr: optional list of n_jobs torch tensors receiving each job's LLRs. shed_late: False/0 (serve every
-    job), True/1 (drop jobs already late at launch) or 2 (deadline-aware admission). Returns (report dict,
-    per-job latencies in seconds (-1 = shed), per-job cell index)."""
+    job_llr: optional list of n_jobs float32 GPU tensors receiving each job's LLRs (each at least the size of
+    its cell's LLRs). job_cells: optional cell index of each job (else drawn from seed); with it, job_H /
+    job_sigma2 / job_Y: optional lists of n_jobs GPU tensors, each job's own inputs in its cell's shapes.
+    shed_late: False/0 (serve every job), True/1 (drop jobs already late at launch) or 2 (deadline-aware
+    admission). Returns (report dict, per-job latencies in seconds (-1 = shed), per-job cell index)."""
     import numpy as np
+    import torch
     arr = (CellDesc * len(cells))(*cells)
-    ptrs = None
-    if job_llr is not None:
-        ptrs = (ctypes.c_void_p * n_jobs)(*[t.data_ptr() for t in job_llr])
+    keep = []
+
+    def ptr_array(ts, name, dtype, shape_of):
+        if ts is None:
+            return None
+        if len(ts) != n_jobs:
+            raise ValueError("%s: %d tensors for %d jobs" % (name, len(ts), n_jobs))
+        for j, t in enumerate(ts):
+            _need(t, "%s[%d]" % (name, j), dtype, *shape_of(j))
+        _same_device(*ts)
+        a = (ctypes.c_void_p * n_jobs)(*[t.data_ptr() for t in ts])
+        keep.append(a)
+        return ctypes.cast(a, _P)
+
+    jc = None
+    if job_cells is not None:
+        jc = np.ascontiguousarray(job_cells, dtype=np.int32)
+        if jc.shape != (n_jobs,) or jc.min() < 0 or jc.max() >= len(cells):
+            raise ValueError("job_cells: %d indices in [0, %d) expected" % (n_jobs, len(cells)))
+    elif job_H is not None or job_sigma2 is not None or job_Y is not None:
+        raise ValueError("per-job inputs need job_cells")
+
+    def cshape(j, f):
+        return (f(cells[int(jc[j])]), None) if jc is not None else (None, None)
+
+    llr_len = lambda c: c.nSC * c.nLayer * c.nSym * 2 * c.nPamBits
+    max_llr = max(llr_len(c) for c in cells)
+    p_llr = ptr_array(job_llr, "job_llr", torch.float32,
+                      lambda j: (None, llr_len(cells[int(jc[j])]) if jc is not None else max_llr))
+    p_H = ptr_array(job_H, "job_H", torch.complex64, lambda j: cshape(j, lambda c: (c.nSC, c.nAnt, c.nLayer)))
+    p_s2 = ptr_array(job_sigma2, "job_sigma2", torch.float32, lambda j: cshape(j, lambda c: (c.nSC,)))
+    p_Y = ptr_array(job_Y, "job_Y", torch.complex64, lambda j: cshape(j, lambda c: (c.nSC, c.nSym, c.nAnt)))
     cfg = ServeConfig(ctypes.cast(arr, _P), len(cells), n_jobs, float(rate_hz), float(deadline_s), int(hbm_budget),
-                      max_batch, max_inflight, int(shed_late), 1 if use_graph else 0, seed,
-                      ctypes.cast(ptrs, _P) if ptrs is not None else None)
+                      max_batch, max_inflight, int(shed_late), 1 if use_graph else 0, seed, p_llr,
+                      jc.ctypes.data_as(_P) if jc is not None else None, p_H, p_s2, p_Y)
     rep = ServeReport()
     lat = np.zeros(n_jobs, dtype=np.float64)
     cell = np.zeros(n_jobs, dtype=np.int32)

@@ -176,25 +176,35 @@ def hbm_ceiling(dev):
     return best
 
 
-def draw_cells(n, seed, dev, prec="f16"):
-    """Cells drawn as in the multi-cell configurations: PRB in {24,52,106,273}, antennas in {16,32,64}, layers in
-    {2,4,8}, QAM in {16,64,256}, SNR uniform in 5-25 dB, Rayleigh CN(0,1/A) channels, 11 data symbols. Cell c
-    is drawn from its own seed, seed + 1000 c (the reference's per-cell convention, 42 + 1000 c,
-    multicell_evidence.cu:110-132)."""
+def draw_shapes(n, seed):
+    """Cell shapes of the multi-cell configurations: PRB in {24,52,106,273}, antennas in {16,32,64}, layers in
+    {2,4,8}, QAM in {16,64,256} (nPamBits 2..4), SNR uniform in 5-25 dB, each drawn uniformly, in that order,
+    from the cell's own generator seeded seed + 1000 c (the reference's per-cell convention, 42 + 1000 c,
+    multicell_evidence.cu:110-132). The generator is returned with the shape, positioned after the draws."""
     import numpy as np
-    import torch
-    import pusch_eq as pe
-    cells = []
+    out = []
     for ci in range(n):
         rng = np.random.default_rng(seed + 1000 * ci)
-        g = torch.Generator(device=dev)
-        g.manual_seed(seed + 1000 * ci)
         prb = int(rng.choice([24, 52, 106, 273]))
         a = int(rng.choice([16, 32, 64]))
         l = int(rng.choice([2, 4, 8]))
         q = int(rng.choice([2, 3, 4]))
         snr = float(rng.uniform(5, 25))
-        nsc = 12 * prb
+        out.append(dict(prb=prb, nsc=12 * prb, A=a, L=l, q=q, snr=snr, seed=seed + 1000 * ci, rng=rng))
+    return out
+
+
+def draw_cells(n, seed, dev, prec="f16"):
+    """Cells of the multi-cell configurations (shapes from draw_shapes), Rayleigh CN(0,1/A) channels, random QAM
+    symbols of the cell's order through them plus AWGN at the cell's SNR, 11 data symbols; data drawn on the GPU
+    by torch from the cell's seed."""
+    import torch
+    import pusch_eq as pe
+    cells = []
+    for sh in draw_shapes(n, seed):
+        g = torch.Generator(device=dev)
+        g.manual_seed(sh["seed"])
+        a, l, q, snr, nsc = sh["A"], sh["L"], sh["q"], sh["snr"], sh["nsc"]
         H = torch.randn((nsc, a, l), dtype=torch.complex64, device=dev, generator=g) * (1.0 / a) ** 0.5
         bits = torch.randint(0, 2, (nsc, l, S, 2 * q), device=dev, generator=g, dtype=torch.uint8)
         s2v = 10.0 ** (-snr / 10.0)

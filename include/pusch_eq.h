@@ -23,7 +23,12 @@
  *     synchronisation; calls on different streams are independent;
  *   - errors come back as pe_status codes with a message in pe_last_error() (thread-local); nothing calls
  *     exit() (the reference's CUDA_CHECK does, aerial_coef_wmma.cu:57-58);
- *   - nSC == 0 is a valid empty call and returns PE_OK without launching anything.
+ *   - nSC == 0 is a valid empty call and returns PE_OK without launching anything;
+ *   - alignment: every buffer aligned to its element size, and further to the widest access the kernels make:
+ *     C, Y_rx and C16 16 bytes (read 16 bytes at a time), H 16 bytes for an even nLayer (8 for odd), float LLR
+ *     buffers 16 bytes for nPamBits 2 and 4 (8 for 1 and 3), binary16 LLR buffers 16 / 8 / 4 bytes for
+ *     nPamBits 4 / 2 / 1 and 3. hipMalloc's 256-byte alignment always qualifies; a misaligned pointer is
+ *     rejected with PE_ERR_INVALID (naming the buffer) before anything is launched.
  */
 #ifndef PUSCH_EQ_H
 #define PUSCH_EQ_H
@@ -208,6 +213,7 @@ pe_status pe_mmse_apply_demap(const pe_c64 *H, const float *sigma2, const pe_c64
                               uint8_t *sc_flags, void *llr, int nSC, int nLayer, int nSym, int nAnt, int nPamBits,
                               pe_prec prec, unsigned flags, float cond_threshold, void *stream);
 
+
 /* ---- SRS filter multiply (channel estimation upstream of the equalizer) ---------------------------- */
 
 #define PE_SRS_MFMA 1u    /* MFMA with FP32 accumulation: one binary16 rounding per output (K % 4 == 0,
@@ -326,7 +332,14 @@ typedef struct {
                                   with the coefficients formed on chip from solve records (as with
                                   PE_SLOT_SOLVE_RECORDS) where the cell shape allows it */
     unsigned seed;             /* arrival process and cell draws */
-    float *const *job_llr;     /* optional [n_jobs]: LLR destination of each job (sized for its cell) */
+    float *const *job_llr;     /* optional [n_jobs]: LLR destination of each job (sized for its cell; aligned
+                                  as the llr of pe_coef_apply_demap) */
+    /* Optional per-job inputs (each array NULL or [n_jobs]); a job's slot then carries its own data instead of
+       its cell's, as a real slot does: */
+    const int *job_cells;               /* cell of each job, 0 .. n_cells - 1 (NULL: drawn uniformly from seed) */
+    const pe_c64 *const *job_H;         /* channel estimate of each job, the shape of its cell's H */
+    const float *const *job_sigma2;     /* noise variance per subcarrier of each job */
+    const pe_c64 *const *job_Y;         /* samples of each job, the shape of its cell's Y_rx */
 } pe_serve_config;
 
 typedef struct {

@@ -194,3 +194,132 @@ def test_tune_table_parser(pe, tmp_path):
     committed = os.path.join(ROOT, "aerial-mimo-tc-optimization_amd", "tune_table.txt")
     lines = [ln for ln in open(committed) if ln.startswith(("coef ", "mmse "))]
     assert h.pe_tune_load(committed.encode()) == len(lines)
+
+
+def _misaligned_cases(pe):
+    """(entry point name, call with every pointer 256-byte aligned except the one named, buffer name)."""
+    h = pe.lib()
+    A = ctypes.c_void_p(256)
+
+    def args(n_ptr, bad, off):
+        return [ctypes.c_void_p(256 + off) if i == bad else A for i in range(n_ptr)]
+
+    cases = []
+    # (C, Y_rx, Y_eq, nSC, L, S, A, prec): C and Y_rx are read 16 bytes at a time, Y_eq written 8
+    for bad, off, name in ((0, 8, "C"), (1, 8, "Y_rx"), (2, 4, "Y_eq")):
+        for prec in (0, 1):
+            cases.append((lambda b=bad, o=off, p=prec: h.pe_coef_apply(*args(3, b, o), 4, 8, 11, 64, p, None), name))
+    cases.append((lambda: h.pe_coef_apply_v1(*args(3, 0, 8), 4, 11, None), "C"))
+    # fused apply + demap, float LLRs: 16-byte stores at q = 2, 4, 8-byte at q = 1, 3
+    for q, off in ((4, 8), (2, 8), (3, 4), (1, 4)):
+        cases.append((lambda q=q, o=off: h.pe_coef_apply_demap(*args(4, 3, o), 4, 8, 11, 64, q, 1, None), "llr"))
+    cases.append((lambda: h.pe_coef_apply_demap(*args(4, 2, 2), 4, 8, 11, 64, 4, 1, None), "noise_inv"))
+    # binary16 LLRs: 16 / 8 / 4 bytes at q = 4 / 2 / 1, 3
+    for q, off in ((4, 8), (2, 4), (1, 2), (3, 2)):
+        cases.append((lambda q=q, o=off: h.pe_coef_apply_demap_f16(*args(4, 3, o), 4, 8, 11, 64, q, 1, None),
+                      "llr"))
+        cases.append((lambda q=q, o=off: h.pe_llr_demap_f16(*args(3, 2, o), 10, q, None), "llrOut"))
+    cases.append((lambda: h.pe_llr_demap_f16(*args(3, 0, 2), 10, 4, None), "softEst"))
+    cases.append((lambda: h.pe_llr_demap(*args(3, 0, 4), 4, 8, 11, 4, None), "Y_eq"))
+    cases.append((lambda: h.pe_llr_demap(*args(3, 2, 8), 4, 8, 11, 4, None), "llr"))
+    # MMSE: H rows of an even layer count are read 16 bytes at a time
+    cases.append((lambda: h.pe_gram_mmse(*args(5, 0, 8), 4, 8, 64, 1, 0.0, None), "H"))
+    cases.append((lambda: h.pe_gram_mmse(*args(5, 2, 4), 4, 8, 64, 1, 0.0, None), "C"))
+    cases.append((lambda: h.pe_gram_mmse_c16(*args(6, 0, 8), 4, 8, 64, 1, 0.0, None), "H"))
+    cases.append((lambda: h.pe_gram(*args(3, 1, 4), 4, 8, 64, None), "H"))
+    cases.append((lambda: h.pe_coef_apply_demap_c16(*args(5, 4, 8), 4, 8, 11, 64, 4, 0, None), "llr"))
+    return cases
+
+
+def test_misaligned_pointers_are_rejected_without_a_device(pe):
+    """Every entry point checks that each buffer is aligned for the widest access its kernels make (16-byte loads
+    of C, Y_rx and H rows, LLR stores of 2 q values) and returns PE_ERR_INVALID, naming the buffer, before
+    anything is launched; the same calls with aligned buffers of an odd layer count pass the check for H."""
+    h = pe.lib()
+    for call, name in _misaligned_cases(pe):
+        assert call() == 1
+        msg = h.pe_last_error().decode()
+        assert name in msg and "aligned" in msg, msg
+    # an odd layer count reads H rows 8 bytes at a time: an 8-byte-aligned H passes the alignment check (the
+    # call then fails on the null flags buffer, not on H)
+    p8 = ctypes.c_void_p(264)
+    A = ctypes.c_void_p(256)
+    assert h.pe_gram_mmse(p8, A, A, A, None, 4, 7, 64, 1, 0.0, None) == 1
+    assert b"null" in h.pe_last_error()
+
+
+def test_slot_and_serve_check_alignment_without_a_device(pe):
+    h = pe.lib()
+    out = ctypes.c_void_p()
+    a = ctypes.c_void_p(256)
+    b8 = ctypes.c_void_p(264)
+    good = pe.CellDesc(None, None, a, a, a, a, 12, 8, 11, 64, 4, 1)
+    bad_y = pe.CellDesc(None, None, a, a, b8, a, 12, 8, 11, 64, 4, 1)
+    bad_llr = pe.CellDesc(None, None, a, a, a, b8, 12, 8, 11, 64, 4, 1)
+    for cell, name in ((bad_y, b"Y_rx"), (bad_llr, b"llr")):
+        arr = (pe.CellDesc * 2)(good, cell)
+        assert h.pe_slot_plan_create(arr, 2, 1, ctypes.byref(out)) == 1
+        assert b"cell 1" in h.pe_last_error() and name in h.pe_last_error()
+    # serving: per-job inputs need job_cells, and each must be aligned
+    cells = (pe.CellDesc * 1)(pe.CellDesc(a, a, None, None, a, None, 12, 8, 11, 64, 4, 1))
+    rep = pe.ServeReport()
+    jc = (ctypes.c_int * 2)(0, 0)
+    ys = (ctypes.c_void_p * 2)(256, 264)
+
+    def run(job_cells, job_y):
+        cfg = pe.ServeConfig(ctypes.cast(cells, ctypes.c_void_p), 1, 2, 0.0, 5e-4, 1 << 30, 8, 1, 0, 0, 1, None,
+                             job_cells, None, None, job_y)
+        return h.pe_serve_run(ctypes.byref(cfg), ctypes.byref(rep), None, None)
+
+    assert run(None, ctypes.cast(ys, ctypes.c_void_p)) == 1
+    assert b"job_cells" in h.pe_last_error()
+    assert run(ctypes.cast(jc, ctypes.c_void_p), ctypes.cast(ys, ctypes.c_void_p)) == 1
+    assert b"job 1" in h.pe_last_error() and b"job_Y" in h.pe_last_error()
+    jc_bad = (ctypes.c_int * 2)(0, 3)
+    assert run(ctypes.cast(jc_bad, ctypes.c_void_p), None) == 1
+    assert b"out of range" in h.pe_last_error()
+
+
+def test_binding_rejects_mismatched_tensors_before_any_launch(pe):
+    """The binding checks dtype, contiguity and shape of every tensor before it calls the library (these run
+    first, so CPU tensors show them here), then that the tensors are on a GPU: nothing reaches a kernel with a
+    buffer smaller than the shapes imply or of the wrong element type."""
+    import torch
+    c64, f32 = torch.complex64, torch.float32
+    C = torch.zeros((4, 8, 64), dtype=c64)
+    Y = torch.zeros((4, 11, 64), dtype=c64)
+    ni = torch.zeros((4, 8), dtype=f32)
+    H = torch.zeros((4, 64, 8), dtype=c64)
+    s2 = torch.zeros(4, dtype=f32)
+    with pytest.raises(TypeError, match="C: dtype"):
+        pe.coef_apply(C.to(torch.complex128), Y)
+    with pytest.raises(ValueError, match="Y: shape"):
+        pe.coef_apply(C, Y[:, :, :32].contiguous())
+    with pytest.raises(ValueError, match="contiguous"):
+        pe.coef_apply(C, Y.transpose(0, 1))
+    with pytest.raises(ValueError, match="GPU tensors"):
+        pe.coef_apply(C, Y)
+    with pytest.raises(TypeError, match="sigma2: dtype"):
+        pe.gram_mmse(H, s2.double())
+    with pytest.raises(ValueError, match="H: shape|contiguous"):
+        pe.gram_mmse(H[:, ::2], s2)
+    with pytest.raises(ValueError, match="noise_inv: shape"):
+        pe.coef_apply_demap(C, Y, ni[:, :4].contiguous(), 4)
+    with pytest.raises(ValueError, match="out: shape"):
+        pe.coef_apply_demap(C, Y, ni, 4, out=torch.zeros((4, 8, 11, 6), dtype=f32))
+    with pytest.raises(ValueError, match="n_pam_bits"):
+        pe.coef_apply_demap(C, Y, ni, 5)
+    with pytest.raises(ValueError, match="Y: shape"):
+        pe.mmse_apply_demap(H, s2, Y[:2].contiguous(), 4)
+    with pytest.raises(TypeError, match="noise_inv: dtype"):
+        pe.llr_demap(torch.zeros((4, 8, 11), dtype=c64), ni.half(), 4)
+    with pytest.raises(ValueError, match="out: at least"):
+        pe.coef_apply_v1(C, Y, torch.zeros(10, dtype=c64), 4, 11)
+    with pytest.raises(ValueError, match="range"):
+        pe.coef_apply_range(C, Y, torch.zeros((4, 8, 11), dtype=c64), 3, 2)
+    with pytest.raises(ValueError, match="H: shape"):
+        pe.cell_desc(C, ni, Y, torch.zeros((4, 8, 11, 8), dtype=f32), 4, H=H[:, :32].contiguous(), sigma2=s2)
+    with pytest.raises(TypeError, match="W: elements"):
+        pe.srs_filter(torch.zeros((2, 64, 24), dtype=torch.int32), torch.zeros((24, 24), dtype=torch.int16),
+                      torch.zeros((4, 8), dtype=torch.int32))
+

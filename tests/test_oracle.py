@@ -197,6 +197,56 @@ def test_llr_folding_is_exact_maxlog(po, q):
         assert np.max(np.abs(out - bf)) <= 1e-5 * max(1.0, np.max(np.abs(bf)))
 
 
+def _gray_levels(q):
+    """TS 38.211 PAM levels of one axis: bits c [2^q, q] (bit i of the code) and amplitudes (unit QAM power)."""
+    codes = np.arange(1 << q)
+    c = ((codes[:, None] >> np.arange(q)[None, :]) & 1).astype(np.float64)
+    t = 1.0 - 2.0 * c[:, q - 1]
+    for i in range(q - 2, -1, -1):
+        t = (1.0 - 2.0 * c[:, i]) * (2.0 ** (q - 1 - i) - t)
+    return c, t / np.sqrt(2.0 * (4.0 ** q - 1.0) / 3.0)
+
+
+def _maxlog_numpy(x, ni, q, levels=None):
+    """Exact max-log per axis, written independently of the oracle's C code: LLR_i = (min over levels with bit
+    i = 1 of (x - a)^2 - min over levels with bit i = 0) * noise_inv. x [...], ni broadcastable -> [..., q]."""
+    c, lev = _gray_levels(q) if levels is None else levels
+    d2 = (x[..., None] - lev) ** 2
+    out = [(np.where(c[:, i] == 1, d2, np.inf).min(-1) - np.where(c[:, i] == 0, d2, np.inf).min(-1)) * ni
+           for i in range(q)]
+    return np.stack(out, -1)
+
+
+@pytest.mark.parametrize("q", [1, 2, 3, 4])
+def test_llr_demap_f64_magnitudes_equal_bruteforce(po, q):
+    """po_llr_demap_f64 (the FP64 end of every chain check, and the source of its hard-decision epsilon) over
+    the equalizer layout [sc][l][s][2q]: every LLR equals the exact max-log of its axis times noise_inv[sc][l], to
+    FP64 rounding (relative to (1 + |y|^2) noise_inv), on a grid of I values through all decision regions, random
+    Q values and a distinct noise_inv per (subcarrier, layer). Negative controls: the same brute force with noise
+    scaled by 1 + 1e-9, or with one PAM level moved by 1e-9, misses the bar, so a noise-scaling slip or a wrong
+    LUT entry in the oracle cannot pass."""
+    rng = np.random.default_rng(40 + q)
+    nSC, nL, nS = 7, 3, 97
+    Y = np.linspace(-1.7, 1.7, nSC * nL * nS).reshape(nSC, nL, nS) + 1j * rng.uniform(-1.7, 1.7, (nSC, nL, nS))
+    ni = rng.uniform(0.5, 40.0, (nSC, nL))
+    got = po.llr_demap_f64(Y, ni, q)
+
+    def expected(noise, levels=None):
+        ref = np.empty_like(got)
+        ref[..., 0::2] = _maxlog_numpy(Y.real, noise[:, :, None], q, levels)
+        ref[..., 1::2] = _maxlog_numpy(Y.imag, noise[:, :, None], q, levels)
+        return np.clip(ref, -65504.0, 65504.0)
+
+    scale = (1.0 + np.abs(Y)[..., None] ** 2) * ni[:, :, None, None]
+    bar = 1e-13
+    assert np.max(np.abs(got - expected(ni)) / scale) <= bar
+    assert np.max(np.abs(got - expected(ni * (1 + 1e-9))) / scale) > bar
+    c, lev = _gray_levels(q)
+    lev = lev.copy()
+    lev[-1] += 1e-9
+    assert np.max(np.abs(got - expected(ni, (c, lev))) / scale) > bar
+
+
 def test_llr_qpsk_closed_form(po):
     d = 1 / np.sqrt(2)
     for x in [-0.9, -0.2, 0.0, 0.3, 1.1]:
