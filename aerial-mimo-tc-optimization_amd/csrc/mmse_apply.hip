@@ -18,10 +18,38 @@
 // Shapes for which this does not pay (fewer than 32 antennas or 4 layers: measured slower, single cells being
 // launch- and latency-bound) or cannot be done (more than 64 antennas: one column per lane) run pe_gram_mmse's
 // and the apply's kernels with C in HBM: the same kernels, hence the same results.
+#include <mutex>
+
 #include "pe_internal.hpp"
+#include "pusch_eq.h"
 #include "solve_record.hpp"
 
+// Coefficient workspace of pe_mmse_apply_demap_noalloc: solve records (and the coefficient blocks of subcarriers
+// solved in FP64) for up to PE_NOALLOC_SCRATCH_BYTES of coefficients, one copy per device, held until the process
+// ends. Host code never passes `scratch` itself (its host-side name is not a device address): it passes the
+// address noalloc_scratch() looked up for the current device.
+static __device__ __align__(256) unsigned char scratch[PE_NOALLOC_SCRATCH_BYTES];
+
 namespace pe {
+
+hipError_t noalloc_scratch(void **ptr) {
+    constexpr int kMaxDevices = 64;
+    static std::mutex mu;
+    static void *addr[kMaxDevices] = {};
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    if (dev < 0 || dev >= kMaxDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lk(mu);
+    if (!addr[dev]) {  // once per device, with that device current
+        void *p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        addr[dev] = p;
+    }
+    *ptr = addr[dev];
+    return hipSuccess;
+}
 
 namespace {
 

@@ -235,6 +235,9 @@ pe_status pe_init(int device) {
     const char *path = getenv("PE_TUNE_TABLE");
     if (path && *path && load_table_file(path) < 0)
         return fail(PE_ERR_INVALID, "cannot read tile table %s", path);
+    void *scratch = nullptr;  // set up the scratch address of pe_mmse_apply_demap_noalloc for this device
+    e = pe::noalloc_scratch(&scratch);
+    if (e != hipSuccess) return from_hip(e, "scratch address");
     return ok();
 }
 
@@ -499,6 +502,28 @@ pe_status pe_mmse_apply_demap(const pe_c64 *H, const float *sigma2, const pe_c64
         noise_inv, sc_flags, llr, nSC, nLayer, nSym, nAnt, nPamBits, (int)prec, (flags & PE_MMSE_UNBIASED) != 0, thr,
         (flags & PE_EQ_LLR_F16) ? 1 : 0, (flags & PE_EQ_WRITE_C) ? 1 : 0, (hipStream_t)stream);
     return from_hip(e, "mmse_apply_demap launch");
+}
+
+pe_status pe_mmse_apply_demap_noalloc(const pe_c64 *H, const float *sigma2, const pe_c64 *Y_rx, float *noise_inv,
+                                      uint8_t *sc_flags, void *llr, int nSC, int nLayer, int nSym, int nAnt,
+                                      int nPamBits, pe_prec prec, unsigned flags, float cond_threshold, void *stream) {
+    pe_status s = check_mmse_apply(H, sigma2, Y_rx, noise_inv, sc_flags, llr, nSC, nLayer, nSym, nAnt, nPamBits, prec,
+                                   flags, PE_MMSE_UNBIASED | PE_EQ_LLR_F16);
+    if (s != PE_OK) return s;
+    const size_t ws = (size_t)nSC * nLayer * nAnt * sizeof(pe_c64);
+    if (ws > PE_NOALLOC_SCRATCH_BYTES)
+        return fail(PE_ERR_CAPACITY, "workspace of %zu bytes (nSC * nLayer * nAnt * 8) exceeds the %u-byte scratch", ws,
+                    (unsigned)PE_NOALLOC_SCRATCH_BYTES);
+    if (nSC == 0) return ok();
+    void *scratch = nullptr;  // the array's address on the current device (no allocation)
+    hipError_t e = pe::noalloc_scratch(&scratch);
+    if (e != hipSuccess) return from_hip(e, "scratch address");
+    const float thr = cond_threshold > 0.f ? cond_threshold : 1e4f;
+    e = pe::launch_mmse_apply_demap(reinterpret_cast<const float2 *>(H), sigma2, reinterpret_cast<const float2 *>(Y_rx),
+                                    reinterpret_cast<float2 *>(scratch), noise_inv, sc_flags, llr, nSC, nLayer, nSym,
+                                    nAnt, nPamBits, (int)prec, (flags & PE_MMSE_UNBIASED) != 0, thr,
+                                    (flags & PE_EQ_LLR_F16) ? 1 : 0, 0, (hipStream_t)stream);
+    return from_hip(e, "mmse_apply_demap_noalloc launch");
 }
 
 pe_status pe_bfp_decompress(const uint8_t *in, pe_c64 *Y_rx, int nPrb, int nSym, int nAnt, int iqWidth,

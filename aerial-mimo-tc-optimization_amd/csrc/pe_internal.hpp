@@ -104,5 +104,9 @@ hipError_t launch_mmse_apply_demap(const float2 *H, const float *sigma2, const f
                                    float *noise_inv, uint8_t *flags, void *llr, int nSC, int L, int S, int A,
                                    int nPamBits, int prec, bool unbiased, float thr, int llr_f16, int write_c,
                                    hipStream_t st);
+// Device address of the file-scope scratch array that pe_mmse_apply_demap_noalloc uses as its coefficient
+// workspace (PE_NOALLOC_SCRATCH_BYTES), on the current device: looked up once per device with
+// hipGetSymbolAddress and kept; no allocation.
+hipError_t noalloc_scratch(void **ptr);
 
 }  // namespace pe

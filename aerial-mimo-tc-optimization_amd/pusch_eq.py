@@ -52,6 +52,7 @@ SIGNATURES = {
     "pe_coef_apply_c16": ([_P, _P, _P, _P, _I, _I, _I, _I, _P], _I),
     "pe_coef_apply_demap_c16": ([_P, _P, _P, _P, _P, _I, _I, _I, _I, _I, _U, _P], _I),
     "pe_mmse_apply_demap": ([_P, _P, _P, _P, _P, _P, _P, _I, _I, _I, _I, _I, _I, _U, _F, _P], _I),
+    "pe_mmse_apply_demap_noalloc": ([_P, _P, _P, _P, _P, _P, _I, _I, _I, _I, _I, _I, _U, _F, _P], _I),
     "pe_slot_plan_create": ([_P, _I, _U, _P], _I),
     "pe_slot_plan_update": ([_P, _P, _I, _P], _I),
     "pe_slot_plan_launch": ([_P, _P], _I),
@@ -333,6 +334,37 @@ def mmse_apply_demap(H, sigma2, Y, n_pam_bits, prec="f16", unbiased=True, cond_t
                                      _ptr(out), nSC, L, S, A, n_pam_bits, _prec(prec), f, float(cond_threshold),
                                      _stream(stream)))
     return out, C, noise_inv, flags
+
+
+NOALLOC_SCRATCH_BYTES = 32 << 20
+
+
+def mmse_apply_demap_noalloc(H, sigma2, Y, n_pam_bits, prec="f16", unbiased=True, cond_threshold=0.0,
+                             llr_f16=False, noise_inv=None, flags=None, out=None, stream=None):
+    """pe_mmse_apply_demap_noalloc: the one-pass chain with its workspace in the library's fixed device scratch
+    (nSC * nLayer * nAnt * 8 <= NOALLOC_SCRATCH_BYTES); the library allocates nothing. Output tensors are allocated
+    here by torch unless given. Returns (llr, noise_inv, flags)."""
+    import torch
+    q = _q(n_pam_bits)
+    nSC, A, L = _need(H, "H", torch.complex64, (None, None, None)).shape
+    _need(sigma2, "sigma2", torch.float32, (nSC,))
+    S = _need(Y, "Y", torch.complex64, (nSC, None, A)).shape[1]
+    dev = H.device
+    ldt = torch.float16 if llr_f16 else torch.float32
+    if noise_inv is None:
+        noise_inv = torch.empty((nSC, L), dtype=torch.float32, device=dev)
+    if flags is None:
+        flags = torch.empty((nSC,), dtype=torch.uint8, device=dev)
+    if out is None:
+        out = torch.empty((nSC, L, S, 2 * q), dtype=ldt, device=dev)
+    _need(noise_inv, "noise_inv", torch.float32, (nSC, L))
+    _need(flags, "flags", torch.uint8, (nSC,))
+    _need(out, "out", ldt, (nSC, L, S, 2 * q))
+    _same_device(H, sigma2, Y, noise_inv, flags, out)
+    f = (MMSE_UNBIASED if unbiased else 0) | (EQ_LLR_F16 if llr_f16 else 0)
+    _check(lib().pe_mmse_apply_demap_noalloc(_ptr(H), _ptr(sigma2), _ptr(Y), _ptr(noise_inv), _ptr(flags), _ptr(out),
+                                             nSC, L, S, A, q, _prec(prec), f, float(cond_threshold), _stream(stream)))
+    return out, noise_inv, flags
 
 
 SRS_MFMA = 1

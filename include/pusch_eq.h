@@ -213,6 +213,21 @@ pe_status pe_mmse_apply_demap(const pe_c64 *H, const float *sigma2, const pe_c64
                               uint8_t *sc_flags, void *llr, int nSC, int nLayer, int nSym, int nAnt, int nPamBits,
                               pe_prec prec, unsigned flags, float cond_threshold, void *stream);
 
+/* No counterpart in the reference. For callers that cannot allocate device memory (code under hipGraph capture,
+   tight serving loops, harnesses that forbid hipMalloc): pe_mmse_apply_demap without the C workspace argument.
+   Its workspace is a fixed array in the library's device memory, PE_NOALLOC_SCRATCH_BYTES per device, set aside
+   when the library's code is loaded on that device and held until the process ends; the call itself allocates
+   nothing (no hipMalloc, no allocating library routine). nSC * nLayer * nAnt * 8 bytes must fit the array (one
+   cell of 273 PRB = 3276 subcarriers at 8 layers x 128 antennas does), else PE_ERR_CAPACITY. Results (llr,
+   noise_inv, sc_flags) are bit-identical to pe_mmse_apply_demap with the same arguments. flags:
+   PE_MMSE_UNBIASED and PE_EQ_LLR_F16 (PE_EQ_WRITE_C has no C to write: PE_ERR_INVALID). Every call on one device
+   shares that device's array: calls whose work overlaps on the device (different streams, or another thread)
+   race on it and must be serialized by the caller; calls ordered on one stream are safe. No state carries over
+   between calls (each call writes every workspace byte it reads). */
+#define PE_NOALLOC_SCRATCH_BYTES (32u << 20)
+pe_status pe_mmse_apply_demap_noalloc(const pe_c64 *H, const float *sigma2, const pe_c64 *Y_rx, float *noise_inv,
+                                      uint8_t *sc_flags, void *llr, int nSC, int nLayer, int nSym, int nAnt,
+                                      int nPamBits, pe_prec prec, unsigned flags, float cond_threshold, void *stream);
 
 /* ---- SRS filter multiply (channel estimation upstream of the equalizer) ---------------------------- */
 

@@ -323,3 +323,19 @@ def test_binding_rejects_mismatched_tensors_before_any_launch(pe):
         pe.srs_filter(torch.zeros((2, 64, 24), dtype=torch.int32), torch.zeros((24, 24), dtype=torch.int16),
                       torch.zeros((4, 8), dtype=torch.int32))
 
+
+def test_noalloc_entry_point_validates_without_a_device(pe):
+    """pe_mmse_apply_demap_noalloc: work beyond the fixed scratch is PE_ERR_CAPACITY, PE_EQ_WRITE_C (no C to
+    write) is an unknown flag, both before any device call."""
+    h = pe.lib()
+    a = ctypes.c_void_p(256)
+    n = ctypes.c_void_p(0)
+    na = h.pe_mmse_apply_demap_noalloc
+    # (H, sigma2, Y, noise_inv, flags, llr, nSC, L, S, A, q, prec, flags, thr, stream)
+    assert na(n, n, n, n, n, n, 0, 8, 11, 64, 4, 1, 1, 0.0, None) == 0                 # empty
+    assert na(a, a, a, a, a, a, 4097, 8, 11, 128, 4, 1, 1, 0.0, None) == 5             # 33.6 MB > 32 MiB
+    assert b"scratch" in h.pe_last_error()
+    assert na(a, a, a, a, a, a, 16, 8, 11, 64, 4, 1, 1 | 2, 0.0, None) == 1            # PE_EQ_WRITE_C
+    assert na(a, a, a, a, a, a, 16, 8, 11, 64, 4, 0, 1, 0.0, None) == 2                # FP32: no MFMA
+    assert na(a, a, ctypes.c_void_p(264), a, a, a, 16, 8, 11, 64, 4, 1, 1, 0.0, None) == 1  # unaligned Y
+    assert pe.NOALLOC_SCRATCH_BYTES == 32 << 20
