@@ -705,6 +705,9 @@ def main():
     ap.add_argument("--sweep-out", default=None, help="write the deployment sweep table (JSON) here")
     ap.add_argument("--multicell-only", action="store_true",
                     help="only the 24-cell slot section (graph and eager slots), printed as JSON (for profiling)")
+    ap.add_argument("--headline-only", action="store_true",
+                    help="only the timed headline launches (fused apply + demap over the 32-cell batches), for "
+                         "profiling")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run 5 plain 256 MiB device copies (scale reference for the HBM byte counters)")
     args = ap.parse_args()
@@ -816,6 +819,10 @@ def main():
     flops_launch = 8.0 * nsc * L * S * A
     achieved_gbs = bytes_launch / per_launch / 1e9
     value = nsc / per_launch
+    if args.headline_only:
+        print(json.dumps({"subcarriers": nsc, "ms_per_launch": per_launch * 1e3, "hbm_gbs": achieved_gbs,
+                          "bytes_per_launch": bytes_launch, "steps": args.steps, "warmup": args.warmup}), flush=True)
+        return
 
     # per-launch distribution (events between back-to-back launches, same alternation), cold = from HBM
     evs = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps + 1)]
