@@ -23,6 +23,7 @@
 // The split variants add the rounding residuals: A*B ~ Ahi*Bhi + (Ahi*Blo + Alo*Bhi).
 #pragma once
 #include "pe_device.hpp"
+#include "pe_internal.hpp"
 
 namespace pe {
 namespace mfma {
@@ -242,18 +243,82 @@ __device__ __forceinline__ void demap_store(float2 s0, float2 s1, size_t sc, int
     }
 }
 
+// ---- codeword output: descrambling and the rate-recovery layout (TS 38.211 6.3.1, TS 38.212 5.4.2) ----------
+// Codeword records (CwLayer, pe_internal.hpp): LLR i of layer l's symbol at (subcarrier sc, data symbol n) is
+// codeword bit g = ((n nSC + sc) nu + v) Qm + i; it is negated where c(g) = 1 and stored at g (codeword order) or at
+// its place in the de-interleaved code block.
+template <int NPB>
+__device__ __forceinline__ void cw_store(float (&v)[2 * NPB], const CwLayer &cl, size_t sc, int n, int nSC,
+                                         int f16) {
+    constexpr uint32_t QM = 2 * NPB;
+    const uint32_t nu = cl.nu_v & 255u, vr = cl.nu_v >> 8;
+    const uint32_t g = (((uint32_t)n * (uint32_t)nSC + (uint32_t)sc) * nu + vr) * QM;
+    if (cl.cseq) {
+        const uint32_t *cs = reinterpret_cast<const uint32_t *>(cl.cseq);
+        const uint32_t w = g >> 5, sh = g & 31u;
+        uint32_t bits = cs[w] >> sh;
+        if (sh + QM > 32u) bits |= cs[w + 1] << (32u - sh);
+#pragma unroll
+        for (uint32_t i = 0; i < QM; ++i)
+            if ((bits >> i) & 1u) v[i] = -v[i];
+    }
+    if (cl.e_lo == 0u) {  // codeword order: the symbol's Qm LLRs are consecutive
+        if (f16)
+            store_llr_f16<NPB>(reinterpret_cast<uint16_t *>(cl.out) + g, v);
+        else
+            store_llr<NPB>(reinterpret_cast<float *>(cl.out) + g, v);
+        return;
+    }
+    const uint32_t lo_len = cl.r0 * cl.e_lo;
+    uint32_t start, E;
+    if (g < lo_len) {
+        E = cl.e_lo;
+        start = g / E * E;
+    } else {
+        E = cl.e_lo + nu * QM;
+        start = lo_len + (g - lo_len) / E * E;
+    }
+    const uint32_t j = (g - start) / QM, stride = E / QM;  // symbol j of its block; bit i goes to plane i
+#pragma unroll
+    for (uint32_t i = 0; i < QM; ++i) {
+        const uint32_t pos = start + i * stride + j;
+        if (f16)
+            reinterpret_cast<uint16_t *>(cl.out)[pos] = (uint16_t)float_to_half_bits(v[i]);
+        else
+            reinterpret_cast<float *>(cl.out)[pos] = v[i];
+    }
+}
+
+// The demapped LLRs of the two symbols a lane holds (layers l0, l0 + 1, data symbol n) in codeword order.
+template <int NPB>
+__device__ __forceinline__ void cw_demap_store(float2 s0, float2 s1, size_t sc, int l0, int n, int L, int nSC,
+                                               const float *__restrict__ noise_inv, const CwLayer *cwl, int f16) {
+    if (l0 < L) {
+        float v[2 * NPB];
+        qam_llr<NPB>(s0.x, s0.y, noise_inv[sc * L + l0], v);
+        cw_store<NPB>(v, cwl[l0], sc, n, nSC, f16);
+    }
+    if (l0 + 1 < L) {
+        float v[2 * NPB];
+        qam_llr<NPB>(s1.x, s1.y, noise_inv[sc * L + l0 + 1], v);
+        cw_store<NPB>(v, cwl[l0 + 1], sc, n, nSC, f16);
+    }
+}
+
 // One subcarrier: range scaling, operand conversion into LDS, MFMA, epilogue to Y_eq or LLRs.
 // NPB: 0 -> store Y_eq; 1..4 -> fused demap of that QAM order; -1 -> fused demap of order npb_rt.
 // C16IN: R.c holds binary16 coefficient chunks (load_raw_c16; nC, rc count those) scaled by 2^ec_in; only the
 // samples are range-checked here, and only PREC 1 applies.
 // CCOLS: the coefficients come as columns in *cc instead of R.c (nC, rc unused). The values, the scale
 // decision and every operand written to LDS are those of the chunk path on the same complex64 coefficients.
-template <int KS, int PREC, int NPB, bool C16IN = false, bool CCOLS = false>
+// CW: the LLRs go to their codewords (cwl: the eight layer records, nSC: subcarriers of the allocation) instead
+// of llr; the QAM order is npb_rt.
+template <int KS, int PREC, int NPB, bool C16IN = false, bool CCOLS = false, bool CW = false>
 __device__ __forceinline__ void process_sc(RawSC<KS> &R, size_t sc, unsigned char *lds, float2 *__restrict__ O,
                                            const float *__restrict__ noise_inv, float *__restrict__ llr, int L,
                                            int S, int A, int lane, int nC, int nY, int rc, int ry,
                                            int npb_rt = 0, int f16 = 0, int ec_in = 0,
-                                           const CCols *cc = nullptr) {
+                                           const CCols *cc = nullptr, const CwLayer *cwl = nullptr, int nSC = 0) {
     static_assert(!C16IN || PREC == 1, "binary16 coefficient input is for the binary16 product");
     static_assert(!(C16IN && CCOLS), "one coefficient source");
     constexpr int NU = KS > 4 ? 2 : 1;  // antennas per lane in the column form
@@ -426,6 +491,17 @@ __device__ __forceinline__ void process_sc(RawSC<KS> &R, size_t sc, unsigned cha
     const float2 s0 = lo ? make_float2(d[0], y0) : make_float2(y0, d[2]);
     const float2 s1 = lo ? make_float2(d[1], y1) : make_float2(y1, d[3]);
     const int n = r;
+    if constexpr (CW) {
+        if (n < S) {
+            switch (npb_rt) {
+            case 1: cw_demap_store<1>(s0, s1, sc, l0, n, L, nSC, noise_inv, cwl, f16); break;
+            case 2: cw_demap_store<2>(s0, s1, sc, l0, n, L, nSC, noise_inv, cwl, f16); break;
+            case 3: cw_demap_store<3>(s0, s1, sc, l0, n, L, nSC, noise_inv, cwl, f16); break;
+            default: cw_demap_store<4>(s0, s1, sc, l0, n, L, nSC, noise_inv, cwl, f16); break;
+            }
+        }
+        return;
+    }
     if (n < S) {
         if constexpr (NPB == 0) {
             if (l0 < L) O[(sc * L + l0) * S + n] = s0;

@@ -526,6 +526,73 @@ pe_status pe_mmse_apply_demap_noalloc(const pe_c64 *H, const float *sigma2, cons
     return from_hip(e, "mmse_apply_demap_noalloc launch");
 }
 
+pe_status pe_gold_sequence(uint32_t c_init, uint32_t *c_words, int n_bits, void *stream) {
+    if (n_bits < 1 || n_bits >= (1 << 30)) return fail(PE_ERR_INVALID, "n_bits = %d outside [1, 2^30)", n_bits);
+    if (!c_words) return fail(PE_ERR_INVALID, "null pointer");
+    pe_status s = check_align({{c_words, 4, "c_words"}});
+    if (s != PE_OK) return s;
+    return from_hip(pe::launch_gold_sequence(c_init, c_words, n_bits, (hipStream_t)stream), "gold_sequence launch");
+}
+
+pe_status pe_coef_apply_demap_cw(const pe_c64 *C, const pe_c64 *Y_rx, const float *noise_inv, const pe_cw_layout *cw,
+                                 int nSC, int nLayer, int nSym, int nAnt, int nPamBits, pe_prec prec, unsigned flags,
+                                 void *stream) {
+    pe_status s = check_coef_shape(nSC, nLayer, nSym, nAnt, (int)prec);
+    if (s != PE_OK) return s;
+    if (prec == PE_PREC_F32) return fail(PE_ERR_UNSUPPORTED, "the codeword layout runs with the MFMA precisions");
+    if (nPamBits < 1 || nPamBits > 4) return fail(PE_ERR_INVALID, "nPamBits = %d outside [1, 4]", nPamBits);
+    if (flags & ~PE_LLR_F16) return fail(PE_ERR_INVALID, "unknown flag bits 0x%x", flags);
+    if (!cw) return fail(PE_ERR_INVALID, "null codeword layout");
+    const bool f16 = (flags & PE_LLR_F16) != 0;
+    const int qm = 2 * nPamBits;
+    int nu[8] = {0}, rank[8] = {0}, ncw = 0;
+    for (int l = 0; l < nLayer; ++l) {
+        const int k = cw->layer_cw[l];
+        if (k < 0 || k >= nLayer) return fail(PE_ERR_INVALID, "layer_cw[%d] = %d outside [0, nLayer)", l, k);
+        rank[l] = nu[k]++;
+        ncw = k + 1 > ncw ? k + 1 : ncw;
+    }
+    for (int k = 0; k < ncw; ++k)
+        if (nu[k] == 0) return fail(PE_ERR_INVALID, "codeword %d has no layer (codewords must be numbered 0 .. n-1)", k);
+    if (nSC == 0) return ok();
+    if (!check_ptrs(C, Y_rx, noise_inv)) return fail(PE_ERR_INVALID, "null pointer");
+    s = check_align({{C, 16, "C"}, {Y_rx, 16, "Y_rx"}, {noise_inv, 4, "noise_inv"}});
+    if (s != PE_OK) return s;
+    const long long per_layer = (long long)nSC * nSym;  // symbols per layer
+    pe::CwLayer layers[8] = {};
+    for (int k = 0; k < ncw; ++k) {
+        const long long G = per_layer * nu[k] * qm;
+        if (G >= (1ll << 31)) return fail(PE_ERR_UNSUPPORTED, "codeword %d: %lld bits >= 2^31", k, G);
+        if (!cw->llr[k]) return fail(PE_ERR_INVALID, "codeword %d: null llr", k);
+        if (!pe::is_aligned(cw->llr[k], pe::llr_align(nPamBits, f16)))
+            return fail(PE_ERR_INVALID, "codeword %d: llr must be %zu-byte aligned", k, pe::llr_align(nPamBits, f16));
+        if (cw->c_seq[k] && !pe::is_aligned(cw->c_seq[k], 4))
+            return fail(PE_ERR_INVALID, "codeword %d: c_seq must be 4-byte aligned", k);
+        const int C_blocks = cw->nCodeBlocks[k];
+        if (C_blocks < 0 || C_blocks > per_layer)
+            return fail(PE_ERR_INVALID, "codeword %d: nCodeBlocks = %d outside [0, nSC nSym = %lld]", k, C_blocks,
+                        per_layer);
+    }
+    for (int l = 0; l < nLayer; ++l) {
+        const int k = cw->layer_cw[l];
+        pe::CwLayer &r = layers[l];
+        r.out = (unsigned long long)(uintptr_t)cw->llr[k];
+        r.cseq = (unsigned long long)(uintptr_t)cw->c_seq[k];
+        r.nu_v = (uint32_t)nu[k] | ((uint32_t)rank[l] << 8);
+        const int C_blocks = cw->nCodeBlocks[k];
+        if (C_blocks > 0) {  // 38.212 5.4.2.1: r < C - mod(N, C) get floor(N / C) symbols per layer, the rest one more
+            const long long N = per_layer;
+            r.e_lo = (uint32_t)((long long)nu[k] * qm * (N / C_blocks));
+            r.r0 = (uint32_t)(C_blocks - N % C_blocks);
+        }
+    }
+    hipError_t e = pe::launch_coef_apply_demap_cw(reinterpret_cast<const float2 *>(C),
+                                                  reinterpret_cast<const float2 *>(Y_rx), noise_inv, layers, nSC,
+                                                  nLayer, nSym, nAnt, nPamBits, (int)prec, f16 ? 1 : 0,
+                                                  (hipStream_t)stream);
+    return from_hip(e, "coef_apply_demap_cw launch");
+}
+
 pe_status pe_bfp_decompress(const uint8_t *in, pe_c64 *Y_rx, int nPrb, int nSym, int nAnt, int iqWidth,
                             size_t row_stride, float scale, void *stream) {
     if (nPrb < 0) return fail(PE_ERR_INVALID, "nPrb = %d < 0", nPrb);

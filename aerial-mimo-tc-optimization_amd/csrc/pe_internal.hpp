@@ -104,6 +104,23 @@ hipError_t launch_mmse_apply_demap(const float2 *H, const float *sigma2, const f
                                    float *noise_inv, uint8_t *flags, void *llr, int nSC, int L, int S, int A,
                                    int nPamBits, int prec, bool unbiased, float thr, int llr_f16, int write_c,
                                    hipStream_t st);
+// Codeword output of the fused demapper (descrambling and the rate-recovery layout). Record of layer l, 32 bytes,
+// eight per launch: its codeword's LLR array and scrambling sequence, the codeword's layer count nu and the layer's
+// rank v in it, and the code-block split (blocks 0 .. r0 - 1 of e_lo bits, the rest of e_lo + nu Qm bits).
+struct CwLayer {
+    unsigned long long out;   // the codeword's LLR array (float or binary16)
+    unsigned long long cseq;  // c(0 ..) of the codeword, bit g % 32 of word g / 32; 0: no descrambling
+    uint32_t e_lo;            // 0: codeword order (no de-interleaving)
+    uint32_t nu_v;            // nu | v << 8
+    uint32_t r0;
+    uint32_t pad;
+};
+// TS 38.211 5.2.1 Gold sequence c(0 .. n_bits - 1) for c_init, packed (bit g % 32 of words[g / 32]).
+hipError_t launch_gold_sequence(uint32_t c_init, uint32_t *words, int n_bits, hipStream_t st);
+// Fused MFMA apply + demap (prec 1..4) with the LLRs descrambled and laid out per codeword.
+hipError_t launch_coef_apply_demap_cw(const float2 *C, const float2 *Y, const float *noise_inv,
+                                      const CwLayer (&layers)[8], int nSC, int L, int S, int A, int nPamBits, int prec,
+                                      int llr_f16, hipStream_t st);
 // Device address of the file-scope scratch array that pe_mmse_apply_demap_noalloc uses as its coefficient
 // workspace (PE_NOALLOC_SCRATCH_BYTES), on the current device: looked up once per device with
 // hipGetSymbolAddress and kept; no allocation.

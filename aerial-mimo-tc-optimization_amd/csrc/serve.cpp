@@ -86,17 +86,23 @@ struct JobBuf {
 // Online batch-latency model for deadline-aware admission. The latencies (launch to observed completion) of
 // the last 16 completed batches, which already include the sharing of the GPU with the batches in flight at
 // the time, are fitted as latency = a + b * subcarriers by a Theil-Sen line (b: median of the pairwise
-// slopes, a: median of latency - b * subcarriers), robust to the occasional batch delayed by the host; a
-// batch of n subcarriers is predicted to take a + b n plus a margin of six median absolute residuals. Until
-// two batches have completed nothing is predicted late.
+// slopes, a: median of latency - b * subcarriers), robust to the occasional batch delayed by the host. A batch
+// of n subcarriers is predicted to take a + b n plus a margin calibrated from observed lateness: every served job
+// that completes after its deadline raises the margin by kUp, every one on time lowers it by kDown (floor 0). The
+// margin settles where the late share of served jobs is kDown / (kUp + kDown) = 0.5 %: under overload it grows
+// until the served jobs' p99 is inside the deadline; when nothing is late it falls to zero, so below capacity a
+// job is shed only when the fit itself puts it past the deadline. Until two batches have completed nothing is
+// predicted late.
 struct LatencyModel {
     static constexpr size_t kWindow = 16;
+    static constexpr double kUp = 40e-6, kDown = 40e-6 / 199.0;  // seconds per late / on-time job
     double iso = 0.0;  // one-job batch latency at warm-up (seconds), reported
     std::deque<std::pair<double, double>> recent;  // (subcarriers, latency)
     double a = 0.0, b = 0.0, margin = 0.0;
     bool ready = false;
     double fit(double n_sc) const { return ready ? a + b * n_sc : 0.0; }
     double predict(double n_sc) const { return ready ? a + b * n_sc + margin : 0.0; }
+    void outcome(bool late, double cap) { margin = std::min(cap, std::max(0.0, margin + (late ? kUp : -kDown))); }
     static double median(std::vector<double> &v) {
         std::nth_element(v.begin(), v.begin() + v.size() / 2, v.end());
         return v[v.size() / 2];
@@ -115,9 +121,6 @@ struct LatencyModel {
         v.clear();
         for (const auto &r : recent) v.push_back(r.second - b * r.first);
         a = std::max(0.0, median(v));
-        v.clear();
-        for (const auto &r : recent) v.push_back(std::fabs(r.second - (a + b * r.first)));
-        margin = 6.0 * median(v);
         ready = true;
     }
 };
@@ -350,6 +353,7 @@ extern "C" pe_status pe_serve_run(const pe_serve_config *cfg, pe_serve_report *r
             if (first_batches.size() < 5) first_batches.push_back(t - batch_t0[k]);
             for (int j : batch_jobs[k]) {
                 lat[j] = t - arrival[j];
+                model.outcome(lat[j] > cfg->deadline_s, cfg->deadline_s);
                 arena.release(buf[j].off, buf[j].bytes);
                 ++finished;
             }
@@ -421,7 +425,7 @@ extern "C" pe_status pe_serve_run(const pe_serve_config *cfg, pe_serve_report *r
     if (status != PE_OK) return serve_fail(status, err);
 
     std::vector<double> served;
-    int missed = shed;
+    int missed = shed, late = 0;
     double sum = 0.0, mx = 0.0;
     long sc_done = 0;
     for (int j = 0; j < nj; ++j) {
@@ -431,7 +435,10 @@ extern "C" pe_status pe_serve_run(const pe_serve_config *cfg, pe_serve_report *r
         served.push_back(lat[j]);
         sum += lat[j];
         mx = std::max(mx, lat[j]);
-        if (lat[j] > cfg->deadline_s) ++missed;
+        if (lat[j] > cfg->deadline_s) {
+            ++missed;
+            ++late;
+        }
         sc_done += pop[cellof[j]].nSC;
     }
     rep->n_jobs = nj;
@@ -455,6 +462,8 @@ extern "C" pe_status pe_serve_run(const pe_serve_config *cfg, pe_serve_report *r
     for (double v : first_batches) fb += v;
     rep->contention_index = first_batches.empty() || model.iso <= 0 ? 0.0 : fb / first_batches.size() / model.iso;
     rep->model_s_per_sc = model.b;
+    rep->n_late = late;
+    rep->model_margin_s = model.margin;
     pe::set_error("");
     return PE_OK;
 }

@@ -53,6 +53,8 @@ SIGNATURES = {
     "pe_coef_apply_demap_c16": ([_P, _P, _P, _P, _P, _I, _I, _I, _I, _I, _U, _P], _I),
     "pe_mmse_apply_demap": ([_P, _P, _P, _P, _P, _P, _P, _I, _I, _I, _I, _I, _I, _U, _F, _P], _I),
     "pe_mmse_apply_demap_noalloc": ([_P, _P, _P, _P, _P, _P, _I, _I, _I, _I, _I, _I, _U, _F, _P], _I),
+    "pe_gold_sequence": ([_U, _P, _I, _P], _I),
+    "pe_coef_apply_demap_cw": ([_P, _P, _P, _P, _I, _I, _I, _I, _I, _I, _U, _P], _I),
     "pe_slot_plan_create": ([_P, _I, _U, _P], _I),
     "pe_slot_plan_update": ([_P, _P, _I, _P], _I),
     "pe_slot_plan_launch": ([_P, _P], _I),
@@ -336,6 +338,60 @@ def mmse_apply_demap(H, sigma2, Y, n_pam_bits, prec="f16", unbiased=True, cond_t
     return out, C, noise_inv, flags
 
 
+class CwLayout(ctypes.Structure):
+    """pe_cw_layout of include/pusch_eq.h."""
+    _fields_ = [("layer_cw", _I * 8), ("llr", _P * 8), ("c_seq", _P * 8), ("nCodeBlocks", _I * 8)]
+
+
+def gold_sequence(c_init, n_bits, out=None, stream=None):
+    """TS 38.211 5.2.1 sequence c(0 .. n_bits - 1) for c_init (pe_gold_sequence): int32 tensor of ceil(n_bits / 32)
+    words, bit g % 32 of word g / 32 = c(g)."""
+    import torch
+    nw = (int(n_bits) + 31) // 32
+    if out is None:
+        out = torch.empty(nw, dtype=torch.int32, device=torch.device("cuda", torch.cuda.current_device()))
+    _need(out, "out", 4, min_numel=nw)
+    _same_device(out)
+    _check(lib().pe_gold_sequence(int(c_init) & 0xFFFFFFFF, _ptr(out), int(n_bits), _stream(stream)))
+    return out
+
+
+def coef_apply_demap_cw(C, Y, noise_inv, n_pam_bits, layer_cw, c_seqs=None, n_code_blocks=None, prec="f16",
+                        llr_f16=False, outs=None, stream=None):
+    """Fused apply + demap with the LLRs descrambled and laid out per codeword (pe_coef_apply_demap_cw). layer_cw:
+    codeword of each layer; c_seqs: per codeword an int32 word tensor from gold_sequence, or None; n_code_blocks:
+    per codeword 0 (codeword order) or the number of code blocks to de-interleave. Returns the list of per-codeword
+    LLR tensors (float32, or float16 with llr_f16), G_k = nSC nSym nu_k 2q values each."""
+    import torch
+    q = _q(n_pam_bits)
+    nSC, L, A = _need(C, "C", torch.complex64, (None, None, None)).shape
+    S = _need(Y, "Y", torch.complex64, (nSC, None, A)).shape[1]
+    _need(noise_inv, "noise_inv", torch.float32, (nSC, L))
+    if len(layer_cw) != L:
+        raise ValueError("layer_cw has %d entries for %d layers" % (len(layer_cw), L))
+    ncw = max(layer_cw) + 1
+    nu = [list(layer_cw).count(k) for k in range(ncw)]
+    ldt = torch.float16 if llr_f16 else torch.float32
+    G = [nSC * S * nu[k] * 2 * q for k in range(ncw)]
+    if outs is None:
+        outs = [torch.empty(G[k], dtype=ldt, device=C.device) for k in range(ncw)]
+    for k in range(ncw):
+        _need(outs[k], "outs[%d]" % k, ldt, (G[k],))
+        if c_seqs is not None and c_seqs[k] is not None:
+            _need(c_seqs[k], "c_seqs[%d]" % k, 4, min_numel=(G[k] + 31) // 32)
+    _same_device(C, Y, noise_inv, *outs, *[c for c in (c_seqs or []) if c is not None])
+    lay = CwLayout()
+    for l in range(L):
+        lay.layer_cw[l] = int(layer_cw[l])
+    for k in range(ncw):
+        lay.llr[k] = outs[k].data_ptr()
+        lay.c_seq[k] = c_seqs[k].data_ptr() if c_seqs is not None and c_seqs[k] is not None else None
+        lay.nCodeBlocks[k] = int(n_code_blocks[k]) if n_code_blocks is not None else 0
+    _check(lib().pe_coef_apply_demap_cw(_ptr(C), _ptr(Y), _ptr(noise_inv), ctypes.byref(lay), nSC, L, S, A, q,
+                                        _prec(prec), LLR_F16 if llr_f16 else 0, _stream(stream)))
+    return outs
+
+
 NOALLOC_SCRATCH_BYTES = 32 << 20
 
 
@@ -598,7 +654,8 @@ class ServeReport(ctypes.Structure):
                 ("miss_rate", ctypes.c_double), ("jobs_per_s", ctypes.c_double),
                 ("subcarriers_per_s", ctypes.c_double), ("mean_batch_jobs", ctypes.c_double),
                 ("offered_rate_hz", ctypes.c_double), ("iso_job_s", ctypes.c_double),
-                ("contention_index", ctypes.c_double), ("model_s_per_sc", ctypes.c_double)]
+                ("contention_index", ctypes.c_double), ("model_s_per_sc", ctypes.c_double), ("n_late", _I),
+                ("model_margin_s", ctypes.c_double)]
 
 
 def serve_run(cells, n_jobs, rate_hz, deadline_s=0.5e-3, hbm_budget=1 << 30, max_batch=64, max_inflight=2,

@@ -220,6 +220,37 @@ def draw_cells(n, seed, dev, prec="f16"):
     return cells
 
 
+def codeword_bench(dev, C, Y, ni, steps=50):
+    """Descrambling and the rate-recovery layout in the fused epilogue: the 32-cell batch as one allocation of
+    50,688 subcarriers carrying one 8-layer codeword (G = 35.7 M bits), LLRs descrambled with its Gold sequence
+    and written in codeword order, or de-interleaved per code block (C = ceil(G / 8448): LDPC base graph 1 blocks),
+    against the same launch writing the per-symbol layout. LLR bytes are the same; the epilogue also reads one bit
+    of the sequence per LLR."""
+    import torch
+    import pusch_eq as pe
+    nsc = C.shape[0]
+    G = nsc * S * L * 2 * Q
+    c = pe.gold_sequence((20000 << 15) | 1, G)  # n_RNTI = 20000, n_ID = 1
+    ncb = -(-G // 8448)
+    outs = [torch.empty(G, dtype=torch.float32, device=dev)]
+    llr = torch.empty((nsc, L, S, 2 * Q), dtype=torch.float32, device=dev)
+    runs = {"per_symbol": lambda: pe.coef_apply_demap(C, Y, ni, Q, prec="f16", out=llr),
+            "codeword_order": lambda: pe.coef_apply_demap_cw(C, Y, ni, Q, [0] * L, [c], [0], outs=outs),
+            "deinterleaved": lambda: pe.coef_apply_demap_cw(C, Y, ni, Q, [0] * L, [c], [ncb], outs=outs)}
+    base_bytes = nsc * (L * A * 8 + S * A * 8 + L * S * 2 * Q * 4 + L * 4)
+    res = {}
+    for name, fn in runs.items():
+        el, _ = time_launches(lambda: fn(), [()], steps, 5)
+        by = base_bytes + (0 if name == "per_symbol" else G // 8)
+        res[name] = {"ms_per_launch": el / steps * 1e3, "bytes_per_launch": by, "hbm_gbs": by / (el / steps) / 1e9}
+    gel, _ = time_launches(lambda: pe.gold_sequence((20000 << 15) | 1, G, out=c), [()], 20, 2)
+    res["gold_sequence_ms"] = gel / 20 * 1e3
+    res["workload"] = ("%d subcarriers x 8 layers x 11 symbols x 64 antennas, 256-QAM, f16 MFMA, FP32 LLRs; one "
+                       "codeword of G = %d bits, %d code blocks when de-interleaved; the sequence (one bit per LLR, "
+                       "%d bytes) is made once per UE and reused every slot" % (nsc, G, ncb, G // 8))
+    return res
+
+
 def slot_bytes(cells):
     """Algorithmic HBM bytes of one slot: MMSE (H, sigma2 in; C, noise_inv, flags out) + fused apply/demap
     (C, Y_rx, noise_inv in; LLR out)."""
@@ -708,6 +739,8 @@ def main():
     ap.add_argument("--headline-only", action="store_true",
                     help="only the timed headline launches (fused apply + demap over the 32-cell batches), for "
                          "profiling")
+    ap.add_argument("--codeword-only", action="store_true",
+                    help="only the descrambling / codeword-layout launches over the first 32-cell batch, for profiling")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run 5 plain 256 MiB device copies (scale reference for the HBM byte counters)")
     args = ap.parse_args()
@@ -823,6 +856,13 @@ def main():
         print(json.dumps({"subcarriers": nsc, "ms_per_launch": per_launch * 1e3, "hbm_gbs": achieved_gbs,
                           "bytes_per_launch": bytes_launch, "steps": args.steps, "warmup": args.warmup}), flush=True)
         return
+    if args.codeword_only:
+        print(json.dumps(codeword_bench(dev, sets[0]["C"], sets[0]["Y"], sets[0]["ni"], 10)), flush=True)
+        return
+    log("descrambling + codeword layout in the fused epilogue")
+    codeword = codeword_bench(dev, sets[0]["C"], sets[0]["Y"], sets[0]["ni"])
+    log("per-symbol %.1f us, codeword order %.1f us, de-interleaved %.1f us" % tuple(
+        codeword[k]["ms_per_launch"] * 1e3 for k in ("per_symbol", "codeword_order", "deinterleaved")))
 
     # per-launch distribution (events between back-to-back launches, same alternation), cold = from HBM
     evs = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps + 1)]
@@ -1001,6 +1041,7 @@ def main():
         "plain_f32": {"value": nsc / per32, "ms_per_step": per32 * 1e3,
                       "hbm_gbs": bytes_launch / per32 / 1e9},
         "f16_split": {"value": nsc / persp, "ms_per_step": persp * 1e3, "hbm_gbs": bytes_launch / persp / 1e9},
+        "llr_codeword": codeword,
         "llr_f16_wire": {"value": nsc / per16, "ms_per_step": per16 * 1e3, "hbm_gbs": bytes16 / per16 / 1e9,
                          "bytes_per_launch": bytes16,
                          "workload": "same launch, LLRs written as binary16 (pe_coef_apply_demap_f16)"},

@@ -166,6 +166,43 @@ pe_status pe_coef_apply_demap_f16(const pe_c64 *C, const pe_c64 *Y_rx, const flo
                                   int nSC, int nLayer, int nSym, int nAnt, int nPamBits, pe_prec prec,
                                   void *stream);
 
+/* ---- descrambling and the rate-recovery layout of the LLRs ------------------------------------------------ */
+
+/* The steps after the reference's demapper output ([b0_I, b0_Q, b1_I, ...] per symbol, aerial_llr_bench_v2.cu:117,
+   221-227), which the reference does not implement; fixed by the 3GPP specifications:
+   pe_gold_sequence: the TS 38.211 5.2.1 pseudo-random sequence c(0 .. n_bits - 1) for c_init (PUSCH: n_RNTI 2^15
+   + n_ID, 6.3.1.1), c(n) = x1(n + 1600) ^ x2(n + 1600), packed: bit g % 32 of c_words[g / 32] = c(g); the unused
+   bits of the last word are zero. 1 <= n_bits < 2^30; c_words 4-byte aligned. It depends on c_init only: make it
+   once per UE and reuse it every slot. */
+pe_status pe_gold_sequence(uint32_t c_init, uint32_t *c_words, int n_bits, void *stream);
+
+/* Where pe_coef_apply_demap_cw puts the LLRs. Codeword k holds the layers l with layer_cw[l] == k (layer mapping of
+   TS 38.211 6.3.1.3: its nu_k layers, in increasing l, carry codeword symbols m = nu_k i + v at layer symbol i,
+   v = the layer's rank among them), codewords numbered 0 .. nCodewords - 1 with every number used. Resource
+   elements are taken frequency first (6.3.1.6): layer symbol i = s nSC + sc for data symbol s and subcarrier sc
+   of the nSC-subcarrier allocation. Bit i of codeword symbol m is codeword bit g = m Qm + i (Qm = 2 nPamBits),
+   G_k = nSC nSym nu_k Qm bits per codeword. */
+typedef struct {
+    int layer_cw[8];           /* codeword of each of the nLayer layers */
+    void *llr[8];              /* per codeword: G_k LLRs (float, or binary16 with PE_LLR_F16), aligned as the llr
+                                  of pe_coef_apply_demap / pe_coef_apply_demap_f16 */
+    const uint32_t *c_seq[8];  /* per codeword: c(0 .. G_k - 1) as pe_gold_sequence packs it (4-byte aligned), or
+                                  NULL: no descrambling */
+    int nCodeBlocks[8];        /* per codeword: 0 = codeword order (LLR of bit g at g); C >= 1 = the codeword's C
+                                  code blocks of TS 38.212 5.4.2.1 (E_r = nu Qm floor(G / (nu Qm C)) for
+                                  r < C - mod(G / (nu Qm), C), else nu Qm ceil(G / (nu Qm C))), each bit de-interleaved
+                                  (5.4.2.2 undone: bit t = i + j Qm of block r at e-index i E_r/Qm + j), blocks
+                                  concatenated: the order rate recovery consumes. 1 <= C <= nSC nSym. */
+} pe_cw_layout;
+
+/* pe_coef_apply_demap (MFMA precisions) with each LLR negated where c(g) = 1 (descrambling, 6.3.1.1 undone; the
+   LLR sign convention is positive = bit 0) and stored at its codeword position as pe_cw_layout says. The values are
+   those of pe_coef_apply_demap (or, with PE_LLR_F16, pe_coef_apply_demap_f16) for the same inputs, sign-flipped and
+   moved: no other arithmetic. G_k < 2^31. flags: PE_LLR_F16. */
+pe_status pe_coef_apply_demap_cw(const pe_c64 *C, const pe_c64 *Y_rx, const float *noise_inv, const pe_cw_layout *cw,
+                                 int nSC, int nLayer, int nSym, int nAnt, int nPamBits, pe_prec prec, unsigned flags,
+                                 void *stream);
+
 /* ---- coefficients in the binary16 operand format ---------------------------------------------------- */
 
 /* No reference counterpart; the reference names this as future work ("FP16 native data path: eliminate the
@@ -342,7 +379,9 @@ typedef struct {
     int max_inflight;          /* batches in flight, one stream each, 1..8 */
     int shed_late;             /* 0: serve every job; 1: drop jobs whose deadline passed before launch; 2:
                                   deadline-aware admission, drop jobs whose predicted completion (age + predicted
-                                  batch latency, online model) is past the deadline. Dropped jobs count as misses. */
+                                  batch latency: an online latency fit plus a margin calibrated so that about 0.5 %
+                                  of served jobs complete late) is past the deadline. Dropped jobs count as
+                                  misses. */
     int use_graph;             /* nonzero: each stream replays one captured slot graph; zero: eager launches,
                                   with the coefficients formed on chip from solve records (as with
                                   PE_SLOT_SOLVE_RECORDS) where the cell shape allows it */
@@ -367,6 +406,9 @@ typedef struct {
     double contention_index;   /* mean latency of the first 5 batches / iso_job_s: the online contention
                                   indicator of pmu_contention.cu:230-245 */
     double model_s_per_sc;     /* final per-subcarrier slope of the batch-latency model (shed_late = 2) */
+    int n_late;                /* served jobs that completed after their deadline (n_missed = n_late + n_shed) */
+    double model_margin_s;     /* final admission margin of the latency model (shed_late = 2), calibrated from
+                                  observed lateness */
 } pe_serve_report;
 
 /* Runs the serving loop to completion on the current device (blocking). Allocates its arena, streams,

@@ -870,3 +870,84 @@ void po_freq_interp_f64(const double *W, const double *X, double *Hf, int nBatch
             }
     }
 }
+
+/* ------------------------------------------------------------------------------------------------------
+ * Descrambling and rate-recovery layout of the demapper output. The reference hands its binary16 LLRs to the
+ * decoder in the per-symbol order [b0_I, b0_Q, b1_I, ...] (aerial_llr_bench_v2.cu:117,221-227) and stops there;
+ * the receiver steps that follow are fixed by the 3GPP specifications, restated here bit by bit:
+ *   TS 38.211 5.2.1   pseudo-random (Gold) sequence c(n) = (x1(n + Nc) + x2(n + Nc)) mod 2, Nc = 1600,
+ *                     x1(n + 31) = (x1(n + 3) + x1(n)) mod 2,  x1(0) = 1, x1(1..30) = 0,
+ *                     x2(n + 31) = (x2(n + 3) + x2(n + 2) + x2(n + 1) + x2(n)) mod 2,  x2(i) = bit i of c_init;
+ *   TS 38.211 6.3.1.1 scrambling b~(i) = (b(i) + c(i)) mod 2 over the codeword's bits (c_init = n_RNTI 2^15 +
+ *                     n_ID for PUSCH), so the receiver negates the LLR of bit i where c(i) = 1;
+ *   TS 38.211 6.3.1.2-6.3.1.6 modulation (Qm bits per symbol, bit i of the symbol = codeword bit m Qm + i),
+ *                     layer mapping of one codeword onto nu layers (layer v carries codeword symbol m = nu i + v)
+ *                     and mapping to resource elements frequency first (layer symbol i = s nSC + sc over the
+ *                     allocation's nSC subcarriers and data symbols s);
+ *   TS 38.212 5.4.2.1 rate-matching output length of code block r of C: E_r = nu Qm floor(G / (nu Qm C)) for
+ *                     r < C - mod(G / (nu Qm), C), else nu Qm ceil(G / (nu Qm C)); blocks concatenated (5.5);
+ *   TS 38.212 5.4.2.2 bit interleaving f(i + j Qm) = e(i E/Qm + j), undone here: e(i E/Qm + j) = f(i + j Qm).
+ * ------------------------------------------------------------------------------------------------------ */
+
+/* c(0 .. n_bits - 1), one byte per bit, by running both registers bit by bit from their initial states. */
+void po_gold_sequence(uint32_t c_init, int n_bits, uint8_t *c) {
+    const int NC = 1600, len = n_bits + NC + 31;
+    uint8_t *x1 = (uint8_t *)calloc((size_t)len, 1), *x2 = (uint8_t *)calloc((size_t)len, 1);
+    x1[0] = 1;
+    for (int i = 0; i < 31; i++) x2[i] = (uint8_t)((c_init >> i) & 1u);
+    for (int n = 0; n + 31 < len; n++) {
+        x1[n + 31] = (uint8_t)((x1[n + 3] + x1[n]) & 1);
+        x2[n + 31] = (uint8_t)((x2[n + 3] + x2[n + 2] + x2[n + 1] + x2[n]) & 1);
+    }
+    for (int n = 0; n < n_bits; n++) c[n] = (uint8_t)((x1[n + NC] + x2[n + NC]) & 1);
+    free(x1);
+    free(x2);
+}
+
+/* E_r of the C code blocks of a codeword of G bits on nu layers with Qm bits per symbol (38.212 5.4.2.1, all C
+ * blocks scheduled). Returns 0, or -1 when C is out of range (C < 1 or more blocks than symbols per layer). */
+int po_cw_block_sizes(int G, int nu, int qm, int C, int *E) {
+    const int per = nu * qm, N = G / per;
+    if (C < 1 || C > N || G % per) return -1;
+    for (int r = 0; r < C; r++) E[r] = (r < C - N % C) ? per * (N / C) : per * ((N + C - 1) / C);
+    return 0;
+}
+
+/* The demapper's LLRs llr[sc][l][s][2 nPamBits] moved into per-codeword arrays: codeword k holds the layers l with
+ * layer_cw[l] == k (their rank v in increasing l), G_k = nSC S nu_k Qm LLRs. Codeword bit g = m Qm + i with
+ * m = (s nSC + sc) nu_k + v is descrambled (negated where c_k[g] = 1; c_k == NULL: no descrambling) and stored at
+ * g (nCodeBlocks[k] == 0: codeword order) or, with C = nCodeBlocks[k] blocks, de-interleaved within its block. */
+void po_cw_layout(const float *llr, int nSC, int L, int S, int nPamBits, const int *layer_cw, int nCodewords,
+                  const uint8_t *const *c, const int *nCodeBlocks, float *const *out) {
+    const int qm = 2 * nPamBits;
+    int nu[8] = {0}, v[8], start[8][64], E[8][64];
+    for (int l = 0; l < L; l++) v[l] = nu[layer_cw[l]]++;
+    for (int k = 0; k < nCodewords; k++) {
+        const int G = nSC * S * nu[k] * qm, C = nCodeBlocks[k];
+        if (C > 0) {
+            po_cw_block_sizes(G, nu[k], qm, C, E[k]);
+            start[k][0] = 0;
+            for (int r = 1; r < C; r++) start[k][r] = start[k][r - 1] + E[k][r - 1];
+        }
+    }
+    for (int sc = 0; sc < nSC; sc++)
+        for (int l = 0; l < L; l++) {
+            const int k = layer_cw[l];
+            for (int s = 0; s < S; s++) {
+                const int m = (s * nSC + sc) * nu[k] + v[l];
+                for (int i = 0; i < qm; i++) {
+                    const int g = m * qm + i;
+                    float x = llr[(((size_t)sc * L + l) * S + s) * qm + i];
+                    if (c && c[k] && c[k][g]) x = -x;
+                    int pos = g;
+                    if (nCodeBlocks[k] > 0) {
+                        int r = 0;
+                        while (r + 1 < nCodeBlocks[k] && g >= start[k][r + 1]) r++;
+                        const int t = g - start[k][r];
+                        pos = start[k][r] + (t % qm) * (E[k][r] / qm) + t / qm;
+                    }
+                    out[k][pos] = x;
+                }
+            }
+        }
+}

@@ -79,4 +79,9 @@ void po_freq_interp_f32(const float *W, const float *X, float *Hf, int nBatch, i
 int po_freq_interp_f32_omp(const float *W, const float *X, float *Hf, int nBatch, int M, int K, int NC,
                            int nthreads);
 void po_freq_interp_f64(const double *W, const double *X, double *Hf, int nBatch, int M, int K, int NC);
+/* -- descrambling and rate-recovery layout of the demapper output (TS 38.211 5.2.1, 6.3.1; TS 38.212 5.4.2) -- */
+void po_gold_sequence(uint32_t c_init, int n_bits, uint8_t *c);
+int po_cw_block_sizes(int G, int nu, int qm, int C, int *E);
+void po_cw_layout(const float *llr, int nSC, int L, int S, int nPamBits, const int *layer_cw, int nCodewords,
+                  const uint8_t *const *c, const int *nCodeBlocks, float *const *out);
 #endif

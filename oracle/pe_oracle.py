@@ -58,6 +58,9 @@ def lib():
             "po_freq_interp_f32": ([P, P, P, I, I, I, I], None),
             "po_freq_interp_f32_omp": ([P, P, P, I, I, I, I, I], I),
             "po_freq_interp_f64": ([P, P, P, I, I, I, I], None),
+            "po_gold_sequence": ([U, I, P], None),
+            "po_cw_block_sizes": ([I, I, I, I, P], I),
+            "po_cw_layout": ([P, I, I, I, I, P, I, P, P, P], None),
         }
         for name, (args, res) in sig.items():
             fn = getattr(h, name)
@@ -473,3 +476,47 @@ def round_bf16_complex(z):
     """complex64 -> complex64 with real and imaginary parts rounded to bfloat16."""
     z = np.ascontiguousarray(z, dtype=np.complex64)
     return (round_bf16(z.real) + 1j * round_bf16(z.imag)).astype(np.complex64)
+
+
+# ---------------------------------------------------------------------------------------------------------
+# descrambling and rate-recovery layout (TS 38.211 5.2.1, 6.3.1; TS 38.212 5.4.2)
+# ---------------------------------------------------------------------------------------------------------
+def gold_sequence(c_init, n_bits):
+    """c(0 .. n_bits - 1) of TS 38.211 5.2.1 (uint8, one bit per entry)."""
+    out = np.empty(n_bits, dtype=np.uint8)
+    lib().po_gold_sequence(int(c_init) & 0xFFFFFFFF, int(n_bits), _p(out))
+    return out
+
+
+def cw_block_sizes(G, nu, qm, C):
+    """E_r of the C code blocks (TS 38.212 5.4.2.1); None when C is out of range."""
+    E = np.empty(max(int(C), 1), dtype=np.int32)
+    return None if lib().po_cw_block_sizes(int(G), int(nu), int(qm), int(C), _p(E)) else E[:C].copy()
+
+
+def cw_layout(llr, layer_cw, c_seqs=None, n_code_blocks=None):
+    """The demapper's LLRs [nSC, L, S, 2q] (float32) moved into per-codeword arrays: codeword k holds the layers
+    with layer_cw[l] == k; c_seqs[k] (uint8 bits or None) descrambles it; n_code_blocks[k] > 0 de-interleaves it per
+    code block. Returns a list of float32 arrays of G_k LLRs."""
+    llr = np.ascontiguousarray(llr, dtype=np.float32)
+    nSC, L, S, qm = llr.shape
+    lc = np.ascontiguousarray(layer_cw, dtype=np.int32)
+    ncw = int(lc.max()) + 1
+    nu = np.bincount(lc, minlength=ncw)
+    ncb = np.ascontiguousarray(n_code_blocks if n_code_blocks is not None else [0] * ncw, dtype=np.int32)
+    outs = [np.full(nSC * S * int(nu[k]) * qm, np.nan, dtype=np.float32) for k in range(ncw)]
+    cs = None
+    keep = []
+    if c_seqs is not None:
+        arr = []
+        for k in range(ncw):
+            if c_seqs[k] is None:
+                arr.append(None)
+            else:
+                a = np.ascontiguousarray(c_seqs[k], dtype=np.uint8)
+                keep.append(a)
+                arr.append(a.ctypes.data)
+        cs = (ctypes.c_void_p * ncw)(*arr)
+    op = (ctypes.c_void_p * ncw)(*[o.ctypes.data for o in outs])
+    lib().po_cw_layout(_p(llr), nSC, L, S, qm // 2, _p(lc), ncw, cs, _p(ncb), op)
+    return outs

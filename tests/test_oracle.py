@@ -563,3 +563,155 @@ def test_published_fixture_is_consistent():
     """The fixture's EVM in percent and in dB describe the same ratio (20 log10), and its shape is the headline's."""
     assert abs(20 * np.log10(PUBLISHED["coef_rigor_evm_pct"] / 100) - RIGOR_EVM_DB) < 0.05
     assert PUBLISHED["coef_rigor_shape"] == [NSC, L, S, A]
+
+
+# ---------------------------------------------------------------------------------------------------------
+# descrambling and rate-recovery layout (TS 38.211 5.2.1, 6.3.1; TS 38.212 5.4.2)
+# ---------------------------------------------------------------------------------------------------------
+F1 = (1 << 31) | (1 << 3) | 1                          # x1: t^31 + t^3 + 1
+F2 = (1 << 31) | (1 << 3) | (1 << 2) | (1 << 1) | 1    # x2: t^31 + t^3 + t^2 + t + 1
+
+
+def _gf2_mulmod(a, b, f):
+    """a * b mod f over GF(2), polynomials as Python ints."""
+    r = 0
+    while b:
+        if b & 1:
+            r ^= a
+        b >>= 1
+        a <<= 1
+        if a >> 31 & 1:
+            a ^= f
+    return r
+
+
+def _gf2_powmod(k, f):
+    r, base = 1, 2  # t^0, t^1
+    while k:
+        if k & 1:
+            r = _gf2_mulmod(r, base, f)
+        base = _gf2_mulmod(base, base, f)
+        k >>= 1
+    return r
+
+
+def _lfsr_term(k, f, init):
+    """x(k) of the linear recurrence with characteristic polynomial f (degree 31) from x(0..30) = bits of init:
+    x(k) = sum_i [t^k mod f]_i x(i) over GF(2) (the shift operator t acting on the sequence space)."""
+    return bin(_gf2_powmod(k, f) & init).count("1") & 1
+
+
+@pytest.mark.parametrize("c_init", [0, 1, 0x4000_0000, 0x7FFF_FFFF, (1234 << 15) + 567, 0x2468ACE])
+def test_gold_sequence_equals_the_polynomial_residue_route(po, c_init):
+    """c(n) = x1(n + 1600) + x2(n + 1600) with each term computed as sum_i [t^(n+1600) mod f]_i x(i) from the
+    initial values (x1: x1(0) = 1; x2: bit i of c_init), a route that shares nothing with the oracle's bit-serial
+    loop but the definition: pins the taps, Nc = 1600 and the c_init bit order."""
+    c = po.gold_sequence(c_init, 300)
+    idx = list(range(0, 40)) + [97, 128, 255, 299]
+    ref = [_lfsr_term(n + 1600, F1, 1) ^ _lfsr_term(n + 1600, F2, c_init & 0x7FFF_FFFF) for n in idx]
+    assert list(c[idx]) == ref
+
+
+def test_gold_sequence_recurrence_and_linearity(po):
+    """c obeys the degree-62 recurrence of f1 f2 (it is the sum of the two register sequences), c_init = 0 leaves
+    the x1 m-sequence alone (recurrence of f1), and c is affine in c_init: c(a ^ b) = c(a) ^ c(b) ^ c(0)."""
+    n = 4000
+    fprod = 0
+    for i in range(32):  # carry-less product f1 * f2
+        if F2 >> i & 1:
+            fprod ^= F1 << i
+    taps = [i for i in range(62) if fprod >> i & 1]
+    rng = np.random.default_rng(0)
+    for c_init in (0, 1, int(rng.integers(0, 2 ** 31)), 0x7FFF_FFFF):
+        c = po.gold_sequence(c_init, n).astype(np.int64)
+        lhs = c[62:]
+        rhs = np.zeros(n - 62, np.int64)
+        for t in taps:
+            rhs ^= c[t:n - 62 + t]
+        assert np.array_equal(lhs, rhs)
+    c0 = po.gold_sequence(0, n)
+    assert np.array_equal(c0[31:], c0[3:n - 28] ^ c0[:n - 31]) and c0.any()
+    for _ in range(5):
+        a, b = (int(x) for x in rng.integers(0, 2 ** 31, 2))
+        assert np.array_equal(po.gold_sequence(a ^ b, 999), po.gold_sequence(a, 999) ^ po.gold_sequence(b, 999) ^
+                              po.gold_sequence(0, 999))
+
+
+def test_code_block_sizes_known_answer(po):
+    """38.212 5.4.2.1 by hand: G = 2112 bits on 2 layers of 256-QAM (N = 132 symbols per layer) in C = 5 blocks:
+    floor(132/5) = 26, ceil = 27, mod(132, 5) = 2, so blocks 0..2 get 2*8*26 = 416 bits and blocks 3, 4 get 432."""
+    assert list(po.cw_block_sizes(2112, 2, 8, 5)) == [416, 416, 416, 432, 432]
+    assert list(po.cw_block_sizes(2112, 2, 8, 4)) == [528] * 4
+    assert po.cw_block_sizes(2112, 2, 8, 133) is None and po.cw_block_sizes(2112, 2, 8, 0) is None
+    for G, nu, qm, C in ((9504, 4, 6, 7), (288, 1, 2, 144), (1_368_576, 8, 8, 43)):
+        E = po.cw_block_sizes(G, nu, qm, C)
+        assert E.sum() == G and np.all(E % (nu * qm) == 0) and E.max() - E.min() <= nu * qm
+        assert np.all(np.diff(E) >= 0)
+
+
+def _interleave(e, qm):
+    """38.212 5.4.2.2 as written: for j < E/Qm, i < Qm: f(i + j Qm) = e(i E/Qm + j)."""
+    E = e.size
+    f = np.empty_like(e)
+    for j in range(E // qm):
+        for i in range(qm):
+            f[i + j * qm] = e[i * (E // qm) + j]
+    return f
+
+
+@pytest.mark.parametrize("q,nu,C", [(4, 2, 5), (3, 3, 7), (1, 1, 1), (2, 4, 3)])
+def test_deinterleave_inverts_the_bit_interleaver(po, q, nu, C):
+    """Per code block, applying the 38.212 interleaver to the oracle's de-interleaved output gives back the
+    codeword-order output (the f sequence)."""
+    rng = np.random.default_rng(q * 10 + nu)
+    nSC, S, qm = 12, 11, 2 * q
+    llr = rng.standard_normal((nSC, nu, S, qm)).astype(np.float32)
+    f = po.cw_layout(llr, [0] * nu)[0]
+    e = po.cw_layout(llr, [0] * nu, n_code_blocks=[C])[0]
+    E = po.cw_block_sizes(f.size, nu, qm, C)
+    starts = np.concatenate([[0], np.cumsum(E)[:-1]])
+    for r in range(C):
+        blk = slice(starts[r], starts[r] + E[r])
+        assert np.array_equal(_interleave(e[blk], qm), f[blk])
+
+
+@pytest.mark.parametrize("q,layer_cw", [(4, [0, 0, 0, 0]), (2, [0, 1, 0, 1, 2, 2, 2, 3]), (3, [0]), (1, [1, 0])])
+def test_descrambled_layout_recovers_the_codeword_bits(po, q, layer_cw):
+    """End to end against a transmitter written from TS 38.211 6.3.1: codeword bits b -> scrambling with c ->
+    modulation (Qm consecutive bits per symbol) -> layer mapping (codeword symbol m on layer m mod nu, layer symbol
+    m div nu) -> resource elements frequency first (layer symbol i at subcarrier i mod nSC of data symbol
+    i div nSC). The oracle's FP64 demapper on the noise-free symbols, then its codeword layout with descrambling,
+    returns LLRs whose signs are b (positive = 0) in codeword order, and de-interleaved per block they are the
+    de-interleaved b."""
+    rng = np.random.default_rng(q + 7 * len(layer_cw))
+    nSC, S, qm = 24, 5, 2 * q
+    L = len(layer_cw)
+    ncw = max(layer_cw) + 1
+    nu = np.bincount(layer_cw, minlength=ncw)
+    yeq = np.zeros((nSC, L, S), dtype=np.complex128)
+    bits, cs = [], []
+    for k in range(ncw):
+        G = nSC * S * nu[k] * qm
+        b = rng.integers(0, 2, G, dtype=np.uint8)
+        c = po.gold_sequence((1000 + k) << 15 | 77, G)
+        sym = po.qam_map((b ^ c).reshape(-1, qm), q)
+        layers = [l for l in range(L) if layer_cw[l] == k]
+        for m in range(sym.size):
+            i = m // nu[k]
+            yeq[i % nSC, layers[m % nu[k]], i // nSC] = sym[m]
+        bits.append(b)
+        cs.append(c)
+    llr = po.llr_demap_f64(yeq, np.ones((nSC, L)), q).astype(np.float32)
+    outs = po.cw_layout(llr, layer_cw, cs)
+    for k in range(ncw):
+        assert np.array_equal((outs[k] < 0).astype(np.uint8), bits[k]), "codeword %d" % k
+    outs_e = po.cw_layout(llr, layer_cw, cs, [3] * ncw)
+    for k in range(ncw):
+        E = po.cw_block_sizes(bits[k].size, nu[k], qm, 3)
+        starts = np.concatenate([[0], np.cumsum(E)[:-1]])
+        for r in range(3):
+            blk = slice(starts[r], starts[r] + E[r])
+            assert np.array_equal(_interleave((outs_e[k][blk] < 0).astype(np.uint8), qm), bits[k][blk])
+    # without descrambling the signs are the scrambled bits
+    raw = po.cw_layout(llr, layer_cw)
+    assert np.array_equal((raw[0] < 0).astype(np.uint8), bits[0] ^ cs[0])
