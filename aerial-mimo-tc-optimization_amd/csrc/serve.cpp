@@ -83,26 +83,16 @@ struct JobBuf {
     size_t off = 0, bytes = 0;
 };
 
-// Online batch-latency model for deadline-aware admission. The latencies (launch to observed completion) of
-// the last 16 completed batches, which already include the sharing of the GPU with the batches in flight at
-// the time, are fitted as latency = a + b * subcarriers by a Theil-Sen line (b: median of the pairwise
-// slopes, a: median of latency - b * subcarriers), robust to the occasional batch delayed by the host. A batch
-// of n subcarriers is predicted to take a + b n plus a margin calibrated from observed lateness: every served job
-// that completes after its deadline raises the margin by kUp, every one on time lowers it by kDown (floor 0). The
-// margin settles where the late share of served jobs is kDown / (kUp + kDown) = 0.5 %: under overload it grows
-// until the served jobs' p99 is inside the deadline; when nothing is late it falls to zero, so below capacity a
-// job is shed only when the fit itself puts it past the deadline. Until two batches have completed nothing is
-// predicted late.
-struct LatencyModel {
-    static constexpr size_t kWindow = 16;
-    static constexpr double kUp = 40e-6, kDown = 40e-6 / 199.0;  // seconds per late / on-time job
-    double iso = 0.0;  // one-job batch latency at warm-up (seconds), reported
+// Latency of a batch as a function of its size: the latencies (launch to observed completion) of the last 16
+// completed batches fitted as latency = a + b * subcarriers by a Theil-Sen line (b: median of the pairwise slopes,
+// a: median of latency - b * subcarriers), robust to the occasional batch delayed by the host. It predicts only
+// once 8 batches are in the window (a line through two or three batches of similar size extrapolates wildly).
+struct LatencyFit {
+    static constexpr size_t kWindow = 16, kMinPoints = 8;
     std::deque<std::pair<double, double>> recent;  // (subcarriers, latency)
-    double a = 0.0, b = 0.0, margin = 0.0;
+    double a = 0.0, b = 0.0;
     bool ready = false;
-    double fit(double n_sc) const { return ready ? a + b * n_sc : 0.0; }
-    double predict(double n_sc) const { return ready ? a + b * n_sc + margin : 0.0; }
-    void outcome(bool late, double cap) { margin = std::min(cap, std::max(0.0, margin + (late ? kUp : -kDown))); }
+    double at(double n_sc) const { return a + b * n_sc; }
     static double median(std::vector<double> &v) {
         std::nth_element(v.begin(), v.begin() + v.size() / 2, v.end());
         return v[v.size() / 2];
@@ -111,7 +101,7 @@ struct LatencyModel {
         if (n_sc <= 0) return;
         recent.emplace_back(n_sc, latency);
         if (recent.size() > kWindow) recent.pop_front();
-        if (recent.size() < 2) return;
+        if (recent.size() < kMinPoints) return;
         std::vector<double> v;
         for (size_t i = 0; i < recent.size(); ++i)
             for (size_t j = i + 1; j < recent.size(); ++j)
@@ -123,6 +113,41 @@ struct LatencyModel {
         a = std::max(0.0, median(v));
         ready = true;
     }
+};
+
+// Batch-latency model of deadline-aware admission. A batch's latency depends on its size and on how many other
+// batches share the GPU while it runs, so batches are fitted separately by the number of batches already in flight
+// when they were launched (and once pooled, the fallback while a class has too few batches). A batch is
+// predicted to take its class's fit plus, when other batches are in flight, a margin calibrated from observed
+// lateness once per completed batch: a batch with a job past its deadline raises the margin by kUp, a batch
+// without lowers it by kDown (floor 0, cap one deadline). The margin settles where kDown / (kUp + kDown) = 0.5 % of
+// batches hold a late job, so under overload the served jobs' tail stays inside the deadline; when nothing is late
+// it falls back to zero within about two hundred batches, so below capacity a job is shed only when the fit itself
+// puts it past the deadline. Measured alternatives: feedback per job instead of per batch let one late batch of
+// many jobs raise the margin to the cap (28 % shed at 90 % load); gating shedding on sustained lateness let the
+// margin reach the cap while disengaged and oscillate under overload. Until a fit is ready nothing is predicted
+// late.
+struct LatencyModel {
+    static constexpr int kClasses = 9;  // 0 .. 8 batches in flight at launch
+    static constexpr double kUp = 50e-6, kDown = 50e-6 / 199.0;  // seconds per batch with / without a late job
+    double iso = 0.0;  // one-job batch latency at warm-up (seconds), reported
+    LatencyFit by_load[kClasses], pooled;
+    double margin = 0.0;
+    const LatencyFit *fit_for(int inflight) const {
+        const LatencyFit &f = by_load[std::min(inflight, kClasses - 1)];
+        return f.ready ? &f : pooled.ready ? &pooled : nullptr;
+    }
+    // predicted latency of a batch of n_sc subcarriers launched with `inflight` other batches on the GPU
+    double predict(double n_sc, int inflight) const {
+        const LatencyFit *f = fit_for(inflight);
+        return f ? f->at(n_sc) + (inflight > 0 ? margin : 0.0) : 0.0;
+    }
+    void observe(double latency, double n_sc, int inflight_at_launch) {
+        by_load[std::min(inflight_at_launch, kClasses - 1)].observe(latency, n_sc);
+        pooled.observe(latency, n_sc);
+    }
+    void outcome(bool late, double cap) { margin = std::min(cap, std::max(0.0, margin + (late ? kUp : -kDown))); }
+    double slope() const { return pooled.b; }
 };
 
 // Coefficient bytes of a job: complex64, or binary16 pairs plus one exponent per subcarrier.
@@ -325,6 +350,7 @@ extern "C" pe_status pe_serve_run(const pe_serve_config *cfg, pe_serve_report *r
     std::vector<std::vector<int>> batch_jobs(K);
     std::vector<bool> busy(K, false);
     std::vector<double> batch_t0(K, 0.0), batch_sc(K, 0.0);  // per stream: launch time, subcarriers
+    std::vector<int> batch_inflight(K, 0);                   // per stream: other batches in flight at launch
     std::vector<double> first_batches;  // latencies of the first 5 batches (contention indicator)
     std::deque<int> ready;
     int next = 0, finished = 0, shed = 0, batches = 0, max_inflight_seen = 0;
@@ -349,14 +375,16 @@ extern "C" pe_status pe_serve_run(const pe_serve_config *cfg, pe_serve_report *r
                 break;
             }
             const double t = now_s();
-            model.observe(t - batch_t0[k], batch_sc[k]);
+            model.observe(t - batch_t0[k], batch_sc[k], batch_inflight[k]);
             if (first_batches.size() < 5) first_batches.push_back(t - batch_t0[k]);
+            bool batch_late = false;
             for (int j : batch_jobs[k]) {
                 lat[j] = t - arrival[j];
-                model.outcome(lat[j] > cfg->deadline_s, cfg->deadline_s);
+                batch_late = batch_late || lat[j] > cfg->deadline_s;
                 arena.release(buf[j].off, buf[j].bytes);
                 ++finished;
             }
+            model.outcome(batch_late, cfg->deadline_s);
             batch_jobs[k].clear();
             busy[k] = false;
         }
@@ -371,20 +399,19 @@ extern "C" pe_status pe_serve_run(const pe_serve_config *cfg, pe_serve_report *r
             const double t_adm = now_s();
             cells.clear();
             double sc_batch = 0.0;
-            // deadline-aware admission predicts with the batch this launch will have if nothing is shed: the
-            // first max_batch_jobs jobs of the queue (a batch only grows while it is formed)
-            double pred = 0.0;
-            if (cfg->shed_late == 2) {
-                double sc_full = 0.0;
-                int m = 0;
+            // deadline-aware admission predicts with the batch this launch will have if nothing more is shed: the
+            // jobs admitted so far plus the queue's head up to max_batch_jobs (a batch only grows while it is
+            // formed; a shed job leaves it, and the prediction is made again for what remains)
+            int n_busy = 0;
+            for (int kk = 0; kk < K; ++kk) n_busy += busy[kk] ? 1 : 0;
+            auto prospective = [&]() {
+                double sc = sc_batch;
+                int m = (int)cells.size();
                 for (auto it = ready.begin(); it != ready.end() && m < cfg->max_batch_jobs; ++it, ++m)
-                    sc_full += pop[cellof[*it]].nSC;
-                // with no batch on the GPU the launch runs alone: the fit without the margin (this also lets an
-                // overly pessimistic model see new batches)
-                bool any_busy = false;
-                for (int kk = 0; kk < K; ++kk) any_busy = any_busy || busy[kk];
-                pred = any_busy ? model.predict(sc_full) : model.fit(sc_full);
-            }
+                    sc += pop[cellof[*it]].nSC;
+                return model.predict(sc, n_busy);  // no batch on the GPU: the fit without the margin
+            };
+            double pred = cfg->shed_late == 2 ? prospective() : 0.0;
             while (!ready.empty() && (int)cells.size() < cfg->max_batch_jobs) {
                 const int j = ready.front();
                 const double nsc_j = pop[cellof[j]].nSC;
@@ -393,6 +420,7 @@ extern "C" pe_status pe_serve_run(const pe_serve_config *cfg, pe_serve_report *r
                 if (late || predicted_late) {
                     ready.pop_front();
                     ++shed;
+                    if (predicted_late) pred = prospective();
                     continue;
                 }
                 JobBuf b;
@@ -410,6 +438,8 @@ extern "C" pe_status pe_serve_run(const pe_serve_config *cfg, pe_serve_report *r
             }
             batch_t0[k] = now_s();
             batch_sc[k] = sc_batch;
+            batch_inflight[k] = 0;
+            for (int kk = 0; kk < K; ++kk) batch_inflight[k] += busy[kk] ? 1 : 0;
             status = submit(k, cells, false);
             if (status != PE_OK) break;
             busy[k] = true;
@@ -461,7 +491,7 @@ extern "C" pe_status pe_serve_run(const pe_serve_config *cfg, pe_serve_report *r
     double fb = 0.0;
     for (double v : first_batches) fb += v;
     rep->contention_index = first_batches.empty() || model.iso <= 0 ? 0.0 : fb / first_batches.size() / model.iso;
-    rep->model_s_per_sc = model.b;
+    rep->model_s_per_sc = model.slope();
     rep->n_late = late;
     rep->model_margin_s = model.margin;
     pe::set_error("");

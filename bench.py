@@ -563,15 +563,16 @@ def serving_bench(dev, n_jobs):
         rep, _, _ = pe.serve_run(descs, n_jobs, f * capacity, seed=2, **kw)
         sweep.append({"load": f, "offered_jobs_per_s": f * capacity, "jobs_per_s": rep["jobs_per_s"],
                       "p50_ms": rep["p50_s"] * 1e3, "p99_ms": rep["p99_s"] * 1e3, "max_ms": rep["max_s"] * 1e3,
-                      "miss_rate": rep["miss_rate"], "mean_batch_jobs": rep["mean_batch_jobs"]})
-    # deadline-aware admission (shed jobs predicted to finish late) at and beyond capacity
+                      "miss_rate": rep["miss_rate"], "late": rep["n_late"], "mean_batch_jobs": rep["mean_batch_jobs"]})
+    # deadline-aware admission (shed jobs predicted to finish late, margin calibrated from observed lateness), same
+    # arrival streams
     admission = []
-    for f in (0.9, 1.0, 1.1, 1.2, 1.5):
+    for f in (0.1, 0.3, 0.5, 0.7, 0.9, 1.0, 1.1, 1.2, 1.5):
         rep, _, _ = pe.serve_run(descs, n_jobs, f * capacity, seed=2, **dict(kw, shed_late=2))
         admission.append({"load": f, "jobs_per_s": rep["jobs_per_s"], "p50_ms": rep["p50_s"] * 1e3,
                           "p99_ms": rep["p99_s"] * 1e3, "max_ms": rep["max_s"] * 1e3,
-                          "miss_rate": rep["miss_rate"], "shed": rep["n_shed"],
-                          "mean_batch_jobs": rep["mean_batch_jobs"]})
+                          "miss_rate": rep["miss_rate"], "late": rep["n_late"], "shed": rep["n_shed"],
+                          "final_margin_ms": rep["model_margin_s"] * 1e3, "mean_batch_jobs": rep["mean_batch_jobs"]})
     # plain FP32 path, one job at a time (per-cell launches: MMSE then fused FP32 apply/demap)
     torch.cuda.synchronize()
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
@@ -739,6 +740,7 @@ def main():
     ap.add_argument("--headline-only", action="store_true",
                     help="only the timed headline launches (fused apply + demap over the 32-cell batches), for "
                          "profiling")
+    ap.add_argument("--serving-only", action="store_true", help="only the serving-loop section, printed as JSON")
     ap.add_argument("--codeword-only", action="store_true",
                     help="only the descrambling / codeword-layout launches over the first 32-cell batch, for profiling")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -762,6 +764,9 @@ def main():
     import pusch_eq as pe
     dev = torch.device("cuda:0")
     pe.init(0)
+    if args.serving_only:
+        print(json.dumps(serving_bench(dev, args.serve_jobs)), flush=True)
+        return
     if args.multicell_only:
         print(json.dumps(multicell_bench(dev, 20)), flush=True)
         return

@@ -379,8 +379,8 @@ typedef struct {
     int max_inflight;          /* batches in flight, one stream each, 1..8 */
     int shed_late;             /* 0: serve every job; 1: drop jobs whose deadline passed before launch; 2:
                                   deadline-aware admission, drop jobs whose predicted completion (age + predicted
-                                  batch latency: an online latency fit plus a margin calibrated so that about 0.5 %
-                                  of served jobs complete late) is past the deadline. Dropped jobs count as
+                                  batch latency: online latency fits by GPU occupancy plus a margin calibrated so
+                                  that about 0.5 % of batches hold a late job) is past the deadline. Dropped jobs count as
                                   misses. */
     int use_graph;             /* nonzero: each stream replays one captured slot graph; zero: eager launches,
                                   with the coefficients formed on chip from solve records (as with
