@@ -14,8 +14,10 @@ Candidates (see pe_tune_load in include/pusch_eq.h):
                 workgroup {1, 2, 4, 8} x {one Gram round per wave, two, one with the antenna rows kept in
                 registers} (tile = rounds x waves x 64 / lanes <= 128 subcarriers) per batch bucket (single
                 cell, large batch)
-    slot:       MMSE (tile, waves, rows kept) in {(4,1,0) .. (32,8,1)} (24-cell plans; apply tile 8, used by the
-                serving loop's workgroups that stride over tiles)
+    solve:      tile of pe_mmse_apply_demap's solve kernel, (subcarriers, waves) in {(16,4) .. (128,8)}, per
+                (L, A) that takes solve records (L >= 4, 32 <= A <= 64), timed on the whole one-pass chain at
+                32 cells x 1584 subcarriers
+    slot:       MMSE (tile, waves, rows kept) in {(4,1,0) .. (32,8,1)} (24-cell plans)
 Timing: HIP events around 20 launches after 5 warm-up launches, alternating two buffer sets (for the apply
 their sum exceeds the 256 MB Infinity Cache, so launches stream from HBM as in production); single-cell MMSE
 launches are timed one at a time (median of 15).
@@ -178,24 +180,45 @@ def slot_cells(dev, n, seed):
     return cells, keep
 
 
+def tune_solve(pe, dev, L, A, base):
+    """Subcarriers and waves per workgroup of the one-pass chain's solve kernel: pe_mmse_apply_demap on 32 cells
+    of 1584 subcarriers, 11 symbols, 16-QAM, two input sets alternating."""
+    import torch
+    nsc, S = 32 * 1584, 11
+    sets = []
+    for k in range(2):
+        g = torch.Generator(device=dev).manual_seed(20 + k)
+        H = torch.randn((nsc, A, L), dtype=torch.complex64, device=dev, generator=g) * (1.0 / A) ** 0.5
+        s2 = torch.full((nsc,), 0.03, device=dev)
+        Y = torch.randn((nsc, S, A), dtype=torch.complex64, device=dev, generator=g)
+        sets.append((H, s2, Y))
+    res = []
+    for T, W in ((16, 4), (32, 4), (32, 8), (64, 4), (64, 8), (128, 4), (128, 8)):
+        with_table(pe, ["solve %d %d -1 %d %d" % (L, A, T, W)] + base)
+        t = timed(lambda H, s2, Y: pe.mmse_apply_demap(H, s2, Y, 2, prec="f16"), sets)
+        res.append({"T": T, "W": W, "us": t * 1e6})
+    del sets
+    torch.cuda.empty_cache()
+    best = min(res, key=lambda r: r["us"])
+    return "solve %d %d -1 %d %d" % (L, A, best["T"], best["W"]), best, res
+
+
 def tune_slot(pe, dev, base):
-    """Subcarriers per MMSE tile and per apply tile of the multi-cell executor, on two alternating 24-cell
-    slot plans replayed as graphs."""
+    """Subcarriers and waves per MMSE tile of the multi-cell executor, on two alternating 24-cell slot plans
+    replayed as graphs."""
     import torch
     sets = [slot_cells(dev, 24, 77 + k) for k in range(2)]
     res = []
     for mt, mw, mk in ((4, 1, 0), (8, 1, 0), (8, 2, 0), (16, 1, 0), (16, 2, 0), (16, 4, 0), (32, 2, 0), (32, 4, 0),
                        (32, 8, 0), (64, 4, 0), (64, 8, 0), (128, 8, 0), (8, 2, 1), (16, 4, 1), (32, 8, 1)):
-        for at in (8,):  # slot plans run one subcarrier per wave whatever the tile; the serving loop uses it
-            with_table(pe, ["slot %d %d %d %d" % (mt, at, mw, mk)] + base)
-            plans = [pe.SlotPlan(c) for c, _ in sets]
-            t = timed(lambda pl: pl.graph_launch(), [(plans[0],), (plans[1],)], iters=30, warm=6)
-            for pl in plans:
-                pl.close()
-            res.append({"mmse_tile": mt, "mmse_waves": mw, "mmse_keep": mk, "apply_tile": at, "us": t * 1e6})
+        with_table(pe, ["slot %d %d %d" % (mt, mw, mk)] + base)
+        plans = [pe.SlotPlan(c) for c, _ in sets]
+        t = timed(lambda pl: pl.graph_launch(), [(plans[0],), (plans[1],)], iters=30, warm=6)
+        for pl in plans:
+            pl.close()
+        res.append({"mmse_tile": mt, "mmse_waves": mw, "mmse_keep": mk, "us": t * 1e6})
     best = min(res, key=lambda r: r["us"])
-    return ("slot %d %d %d %d" % (best["mmse_tile"], best["apply_tile"], best["mmse_waves"], best["mmse_keep"]),
-            best, res)
+    return "slot %d %d %d" % (best["mmse_tile"], best["mmse_waves"], best["mmse_keep"]), best, res
 
 
 def main():
@@ -220,8 +243,9 @@ def main():
     lines = ["# tile table written by autotune.py on %s" % torch.cuda.get_device_name(0),
              "# coef <L> <S> <A> <prec> <f32_kc> <mfma_wpb> <mfma_spw>",
              "# mmse <L> <A> <nSC_max> <lanes per subcarrier> <subcarriers per tile>",
-             "# slot <mmse subcarriers per tile> <apply subcarriers per tile>"]
-    report = {"apply": [], "mmse": [], "slot": []}
+             "# solve <L> <A> <nSC_max> <subcarriers per tile> <waves>",
+             "# slot <mmse subcarriers per tile> <mmse waves> <keep rows>"]
+    report = {"apply": [], "mmse": [], "solve": [], "slot": []}
     kept = []
     if args.keep_coef and os.path.exists(args.out):
         kept = [ln.strip() for ln in open(args.out) if ln.startswith("coef ")]
@@ -239,6 +263,13 @@ def main():
         report["mmse"].append({"L": L, "A": A, "P": P, "candidates": rep})
         print("mmse  L=%d A=%3d lanes=%2d  %s" % (L, A, P, "  ".join(
             "nsc %d: T=%d W=%d keep=%d %.1f us" % (n, T, W, K, t * 1e6) for n, T, W, K, t in chosen)), flush=True)
+    for A, L in itertools.product(As, Ls):
+        if L < 4 or not 32 <= A <= 64:  # shapes without solve records run the coefficient kernels
+            continue
+        sl, best, rep = tune_solve(pe, dev, L, A, lines)
+        lines.append(sl)
+        report["solve"].append({"L": L, "A": A, "candidates": rep})
+        print("solve L=%d A=%3d  %s  %.1f us" % (L, A, sl, best["us"]), flush=True)
     sline, best, rep = tune_slot(pe, dev, lines)
     lines.append(sline)
     report["slot"] = rep

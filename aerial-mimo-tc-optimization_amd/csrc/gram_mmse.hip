@@ -1,5 +1,5 @@
-// gram_mmse.hip -- single-launch MMSE coefficient kernels: the FP32 tile pass and the FP64 pass over flagged
-// subcarriers; the per-subcarrier arithmetic is in mmse_device.hpp.
+// gram_mmse.hip -- the single-cell MMSE coefficient kernel: one launch, the FP32 tile pass with the FP64 solves of
+// the subcarriers it flags done by the same workgroup; the per-subcarrier arithmetic is in mmse_device.hpp.
 #include "mmse_device.hpp"
 #include "pe_internal.hpp"
 
@@ -23,20 +23,7 @@ __global__ __launch_bounds__(512) void gram_mmse_f32_kernel(const float2 *__rest
     const long sc0 = (long)blockIdx.x * T;
     const int n = (int)min((long)T, (long)nSC - sc0);
     tile_fp32<L, KR, C16, NA>(H, sigma2, C, noise_inv, flags, sc0, n, A, P, unbiased != 0, thr, lds,
-                              threadIdx.x, blockDim.x >> 6, nullptr, c16, cexp);
-}
-
-// One wave per 64 subcarriers; leaves after one flag load unless one of them was flagged.
-template <int L, bool C16>
-__global__ __launch_bounds__(64) void gram_mmse_f64_kernel(const float2 *__restrict__ H,
-                                                           const float *__restrict__ sigma2,
-                                                           float2 *__restrict__ C, float *__restrict__ noise_inv,
-                                                           uint8_t *__restrict__ flags, int nSC, int A, int P,
-                                                           int unbiased, uint32_t *__restrict__ c16,
-                                                           int *__restrict__ cexp) {
-    const long sc0 = (long)blockIdx.x * 64;
-    const int n = (int)min(64L, (long)nSC - sc0);
-    tile_fp64<L, C16>(H, sigma2, C, noise_inv, flags, sc0, n, A, P, unbiased != 0, threadIdx.x, c16, cexp);
+                              threadIdx.x, blockDim.x >> 6, c16, cexp);
 }
 
 template <int L, bool C16>
@@ -59,10 +46,6 @@ hipError_t launch_mmse_L(const float2 *H, const float *sigma2, float2 *C, float 
     else
         hipLaunchKernelGGL((gram_mmse_f32_kernel<L, 0, C16>), g, b, lds, st, H, sigma2, C, ni, flags, nSC, A, P,
                            T, unbiased ? 1 : 0, thr, c16, cexp);
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return e;
-    hipLaunchKernelGGL((gram_mmse_f64_kernel<L, C16>), dim3((nSC + 63) / 64), dim3(64), 0, st, H, sigma2, C, ni,
-                       flags, nSC, A, P, unbiased ? 1 : 0, c16, cexp);
     return hipGetLastError();
 }
 
@@ -87,31 +70,6 @@ hipError_t launch_mmse(const float2 *H, const float *sigma2, float2 *C, float *n
 hipError_t launch_gram_mmse(const float2 *H, const float *sigma2, float2 *C, float *noise_inv, uint8_t *flags,
                             int nSC, int L, int A, bool unbiased, float thr, hipStream_t st) {
     return launch_mmse<false>(H, sigma2, C, noise_inv, flags, nSC, L, A, unbiased, thr, nullptr, nullptr, st);
-}
-
-hipError_t launch_gram_mmse_fp64_pass(const float2 *H, const float *sigma2, float2 *C, float *noise_inv,
-                                      uint8_t *flags, int nSC, int L, int A, bool unbiased, hipStream_t st) {
-    const int P = mmse_lanes_for(L, A);
-    const dim3 g((unsigned)((nSC + 63) / 64)), b(64);
-    const int u = unbiased ? 1 : 0;
-    switch (L) {
-#define PE_F64_L(LL)                                                                                              \
-    case LL:                                                                                                      \
-        hipLaunchKernelGGL((gram_mmse_f64_kernel<LL, false>), g, b, 0, st, H, sigma2, C, noise_inv, flags, nSC, A, P, \
-                           u, nullptr, nullptr);                                                                  \
-        break;
-        PE_F64_L(1)
-        PE_F64_L(2)
-        PE_F64_L(3)
-        PE_F64_L(4)
-        PE_F64_L(5)
-        PE_F64_L(6)
-        PE_F64_L(7)
-        PE_F64_L(8)
-#undef PE_F64_L
-    default: return hipErrorInvalidValue;
-    }
-    return hipGetLastError();
 }
 
 hipError_t launch_gram_mmse_c16(const float2 *H, const float *sigma2, uint32_t *C16, int *cexp, float *noise_inv,

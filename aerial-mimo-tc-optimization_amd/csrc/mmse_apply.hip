@@ -5,9 +5,9 @@
 //
 //   1. mmse_solve_kernel: the MMSE kernel's Gram and L x L algebra (tile_solve) per tile, then the solve
 //      record of every subcarrier is copied from its LDS slot to the start of that subcarrier's block of the
-//      caller's C workspace; noise_inv and flags are written as by pe_gram_mmse.
-//   2. the MMSE FP64 pass over the flagged subcarriers (their coefficient blocks are written to C).
-//   3. apply_solve_kernel, one subcarrier per wave: lane a reads antenna row a of H and forms the column
+//      caller's C workspace; the subcarriers the FP32 algebra flags are solved in FP64 by the same workgroup,
+//      their coefficient blocks written to C; noise_inv and flags are written as by pe_gram_mmse.
+//   2. apply_solve_kernel, one subcarrier per wave: lane a reads antenna row a of H and forms the column
 //      C[.][a] with the operation sequence of the MMSE kernel's c_values (Ginv entries expanded once into a
 //      full matrix in LDS and read there by broadcast); the column goes straight into the operand layout of
 //      the fused apply (process_sc, column form), which runs the MFMA product and the demapper as the
@@ -73,10 +73,10 @@ __global__ __launch_bounds__(512) void mmse_solve_kernel(const float2 *__restric
     const long sc0 = (long)blockIdx.x * T;
     const int n = (int)min((long)T, (long)nSC - sc0);
     float kr[1][L], ki[1][L];
-    tile_solve<L, 1>(H, sigma2, noise_inv, flags, sc0, n, A, P, unbiased != 0, thr, lds_f, threadIdx.x, W, nullptr,
-                     false, kr, ki);
-    // flagged subcarriers' blocks are rewritten with coefficients by the FP64 pass
-    rec::write_records<L>(C, sc0, n, A, lds_f, threadIdx.x, blockDim.x);
+    tile_solve<L, 1>(H, sigma2, noise_inv, flags, sc0, n, A, P, unbiased != 0, thr, lds_f, threadIdx.x, W, false,
+                     kr, ki);
+    rec::write_records<L>(C, sc0, n, A, lds_f, threadIdx.x, blockDim.x);  // not for flagged subcarriers
+    tile_fp64_inline<L>(H, sigma2, C, noise_inv, flags, sc0, n, A, unbiased != 0, lds_f, threadIdx.x, W);
 }
 
 template <int L, int KS, int PREC>
@@ -91,7 +91,7 @@ __global__ __launch_bounds__(64 * kWpb) void apply_solve_kernel(
     const long sc = (long)blockIdx.x * kWpb + wave;
     if (sc >= nSC) return;  // uniform per wave; no workgroup barrier follows
     if (A < 16 * KS) zero_operands<KS, PREC>(ops, lane);
-    if (flags[sc] & PE_SCFLAG_FP64) {  // the FP64 pass wrote the coefficient block
+    if (flags[sc] & PE_SCFLAG_FP64) {  // solved in FP64: the coefficient block is in C
         const int nC = L * (A >> 1), rc = (nC + 63) >> 6, nY = S * (A >> 1), ry = (nY + 63) >> 6;
         RawSC<KS> ra;
         load_raw<KS>(ra, C, Y, (size_t)sc, L, S, A, lane, nC, nY, rc, ry);
@@ -107,13 +107,11 @@ hipError_t launch_l(const float2 *H, const float *s2, const float2 *Y, float2 *C
                     float *llr, int nSC, int S, int A, int npb, bool unbiased, float thr, int f16, int write_c,
                     hipStream_t st) {
     const int P = mmse_lanes_for(L, A);
-    const MmseTile mt = mmse_tile_for(L, A, nSC);
+    const MmseTile mt = solve_tile_for(L, A, nSC);
     const int T = mt.tile, W = mt.waves;
     hipLaunchKernelGGL((mmse_solve_kernel<L>), dim3((unsigned)((nSC + T - 1) / T)), dim3(64 * W),
                        (size_t)tile_lds_bytes(L, T), st, H, s2, C, ni, flags, nSC, A, P, T, unbiased ? 1 : 0, thr);
     hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return e;
-    e = launch_gram_mmse_fp64_pass(H, s2, C, ni, flags, nSC, L, A, unbiased, st);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL((apply_solve_kernel<L, KS, PREC>), dim3((unsigned)((nSC + kWpb - 1) / kWpb)), dim3(64 * kWpb),
                        0, st, H, Y, C, ni, flags, llr, nSC, S, A, npb, f16, write_c);

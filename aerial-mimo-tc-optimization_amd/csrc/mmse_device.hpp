@@ -7,7 +7,7 @@
 //   C = diag(s) Ginv H^H  with s_l = 1 / (1 - sigma2 Ginv_ll) when unbiased (else 1),
 //   noise_inv_l = (1 - x_l) / x_l  with x_l = sigma2 Ginv_ll,
 //   condition estimate kappa_1(G) = ||G||_1 ||Ginv||_1; above the threshold, or when the factorization
-//   breaks down, the FP32 pass hands the subcarrier to an FP64 pass.
+//   breaks down, the subcarrier is solved again in FP64 by the same workgroup (wave_fp64).
 //
 // Lanes and subcarriers. P lanes (P = 4, 8 or 16) share a subcarrier wherever antennas are summed: lane p of
 // the group takes antenna rows a = p, p + P, ... of H[sc] (rows of L complex values, contiguous, so a group's
@@ -19,9 +19,9 @@
 //     one lane per antenna (up to 64 lanes per subcarrier), reading Ginv from LDS, so stores cover whole C
 //     rows. With 64/P subcarriers per tile the Gram is a single round (lowest latency); with 64 the algebra
 //     runs on every lane without redundancy. The tile size changes no result.
-//   * group solve (FP64 pass): the P lanes of a group run the whole solve for one flagged subcarrier, the
-//     L x L algebra redundantly in each lane.
-// Results depend on P only through the Gram summation order; every path takes P from the same per-(L, A)
+//   * FP64 solve of a flagged subcarrier: one wave, lane 8 i + j holding element (i, j) of the L x L matrices;
+//     independent of P and of the tile.
+// Results depend on P only through the FP32 Gram summation order; every path takes P from the same per-(L, A)
 // choice, and the arithmetic is written with explicit fused multiply-adds and contraction off, so the
 // single-cell kernels and the multi-cell kernels produce bit-identical coefficients.
 #pragma once
@@ -30,8 +30,6 @@
 
 namespace pe {
 namespace mmse {
-
-constexpr uint8_t kPendingFp64 = 0x80;  // internal flag: FP32 pass handed the subcarrier to the FP64 pass
 
 __host__ __device__ constexpr int tri(int i, int j) { return i * (i + 1) / 2 + j; }  // i >= j
 // Floats per subcarrier slot in LDS: Ginv (2 NT) + scales (L) + ok flag, rounded up to 4 x an odd number so
@@ -360,35 +358,6 @@ __device__ __forceinline__ void write_c16_group(const float2 *__restrict__ h, in
     if (store && q == 0) *cexp = e;
 }
 
-// The same for the FP64 pass's group of P lanes (lane p: antennas p, p + P, ...): two passes over the
-// antennas, the first for the maximum, the second recomputing the values to store them.
-template <int L>
-__device__ __forceinline__ void write_c16_twopass(const float2 *__restrict__ h, int A, int p, int P,
-                                                  const double (&vr)[Dim<L>::NT], const double (&vi)[Dim<L>::NT],
-                                                  const double (&scl)[L], uint32_t *__restrict__ c16,
-                                                  int *__restrict__ cexp) {
-    float m = 0.f;
-    for (int a = p; a < A; a += P) {
-        double hr[L], hi[L];
-        float cr[L], ci[L];
-        load_row<double, L>(h + (size_t)a * L, hr, hi);
-        c_values<double, L>(hr, hi, vr, vi, scl, cr, ci);
-#pragma unroll
-        for (int l = 0; l < L; ++l) m = fmaxf(m, fmaxf(abs_(cr[l]), abs_(ci[l])));
-    }
-    const int e = c16_exp(wave_max_xor(m, 1, P));
-    const float s = ldexpf(1.f, -e);
-    for (int a = p; a < A; a += P) {
-        double hr[L], hi[L];
-        float cr[L], ci[L];
-        load_row<double, L>(h + (size_t)a * L, hr, hi);
-        c_values<double, L>(hr, hi, vr, vi, scl, cr, ci);
-#pragma unroll
-        for (int l = 0; l < L; ++l) c16[(size_t)l * A + a] = pack_half2(cr[l] * s, ci[l] * s);
-    }
-    if (p == 0) *cexp = e;
-}
-
 template <typename T, int L>
 __device__ __forceinline__ void write_c(const float2 *__restrict__ h, int A, int p, int P,
                                         const T (&vr)[Dim<L>::NT], const T (&vi)[Dim<L>::NT], const T (&scl)[L],
@@ -398,6 +367,215 @@ __device__ __forceinline__ void write_c(const float2 *__restrict__ h, int A, int
         T hr[L], hi[L];
         load_row<T, L>(h + (size_t)a * L, hr, hi);
         c_column<T, L>(hr, hi, vr, vi, scl, A, a, c_out, store);
+    }
+}
+
+// ---- FP64 solve of one flagged subcarrier by one wave ---------------------------------------------------------
+// Lane 8 i + j holds element (i, j) of the L x L matrices (lower triangle, i >= j, in use) as complex doubles; values
+// of other lanes are read by shuffles. G = H^H H + sigma2 I (antenna-serial sums), right-looking Cholesky G = R R^H,
+// W = R^-1 row by row, Ginv = W^H W, then one lane per antenna forms C[l][a] = s_l sum_j Ginv[l][j] conj(H[a][j]).
+// Loops are kept rolled (this path is rare): about 60 registers, so it runs inside the FP32 kernels for the
+// subcarriers they flag (no separate launch) without raising their register count much.
+// Writes C (or the binary16 block and its exponent), noise_inv and the final flag (FP64, plus SINGULAR with zero C
+// and noise_inv when the factorization breaks down). Every lane of the wave calls it.
+__device__ __forceinline__ double shfl_d(double v, int src) {
+    const unsigned long long u = __builtin_bit_cast(unsigned long long, v);
+    const unsigned lo = (unsigned)__shfl((int)(unsigned)u, src, 64);
+    const unsigned hi = (unsigned)__shfl((int)(unsigned)(u >> 32), src, 64);
+    return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
+}
+
+template <int L, bool C16>
+__device__ __forceinline__ void wave_fp64(const float2 *__restrict__ h, float s2f, float2 *__restrict__ c,
+                                                    float *__restrict__ ni, uint8_t *__restrict__ flag, int A,
+                                                    bool unbiased, int lane, uint32_t *__restrict__ c16,
+                                                    int *__restrict__ cexp) {
+    const int i = lane >> 3, j = lane & 7;
+    const bool low = i < L && j <= i;  // this lane's element is in the lower triangle
+    double rr = 0.0, ri = 0.0;
+    if (low) {
+#pragma unroll 1
+        for (int a = 0; a < A; ++a) {
+            const float2 hi_ = h[a * L + i], hj = h[a * L + j];
+            rr = fma_((double)hi_.x, (double)hj.x, fma_((double)hi_.y, (double)hj.y, rr));
+            ri = fma_((double)hi_.x, (double)hj.y, fma_(-(double)hi_.y, (double)hj.x, ri));
+        }
+    }
+    if (i == j) {
+        rr += (double)s2f;
+        ri = 0.0;
+    }
+    // Cholesky, right-looking: column k scaled by 1 / R_kk, then the trailing lower triangle updated
+    bool ok = true;
+#pragma unroll 1
+    for (int k = 0; k < L; ++k) {
+        const double d = shfl_d(rr, 9 * k);
+        if (!(d > 0.0)) {  // uniform
+            ok = false;
+            break;
+        }
+        const double rk = dev_sqrt<double>(d), rinv = 1.0 / rk;
+        if (j == k && i == k) {
+            rr = rk;
+            ri = 0.0;
+        } else if (j == k && i > k && i < L) {
+            rr *= rinv;
+            ri *= rinv;
+        }
+        const double aik_r = shfl_d(rr, 8 * i + k), aik_i = shfl_d(ri, 8 * i + k);  // R[i][k]
+        const double ajk_r = shfl_d(rr, 8 * j + k), ajk_i = shfl_d(ri, 8 * j + k);  // R[j][k]
+        if (low && j > k) {  // G[i][j] -= R[i][k] conj(R[j][k])
+            rr = fma_(-aik_r, ajk_r, fma_(-aik_i, ajk_i, rr));
+            ri = fma_(-aik_i, ajk_r, fma_(aik_r, ajk_i, ri));
+        }
+    }
+    double scale = 1.0, x = 0.0, vr = 0.0, vi = 0.0;
+    if (ok) {
+        // W = R^-1: W_ii = 1 / R_ii, W_ij = -(sum_{k=j}^{i-1} R_ik W_kj) / R_ii, one row at a time
+        const double rdiag = shfl_d(rr, 9 * (i < L ? i : 0));
+        double wr = (i == j && i < L) ? 1.0 / rdiag : 0.0, wi = 0.0;
+#pragma unroll 1
+        for (int row = 1; row < L; ++row) {
+            double sr = 0.0, si = 0.0;
+            for (int k = 0; k < row; ++k) {
+                const double Rr = shfl_d(rr, 8 * row + k), Ri = shfl_d(ri, 8 * row + k);  // R[row][k]
+                const double Wr = shfl_d(wr, 8 * k + j), Wi = shfl_d(wi, 8 * k + j);      // W[k][j]
+                if (j <= k) {
+                    sr = fma_(Rr, Wr, fma_(-Ri, Wi, sr));
+                    si = fma_(Rr, Wi, fma_(Ri, Wr, si));
+                }
+            }
+            if (i == row && j < row) {
+                const double dinv = 1.0 / rdiag;
+                wr = -sr * dinv;
+                wi = -si * dinv;
+            }
+        }
+        // Ginv = W^H W: V_ij = sum_{k >= i} conj(W_ki) W_kj
+#pragma unroll 1
+        for (int k = 0; k < L; ++k) {
+            const double a_r = shfl_d(wr, 8 * k + i), a_i = shfl_d(wi, 8 * k + i);  // W[k][i]
+            const double b_r = shfl_d(wr, 8 * k + j), b_i = shfl_d(wi, 8 * k + j);  // W[k][j]
+            if (low && k >= i) {
+                vr = fma_(a_r, b_r, fma_(a_i, b_i, vr));
+                vi = fma_(a_r, b_i, fma_(-a_i, b_r, vi));
+            }
+        }
+        if (i == j) vi = 0.0;  // Ginv is Hermitian: a real diagonal
+        if (i == j && i < L) {
+            x = (double)s2f * vr;
+            const double mu = 1.0 - x;
+            scale = (unbiased && mu != 0.0) ? 1.0 / mu : 1.0;
+            ni[i] = noise_inv_of(x);
+        }
+    } else if (i == j && i < L) {
+        ni[i] = 0.f;
+    }
+    // coefficients: one lane per antenna (a = lane, lane + 64, ...), one layer row at a time (a rolled loop, so
+    // that only that row of Ginv is in registers); C16: a first sweep for the block maximum, a second to store
+    auto coef = [&](int l, const float (&hr)[L], const float (&hi)[L], float &cr, float &ci) {
+        double re = 0.0, im = 0.0;
+#pragma unroll
+        for (int jj = 0; jj < L; ++jj) {  // Ginv[l][jj]: V[l][jj] for jj <= l, else conj(V[jj][l])
+            const int src = jj <= l ? 8 * l + jj : 8 * jj + l;
+            const double g_r = shfl_d(vr, src), g_i0 = shfl_d(vi, src);
+            const double g_i = jj <= l ? g_i0 : -g_i0;
+            re = fma_(g_r, (double)hr[jj], fma_(g_i, (double)hi[jj], re));
+            im = fma_(g_i, (double)hr[jj], fma_(-g_r, (double)hi[jj], im));
+        }
+        const double s = shfl_d(scale, 9 * l);
+        cr = ok ? (float)(re * s) : 0.f;
+        ci = ok ? (float)(im * s) : 0.f;
+    };
+    auto row = [&](int a, float (&hr)[L], float (&hi)[L]) {
+#pragma unroll
+        for (int jj = 0; jj < L; ++jj) {
+            const float2 v = a < A ? h[a * L + jj] : make_float2(0.f, 0.f);
+            hr[jj] = v.x;
+            hi[jj] = v.y;
+        }
+    };
+    if constexpr (!C16) {
+        for (int a = lane; a < A; a += 64) {
+            float hr[L], hi[L];
+            row(a, hr, hi);
+#pragma unroll 1
+            for (int l = 0; l < L; ++l) {
+                float cr, ci;
+                coef(l, hr, hi, cr, ci);
+                c[(size_t)l * A + a] = make_float2(cr, ci);
+            }
+        }
+    } else {  // A <= 128
+        float m = 0.f;
+        for (int a = lane; a < A; a += 64) {
+            float hr[L], hi[L];
+            row(a, hr, hi);
+#pragma unroll 1
+            for (int l = 0; l < L; ++l) {
+                float cr, ci;
+                coef(l, hr, hi, cr, ci);
+                m = fmaxf(m, fmaxf(abs_(cr), abs_(ci)));
+            }
+        }
+        const int e = c16_exp(wave_max_xor(m, 1, 64));
+        const float sc = ldexpf(1.f, -e);
+        for (int a = lane; a < A; a += 64) {
+            float hr[L], hi[L];
+            row(a, hr, hi);
+#pragma unroll 1
+            for (int l = 0; l < L; ++l) {
+                float cr, ci;
+                coef(l, hr, hi, cr, ci);
+                c16[(size_t)l * A + a] = pack_half2(cr * sc, ci * sc);
+            }
+        }
+        if (lane == 0) *cexp = e;
+    }
+    if (lane == 0) *flag = ok ? PE_SCFLAG_FP64 : (PE_SCFLAG_FP64 | PE_SCFLAG_SINGULAR);
+}
+
+// The same as a called function: in the multi-cell kernels, whose registers are set by the largest layer count,
+// a call keeps the kernel's register count where it is without the FP64 code (the call itself is only made for a
+// flagged subcarrier).
+template <int L, bool C16>
+__device__ __noinline__ void wave_fp64_call(const float2 *__restrict__ h, float s2f, float2 *__restrict__ c,
+                                            float *__restrict__ ni, uint8_t *__restrict__ flag, int A, bool unbiased,
+                                            int lane, uint32_t *__restrict__ c16, int *__restrict__ cexp) {
+    wave_fp64<L, C16>(h, s2f, c, ni, flag, A, unbiased, lane, c16, cexp);
+}
+
+// The FP64 solves of a tile's flagged subcarriers (those whose slot ok flag is 0), spread over the workgroup's W
+// waves; called by every thread after the tile's algebra, while the slots are intact. CALL: through
+// wave_fp64_call.
+template <int L, bool C16 = false, bool CALL = false>
+__device__ __forceinline__ void tile_fp64_inline(const float2 *__restrict__ H, const float *__restrict__ sigma2,
+                                                 float2 *__restrict__ C, float *__restrict__ noise_inv,
+                                                 uint8_t *__restrict__ flags, long sc0, int n, int A, bool unbiased,
+                                                 const float *__restrict__ lds, int tid, int W,
+                                                 uint32_t *__restrict__ c16 = nullptr, int *__restrict__ cexp = nullptr) {
+    constexpr int NT = Dim<L>::NT;
+    constexpr int SLOT = Dim<L>::SLOT;
+    const int lane = tid & 63, wave = tid >> 6;
+    int idx = 0;
+    for (int t0 = 0; t0 < n; t0 += 64) {
+        const bool f = t0 + lane < n && lds[(t0 + lane) * SLOT + 2 * NT + L] == 0.f;
+        unsigned long long m = __ballot(f);  // the same in every wave
+        while (m) {
+            const int b = __builtin_ctzll(m);
+            m &= m - 1;
+            if (idx++ % W != wave) continue;
+            const long sc = sc0 + t0 + b;
+            float2 *c = C ? C + (size_t)sc * L * A : nullptr;
+            uint32_t *c16s = c16 ? c16 + (size_t)sc * L * A : nullptr;
+            int *ce = cexp ? cexp + sc : nullptr;
+            if constexpr (CALL)
+                wave_fp64_call<L, C16>(H + (size_t)sc * A * L, sigma2[sc], c, noise_inv + (size_t)sc * L, flags + sc,
+                                       A, unbiased, lane, c16s, ce);
+            else
+                wave_fp64<L, C16>(H + (size_t)sc * A * L, sigma2[sc], c, noise_inv + (size_t)sc * L, flags + sc, A,
+                                  unbiased, lane, c16s, ce);
+        }
     }
 }
 
@@ -434,13 +612,14 @@ __device__ __forceinline__ void tile_barrier(int W) {
 
 // ---- FP32 pass over one tile: subcarriers sc0 .. sc0 + n - 1 of one cell, by a workgroup of W waves -------
 // Phases 1 and 2 of a tile (Gram, algebra), ending at a barrier with Ginv, the scales and the ok flag of
-// every subcarrier in its LDS slot, noise_inv and flags written. keep: the lane's KR1 antenna rows stay in
-// kr / ki (single Gram round only).
+// every subcarrier in its LDS slot; noise_inv and flags (0) written for the subcarriers solved in FP32, the
+// flagged ones left to tile_fp64_inline. keep: the lane's KR1 antenna rows stay in kr / ki (single Gram round
+// only).
 template <int L, int KR1>
 __device__ __forceinline__ void tile_solve(const float2 *__restrict__ H, const float *__restrict__ sigma2,
                                            float *__restrict__ noise_inv, uint8_t *__restrict__ flags, long sc0,
                                            int n, int A, int P, bool unbiased, float thr, float *__restrict__ lds,
-                                           int tid, int W, unsigned *flagged, bool keep, float (&kr)[KR1][L],
+                                           int tid, int W, bool keep, float (&kr)[KR1][L],
                                            float (&ki)[KR1][L]) {
     constexpr int NT = Dim<L>::NT;
     constexpr int SLOT = Dim<L>::SLOT;
@@ -504,9 +683,8 @@ __device__ __forceinline__ void tile_solve(const float2 *__restrict__ H, const f
         if (ok) {
 #pragma unroll
             for (int l = 0; l < L; ++l) noise_inv[sc * L + l] = noise_inv_of(x[l]);
+            flags[sc] = 0;
         }
-        flags[sc] = ok ? 0 : kPendingFp64;
-        if (!ok && flagged) atomicAdd(flagged, 1u);
     }
     tile_barrier(W);
 }
@@ -514,20 +692,20 @@ __device__ __forceinline__ void tile_solve(const float2 *__restrict__ H, const f
 // n <= 64 W; lds: the workgroup's region of at least n * Dim<L>::SLOT floats; tid = thread in the workgroup.
 // Gram and coefficient rounds are spread over the W waves; the algebra runs on threads 0 .. n - 1, i.e. on as
 // few waves as possible, while the others wait at the barrier and leave their issue slots to other
-// workgroups. flags[sc] = 0 or kPendingFp64 (C and noise_inv of a flagged subcarrier are left to the FP64
-// pass); `flagged` (optional) counts flagged subcarriers.
+// workgroups. Flagged subcarriers are then solved in FP64 by the workgroup's waves (tile_fp64_inline), which write
+// their C, noise_inv and final flags; the tile ends at a barrier, so a workgroup may go on to another tile.
 // KR > 0: when every wave has a single Gram round (n <= 64 W / P) and A <= KR P, each lane keeps its KR antenna
 // rows in registers from the Gram to the coefficients instead of reading H a second time (the coefficient
 // stores then cover P consecutive antennas of a C row per lane group). It trades the second read of H for
 // registers (fewer waves per SIMD); which is faster depends on the shape.
 // C16: coefficients go to c16 / cexp as binary16 pairs and exponents (write_c16_group with NA antennas per
-// lane: A <= 64 NA; no kept rows) instead of complex64 to C.
-template <int L, int KR, bool C16 = false, int NA = 2>
+// lane: A <= 64 NA; no kept rows) instead of complex64 to C. CALL: the FP64 solves through a function call.
+template <int L, int KR, bool C16 = false, int NA = 2, bool CALL = false>
 __device__ __forceinline__ void tile_fp32(const float2 *__restrict__ H, const float *__restrict__ sigma2,
                                           float2 *__restrict__ C, float *__restrict__ noise_inv,
                                           uint8_t *__restrict__ flags, long sc0, int n, int A, int P,
                                           bool unbiased, float thr, float *__restrict__ lds, int tid, int W,
-                                          unsigned *flagged, uint32_t *__restrict__ c16 = nullptr,
+                                          uint32_t *__restrict__ c16 = nullptr,
                                           int *__restrict__ cexp = nullptr) {
     static_assert(!C16 || KR == 0, "binary16 output reads H again for the coefficients");
     constexpr int NT = Dim<L>::NT;
@@ -538,7 +716,7 @@ __device__ __forceinline__ void tile_fp32(const float2 *__restrict__ H, const fl
     const bool keep = KR > 0 && A <= KR * P && n <= W * G;
     constexpr int KR1 = KR > 0 ? KR : 1;
     float kr[KR1][L], ki[KR1][L];
-    tile_solve<L, KR1>(H, sigma2, noise_inv, flags, sc0, n, A, P, unbiased, thr, lds, tid, W, flagged, keep, kr, ki);
+    tile_solve<L, KR1>(H, sigma2, noise_inv, flags, sc0, n, A, P, unbiased, thr, lds, tid, W, keep, kr, ki);
 
     // coefficients: Q = min(64, 2^ceil(log2 A)) lanes per subcarrier, lane q writing antennas q, q + Q, ..., so
     // a store instruction covers up to 512 contiguous bytes of a C row whatever P is (C[l][a] is computed by
@@ -548,7 +726,7 @@ __device__ __forceinline__ void tile_fp32(const float2 *__restrict__ H, const fl
         const int t = min(wave * G + g, n - 1);
         float in[2 * NT + L + 1];
         slot_load<2 * NT + L + 1>(lds + t * SLOT, in);
-        const bool store = wave * G + g < n && in[2 * NT + L] != 0.f;  // flagged: the FP64 pass writes it
+        const bool store = wave * G + g < n && in[2 * NT + L] != 0.f;  // flagged: solved in FP64 below
         float vr[NT], vi[NT], scl[L];
 #pragma unroll
         for (int k = 0; k < NT; ++k) {
@@ -569,7 +747,7 @@ __device__ __forceinline__ void tile_fp32(const float2 *__restrict__ H, const fl
         const int t = min(t0 + g3, n - 1);
         float in[2 * NT + L + 1];
         slot_load<2 * NT + L + 1>(lds + t * SLOT, in);
-        const bool store = t0 + g3 < n && in[2 * NT + L] != 0.f;  // flagged: the FP64 pass writes it
+        const bool store = t0 + g3 < n && in[2 * NT + L] != 0.f;  // flagged: solved in FP64 below
         float vr[NT], vi[NT], scl[L];
 #pragma unroll
         for (int k = 0; k < NT; ++k) {
@@ -591,76 +769,9 @@ __device__ __forceinline__ void tile_fp32(const float2 *__restrict__ H, const fl
 #pragma unroll 2
         for (int t0 = wave * R3; !keep && t0 < n; t0 += W * R3) coef_round(t0);
     }
+    // subcarriers the FP32 pass flagged, solved in FP64 by this workgroup
+    tile_fp64_inline<L, C16, CALL>(H, sigma2, C, noise_inv, flags, sc0, n, A, unbiased, lds, tid, W, c16, cexp);
     tile_barrier(W);  // slot reads done before the workgroup's next tile rewrites them
-}
-
-// ---- FP64 pass for one flagged subcarrier, by one group of P lanes -------------------------------------------
-// Writes C, noise_inv and the final flag (FP64, plus SINGULAR with zero C / noise_inv if the FP64
-// factorization breaks down too).
-template <int L, bool C16 = false>
-__device__ __forceinline__ void group_fp64(const float2 *__restrict__ h, float s2f, float2 *__restrict__ c,
-                                           float *__restrict__ ni, uint8_t *__restrict__ flag, int A, int p,
-                                           int P, bool unbiased, uint32_t *__restrict__ c16 = nullptr,
-                                           int *__restrict__ cexp = nullptr) {
-    constexpr int NT = Dim<L>::NT;
-    double gr[NT], gi[NT], vr[NT], vi[NT], scl[L], x[L];
-    gram_group<double, L>(h, A, p, P, gr, gi);
-    const bool ok = solve<double, L>(gr, gi, (double)s2f, unbiased, 0.0, false, vr, vi, scl, x);
-    if (ok) {
-        if (p == 0) {
-#pragma unroll
-            for (int l = 0; l < L; ++l) ni[l] = noise_inv_of(x[l]);
-        }
-        if constexpr (C16)
-            write_c16_twopass<L>(h, A, p, P, vr, vi, scl, c16, cexp);
-        else
-            write_c<double, L>(h, A, p, P, vr, vi, scl, c);
-    } else {
-        for (int a = p; a < A; a += P)
-            for (int l = 0; l < L; ++l) {
-                if constexpr (C16)
-                    c16[(size_t)l * A + a] = 0u;
-                else
-                    c[(size_t)l * A + a] = make_float2(0.f, 0.f);
-            }
-        if (C16 && p == 0) *cexp = 0;
-        if (p == 0)
-            for (int l = 0; l < L; ++l) ni[l] = 0.f;
-    }
-    if (p == 0) *flag = ok ? PE_SCFLAG_FP64 : (PE_SCFLAG_FP64 | PE_SCFLAG_SINGULAR);
-}
-
-// ---- FP64 pass over one tile: every flagged subcarrier of sc0 .. sc0 + n - 1 (n <= 64) --------------------
-// One flag load per lane; a wave whose tile holds no flagged subcarrier leaves at once. Flagged subcarriers
-// are taken 64 / P at a time, one per lane group.
-template <int L, bool C16 = false>
-__device__ __forceinline__ void tile_fp64(const float2 *__restrict__ H, const float *__restrict__ sigma2,
-                                          float2 *__restrict__ C, float *__restrict__ noise_inv,
-                                          uint8_t *__restrict__ flags, long sc0, int n, int A, int P,
-                                          bool unbiased, int lane, uint32_t *__restrict__ c16 = nullptr,
-                                          int *__restrict__ cexp = nullptr) {
-    const bool pend = lane < n && (flags[sc0 + lane] & kPendingFp64);
-    unsigned long long m = __ballot(pend);
-    const int lgP = __builtin_ctz((unsigned)P);
-    const int p = lane & (P - 1), g = lane >> lgP, G = 64 >> lgP;
-    while (m) {  // m is wave-uniform
-        unsigned long long mm = m;
-        int idx = -1;
-        for (int k = 0; k <= g && mm; ++k) {  // the g-th lowest set bit goes to group g
-            if (k == g) idx = __builtin_ctzll(mm);
-            mm &= mm - 1;
-        }
-        for (int k = 0; k < G && m; ++k) m &= m - 1;
-        if (idx >= 0) {  // uniform per lane group
-            const long sc = sc0 + idx;
-            if constexpr (C16)
-                group_fp64<L, true>(H + (size_t)sc * A * L, sigma2[sc], nullptr, noise_inv + (size_t)sc * L,
-                                    flags + sc, A, p, P, unbiased, c16 + (size_t)sc * L * A, cexp + sc);
-            else
-                group_fp64<L>(H + (size_t)sc * A * L, sigma2[sc], C + (size_t)sc * L * A,
-                              noise_inv + (size_t)sc * L, flags + sc, A, p, P, unbiased);
-        }
-    }
 }
 
 #pragma clang fp contract(fast)  // the HIP default again for code after this header

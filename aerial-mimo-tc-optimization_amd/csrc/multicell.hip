@@ -3,14 +3,13 @@
 //
 // The reference runs one launch per cell on its own stream (multicell_contention.cu:194-283,
 // multicell_evidence.cu:497-566). Here a batch of cell-jobs is cut into subcarrier tiles on the host and
-// executed by a fixed sequence of three kernels, whatever the number and shapes of the cells:
-//   MMSE FP32 pass over the cells with H (the L x L solve is unrolled per L and selected by the tile's cell),
-//   the FP64 pass over flagged subcarriers,
+// executed by a fixed sequence of kernels, whatever the number and shapes of the cells:
+//   the MMSE kernel over the cells with H (the L x L solve is unrolled per L and selected by the tile's cell;
+//   subcarriers the FP32 algebra flags are solved in FP64 by the same workgroup),
 //   one fused coefficient apply + demap kernel per precision (the antenna class and the QAM order are read
 //   per tile).
 // Tiles of a kernel are ordered by decreasing per-subcarrier cost. The kernels read their tile ranges from
-// device memory, so one captured graph serves every batch, and no memset node is needed (the flag counter is
-// zeroed by a kernel).
+// device memory, so one captured graph serves every batch, and no memset node is needed.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -32,8 +31,7 @@ namespace {
 // plan the grid is the class's exact tile count (one tile per workgroup); for the serving loop, whose batches
 // vary, it is bounded by the device's size. Tiles of one class cost about the same, so this static assignment
 // balances without atomics. Tile counts are read from device memory, so one captured graph serves every
-// batch. The FP32 pass counts flagged subcarriers; the first apply kernel, which follows both MMSE passes,
-// zeroes the counter for the next slot (inside a kernel, so a captured graph replays with no memset node).
+// batch.
 struct BlockView {
     const LaunchRange *ranges;
     const CellDev *cells;
@@ -60,7 +58,7 @@ __host__ __device__ __forceinline__ int ks_class_dev(int A) { return A <= 16 ? 1
 // PREC 5 (PE_PREC_F16_C16): binary16 product on binary16 coefficients (C holds the C16 block, cexp the
 // exponents), the code of the single-cell pe_coef_apply_demap_c16.
 // Cells with solve records (rec): stage 2 forms the coefficients from H and the record, except on subcarriers
-// the FP64 pass solved (their coefficient block is in C).
+// solved in FP64 (their coefficient block is in C).
 template <int KS, int PREC>
 __device__ __forceinline__ void apply_one_sc_cell(const CellDev &cd, long sc, unsigned char *lds, int lane) {
     constexpr int P = PREC == 5 ? 1 : PREC;
@@ -96,13 +94,10 @@ __device__ __forceinline__ void apply_sc_class(const CellDev &cd, long sc, unsig
 
 template <int KSMAX, int PREC, bool LOOP>
 __global__ __launch_bounds__(64 * kApplyWaves) void slot_apply_kernel(const unsigned char *__restrict__ block,
-                                                                      int li, int off_cells, int off_tiles,
-                                                                      Counters *reset) {
+                                                                      int li, int off_cells, int off_tiles) {
     using namespace mfma;
     __shared__ __attribute__((aligned(16))) unsigned char lds[kApplyWaves][LdsLayout<KSMAX>::bytes(PREC)];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    // both MMSE passes are complete (stream order): zero their counter for the next slot
-    if (reset && blockIdx.x == 0 && threadIdx.x == 0) reset->flagged = 0u;
     const BlockView v = view(block, off_cells, off_tiles);
     const LaunchRange rg = v.ranges[li];
     if constexpr (!LOOP) {
@@ -134,14 +129,14 @@ __global__ __launch_bounds__(64 * kApplyWaves) void slot_apply_kernel(const unsi
 
 // Cells with solve records (SlotExec::uses_records): their own apply launch per precision, so that the other
 // cells' kernel keeps its registers. Per subcarrier, the coefficients are formed on chip from H and the record,
-// or read from C where the FP64 pass ran; the code is inlined (a called function is scheduled without the
+// or read from C where the solve ran in FP64; the code is inlined (a called function is scheduled without the
 // kernel's occupancy target and takes twice the registers).
 template <int KS, int PREC>
 __device__ __forceinline__ void apply_record_class(const CellDev &cd, long sc, unsigned char *lds, float2 *gfull,
                                                    int lane) {
     using namespace mfma;
     if (cd.A < 16 * KS) zero_operands<KS, PREC>(lds, lane);
-    if (cd.flags[sc] & PE_SCFLAG_FP64) {  // the FP64 pass wrote the coefficient block
+    if (cd.flags[sc] & PE_SCFLAG_FP64) {  // solved in FP64: the coefficient block is in C
         const int nC = cd.L * (cd.A >> 1), rc = (nC + 63) >> 6, nY = cd.S * (cd.A >> 1), ry = (nY + 63) >> 6;
         RawSC<KS> ra;
         load_raw<KS>(ra, cd.C, cd.Y, (size_t)sc, cd.L, cd.S, cd.A, lane, nC, nY, rc, ry);
@@ -159,12 +154,11 @@ __device__ __forceinline__ void apply_record_class(const CellDev &cd, long sc, u
 // spills.
 template <int PREC>
 __global__ __launch_bounds__(64 * kApplyWaves) __attribute__((amdgpu_waves_per_eu(5, 8))) void slot_apply_rec_kernel(
-    const unsigned char *__restrict__ block, int li, int off_cells, int off_tiles, Counters *reset) {
+    const unsigned char *__restrict__ block, int li, int off_cells, int off_tiles) {
     using namespace mfma;
     __shared__ __attribute__((aligned(16))) unsigned char lds[kApplyWaves][LdsLayout<4>::bytes(PREC)];
     __shared__ __attribute__((aligned(16))) float2 gfull_all[kApplyWaves][rec::kGinvBytes / sizeof(float2)];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    if (reset && blockIdx.x == 0 && threadIdx.x == 0) reset->flagged = 0u;
     const BlockView v = view(block, off_cells, off_tiles);
     const LaunchRange rg = v.ranges[li];
     auto one = [&](const CellDev &cd, long sc) {  // 32 <= A <= 64 (record_pays)
@@ -178,32 +172,29 @@ __global__ __launch_bounds__(64 * kApplyWaves) __attribute__((amdgpu_waves_per_e
     one(v.cells[tl.cell], sc);
 }
 
-// Stage 1: a tile is up to mmse_tile subcarriers of one cell. FP32 pass: one workgroup of W waves solves the
-// tile (tile solve). FP64 pass: a one-wave workgroup revisits the tile's flagged subcarriers, 64 at a time.
-// Lanes per subcarrier are the cell shape's choice, the same as in the single-cell kernels, so the
+// Stage 1: a tile is up to mmse_tile subcarriers of one cell, solved by one workgroup of W waves (tile solve),
+// which then solves the subcarriers its FP32 algebra flagged in FP64 (tile_fp64_inline). Lanes per subcarrier are the cell shape's choice, the same as in the single-cell kernels, so the
 // coefficients are bit-identical to the cell solved alone.
 // C16: coefficients written in the binary16 operand format (cells of precision PE_PREC_F16_C16).
-template <bool FP64, int KR, bool C16 = false>
+template <int KR, bool C16 = false>
 __device__ __forceinline__ void mmse_tile(const CellDev &cd, const Tile &tl, float *lds, int tid, int W,
-                                          int unbiased, float thr, Counters *q) {
+                                          int unbiased, float thr) {
     using namespace mmse;
     uint32_t *c16 = reinterpret_cast<uint32_t *>(cd.C);
 #define PE_MMSE_L(LL)                                                                                            \
     case LL:                                                                                                     \
-        if constexpr (FP64) {                                                                                    \
-            for (int c = 0; c < tl.n; c += 64)                                                                   \
-                tile_fp64<LL, C16>(cd.H, cd.sigma2, cd.C, cd.noise_inv, cd.flags, tl.sc0 + c, min(64, tl.n - c), \
-                                   cd.A, cd.mmse_p, unbiased != 0, tid, c16, cd.cexp);                           \
-        } else if (!C16 && cd.rec) {                                                                             \
+        if (!C16 && cd.rec) {                                                                                    \
             float kr[1][LL], ki[1][LL];                                                                          \
             tile_solve<LL, 1>(cd.H, cd.sigma2, cd.noise_inv, cd.flags, tl.sc0, tl.n, cd.A, cd.mmse_p,            \
-                              unbiased != 0, thr, lds, tid, W, &q->flagged, false, kr, ki);                      \
-            rec::write_records<LL>(cd.C, tl.sc0, tl.n, cd.A, lds, tid, 64 * W);                                  \
-            tile_barrier(W); /* records copied before the workgroup's next tile rewrites the slots */            \
+                              unbiased != 0, thr, lds, tid, W, false, kr, ki);                                   \
+            rec::write_records<LL>(cd.C, tl.sc0, tl.n, cd.A, lds, tid, 64 * W); /* not for flagged ones */       \
+            tile_fp64_inline<LL, false, true>(cd.H, cd.sigma2, cd.C, cd.noise_inv, cd.flags, tl.sc0, tl.n, cd.A, \
+                                              unbiased != 0, lds, tid, W);                                       \
+            tile_barrier(W); /* slots read before the workgroup's next tile rewrites them */                     \
         } else {                                                                                                 \
-            tile_fp32<LL, C16 ? 0 : KR, C16, 2>(cd.H, cd.sigma2, cd.C, cd.noise_inv, cd.flags, tl.sc0, tl.n,     \
-                                                cd.A, cd.mmse_p, unbiased != 0, thr, lds, tid, W, &q->flagged,   \
-                                                c16, cd.cexp);                                                   \
+            tile_fp32<LL, C16 ? 0 : KR, C16, 2, true>(cd.H, cd.sigma2, cd.C, cd.noise_inv, cd.flags, tl.sc0,     \
+                                                      tl.n, cd.A, cd.mmse_p, unbiased != 0, thr, lds, tid, W,    \
+                                                      c16, cd.cexp);                                             \
         }                                                                                                        \
         break;
     switch (cd.L) {
@@ -219,8 +210,8 @@ __device__ __forceinline__ void mmse_tile(const CellDev &cd, const Tile &tl, flo
 // workgroups striding over the tiles.
 template <int KR, bool LOOP, bool C16 = false>
 __global__ __launch_bounds__(512) void slot_mmse_fp32_kernel(const unsigned char *__restrict__ block, int li,
-                                                             int off_cells, int off_tiles, Counters *q,
-                                                             int unbiased, float thr) {
+                                                             int off_cells, int off_tiles, int unbiased,
+                                                             float thr) {
     extern __shared__ __attribute__((aligned(16))) float lds[];
     const BlockView v = view(block, off_cells, off_tiles);
     const LaunchRange rg = v.ranges[li];
@@ -228,28 +219,13 @@ __global__ __launch_bounds__(512) void slot_mmse_fp32_kernel(const unsigned char
         if ((int)blockIdx.x >= rg.ntiles) return;  // uniform per workgroup (tile_fp32 has barriers)
         const Tile tl = v.tiles[rg.tile0 + blockIdx.x];
         const CellDev cd = v.cells[tl.cell];
-        mmse_tile<false, KR, C16>(cd, tl, lds, threadIdx.x, blockDim.x >> 6, unbiased, thr, q);
+        mmse_tile<KR, C16>(cd, tl, lds, threadIdx.x, blockDim.x >> 6, unbiased, thr);
     } else {
         for (int t = blockIdx.x; t < rg.ntiles; t += gridDim.x) {  // uniform per workgroup
             const Tile tl = v.tiles[rg.tile0 + t];
             const CellDev cd = v.cells[tl.cell];
-            mmse_tile<false, KR, C16>(cd, tl, lds, threadIdx.x, blockDim.x >> 6, unbiased, thr, q);
+            mmse_tile<KR, C16>(cd, tl, lds, threadIdx.x, blockDim.x >> 6, unbiased, thr);
         }
-    }
-}
-
-// Same tiles, one wave each; leaves at once unless the FP32 pass (counter slot li - 1) flagged a subcarrier.
-template <bool C16 = false>
-__global__ __launch_bounds__(64) void slot_mmse_fp64_kernel(const unsigned char *__restrict__ block, int li,
-                                                            int off_cells, int off_tiles, Counters *q,
-                                                            int unbiased) {
-    if (__atomic_load_n(&(q - 1)->flagged, __ATOMIC_RELAXED) == 0u) return;
-    const BlockView v = view(block, off_cells, off_tiles);
-    const LaunchRange rg = v.ranges[li];
-    for (int t = blockIdx.x; t < rg.ntiles; t += gridDim.x) {
-        const Tile tl = v.tiles[rg.tile0 + t];
-        const CellDev cd = v.cells[tl.cell];
-        mmse_tile<true, 0, C16>(cd, tl, nullptr, threadIdx.x, 1, unbiased, 0.f, q);
     }
 }
 
@@ -265,7 +241,6 @@ SlotExec::~SlotExec() {
     if (exec_) (void)hipGraphExecDestroy(exec_);
     if (graph_) (void)hipGraphDestroy(graph_);
     if (d_block_) (void)hipFree(d_block_);
-    if (d_counters_) (void)hipFree(d_counters_);
     if (d_flags_) (void)hipFree(d_flags_);
     if (h_block_) (void)hipHostFree(h_block_);
 }
@@ -363,7 +338,7 @@ pe_status SlotExec::init(const pe_cell_desc *pop, int npop, int max_cells, unsig
     auto grid_for = [&](int cap, int waves_per_wg, int waves_per_cu) {
         return fixed ? cap : std::max(1, std::min(cap, cus * waves_per_cu / waves_per_wg));
     };
-    // MMSE FP32 and FP64 passes over every cell with H, then one fused apply/demap launch per precision
+    // the MMSE kernel over every cell with H, then one fused apply/demap launch per precision
     max_tiles_ = 0;
     flags_per_cell_ = (size_t)max_sc;
     bool mm = false;
@@ -371,7 +346,6 @@ pe_status SlotExec::init(const pe_cell_desc *pop, int npop, int max_cells, unsig
     if (mm) {
         const int cap = class_tiles([](const pe_cell_desc &c) { return c.H != nullptr; }, mmse_tile_);
         launches_.push_back({0, 0, 0, grid_for(cap, mmse_waves_, 16), 64 * mmse_waves_, cap});
-        launches_.push_back({1, 0, 0, std::min(cap, 64), 64, cap});
         max_tiles_ += cap;
     }
     for (const auto &k : apply_cls) {  // one fused apply/demap launch per (kind, precision), every antenna class
@@ -386,9 +360,7 @@ pe_status SlotExec::init(const pe_cell_desc *pop, int npop, int max_cells, unsig
     block_bytes_ = off_tiles_ + sizeof(Tile) * (size_t)std::max(1, max_tiles_);
     hipError_t e = hipMalloc(&d_block_, block_bytes_);
     if (e == hipSuccess) e = hipHostMalloc(&h_block_, block_bytes_, hipHostMallocDefault);
-    if (e == hipSuccess) e = hipMalloc(&d_counters_, sizeof(Counters) * std::max<size_t>(1, launches_.size()));
     if (e == hipSuccess) e = hipMalloc(&d_flags_, flags_per_cell_ * (size_t)max_cells);
-    if (e == hipSuccess) e = hipMemset(d_counters_, 0, sizeof(Counters) * std::max<size_t>(1, launches_.size()));
     if (e == hipSuccess) e = hipMemset(d_block_, 0, block_bytes_);
     if (e == hipSuccess) e = hipMemset(d_flags_, 0, flags_per_cell_ * (size_t)max_cells);
     if (e == hipSuccess) e = hipDeviceSynchronize();
@@ -446,11 +418,6 @@ pe_status SlotExec::stage(const pe_cell_desc *cells, int n, std::string &err) {
     staged_ntiles_.assign(launches_.size(), 0);
     for (size_t li = 0; li < launches_.size(); ++li) {
         const Launch &l = launches_[li];
-        if (l.kind == 1) {  // FP64 pass: same tiles as the FP32 pass just before it
-            ranges[li] = ranges[li - 1];
-            staged_ntiles_[li] = ranges[li].ntiles;
-            continue;
-        }
         order.clear();
         for (int i = 0; i < n; ++i) {
             const pe_cell_desc &c = cells[i];
@@ -496,44 +463,32 @@ hipError_t SlotExec::launch_one(int i, hipStream_t st, bool exact) const {
     const int unb = (flags_ & PE_MMSE_UNBIASED) ? 1 : 0;
     const float thr = 1e4f;
     const int oc = (int)off_cells_, ot = (int)off_tiles_;
-    Counters *q = d_counters_ + i;
     // loop-free kernels (one tile per workgroup, fewer registers) for fixed plans and for eager launches, whose
     // grid is the staged tile count; the striding kernels for graph replays of varying batches
     const bool lf = fixed_ || exact;
     const int st_tiles = std::max(1, staged_ntiles_[i]);
-    dim3 g(l.kind == 1 ? (exact ? std::min(l.grid, st_tiles) : l.grid) : (exact ? st_tiles : l.grid)), b(l.block);
+    dim3 g(exact ? st_tiles : l.grid), b(l.block);
     if (l.kind == 0) {
         const size_t lds = (size_t)mmse_lds_;
         if (c16_ && lf)
-            hipLaunchKernelGGL((slot_mmse_fp32_kernel<0, false, true>), g, b, lds, st, d_block_, i, oc, ot, q, unb,
-                               thr);
+            hipLaunchKernelGGL((slot_mmse_fp32_kernel<0, false, true>), g, b, lds, st, d_block_, i, oc, ot, unb, thr);
         else if (c16_)
-            hipLaunchKernelGGL((slot_mmse_fp32_kernel<0, true, true>), g, b, lds, st, d_block_, i, oc, ot, q, unb,
-                               thr);
+            hipLaunchKernelGGL((slot_mmse_fp32_kernel<0, true, true>), g, b, lds, st, d_block_, i, oc, ot, unb, thr);
         else if (mmse_keep_ && lf)
-            hipLaunchKernelGGL((slot_mmse_fp32_kernel<4, false>), g, b, lds, st, d_block_, i, oc, ot, q, unb, thr);
+            hipLaunchKernelGGL((slot_mmse_fp32_kernel<4, false>), g, b, lds, st, d_block_, i, oc, ot, unb, thr);
         else if (mmse_keep_)
-            hipLaunchKernelGGL((slot_mmse_fp32_kernel<4, true>), g, b, lds, st, d_block_, i, oc, ot, q, unb, thr);
+            hipLaunchKernelGGL((slot_mmse_fp32_kernel<4, true>), g, b, lds, st, d_block_, i, oc, ot, unb, thr);
         else if (lf)
-            hipLaunchKernelGGL((slot_mmse_fp32_kernel<0, false>), g, b, lds, st, d_block_, i, oc, ot, q, unb, thr);
+            hipLaunchKernelGGL((slot_mmse_fp32_kernel<0, false>), g, b, lds, st, d_block_, i, oc, ot, unb, thr);
         else
-            hipLaunchKernelGGL((slot_mmse_fp32_kernel<0, true>), g, b, lds, st, d_block_, i, oc, ot, q, unb, thr);
+            hipLaunchKernelGGL((slot_mmse_fp32_kernel<0, true>), g, b, lds, st, d_block_, i, oc, ot, unb, thr);
         return hipGetLastError();
     }
-    if (l.kind == 1) {
-        if (c16_)
-            hipLaunchKernelGGL(slot_mmse_fp64_kernel<true>, g, b, 0, st, d_block_, i, oc, ot, q, unb);
-        else
-            hipLaunchKernelGGL(slot_mmse_fp64_kernel<false>, g, b, 0, st, d_block_, i, oc, ot, q, unb);
-        return hipGetLastError();
-    }
-    // the first apply kernel zeroes the MMSE FP32 pass's counter (launch 0) once both MMSE passes are done
-    Counters *rs = (i > 0 && launches_[i - 1].kind < 2 && launches_[0].kind == 0) ? d_counters_ : nullptr;
     if (l.kind == 3) {  // cells with solve records: plans and eager batches only (one tile per workgroup)
         if (!lf) return hipErrorInvalidValue;
 #define PE_REC_CASE(PP)                                                                                          \
     if (l.prec == PP) {                                                                                          \
-        hipLaunchKernelGGL((slot_apply_rec_kernel<PP>), g, b, 0, st, d_block_, i, oc, ot, rs);                   \
+        hipLaunchKernelGGL((slot_apply_rec_kernel<PP>), g, b, 0, st, d_block_, i, oc, ot);                   \
         return hipGetLastError();                                                                                \
     }
         PE_REC_CASE(1) PE_REC_CASE(2) PE_REC_CASE(3) PE_REC_CASE(4)
@@ -543,9 +498,9 @@ hipError_t SlotExec::launch_one(int i, hipStream_t st, bool exact) const {
 #define PE_APPLY_CASE(KK, PP)                                                                                    \
     if (l.cls == KK && l.prec == PP) {                                                                           \
         if (lf)                                                                                                  \
-            hipLaunchKernelGGL((slot_apply_kernel<KK, PP, false>), g, b, 0, st, d_block_, i, oc, ot, rs);        \
+            hipLaunchKernelGGL((slot_apply_kernel<KK, PP, false>), g, b, 0, st, d_block_, i, oc, ot);        \
         else                                                                                                     \
-            hipLaunchKernelGGL((slot_apply_kernel<KK, PP, true>), g, b, 0, st, d_block_, i, oc, ot, rs);         \
+            hipLaunchKernelGGL((slot_apply_kernel<KK, PP, true>), g, b, 0, st, d_block_, i, oc, ot);         \
         return hipGetLastError();                                                                                \
     }
     PE_APPLY_CASE(4, 1) PE_APPLY_CASE(4, 2) PE_APPLY_CASE(4, 3) PE_APPLY_CASE(4, 4)
@@ -555,7 +510,7 @@ hipError_t SlotExec::launch_one(int i, hipStream_t st, bool exact) const {
     return hipErrorInvalidValue;
 }
 
-// The whole sequence on `st`: MMSE FP32 pass, FP64 pass, one fused apply/demap launch per precision.
+// The whole sequence on `st`: the MMSE kernel, one fused apply/demap launch per precision.
 pe_status SlotExec::launch(hipStream_t st, std::string &err, bool exact) {
     hipError_t e = hipSuccess;
     for (size_t i = 0; e == hipSuccess && i < launches_.size(); ++i) e = launch_one((int)i, st, exact);

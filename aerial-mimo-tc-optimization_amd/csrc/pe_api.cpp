@@ -49,6 +49,7 @@ bool valid_prec(int p) { return p >= PE_PREC_F32 && p <= PE_PREC_BF16_SPLIT; }
 //   coef <L> <S> <A> <prec> <f32_kc> <mfma_wpb> <mfma_spw>
 //   mmse <L> <A> <nsc_max> <lanes_per_subcarrier> [<subcarriers_per_tile> [<waves_per_workgroup> [<keep>]]]
 //        (tile <= 64 x waves and <= 128; keep 1: rows kept in registers where the tile allows it)
+//   solve <L> <A> <nsc_max> <subcarriers_per_tile> <waves_per_workgroup>   (the one-pass chain's solve kernel)
 //   slot <mmse_subcarriers_per_tile> <apply_subcarriers_per_tile> [<mmse_waves_per_workgroup> [<mmse_keep>]]
 // Loaded from PE_TUNE_TABLE (pe_init), pe_tune_load(), or else once from tune_table.txt beside the library.
 struct TileEntry {
@@ -58,9 +59,13 @@ struct TileEntry {
 struct MmseEntry {
     int L, A, nsc_max, lanes, tile, waves, keep;  // tile 0: default for the size
 };
+struct SolveEntry {
+    int L, A, nsc_max, tile, waves;
+};
 std::mutex g_table_mu;
 std::vector<TileEntry> g_table;
 std::vector<MmseEntry> g_mmse;
+std::vector<SolveEntry> g_solve;
 pe::SlotTile g_slot_tile;  // zero fields: executor defaults
 bool g_table_set = false;  // a table was loaded or cleared explicitly
 pe::CoefTile g_default_tile;
@@ -70,12 +75,14 @@ int load_table_file(const char *path) {
     if (!f) return -1;
     std::vector<TileEntry> t;
     std::vector<MmseEntry> m;
+    std::vector<SolveEntry> sv;
     pe::SlotTile st;
     char line[256];
     while (fgets(line, sizeof(line), f)) {
         if (line[0] == '#' || line[0] == '\n') continue;
         TileEntry e;
         MmseEntry me;
+        SolveEntry se;
         int kc = 0, wpb = 0, spw = 0, a1 = 0, a2 = 0, a3 = 0, a4 = 0;
         if (sscanf(line, "coef %d %d %d %d %d %d %d", &e.L, &e.S, &e.A, &e.prec, &kc, &wpb, &spw) == 7) {
             e.tile.f32_kc = (kc == 16 || kc == 4) ? kc : 8;
@@ -89,20 +96,23 @@ int load_table_file(const char *path) {
             me.waves = (a3 == 2 || a3 == 4 || a3 == 8) ? a3 : 1;
             me.tile = (a1 >= 1 && a1 <= 64 * me.waves && a1 <= 128) ? a1 : 0;  // 128 x 84 floats: 43 KB of LDS
             m.push_back(me);
-        } else if (sscanf(line, "slot %d %d %d %d", &a1, &a2, &a3, &a4) >= 2) {
-            st.mmse_keep = a4 == 1 ? 1 : 0;
-            st.mmse_waves = (a3 == 2 || a3 == 4 || a3 == 8) ? a3 : 1;
+        } else if (sscanf(line, "solve %d %d %d %d %d", &se.L, &se.A, &se.nsc_max, &se.tile, &se.waves) == 5) {
+            se.waves = (se.waves == 2 || se.waves == 4 || se.waves == 8) ? se.waves : 1;
+            if (se.tile >= 1 && se.tile <= 64 * se.waves && se.tile <= 128) sv.push_back(se);
+        } else if (sscanf(line, "slot %d %d %d", &a1, &a2, &a3) >= 1) {
+            st.mmse_keep = a3 == 1 ? 1 : 0;
+            st.mmse_waves = (a2 == 2 || a2 == 4 || a2 == 8) ? a2 : 1;
             st.mmse_tile = (a1 >= 1 && a1 <= 64 * st.mmse_waves && a1 <= 128) ? a1 : 0;
-            st.apply_tile = (a2 >= 1 && a2 <= 64) ? a2 : 0;
         }
     }
     fclose(f);
     std::lock_guard<std::mutex> lk(g_table_mu);
     g_table.swap(t);
     g_mmse.swap(m);
+    g_solve.swap(sv);
     g_slot_tile = st;
     g_table_set = true;
-    return (int)(g_table.size() + g_mmse.size());
+    return (int)(g_table.size() + g_mmse.size() + g_solve.size());
 }
 
 // tune_table.txt beside this library, read once when no table was set explicitly
@@ -198,6 +208,24 @@ MmseTile mmse_tile_for(int L, int A, int nSC) {
     return t;
 }
 
+MmseTile solve_tile_for(int L, int A, int nSC) {
+    load_default_table();
+    {
+        std::lock_guard<std::mutex> lk(g_table_mu);
+        for (const SolveEntry &e : g_solve)
+            if ((e.L < 0 || e.L == L) && (e.A < 0 || e.A == A) && (e.nsc_max < 0 || nSC <= e.nsc_max)) {
+                MmseTile t;
+                t.tile = e.tile;
+                t.waves = e.waves;
+                return t;
+            }
+    }
+    MmseTile t;
+    t.tile = 64;
+    t.waves = 4;
+    return t;
+}
+
 SlotTile slot_tile() {
     load_default_table();
     std::lock_guard<std::mutex> lk(g_table_mu);
@@ -246,6 +274,7 @@ int pe_tune_load(const char *path) {
         std::lock_guard<std::mutex> lk(g_table_mu);
         g_table.clear();
         g_mmse.clear();
+        g_solve.clear();
         g_slot_tile = pe::SlotTile();
         g_table_set = true;
         return 0;

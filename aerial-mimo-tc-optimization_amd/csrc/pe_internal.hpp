@@ -27,12 +27,15 @@ struct MmseTile {
     int keep = 0;  // 1: antenna rows stay in registers from the Gram to the coefficients
 };
 MmseTile mmse_tile_for(int nLayer, int nAnt, int nSC);
+// Tile of the one-pass chain's solve kernel (Gram and algebra only, no coefficient phase): a "solve" line of the
+// table, else 64 subcarriers on 4 waves, so that the one-lane-per-subcarrier algebra runs on a full wave (measured
+// on 32 cells of 8 layers x 64 antennas: 47.7 us against 63.3 us with the coefficient kernel's 8 on 2 waves).
+MmseTile solve_tile_for(int nLayer, int nAnt, int nSC);
 // Multi-cell executor tiles (zero: executor default).
 struct SlotTile {
     int mmse_tile = 0;   // subcarriers per MMSE tile
     int mmse_waves = 0;  // waves per MMSE workgroup
     int mmse_keep = 0;   // MMSE keeps antenna rows in registers where the tile allows it
-    int apply_tile = 0;  // subcarriers per apply tile
 };
 SlotTile slot_tile();
 
@@ -88,18 +91,15 @@ hipError_t launch_bfp_decompress(const uint8_t *in, float2 *Y, int nPrb, int nSy
                                  size_t row_stride, float scale, hipStream_t st);
 // Streaming probes (HBM ceiling): read_only -> reads src, else copies src to dst.
 hipError_t launch_hbm_probe(const void *src, void *dst, size_t bytes, int read_only, int grid, hipStream_t st);
-// MMSE: FP32 pass (flags subcarriers for FP64) then the FP64 pass over flagged subcarriers.
+// MMSE in one launch: FP32 tile pass; the subcarriers it flags are solved in FP64 by the same workgroup.
 hipError_t launch_gram_mmse(const float2 *H, const float *sigma2, float2 *C, float *noise_inv, uint8_t *flags,
                             int nSC, int L, int A, bool unbiased, float cond_threshold, hipStream_t st);
 // The same with the coefficients written in the binary16 operand format (A <= 128).
 hipError_t launch_gram_mmse_c16(const float2 *H, const float *sigma2, uint32_t *C16, int *cexp, float *noise_inv,
                                 uint8_t *flags, int nSC, int L, int A, bool unbiased, float cond_threshold,
                                 hipStream_t st);
-// The FP64 pass alone (subcarriers the FP32 pass flagged): C, noise_inv and the final flags.
-hipError_t launch_gram_mmse_fp64_pass(const float2 *H, const float *sigma2, float2 *C, float *noise_inv,
-                                      uint8_t *flags, int nSC, int L, int A, bool unbiased, hipStream_t st);
-// MMSE + fused MFMA apply + demap without the coefficients in HBM (prec 1..4; C written for the FP64 pass's
-// subcarriers, or for all with write_c; llr_f16 -> binary16 LLRs).
+// MMSE + fused MFMA apply + demap without the coefficients in HBM (prec 1..4; C written for the subcarriers solved
+// in FP64, or for all with write_c; llr_f16 -> binary16 LLRs).
 hipError_t launch_mmse_apply_demap(const float2 *H, const float *sigma2, const float2 *Y, float2 *C,
                                    float *noise_inv, uint8_t *flags, void *llr, int nSC, int L, int S, int A,
                                    int nPamBits, int prec, bool unbiased, float thr, int llr_f16, int write_c,

@@ -8,7 +8,7 @@
 //   2. arrivals: jobs whose arrival time has passed join a FIFO;
 //   3. admission and launch: whenever a stream is free, the FIFO head is admitted while its buffers (C,
 //      noise_inv, LLR) fit the device arena (the HBM budget), up to max_batch_jobs; the batch runs as one
-//      launch of that stream's slot executor (MMSE, FP64 pass, fused apply + demap; eagerly with grids
+//      launch of that stream's slot executor (MMSE, fused apply + demap; eagerly with grids
 //      sized to the batch, or as a replay of its captured graph).
 // Shedding (shed_late): 1 drops a job whose deadline has passed before launch; 2 (deadline-aware admission)
 // drops a job whose predicted completion -- its age plus the predicted latency of the batch it would join --

@@ -31,11 +31,6 @@ struct Tile {
     int cell, sc0, n, pad;
 };
 
-struct Counters {
-    unsigned flagged;  // subcarriers the MMSE FP32 pass handed to the FP64 pass
-    unsigned pad;
-};
-
 // Per-launch tile range, read by the kernels from device memory so that one captured graph serves any batch.
 struct LaunchRange {
     int tile0, ntiles;
@@ -46,7 +41,7 @@ public:
     static constexpr int kApplyTile = 4;  // subcarriers per stage-2 tile (one per wave of a 4-wave workgroup)
 
     struct Launch {
-        int kind;  // 0 mmse fp32, 1 mmse fp64, 2 apply, 3 apply forming coefficients from solve records
+        int kind;  // 0 mmse, 2 apply, 3 apply forming coefficients from solve records
         int cls;   // apply: largest antenna class KS of the launch (4 or 8)
         int prec;
         int grid, block;  // workgroups (each takes tiles g, g + grid, ...) and threads per workgroup
@@ -98,7 +93,6 @@ private:
     // [Tile x max_tiles]
     size_t off_cells_ = 0, off_tiles_ = 0, block_bytes_ = 0, used_bytes_ = 0;
     unsigned char *h_block_ = nullptr, *d_block_ = nullptr;
-    Counters *d_counters_ = nullptr;
     uint8_t *d_flags_ = nullptr;
     int staged_tiles_ = 0;
     std::vector<int> staged_ntiles_;  // per launch, of the staged batch

@@ -301,7 +301,7 @@ def time_slots(plans, mode, steps):
 
 
 def multicell_bench(dev, steps):
-    """24 uneven cells; one slot = the slot plan's kernel sequence (MMSE FP32 and FP64 passes, one fused
+    """24 uneven cells; one slot = the slot plan's kernel sequence (MMSE with its in-kernel FP64 fallback, one fused
     apply/demap launch) as one graph replay, and the same launched eagerly; two slot buffer sets alternate.
     Also the latency of one slot on an idle GPU (events around the call, so host submission is included)."""
     import torch

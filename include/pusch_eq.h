@@ -82,11 +82,13 @@ const char *pe_status_string(pe_status s);
    device without it. */
 pe_status pe_init(int device);
 /* Replaces the tile table with the file at `path` (NULL: no table, built-in defaults). Returns the number of
-   coef and mmse entries read (0 for NULL), or -1 if the file cannot be read; malformed lines are skipped. Lines, first match wins, -1 = any:
+   coef, mmse and solve entries read (0 for NULL), or -1 if the file cannot be read; malformed lines are
+   skipped. Lines, first match wins, -1 = any:
      coef <nLayer> <nSym> <nAnt> <prec> <f32_kc: 4|8|16> <mfma_wpb: 1|2|4> <mfma_spw: 1..8>
      mmse <nLayer> <nAnt> <nSC_max> <lanes_per_subcarrier: 4|8|16> [<subcarriers_per_tile> [<waves: 1|2|4|8>
           [<keep_rows: 0|1>]]]
-     slot <mmse_subcarriers_per_tile> <apply_subcarriers_per_tile> [<mmse_waves: 1|2|4|8> [<keep_rows: 0|1>]]
+     solve <nLayer> <nAnt> <nSC_max> <subcarriers_per_tile> <waves: 1|2|4|8>   (pe_mmse_apply_demap's solve kernel)
+     slot <mmse_subcarriers_per_tile> [<mmse_waves: 1|2|4|8> [<keep_rows: 0|1>]]   (multi-cell MMSE kernel)
    The lanes of an mmse line set the summation order of the Gram matrix and are taken from the first line
    matching (nLayer, nAnt) whatever nSC_max, so every entry point computes the same coefficients for a
    subcarrier; tile sizes and waves change only speed. Without an explicit table the library reads
@@ -129,10 +131,10 @@ pe_status pe_gram(const pe_c64 *M, const pe_c64 *H, pe_c64 *G, int nSC, int nLay
    W = Ginv H^H (the MMSE matrix), mu_l = 1 - sigma2 * Ginv[l][l], noise_inv_l = mu_l / (1 - mu_l)
    (post-equalization SINR of the unbiased estimate). C = W, or diag(1/mu) W with PE_MMSE_UNBIASED.
    The solve runs in FP32; subcarriers whose 1-norm condition number kappa_1(G) exceeds cond_threshold
-   (<= 0 selects 1e4), or whose FP32 Cholesky breaks down, are recomputed in FP64.
+   (<= 0 selects 1e4), or whose FP32 Cholesky breaks down, are recomputed in FP64 by the same kernel.
    The reference only names this stage (aerial_bottleneck.cu:51-53: eqMmseCoefCompHighMimoKernel).
-   sigma2[nSC] > 0; noise_inv[nSC][nLayer]; sc_flags[nSC] (required: it also carries the hand-over from the
-   FP32 pass to the FP64 pass). 1 <= nLayer <= 8, 1 <= nAnt <= 1024. */
+   sigma2[nSC] > 0; noise_inv[nSC][nLayer]; sc_flags[nSC] (required). 1 <= nLayer <= 8,
+   1 <= nAnt <= 1024. */
 pe_status pe_gram_mmse(const pe_c64 *H, const float *sigma2, pe_c64 *C, float *noise_inv, uint8_t *sc_flags,
                        int nSC, int nLayer, int nAnt, unsigned flags, float cond_threshold, void *stream);
 
@@ -348,13 +350,13 @@ typedef struct pe_slot_plan pe_slot_plan;
                                    as pe_mmse_apply_demap does. Applies to cells with H, an MFMA precision other
                                    than PE_PREC_F16_C16, 32 <= nAnt <= 64 and nLayer >= 4 (where it measured
                                    faster); LLRs and noise_inv are unchanged bit for bit, C holds coefficients only
-                                   where the FP64 pass ran. */
+                                   where the solve ran in FP64. */
 pe_status pe_slot_plan_create(const pe_cell_desc *cells, int nCells, unsigned flags, pe_slot_plan **plan);
 /* Rewrites the per-cell pointers of an existing plan (same shapes) for the next slot, ordered on `stream`. */
 pe_status pe_slot_plan_update(pe_slot_plan *plan, const pe_cell_desc *cells, int nCells, void *stream);
-/* Enqueues the slot on `stream`: the MMSE FP32 pass, the FP64 pass over flagged subcarriers and one fused
-   apply/demap kernel per precision, one workgroup per tile. Graph-capturable: the flag counter is zeroed by
-   a kernel, no memset node is recorded. Each cell's results are bit-identical to the cell run alone through
+/* Enqueues the slot on `stream`: the MMSE kernel (FP32, with the flagged subcarriers solved in FP64 by the
+   same workgroups) and one fused apply/demap kernel per precision, one workgroup per tile. Graph-capturable:
+   no memset node is recorded. Each cell's results are bit-identical to the cell run alone through
    pe_gram_mmse and pe_coef_apply_demap. */
 pe_status pe_slot_plan_launch(pe_slot_plan *plan, void *stream);
 /* Captures pe_slot_plan_launch into a hipGraph on first use and replays it on `stream`. */

@@ -216,7 +216,7 @@ def test_gram_bitexact(po, pe, dev, L, A, nsc):
                                          (1, 8, 33, 0.0), (8, 128, 64, 25.0), (5, 20, 40, 15.0)])
 def test_mmse_vs_fp64(po, pe, dev, unbiased, L, A, nsc, snr):
     """FP32 solve on well-conditioned Rayleigh channels: per-subcarrier norm-wise relative error of C within
-    1e-4 and noise_inv within 1e-3 of the FP64 oracle; no subcarrier needs the FP64 pass."""
+    1e-4 and noise_inv within 1e-3 of the FP64 oracle; no subcarrier needs the FP64 solve."""
     rng = np.random.default_rng(int(snr) + L * 3 + A)
     H = po.rayleigh_h(rng, nsc, A, L)
     s2 = np.full(nsc, 10 ** (-snr / 10), np.float32)
@@ -337,7 +337,7 @@ def test_c16_apply_wide_range_keeps_accuracy(po, pe, dev):
 
 
 def _mmse_c16_case(po, kind, rng):
-    if kind == "ill":  # every other subcarrier goes through the FP64 pass
+    if kind == "ill":  # every other subcarrier goes through the FP64 solve
         H = po.rayleigh_h(rng, 64, 32, 4)
         H[::2, :, 3] = H[::2, :, 2] + 1e-4 * H[::2, :, 3]
         return H, np.full(64, 1e-9, np.float32)
@@ -355,7 +355,7 @@ def _mmse_c16_case(po, kind, rng):
 @pytest.mark.parametrize("kind", ["l8a64", "l2a128", "l5a24", "l8a128", "ill", "singular"])
 def test_gram_mmse_c16_equals_conversion_of_c64(po, pe, dev, kind):
     """pe_gram_mmse_c16 writes exactly what pe_coef_to_c16 makes of pe_gram_mmse's complex64 output (both
-    passes, the FP64 pass and singular subcarriers included); noise_inv and flags unchanged."""
+    FP32 and FP64 solves, singular subcarriers included); noise_inv and flags unchanged."""
     rng = np.random.default_rng(len(kind))
     H, s2 = _mmse_c16_case(po, kind, rng)
     dH, ds2 = dev_c64(H, dev), dev_f32(s2, dev)

@@ -70,8 +70,8 @@ def test_biased_coefficients(pe, dev):
 @pytest.mark.parametrize("L,A,prec,thr", [(8, 64, "f16", 2.0), (4, 32, "bf16_split", 2.0),
                                           (2, 128, "f16_split", 1.01)])
 def test_fp64_fallback(pe, dev, L, A, prec, thr):
-    """A condition threshold just above 1 sends nearly every subcarrier to the FP64 pass: those are applied by
-    the flagged-subcarrier kernel from the coefficients the FP64 pass wrote."""
+    """A condition threshold just above 1 sends nearly every subcarrier to the FP64 solve: those are applied by
+    the flagged-subcarrier path from the coefficients the FP64 solve wrote."""
     fl = _check(pe, dev, 613, L, 11, A, 4, prec, thr=thr, write_c=True)
     assert int((fl != 0).sum()) > 500
 

@@ -136,7 +136,7 @@ def test_plan_rejects_mixed_coefficient_formats(po, pe, dev):
 @pytest.mark.parametrize("graph", [False, True])
 def test_plan_solve_records_match_cells_alone(po, pe, dev, graph):
     """PE_SLOT_SOLVE_RECORDS: stage 1 keeps compact solve records in C and stage 2 forms the coefficients on chip.
-    LLRs and noise_inv equal the cells alone bit for bit, including subcarriers the FP64 pass solves (every third
+    LLRs and noise_inv equal the cells alone bit for bit, including subcarriers solved in FP64 (every third
     subcarrier of one cell has two nearly parallel layers) and cells the flag leaves on the coefficient path
     (128 antennas; 8 layers on 4 antennas, whose record does not fit the block), over repeated launches and a
     pointer update."""

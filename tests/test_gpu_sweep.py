@@ -58,7 +58,7 @@ def test_sweep_cell(po, pe, dev, cell, prec):
     llr_r = cell["llr_r"]
     eps = EPS_REL * np.median(np.abs(llr_r))
     flips = ((llr < 0) != (llr_r < 0)) & (np.abs(llr_r) >= eps)
-    print("A=%d L=%d q=%d snr=%.1f prec=%s apply EVM %.1f dB, fp64 pass %d SCs, flips %d" % (
+    print("A=%d L=%d q=%d snr=%.1f prec=%s apply EVM %.1f dB, fp64 solves %d SCs, flips %d" % (
         cell["A"], cell["L"], q, cell["snr"], prec, db, int((fl.cpu().numpy() != 0).sum()), flips.sum()))
     if prec == "f32":
         assert np.array_equal(yeq, po.coef_apply_v1_f32(Cn, inp["Y"]))

@@ -14,7 +14,7 @@ def _inputs(po, dev, seed, nsc, A, L, q, snr, ill_every=0):
     inp = po.chain_inputs(rng, nsc, A, L, 11, q, snr)
     H = inp["H"].copy()
     s2 = inp["sigma2"].copy()
-    if ill_every:  # nearly parallel layers at low noise: these subcarriers go to the FP64 pass
+    if ill_every:  # nearly parallel layers at low noise: these subcarriers are solved in FP64
         H[::ill_every, :, 1] = H[::ill_every, :, 0] * (1 + 1e-4)
         s2[::ill_every] = 1e-6
     t = lambda a, dt=np.complex64: torch.from_numpy(np.ascontiguousarray(a, dtype=dt)).to(dev)
