@@ -17,6 +17,7 @@
 
 #include "pusch_eq.h"
 #include "pe_internal.hpp"
+#include "tune_parse.hpp"
 
 namespace {
 
@@ -45,74 +46,22 @@ pe_status from_hip(hipError_t e, const char *what) {
 bool valid_prec(int p) { return p >= PE_PREC_F32 && p <= PE_PREC_BF16_SPLIT; }
 
 // ---- tile table -----------------------------------------------------------------------------------------
-// Lines (first match wins, -1 = any):
-//   coef <L> <S> <A> <prec> <f32_kc> <mfma_wpb> <mfma_spw>
-//   mmse <L> <A> <nsc_max> <lanes_per_subcarrier> [<subcarriers_per_tile> [<waves_per_workgroup> [<keep>]]]
-//        (tile <= 64 x waves and <= 128; keep 1: rows kept in registers where the tile allows it)
-//   solve <L> <A> <nsc_max> <subcarriers_per_tile> <waves_per_workgroup>   (the one-pass chain's solve kernel)
-//   slot <mmse_subcarriers_per_tile> <apply_subcarriers_per_tile> [<mmse_waves_per_workgroup> [<mmse_keep>]]
-// Loaded from PE_TUNE_TABLE (pe_init), pe_tune_load(), or else once from tune_table.txt beside the library.
-struct TileEntry {
-    int L, S, A, prec;
-    pe::CoefTile tile;
-};
-struct MmseEntry {
-    int L, A, nsc_max, lanes, tile, waves, keep;  // tile 0: default for the size
-};
-struct SolveEntry {
-    int L, A, nsc_max, tile, waves;
-};
+// Format and parsing in tune_parse.hpp. Loaded from PE_TUNE_TABLE (pe_init), pe_tune_load(), or else once from
+// tune_table.txt beside the library.
 std::mutex g_table_mu;
-std::vector<TileEntry> g_table;
-std::vector<MmseEntry> g_mmse;
-std::vector<SolveEntry> g_solve;
-pe::SlotTile g_slot_tile;  // zero fields: executor defaults
+pe::tune::Table g_tune;    // guarded by g_table_mu
 bool g_table_set = false;  // a table was loaded or cleared explicitly
-pe::CoefTile g_default_tile;
 
 int load_table_file(const char *path) {
     FILE *f = fopen(path, "r");
     if (!f) return -1;
-    std::vector<TileEntry> t;
-    std::vector<MmseEntry> m;
-    std::vector<SolveEntry> sv;
-    pe::SlotTile st;
-    char line[256];
-    while (fgets(line, sizeof(line), f)) {
-        if (line[0] == '#' || line[0] == '\n') continue;
-        TileEntry e;
-        MmseEntry me;
-        SolveEntry se;
-        int kc = 0, wpb = 0, spw = 0, a1 = 0, a2 = 0, a3 = 0, a4 = 0;
-        if (sscanf(line, "coef %d %d %d %d %d %d %d", &e.L, &e.S, &e.A, &e.prec, &kc, &wpb, &spw) == 7) {
-            e.tile.f32_kc = (kc == 16 || kc == 4) ? kc : 8;
-            e.tile.mfma_waves = (wpb == 2 || wpb == 4) ? wpb : 1;
-            e.tile.mfma_spw = spw > 1 ? spw : 1;
-            t.push_back(e);
-        } else if (sscanf(line, "mmse %d %d %d %d %d %d %d", &me.L, &me.A, &me.nsc_max, &me.lanes, &a1, &a3, &a4) >=
-                   4) {
-            me.keep = a4 == 1 ? 1 : 0;
-            me.lanes = (me.lanes == 4 || me.lanes == 16) ? me.lanes : 8;
-            me.waves = (a3 == 2 || a3 == 4 || a3 == 8) ? a3 : 1;
-            me.tile = (a1 >= 1 && a1 <= 64 * me.waves && a1 <= 128) ? a1 : 0;  // 128 x 84 floats: 43 KB of LDS
-            m.push_back(me);
-        } else if (sscanf(line, "solve %d %d %d %d %d", &se.L, &se.A, &se.nsc_max, &se.tile, &se.waves) == 5) {
-            se.waves = (se.waves == 2 || se.waves == 4 || se.waves == 8) ? se.waves : 1;
-            if (se.tile >= 1 && se.tile <= 64 * se.waves && se.tile <= 128) sv.push_back(se);
-        } else if (sscanf(line, "slot %d %d %d", &a1, &a2, &a3) >= 1) {
-            st.mmse_keep = a3 == 1 ? 1 : 0;
-            st.mmse_waves = (a2 == 2 || a2 == 4 || a2 == 8) ? a2 : 1;
-            st.mmse_tile = (a1 >= 1 && a1 <= 64 * st.mmse_waves && a1 <= 128) ? a1 : 0;
-        }
-    }
+    pe::tune::Table t;
+    const int n = pe::tune::parse(f, t);
     fclose(f);
     std::lock_guard<std::mutex> lk(g_table_mu);
-    g_table.swap(t);
-    g_mmse.swap(m);
-    g_solve.swap(sv);
-    g_slot_tile = st;
+    g_tune = std::move(t);
     g_table_set = true;
-    return (int)(g_table.size() + g_mmse.size() + g_solve.size());
+    return n;
 }
 
 // tune_table.txt beside this library, read once when no table was set explicitly
@@ -169,19 +118,24 @@ int device_cus() {
 CoefTile coef_tile_for(int L, int S, int A, int prec) {
     load_default_table();
     std::lock_guard<std::mutex> lk(g_table_mu);
-    for (const TileEntry &e : g_table) {
+    for (const tune::Coef &e : g_tune.coef) {
         if ((e.L < 0 || e.L == L) && (e.S < 0 || e.S == S) && (e.A < 0 || e.A == A) &&
-            (e.prec < 0 || e.prec == prec))
-            return e.tile;
+            (e.prec < 0 || e.prec == prec)) {
+            CoefTile t;
+            t.f32_kc = e.f32_kc;
+            t.mfma_waves = e.mfma_waves;
+            t.mfma_spw = e.mfma_spw;
+            return t;
+        }
     }
-    return g_default_tile;
+    return CoefTile();
 }
 
 int mmse_lanes_for(int L, int A) {
     load_default_table();
     {
         std::lock_guard<std::mutex> lk(g_table_mu);
-        for (const MmseEntry &e : g_mmse)
+        for (const tune::Mmse &e : g_tune.mmse)
             if ((e.L < 0 || e.L == L) && (e.A < 0 || e.A == A)) return e.lanes;
     }
     return A >= 64 ? 16 : A >= 32 ? 8 : 4;
@@ -191,7 +145,7 @@ MmseTile mmse_tile_for(int L, int A, int nSC) {
     load_default_table();
     {
         std::lock_guard<std::mutex> lk(g_table_mu);
-        for (const MmseEntry &e : g_mmse)
+        for (const tune::Mmse &e : g_tune.mmse)
             if ((e.L < 0 || e.L == L) && (e.A < 0 || e.A == A) && (e.nsc_max < 0 || nSC <= e.nsc_max) &&
                 e.tile > 0) {
                 MmseTile t;
@@ -212,7 +166,7 @@ MmseTile solve_tile_for(int L, int A, int nSC) {
     load_default_table();
     {
         std::lock_guard<std::mutex> lk(g_table_mu);
-        for (const SolveEntry &e : g_solve)
+        for (const tune::Solve &e : g_tune.solve)
             if ((e.L < 0 || e.L == L) && (e.A < 0 || e.A == A) && (e.nsc_max < 0 || nSC <= e.nsc_max)) {
                 MmseTile t;
                 t.tile = e.tile;
@@ -229,7 +183,11 @@ MmseTile solve_tile_for(int L, int A, int nSC) {
 SlotTile slot_tile() {
     load_default_table();
     std::lock_guard<std::mutex> lk(g_table_mu);
-    return g_slot_tile;
+    SlotTile t;
+    t.mmse_tile = g_tune.slot.mmse_tile;
+    t.mmse_waves = g_tune.slot.mmse_waves;
+    t.mmse_keep = g_tune.slot.mmse_keep;
+    return t;
 }
 }  // namespace pe
 
@@ -272,10 +230,7 @@ pe_status pe_init(int device) {
 int pe_tune_load(const char *path) {
     if (!path) {
         std::lock_guard<std::mutex> lk(g_table_mu);
-        g_table.clear();
-        g_mmse.clear();
-        g_solve.clear();
-        g_slot_tile = pe::SlotTile();
+        g_tune = pe::tune::Table();
         g_table_set = true;
         return 0;
     }

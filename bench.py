@@ -101,13 +101,21 @@ def time_launches(fn, bufs, steps, warmup):
 
 
 def cpu_model():
+    """The host CPU as /proc/cpuinfo names it: model name, family / model / stepping, cache, cores per socket."""
+    info = {}
     try:
         for ln in open("/proc/cpuinfo"):
-            if ln.startswith("model name"):
-                return ln.split(":", 1)[1].strip()
+            if not ln.strip():
+                break  # first processor only
+            k, _, v = ln.partition(":")
+            info.setdefault(k.strip(), v.strip())
     except OSError:
-        pass
-    return None
+        return None
+    if "model name" not in info:
+        return None
+    extra = ["%s %s" % (k, info[k]) for k in ("vendor_id", "cpu family", "model", "stepping", "microcode",
+                                               "cache size", "cpu cores", "siblings") if k in info]
+    return "%s (%s)" % (info["model name"], ", ".join(extra))
 
 
 def cpu_baseline(C, Y, ni, seconds):
@@ -138,7 +146,8 @@ def cpu_baseline(C, Y, ni, seconds):
                       "scalar antenna-serial FP32 chain + FP32 demapper, OpenMP over (subcarrier, layer), "
                       "%.1f s; then %d repetitions on 1 thread, %.1f s" % (reps, el, reps1, el1),
             "gbs": v * bytes_sc / 1e9, "single_thread_value": v1, "single_thread_gbs": v1 * bytes_sc / 1e9,
-            "cpu_model": cpu_model(), "host_cpus": os.cpu_count()}
+            "cpu_model": cpu_model(), "host_cpus": os.cpu_count(),
+            "build": "oracle/Makefile: gcc -O3 -mfma -mavx2 -ffp-contract=off -fopenmp"}
 
 
 def hbm_ceiling(dev):

@@ -14,7 +14,8 @@ import os
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libpe_oracle.so")
+# PE_ORACLE_LIB names another build of pe_oracle.c (the AddressSanitizer build of the CPU checks)
+LIB_PATH = os.environ.get("PE_ORACLE_LIB") or os.path.join(_HERE, "libpe_oracle.so")
 _lib = None
 
 
