@@ -14,9 +14,9 @@ Candidates (see pe_tune_load in include/pusch_eq.h):
                 workgroup {1, 2, 4, 8} x {one Gram round per wave, two, one with the antenna rows kept in
                 registers} (tile = rounds x waves x 64 / lanes <= 128 subcarriers) per batch bucket (single
                 cell, large batch)
-    solve:      tile of pe_mmse_apply_demap's solve kernel, (subcarriers, waves) in {(16,4) .. (128,8)}, per
-                (L, A) that takes solve records (L >= 4, 32 <= A <= 64), timed on the whole one-pass chain at
-                32 cells x 1584 subcarriers
+    solve:      tile of pe_mmse_apply_demap's solve kernel, (subcarriers, waves) in {(8,1) .. (128,8)}, per
+                (L, A) that takes solve records (L >= 4, 32 <= A <= 64) and batch bucket (one cell, 32 cells),
+                timed on the whole one-pass chain
     slot:       MMSE (tile, waves, rows kept) in {(4,1,0) .. (32,8,1)} (24-cell plans)
 Timing: HIP events around 20 launches after 5 warm-up launches, alternating two buffer sets (for the apply
 their sum exceeds the 256 MB Infinity Cache, so launches stream from HBM as in production); single-cell MMSE
@@ -181,32 +181,40 @@ def slot_cells(dev, n, seed):
 
 
 def tune_solve(pe, dev, L, A, base):
-    """Subcarriers and waves per workgroup of the one-pass chain's solve kernel: pe_mmse_apply_demap on 32 cells
-    of 1584 subcarriers, 11 symbols, 16-QAM, two input sets alternating."""
+    """Subcarriers and waves per workgroup of the one-pass chain's solve kernel, timed on the whole chain
+    (pe_mmse_apply_demap, 11 symbols, QPSK, two input sets alternating) per batch bucket: one cell of 1584
+    subcarriers (single launches, latency) and 32 cells (throughput). Returns the table lines (one per bucket)."""
     import torch
-    nsc, S = 32 * 1584, 11
-    sets = []
-    for k in range(2):
-        g = torch.Generator(device=dev).manual_seed(20 + k)
-        H = torch.randn((nsc, A, L), dtype=torch.complex64, device=dev, generator=g) * (1.0 / A) ** 0.5
-        s2 = torch.full((nsc,), 0.03, device=dev)
-        Y = torch.randn((nsc, S, A), dtype=torch.complex64, device=dev, generator=g)
-        sets.append((H, s2, Y))
-    res = []
-    for T, W in ((16, 4), (32, 4), (32, 8), (64, 4), (64, 8), (128, 4), (128, 8)):
-        with_table(pe, ["solve %d %d -1 %d %d" % (L, A, T, W)] + base)
-        t = timed(lambda H, s2, Y: pe.mmse_apply_demap(H, s2, Y, 2, prec="f16"), sets)
-        res.append({"T": T, "W": W, "us": t * 1e6})
-    del sets
-    torch.cuda.empty_cache()
-    best = min(res, key=lambda r: r["us"])
-    return "solve %d %d -1 %d %d" % (L, A, best["T"], best["W"]), best, res
+    S = 11
+    lines, report = [], []
+    for nsc, bucket in ((1584, 4096), (32 * 1584, -1)):
+        sets = []
+        for k in range(2):
+            g = torch.Generator(device=dev).manual_seed(20 + k)
+            H = torch.randn((nsc, A, L), dtype=torch.complex64, device=dev, generator=g) * (1.0 / A) ** 0.5
+            s2 = torch.full((nsc,), 0.03, device=dev)
+            Y = torch.randn((nsc, S, A), dtype=torch.complex64, device=dev, generator=g)
+            sets.append((H, s2, Y))
+        clock = timed_single if nsc <= 4096 else timed
+        res = []
+        for T, W in ((8, 1), (8, 2), (16, 2), (16, 4), (32, 4), (32, 8), (64, 4), (64, 8), (128, 4), (128, 8)):
+            with_table(pe, ["solve %d %d %d %d %d" % (L, A, bucket, T, W)] + base)
+            t = clock(lambda H, s2, Y: pe.mmse_apply_demap(H, s2, Y, 1, prec="f16"), sets)
+            res.append({"nsc": nsc, "T": T, "W": W, "us": t * 1e6})
+        del sets
+        torch.cuda.empty_cache()
+        best = min(res, key=lambda r: r["us"])
+        lines.append("solve %d %d %d %d %d" % (L, A, bucket, best["T"], best["W"]))
+        report.extend(res)
+    return lines, report
 
 
 def tune_slot(pe, dev, base):
     """Subcarriers and waves per MMSE tile of the multi-cell executor, on two alternating 24-cell slot plans
     replayed as graphs."""
     import torch
+    side = torch.cuda.Stream()  # a created stream, as a receiver runs its slots
+    torch.cuda.set_stream(side)
     sets = [slot_cells(dev, 24, 77 + k) for k in range(2)]
     res = []
     for mt, mw, mk in ((4, 1, 0), (8, 1, 0), (8, 2, 0), (16, 1, 0), (16, 2, 0), (16, 4, 0), (32, 2, 0), (32, 4, 0),
@@ -217,6 +225,8 @@ def tune_slot(pe, dev, base):
         for pl in plans:
             pl.close()
         res.append({"mmse_tile": mt, "mmse_waves": mw, "mmse_keep": mk, "us": t * 1e6})
+    torch.cuda.synchronize()
+    torch.cuda.set_stream(torch.cuda.default_stream())
     best = min(res, key=lambda r: r["us"])
     return "slot %d %d %d" % (best["mmse_tile"], best["mmse_waves"], best["mmse_keep"]), best, res
 
@@ -266,10 +276,10 @@ def main():
     for A, L in itertools.product(As, Ls):
         if L < 4 or not 32 <= A <= 64:  # shapes without solve records run the coefficient kernels
             continue
-        sl, best, rep = tune_solve(pe, dev, L, A, lines)
-        lines.append(sl)
+        sl, rep = tune_solve(pe, dev, L, A, lines)
+        lines.extend(sl)
         report["solve"].append({"L": L, "A": A, "candidates": rep})
-        print("solve L=%d A=%3d  %s  %.1f us" % (L, A, sl, best["us"]), flush=True)
+        print("solve L=%d A=%3d  %s" % (L, A, "  ".join(sl)), flush=True)
     sline, best, rep = tune_slot(pe, dev, lines)
     lines.append(sline)
     report["slot"] = rep

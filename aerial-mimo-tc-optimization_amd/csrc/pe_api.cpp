@@ -174,6 +174,13 @@ MmseTile solve_tile_for(int L, int A, int nSC) {
                 return t;
             }
     }
+    if (nSC <= 4096) {  // a single cell: the coefficient kernel's tile (latency: many small workgroups)
+        const MmseTile m = mmse_tile_for(L, A, nSC);
+        MmseTile t;
+        t.tile = m.tile;
+        t.waves = m.waves;
+        return t;
+    }
     MmseTile t;
     t.tile = 64;
     t.waves = 4;

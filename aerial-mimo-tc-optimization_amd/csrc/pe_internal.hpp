@@ -28,8 +28,9 @@ struct MmseTile {
 };
 MmseTile mmse_tile_for(int nLayer, int nAnt, int nSC);
 // Tile of the one-pass chain's solve kernel (Gram and algebra only, no coefficient phase): a "solve" line of the
-// table, else 64 subcarriers on 4 waves, so that the one-lane-per-subcarrier algebra runs on a full wave (measured
-// on 32 cells of 8 layers x 64 antennas: 47.7 us against 63.3 us with the coefficient kernel's 8 on 2 waves).
+// table, else for a single cell (nSC <= 4096) the coefficient kernel's tile and for batches 64 subcarriers on 4
+// waves, so that the one-lane-per-subcarrier algebra runs on a full wave (measured on 32 cells of 8 layers x 64
+// antennas: 47.7 us against 63.3 us with the coefficient kernel's 8 on 2 waves).
 MmseTile solve_tile_for(int nLayer, int nAnt, int nSC);
 // Multi-cell executor tiles (zero: executor default).
 struct SlotTile {

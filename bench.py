@@ -314,6 +314,12 @@ def multicell_bench(dev, steps):
     apply/demap launch) as one graph replay, and the same launched eagerly; two slot buffer sets alternate.
     Also the latency of one slot on an idle GPU (events around the call, so host submission is included)."""
     import torch
+    with torch.cuda.stream(torch.cuda.Stream()):  # a receiver's own stream, not the legacy default stream
+        return _multicell_bench(dev, steps)
+
+
+def _multicell_bench(dev, steps):
+    import torch
     import pusch_eq as pe
     sets = [draw_cells(24, 42 + k, dev) for k in range(2)]
     plans = [pe.SlotPlan([c["desc"] for c in cs]) for cs in sets]
@@ -365,7 +371,7 @@ def multicell_bench(dev, steps):
             "solve_records": solve_records,
             "workload": "24 cells per slot, PRB {24,52,106,273}, antennas {16,32,64}, layers {2,4,8}, QAM "
                         "{16,64,256}, f16, cell c of a plan drawn from seed (42 or 43) + 1000 c; two plans "
-                        "alternate; one slot = MMSE + fused apply/demap, back-to-back slots",
+                        "alternate; one slot = MMSE + fused apply/demap, back-to-back slots on a created stream",
             "subcarriers_per_plan": nscs, "bytes_per_plan": bys,
             "subcarriers_per_slot": nsc, "slot_ms": t * 1e3, "subcarriers_per_s": nsc / t,
             "hbm_gbs": by / t / 1e9, "hbm_frac": by / t / 1e9 / HBM_PEAK_GBS, "bytes_per_slot": by,
@@ -613,12 +619,17 @@ def deployment_sweep(dev, seed=5, reps=10):
     short spin so that host launch overhead is excluded). The bf16 path at the -60 dB bar is plain bf16 where
     that meets it, else bf16_split; it is compared with the plain FP32 path. Also the chain from channel
     estimates at f16 (pe_gram_mmse + pe_coef_apply_demap against pe_mmse_apply_demap), timed the same way, and
-    whether the two give identical LLRs."""
+    whether the two give identical LLRs.
+    Roofline per configuration, f16 fused apply + demap: algorithmic bytes per launch (C and Y_rx in, LLRs and
+    noise_inv), and GB/s and HBM fraction twice: warm (the single-launch times above: the cell's buffers, at most
+    73 MB, stay in the 256 MB MALL between launches) and cold (each launch preceded by a 512 MB device write that
+    evicts them, so the inputs come from HBM)."""
     import numpy as np
     import torch
     import pusch_eq as pe
     rng = np.random.default_rng(seed)
     rows = []
+    flush = torch.empty(512 << 20, dtype=torch.uint8, device=dev)
     for A_ in (8, 16, 32, 64, 128):
         for prb in (25, 51, 106, 133, 273):
             for L_ in (2, 4, 8):
@@ -678,6 +689,25 @@ def deployment_sweep(dev, seed=5, reps=10):
                 row["us_chain_one_pass"] = dev_us(lambda: pe.mmse_apply_demap(H, sigma2, Y, q, prec="f16", C=C2,
                                                                               noise_inv=ni2, flags=fl2, out=llr2))
                 row["one_pass_identical"] = bool(torch.equal(llr1, llr2))
+                # roofline of the f16 fused apply + demap, warm (above) and cold (MALL evicted before each launch)
+                ts = []
+                for i in range(reps + 2):
+                    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                    flush.fill_(i & 255)
+                    a.record()
+                    pe.coef_apply_demap(C, Y, ni, q, prec="f16", out=llr)
+                    b.record()
+                    b.synchronize()
+                    if i >= 2:
+                        ts.append(a.elapsed_time(b) * 1e3)
+                ts.sort()
+                row["us_f16_cold"] = ts[len(ts) // 2]
+                by = nsc * (L_ * A_ * 8 + S * A_ * 8 + L_ * S * 2 * q * 4 + L_ * 4)
+                row["bytes_f16"] = by
+                row["gbs_f16_warm"] = by / row["us_f16"] / 1e3
+                row["gbs_f16_cold"] = by / row["us_f16_cold"] / 1e3
+                row["hbm_frac_f16_warm"] = row["gbs_f16_warm"] / HBM_PEAK_GBS
+                row["hbm_frac_f16_cold"] = row["gbs_f16_cold"] / HBM_PEAK_GBS
                 row["speedup_one_pass"] = row["us_chain_two_launches"] / row["us_chain_one_pass"]
                 bf = "bf16" if row["evm_db_bf16"] <= -60.0 else "bf16_split"
                 row["bf16_at_bar"] = bf
@@ -702,6 +732,19 @@ def deployment_sweep(dev, seed=5, reps=10):
     summ["one_pass_chain_speedup_median_64_antennas"] = statistics.median(
         r["speedup_one_pass"] for r in rows if r["A"] == 64)
     summ["one_pass_llrs_identical_all"] = all(r["one_pass_identical"] for r in rows)
+    for kind in ("cold", "warm"):
+        fr = sorted(rows, key=lambda r: r["hbm_frac_f16_" + kind])
+        summ["hbm_frac_median_f16_" + kind] = statistics.median(r["hbm_frac_f16_" + kind] for r in rows)
+        w = fr[0]
+        summ["hbm_frac_worst_f16_" + kind] = {"frac": w["hbm_frac_f16_" + kind], "A": w["A"], "prb": w["prb"],
+                                             "L": w["L"], "qam": w["qam"], "us": w["us_f16_" + kind]
+                                             if kind == "cold" else w["us_f16"], "bytes": w["bytes_f16"]}
+        b = fr[-1]
+        summ["hbm_frac_best_f16_" + kind] = {"frac": b["hbm_frac_f16_" + kind], "A": b["A"], "prb": b["prb"],
+                                            "L": b["L"], "qam": b["qam"]}
+    summ["roofline_note"] = ("one cell per launch: cold = MALL evicted before each launch (inputs from HBM), warm = "
+                             "the cell's buffers resident in the 256 MB MALL from the previous launch; "
+                             "small cells are launch-latency-bound either way")
     big = [r for r in rows if r["nsc"] == 3276]
     summ["largest_cells_bf16_at_bar_speedup_vs_f32_median"] = statistics.median(
         r["speedup_bf16_at_bar_vs_f32"] for r in big)
@@ -744,6 +787,7 @@ def main():
     ap.add_argument("--freq-interp-only", action="store_true",
                     help="only the DMRS frequency interpolation (10 launches per precision), for profiling")
     ap.add_argument("--sweep-out", default=None, help="write the deployment sweep table (JSON) here")
+    ap.add_argument("--sweep-only", action="store_true", help="only the deployment sweep, summary printed as JSON")
     ap.add_argument("--multicell-only", action="store_true",
                     help="only the 24-cell slot section (graph and eager slots), printed as JSON (for profiling)")
     ap.add_argument("--headline-only", action="store_true",
@@ -781,6 +825,13 @@ def main():
         return
     if args.srs_only:
         print(json.dumps(srs_bench(dev)), flush=True)
+        return
+    if args.sweep_only:
+        sweep, rows = deployment_sweep(dev)
+        if args.sweep_out:
+            with open(args.sweep_out, "w") as f:
+                json.dump({"summary": sweep, "rows": rows}, f, indent=1)
+        print(json.dumps(sweep), flush=True)
         return
     if args.fronthaul_only:
         print(json.dumps(fronthaul_bench(dev, args.cells, 10)), flush=True)
@@ -1012,8 +1063,11 @@ def main():
     sweep = None
     if not args.no_sweep:
         sweep, rows = deployment_sweep(dev)
-        log("deployment sweep: %d configs, bf16 at -60 dB vs f32 median speedup %.2fx" % (
-            sweep["configs"], sweep["bf16_at_bar_speedup_vs_f32_median"]))
+        log("deployment sweep: %d configs, bf16 at -60 dB vs f32 median speedup %.2fx; f16 HBM fraction cold "
+            "median %.3f worst %.3f (A=%d PRB=%d L=%d)" % (
+                sweep["configs"], sweep["bf16_at_bar_speedup_vs_f32_median"], sweep["hbm_frac_median_f16_cold"],
+                sweep["hbm_frac_worst_f16_cold"]["frac"], sweep["hbm_frac_worst_f16_cold"]["A"],
+                sweep["hbm_frac_worst_f16_cold"]["prb"], sweep["hbm_frac_worst_f16_cold"]["L"]))
         if args.sweep_out:
             with open(args.sweep_out, "w") as f:
                 json.dump({"summary": sweep, "rows": rows}, f, indent=1)
