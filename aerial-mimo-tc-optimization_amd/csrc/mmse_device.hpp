@@ -82,7 +82,21 @@ template <typename T> __device__ __forceinline__ T group_sum(T v, int P) {
 
 #pragma clang fp contract(off)
 
+// Group sums of a Gram matrix's lower triangle (diagonal imaginary parts set to zero).
+template <typename T, int L>
+__device__ __forceinline__ void group_sums(T (&gr)[Dim<L>::NT], T (&gi)[Dim<L>::NT], int P) {
+#pragma unroll
+    for (int i = 0; i < L; ++i)
+#pragma unroll
+        for (int j = 0; j <= i; ++j) {
+            gr[tri(i, j)] = group_sum(gr[tri(i, j)], P);
+            gi[tri(i, j)] = j < i ? group_sum(gi[tri(i, j)], P) : T(0);
+        }
+}
+
 // ---- Gram: this lane's partial sum G[i][j] = sum_a conj(H[a][i]) H[a][j], i >= j ---------------------------
+// The diagonal is real: its imaginary parts are neither accumulated nor summed (zero; the solve sets them to zero
+// anyway), which leaves every other entry's bits as they are.
 template <typename T, int L>
 __device__ __forceinline__ void load_row(const float2 *__restrict__ row, T (&vr)[L], T (&vi)[L]) {
     if constexpr (L % 2 == 0) {
@@ -129,14 +143,10 @@ __device__ __forceinline__ void gram_group_keep(const float2 *__restrict__ h, in
 #pragma unroll
             for (int j = 0; j <= i; ++j) {
                 gr[tri(i, j)] = fma_(hr[r][i], hr[r][j], fma_(hi[r][i], hi[r][j], gr[tri(i, j)]));
-                gi[tri(i, j)] = fma_(hr[r][i], hi[r][j], fma_(-hi[r][i], hr[r][j], gi[tri(i, j)]));
+                if (j < i) gi[tri(i, j)] = fma_(hr[r][i], hi[r][j], fma_(-hi[r][i], hr[r][j], gi[tri(i, j)]));
             }
     }
-#pragma unroll
-    for (int k = 0; k < Dim<L>::NT; ++k) {
-        gr[k] = group_sum(gr[k], P);
-        gi[k] = group_sum(gi[k], P);
-    }
+    group_sums<float, L>(gr, gi, P);
 }
 
 template <typename T, int L>
@@ -154,14 +164,10 @@ __device__ __forceinline__ void gram_group(const float2 *__restrict__ h, int A, 
 #pragma unroll
             for (int j = 0; j <= i; ++j) {
                 gr[tri(i, j)] = fma_(vr[i], vr[j], fma_(vi[i], vi[j], gr[tri(i, j)]));
-                gi[tri(i, j)] = fma_(vr[i], vi[j], fma_(-vi[i], vr[j], gi[tri(i, j)]));
+                if (j < i) gi[tri(i, j)] = fma_(vr[i], vi[j], fma_(-vi[i], vr[j], gi[tri(i, j)]));
             }
     }
-#pragma unroll
-    for (int k = 0; k < Dim<L>::NT; ++k) {
-        gr[k] = group_sum(gr[k], P);
-        gi[k] = group_sum(gi[k], P);
-    }
+    group_sums<T, L>(gr, gi, P);
 }
 
 // ---- the L x L algebra --------------------------------------------------------------------------------------
