@@ -3,10 +3,12 @@
 // 320 B at 8 layers, instead of the 4 KB coefficient block) and the fused apply forms the coefficients from
 // it and the antenna rows of H while the Y block is in flight.
 //
-//   1. mmse_solve_kernel: the MMSE kernel's Gram and L x L algebra (tile_solve) per tile, then the solve
-//      record of every subcarrier is copied from its LDS slot to the start of that subcarrier's block of the
-//      caller's C workspace; the subcarriers the FP32 algebra flags are solved in FP64 by the same workgroup,
-//      their coefficient blocks written to C; noise_inv and flags are written as by pe_gram_mmse.
+//   1. mmse_solve_kernel: the MMSE kernel's Gram and L x L algebra (tile_solve) per tile; the lane that solves
+//      a subcarrier stores its record at the start of that subcarrier's block of the caller's C workspace; the
+//      subcarriers the FP32 algebra flags are solved in FP64 by the same workgroup, their coefficient blocks
+//      written to C; noise_inv and flags are written as by pe_gram_mmse. Tiles are sized so that the batch
+//      spreads evenly over the workgroups resident at once (the launch lasts as long as its busiest compute
+//      unit; the tile changes no result).
 //   2. apply_solve_kernel, one subcarrier per wave: lane a reads antenna row a of H and forms the column
 //      C[.][a] with the operation sequence of the MMSE kernel's c_values (Ginv entries expanded once into a
 //      full matrix in LDS and read there by broadcast); the column goes straight into the operand layout of
@@ -18,6 +20,7 @@
 // Shapes for which this does not pay (fewer than 32 antennas or 4 layers: measured slower, single cells being
 // launch- and latency-bound) or cannot be done (more than 64 antennas: one column per lane) run pe_gram_mmse's
 // and the apply's kernels with C in HBM: the same kernels, hence the same results.
+#include <atomic>
 #include <mutex>
 
 #include "pe_internal.hpp"
@@ -73,9 +76,8 @@ __global__ __launch_bounds__(512) void mmse_solve_kernel(const float2 *__restric
     const long sc0 = (long)blockIdx.x * T;
     const int n = (int)min((long)T, (long)nSC - sc0);
     float kr[1][L], ki[1][L];
-    tile_solve<L, 1>(H, sigma2, noise_inv, flags, sc0, n, A, P, unbiased != 0, thr, lds_f, threadIdx.x, W, false,
-                     kr, ki);
-    rec::write_records<L>(C, sc0, n, A, lds_f, threadIdx.x, blockDim.x);  // not for flagged subcarriers
+    tile_solve<L, 1, true>(H, sigma2, noise_inv, flags, sc0, n, A, P, unbiased != 0, thr, lds_f, threadIdx.x, W,
+                           false, kr, ki, C);  // records of the subcarriers solved in FP32
     tile_fp64_inline<L>(H, sigma2, C, noise_inv, flags, sc0, n, A, unbiased != 0, lds_f, threadIdx.x, W);
 }
 
@@ -102,13 +104,37 @@ __global__ __launch_bounds__(64 * kWpb) void apply_solve_kernel(
     rec::apply_from_record<L, KS, PREC>(L, H, Y, C, noise_inv, llr, sc, S, A, npb, f16, write_c, ops, gfull, lane);
 }
 
+// Subcarriers per tile for nSC subcarriers on tiles of at most tmax: tmax when the launch has at most one
+// workgroup per compute unit; else the size that spreads the batch evenly over the rounds of `resident`
+// workgroups (e.g. 50,688 subcarriers, at most 128 per tile, 512 resident: 99 each, where tiles of 128 would
+// leave 140 compute units with two and 116 with one).
+int balanced_tile(long nSC, int tmax, long resident) {
+    const long tiles = (nSC + tmax - 1) / tmax;
+    if (tiles <= device_cus() || resident <= 0) return tmax;
+    const long rounds = (tiles + resident - 1) / resident;
+    return (int)((nSC + resident * rounds - 1) / (resident * rounds));
+}
+
 template <int L, int KS, int PREC>
 hipError_t launch_l(const float2 *H, const float *s2, const float2 *Y, float2 *C, float *ni, uint8_t *flags,
                     float *llr, int nSC, int S, int A, int npb, bool unbiased, float thr, int f16, int write_c,
                     hipStream_t st) {
     const int P = mmse_lanes_for(L, A);
     const MmseTile mt = solve_tile_for(L, A, nSC);
-    const int T = mt.tile, W = mt.waves;
+    const int W = mt.waves;
+    if (W < 1 || W > 16 || mt.tile < 1 || mt.tile > 128 || mt.tile > 64 * W) return hipErrorInvalidValue;
+    // workgroups of this shape resident per compute unit, per waves-per-workgroup (cached; LDS at the largest tile)
+    static std::atomic<int> occ_cache[17][129];
+    std::atomic<int> &oc = occ_cache[W][mt.tile];
+    int occ = oc.load(std::memory_order_relaxed);
+    if (occ <= 0) {
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, (const void *)&mmse_solve_kernel<L>, 64 * W,
+                                                         (size_t)tile_lds_bytes(L, mt.tile)) != hipSuccess ||
+            occ <= 0)
+            occ = 1;
+        oc.store(occ, std::memory_order_relaxed);
+    }
+    const int T = balanced_tile(nSC, mt.tile, (long)occ * device_cus());
     hipLaunchKernelGGL((mmse_solve_kernel<L>), dim3((unsigned)((nSC + T - 1) / T)), dim3(64 * W),
                        (size_t)tile_lds_bytes(L, T), st, H, s2, C, ni, flags, nSC, A, P, T, unbiased ? 1 : 0, thr);
     hipError_t e = hipGetLastError();

@@ -616,17 +616,43 @@ __device__ __forceinline__ void tile_barrier(int W) {
         lds_order_wave();
 }
 
+// Stores the first K floats of v at dst, 16 bytes at a time when dst is 16-byte aligned (a16) and K is a multiple
+// of 4, else 8 bytes at a time (dst is always 8-byte aligned here) and a last float for odd K.
+template <int K, int N>
+__device__ __forceinline__ void store_floats(float *__restrict__ dst, const float (&v)[N], bool a16) {
+    static_assert(K <= N, "short vector");
+    typedef float f4v __attribute__((ext_vector_type(4)));
+    typedef float f2v __attribute__((ext_vector_type(2)));
+    if (K % 4 == 0 && a16) {
+#pragma unroll
+        for (int k = 0; k < K / 4; ++k) {
+            const f4v q = {v[4 * k], v[4 * k + 1], v[4 * k + 2], v[4 * k + 3]};
+            reinterpret_cast<f4v *>(dst)[k] = q;
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < K / 2; ++k) {
+            const f2v q = {v[2 * k], v[2 * k + 1]};
+            reinterpret_cast<f2v *>(dst)[k] = q;
+        }
+        if (K % 2) dst[K - 1] = v[K - 1];
+    }
+}
+
 // ---- FP32 pass over one tile: subcarriers sc0 .. sc0 + n - 1 of one cell, by a workgroup of W waves -------
 // Phases 1 and 2 of a tile (Gram, algebra), ending at a barrier with Ginv, the scales and the ok flag of
 // every subcarrier in its LDS slot; noise_inv and flags (0) written for the subcarriers solved in FP32, the
 // flagged ones left to tile_fp64_inline. keep: the lane's KR1 antenna rows stay in kr / ki (single Gram round
-// only).
-template <int L, int KR1>
+// only). REC: the solve is kept as a record instead (Ginv lower triangle, real parts then imaginary parts, then
+// the L scales: the slot's first 2 NT + L floats) at the start of the subcarrier's coefficient block in C, stored
+// by the lane that solved it; the slot then holds only the ok flag (flagged subcarriers get no record: their block
+// receives the FP64 coefficients).
+template <int L, int KR1, bool REC = false>
 __device__ __forceinline__ void tile_solve(const float2 *__restrict__ H, const float *__restrict__ sigma2,
                                            float *__restrict__ noise_inv, uint8_t *__restrict__ flags, long sc0,
                                            int n, int A, int P, bool unbiased, float thr, float *__restrict__ lds,
                                            int tid, int W, bool keep, float (&kr)[KR1][L],
-                                           float (&ki)[KR1][L]) {
+                                           float (&ki)[KR1][L], float2 *__restrict__ C = nullptr) {
     constexpr int NT = Dim<L>::NT;
     constexpr int SLOT = Dim<L>::SLOT;
     const int lane = tid & 63, wave = tid >> 6;
@@ -685,11 +711,16 @@ __device__ __forceinline__ void tile_solve(const float2 *__restrict__ H, const f
 #pragma unroll
         for (int l = 0; l < L; ++l) out[2 * NT + l] = scl[l];
         out[2 * NT + L] = ok ? 1.f : 0.f;
-        slot_store<2 * NT + L + 1>(lds + tid * SLOT, out);
+        if constexpr (REC)
+            lds[tid * SLOT + 2 * NT + L] = out[2 * NT + L];
+        else
+            slot_store<2 * NT + L + 1>(lds + tid * SLOT, out);
         if (ok) {
 #pragma unroll
             for (int l = 0; l < L; ++l) noise_inv[sc * L + l] = noise_inv_of(x[l]);
             flags[sc] = 0;
+            if constexpr (REC)  // the block starts 16-byte aligned when L A is even
+                store_floats<2 * NT + L>(reinterpret_cast<float *>(C + (size_t)sc * L * A), out, (L * A) % 2 == 0);
         }
     }
     tile_barrier(W);

@@ -185,9 +185,8 @@ __device__ __forceinline__ void mmse_tile(const CellDev &cd, const Tile &tl, flo
     case LL:                                                                                                     \
         if (!C16 && cd.rec) {                                                                                    \
             float kr[1][LL], ki[1][LL];                                                                          \
-            tile_solve<LL, 1>(cd.H, cd.sigma2, cd.noise_inv, cd.flags, tl.sc0, tl.n, cd.A, cd.mmse_p,            \
-                              unbiased != 0, thr, lds, tid, W, false, kr, ki);                                   \
-            rec::write_records<LL>(cd.C, tl.sc0, tl.n, cd.A, lds, tid, 64 * W); /* not for flagged ones */       \
+            tile_solve<LL, 1, true>(cd.H, cd.sigma2, cd.noise_inv, cd.flags, tl.sc0, tl.n, cd.A, cd.mmse_p,      \
+                                    unbiased != 0, thr, lds, tid, W, false, kr, ki, cd.C); /* FP32 records */    \
             tile_fp64_inline<LL, false, true>(cd.H, cd.sigma2, cd.C, cd.noise_inv, cd.flags, tl.sc0, tl.n, cd.A, \
                                               unbiased != 0, lds, tid, W);                                       \
             tile_barrier(W); /* slots read before the workgroup's next tile rewrites them */                     \

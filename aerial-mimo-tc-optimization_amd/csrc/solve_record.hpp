@@ -23,21 +23,6 @@ __host__ __device__ constexpr bool record_shape(int L, int A) { return A <= 64 &
 __host__ __device__ constexpr bool record_pays(int L, int A) { return record_shape(L, A) && A >= 32 && L >= 4; }
 constexpr int kGinvBytes = 8 * 8 * 8;  // per-wave full Ginv, complex [8][8], in LDS
 
-// Records of a solved tile (subcarriers sc0 .. sc0 + n - 1, slots in lds as tile_solve leaves them) into the
-// coefficient blocks of C; every thread of the workgroup takes part. Flagged subcarriers (slot ok flag 0) get no
-// record: their block receives the FP64 coefficients (tile_fp64_inline), written by other threads, so the two
-// never store to the same address.
-template <int L>
-__device__ __forceinline__ void write_records(float2 *__restrict__ C, long sc0, int n, int A,
-                                              const float *__restrict__ lds, int tid, int nthreads) {
-    constexpr int REC = record_floats(L);
-    constexpr int SLOT = mmse::Dim<L>::SLOT;
-    for (int i = tid; i < n * REC; i += nthreads) {
-        const int t = i / REC, k = i - t * REC;
-        if (lds[t * SLOT + REC] != 0.f) reinterpret_cast<float *>(C + (size_t)(sc0 + t) * L * A)[k] = lds[t * SLOT + k];
-    }
-}
-
 // The coefficient columns of subcarrier sc from its record, by one wave: lane a forms C[.][a] (gfull: the wave's
 // Ginv buffer in LDS). write_c: the columns are stored over the block as well.
 template <int L>
