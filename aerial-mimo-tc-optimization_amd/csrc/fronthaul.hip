@@ -5,7 +5,8 @@
 // iqWidth bits, two's complement, packed most significant bit first (1 + 3 iqWidth bytes). A sample is
 // mantissa * 2^exponent, times the caller's scale in Y_rx.
 //
-// One 256-lane workgroup per (symbol, 8 PRBs, 64 antennas). The input is read row by row (each antenna row's
+// One 256-lane workgroup per (symbol, 8 PRBs, 64 antennas), numbered so that each XCD decodes contiguous PRB
+// ranges (the 8-PRB spans are not whole cache lines). The input is read row by row (each antenna row's
 // 8 PRBs are contiguous bytes: coalesced loads, 4-byte words when rows and blocks are word-aligned, bytes
 // otherwise) into LDS; then each lane decodes one PRB of one antenna from LDS (W, the mantissa width, is a
 // template parameter, so every bit offset is a constant) and stores its 12 complex samples. For each
@@ -19,6 +20,7 @@ namespace pe {
 namespace {
 
 constexpr int kPrbPerWg = 8;  // measured on the 32-cell batch: 4 -> 68.5 us, 8 -> 64.5 us, 16 -> 74.4 us
+constexpr unsigned kXcds = 8;  // XCDs of an MI355X
 
 template <int W, bool WORDS>
 __global__ __launch_bounds__(256) void bfp_decompress_kernel(const uint8_t *__restrict__ in, float2 *__restrict__ Y,
@@ -28,7 +30,15 @@ __global__ __launch_bounds__(256) void bfp_decompress_kernel(const uint8_t *__re
     constexpr int SPAN = kPrbPerWg * BPB;              // bytes of one antenna row in this workgroup
     constexpr int PITCH = ((SPAN + 3) / 4) | 1;        // LDS row pitch in words, odd: lanes on distinct banks
     __shared__ uint32_t rows[64 * PITCH];
-    const int prb0 = blockIdx.x * kPrbPerWg, sym = blockIdx.y, ant0 = blockIdx.z * 64;
+    // Workgroups go to the 8 XCDs round-robin by launch order; renumbered so that each XCD takes a contiguous run
+    // of PRB groups of a row: neighbouring groups share the cache lines at their boundaries, which then hit in
+    // that XCD's L2 instead of being fetched twice.
+    const unsigned gx = gridDim.x, gy = gridDim.y;
+    const unsigned nwg = gx * gy * gridDim.z;
+    const unsigned id = blockIdx.x + gx * (blockIdx.y + gy * blockIdx.z);
+    const unsigned q = nwg / kXcds, r = nwg % kXcds, x = id % kXcds, k = id / kXcds;
+    const unsigned nid = x < r ? x * (q + 1) + k : r * (q + 1) + (x - r) * q + k;
+    const int prb0 = (int)(nid % gx) * kPrbPerWg, sym = (int)((nid / gx) % gy), ant0 = (int)(nid / (gx * gy)) * 64;
     const int np = min(kPrbPerWg, nPrb - prb0), na = min(64, nAnt - ant0);
     const int span = np * BPB;
     const uint8_t *base = in + ((size_t)sym * nAnt + ant0) * row_stride + (size_t)prb0 * BPB;
