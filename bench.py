@@ -100,6 +100,20 @@ def time_launches(fn, bufs, steps, warmup):
     return e0.elapsed_time(e1) / 1e3, wall
 
 
+def mmse_stage_times(pe, cells, steps):
+    """pe_gram_mmse and pe_gram_mmse_c16 over batches of (H, Y, sigma2), `steps` launches each alternating over
+    the batches (beyond the Infinity Cache: every launch reads its H cold). Returns the two elapsed times in s."""
+    bufs, bufs16 = [], []
+    for H, _, s2 in cells:
+        C, ni, fl = pe.gram_mmse(H, s2, unbiased=True)
+        bufs.append((H, s2, C, ni, fl))
+        bufs16.append((H, s2) + tuple(pe.gram_mmse_c16(H, s2, unbiased=True)))
+    el, _ = time_launches(lambda H, s2, C, ni, fl: pe.gram_mmse(H, s2, unbiased=True, C=C, noise_inv=ni, flags=fl),
+                          bufs, steps, 4)
+    el16, _ = time_launches(lambda H, s2, *o: pe.gram_mmse_c16(H, s2, True, 0.0, *o), bufs16, steps, 4)
+    return el, el16
+
+
 def cpu_model():
     """The host CPU as /proc/cpuinfo names it: model name, family / model / stepping, cache, cores per socket."""
     info = {}
@@ -839,33 +853,30 @@ def main():
     if args.freq_interp_only:
         print(json.dumps(freq_interp_bench(dev, 10, 0.0)), flush=True)
         return
-    if args.chain_only:
-        H, Y, s2 = make_cells(args.cells, 1234, dev)
+    if args.chain_only:  # two buffer sets alternate, as in the full run: every launch reads its inputs cold
         nsc = args.cells * NSC_CELL
-        C = torch.empty((nsc, L, A), dtype=torch.complex64, device=dev)
-        ni = torch.empty((nsc, L), dtype=torch.float32, device=dev)
-        fl = torch.empty((nsc,), dtype=torch.uint8, device=dev)
-        llr = torch.empty((nsc, L, S, 2 * Q), dtype=torch.float32, device=dev)
+        cbufs = []
+        for k in range(2):
+            H, Y, s2 = make_cells(args.cells, 1234 + k, dev)
+            cbufs.append((H, s2, torch.empty((nsc, L, A), dtype=torch.complex64, device=dev),
+                          torch.empty((nsc, L), dtype=torch.float32, device=dev),
+                          torch.empty((nsc,), dtype=torch.uint8, device=dev), Y,
+                          torch.empty((nsc, L, S, 2 * Q), dtype=torch.float32, device=dev)))
 
-        def two():
+        def two(H, s2, C, ni, fl, Y, llr):
             pe.gram_mmse(H, s2, unbiased=True, C=C, noise_inv=ni, flags=fl)
             pe.coef_apply_demap(C, Y, ni, Q, prec=args.prec, out=llr)
 
-        def one():
+        def one(H, s2, C, ni, fl, Y, llr):
             pe.mmse_apply_demap(H, s2, Y, Q, prec=args.prec, C=C, noise_inv=ni, flags=fl, out=llr)
 
-        el2, _ = time_launches(two, [()], 40, 5)
-        el1, _ = time_launches(one, [()], 40, 5)
+        el2, _ = time_launches(two, cbufs, 40, 5)
+        el1, _ = time_launches(one, cbufs, 40, 5)
         print(json.dumps({"subcarriers": nsc, "two_launches_ms": el2 / 40 * 1e3, "one_pass_ms": el1 / 40 * 1e3}),
               flush=True)
         return
     if args.mmse_only:
-        H, Y, s2 = make_cells(args.cells, 1234, dev)
-        del Y
-        C, ni, fl = pe.gram_mmse(H, s2, unbiased=True)
-        c16o = pe.gram_mmse_c16(H, s2, unbiased=True)
-        el_m, _ = time_launches(lambda: pe.gram_mmse(H, s2, unbiased=True, C=C, noise_inv=ni, flags=fl), [()], 40, 5)
-        el_m16, _ = time_launches(lambda: pe.gram_mmse_c16(H, s2, True, 0.0, *c16o), [()], 40, 5)
+        el_m, el_m16 = mmse_stage_times(pe, [make_cells(args.cells, 1234 + k, dev) for k in range(2)], 40)
         print(json.dumps({"subcarriers": args.cells * NSC_CELL, "c64_ms": el_m / 40 * 1e3,
                           "c16_ms": el_m16 / 40 * 1e3}), flush=True)
         return
@@ -873,20 +884,19 @@ def main():
     nsc = n * NSC_CELL
     log("generating %d cells x 2 buffer sets" % n)
     sets = []
+    cells = [make_cells(n, 1234 + k, dev) for k in range(2)]
+    # the MMSE stage over the same batches, complex64 and binary16 coefficient output, the two sets alternating
+    el_m, el_m16 = mmse_stage_times(pe, cells, 20)
+    mmse_ms = {"c64": el_m / 20 * 1e3, "c16": el_m16 / 20 * 1e3}
     for k in range(2):
-        H, Y, s2 = make_cells(n, 1234 + k, dev)
+        H, Y, s2 = cells[k]
+        cells[k] = None
         C, ni, fl = pe.gram_mmse(H, s2, unbiased=True)
-        if k == 0:  # the MMSE stage over the same batch, complex64 and binary16 coefficient output
-            c16o = pe.gram_mmse_c16(H, s2, unbiased=True)
-            el_m, _ = time_launches(lambda: pe.gram_mmse(H, s2, unbiased=True, C=C, noise_inv=ni, flags=fl), [()],
-                                    10, 2)
-            el_m16, _ = time_launches(lambda: pe.gram_mmse_c16(H, s2, True, 0.0, *c16o), [()], 10, 2)
-            mmse_ms = {"c64": el_m / 10 * 1e3, "c16": el_m16 / 10 * 1e3}
-            del c16o
         H0 = H[:NSC_CELL].clone() if k == 0 else None  # channels are not an input of the timed launch: keep one cell
         del H
         llr = torch.empty((nsc, L, S, 2 * Q), dtype=torch.float32, device=dev)
         sets.append({"H": H0, "Y": Y, "s2": s2, "C": C, "ni": ni, "fl": fl, "llr": llr})
+    del cells
     torch.cuda.synchronize()
     n_fp64 = int(sum(int((s["fl"] != 0).sum()) for s in sets))
 
@@ -1117,7 +1127,8 @@ def main():
                        "hbm_gbs_c64": nsc * (A * L * 8 + 4 + L * A * 8 + L * 4 + 1) / (mmse_ms["c64"] * 1e-3) / 1e9,
                        "hbm_gbs_c16": nsc * (A * L * 8 + 4 + L * A * 4 + 4 + L * 4 + 1) / (mmse_ms["c16"] * 1e-3)
                        / 1e9,
-                       "workload": "pe_gram_mmse / pe_gram_mmse_c16 over the batch (H read, C written)"},
+                       "workload": "pe_gram_mmse / pe_gram_mmse_c16 over the batch (H read, C written), two "
+                                   "batches alternating (H read cold)"},
         "c16_coefficients": {"value": nsc / perc, "ms_per_step": perc * 1e3, "hbm_gbs": bytes_c16 / perc / 1e9,
                              "bytes_per_launch": bytes_c16,
                              "workload": "same batches, coefficients held as binary16 pairs + exponent "
