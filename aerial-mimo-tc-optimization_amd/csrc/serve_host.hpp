@@ -101,7 +101,10 @@ struct LatencyFit {
 // margin reach the cap while disengaged and oscillate under overload. Step sizes (64-cell serving, 10,000 jobs per
 // load): kUp = 100 us kept p99 at 0.46-0.48 ms for 100-120 % load with 6-18 % of jobs shed or late, against
 // 0.48-0.51 ms at 50 us (arrivals change faster than the smaller steps follow); 200 us shed 0.2 % at 90 % load, and
-// a 0.2 % target (kDown = kUp / 499) shed up to 23 % at 110 %. Until a fit is ready nothing is predicted late.
+// a 0.2 % target (kDown = kUp / 499) shed up to 23 % at 110 %; raising the margin from zero only after 3 late
+// batches among the last 100 (to ignore isolated late batches below capacity) let lateness come in bursts each time
+// it had decayed to zero: p99 0.47-0.55 against 0.45-0.49 ms at 100-150 % load, with no gain at 90 % (0-7 jobs shed
+// in four runs either way). Until a fit is ready nothing is predicted late.
 struct LatencyModel {
     static constexpr int kClasses = 9;  // 0 .. 8 batches in flight at launch
     static constexpr double kUp = 100e-6, kDown = 100e-6 / 199.0;  // seconds per batch with / without a late job
