@@ -810,6 +810,8 @@ def main():
     ap.add_argument("--serving-only", action="store_true", help="only the serving-loop section, printed as JSON")
     ap.add_argument("--codeword-only", action="store_true",
                     help="only the descrambling / codeword-layout launches over the first 32-cell batch, for profiling")
+    ap.add_argument("--ceiling-only", action="store_true",
+                    help="only the measured streaming ceilings (pe_hbm_probe), printed as JSON")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run 5 plain 256 MiB device copies (scale reference for the HBM byte counters)")
     args = ap.parse_args()
@@ -831,6 +833,9 @@ def main():
     import pusch_eq as pe
     dev = torch.device("cuda:0")
     pe.init(0)
+    if args.ceiling_only:
+        print(json.dumps(hbm_ceiling(dev)), flush=True)
+        return
     if args.serving_only:
         print(json.dumps(serving_bench(dev, args.serve_jobs)), flush=True)
         return
