@@ -105,40 +105,64 @@ def _check(status):
 
 
 def _ptr(t):
-    return ctypes.c_void_p(t.data_ptr()) if t is not None else ctypes.c_void_p(0)
+    """A tensor's device address for a c_void_p argument (plain ints convert; None is NULL)."""
+    return t.data_ptr() if t is not None else None
 
 
 def _need(t, name, dtype, shape=None, min_numel=None):
     """Checks a tensor handed to the library before any launch: its dtype (a torch dtype, or an element size in
     bytes for the binary16-pair buffers), contiguity, and shape (None entries match any size) or minimum number of
     elements. Every wrapper then checks with _same_device that all its tensors are on one GPU, so dtype and shape
-    mistakes are reported whatever the device. Raises TypeError / ValueError; returns t."""
-    import torch
-    if not isinstance(t, torch.Tensor):
+    mistakes are reported whatever the device. Raises TypeError / ValueError; returns t. (Written for speed: it
+    runs for every tensor of every call.)"""
+    if not isinstance(t, _tensor_type()):
         raise TypeError("%s must be a torch tensor, got %s" % (name, type(t).__name__))
-    if isinstance(dtype, int):
+    if type(dtype) is int:
         if t.element_size() != dtype:
             raise TypeError("%s: elements of %d bytes expected, got %s" % (name, dtype, t.dtype))
     elif t.dtype != dtype:
         raise TypeError("%s: dtype %s expected, got %s" % (name, dtype, t.dtype))
     if not t.is_contiguous():
         raise ValueError("%s must be contiguous" % name)
-    if shape is not None and (t.dim() != len(shape) or
-                              any(w is not None and int(w) != int(g) for w, g in zip(shape, t.shape))):
-        raise ValueError("%s: shape %s expected, got %s" % (name, tuple(shape), tuple(t.shape)))
+    if shape is not None:
+        got = t.shape
+        ok = len(got) == len(shape)
+        if ok:
+            for w, g in zip(shape, got):
+                if w is not None and w != g:
+                    ok = False
+                    break
+        if not ok:
+            raise ValueError("%s: shape %s expected, got %s" % (name, tuple(shape), tuple(got)))
     if min_numel is not None and t.numel() < min_numel:
         raise ValueError("%s: at least %d elements expected, got %d" % (name, min_numel, t.numel()))
     return t
 
 
+_TENSOR = None
+
+
+def _tensor_type():
+    global _TENSOR
+    if _TENSOR is None:
+        import torch
+        _TENSOR = torch.Tensor
+    return _TENSOR
+
+
 def _same_device(*ts):
     """All given tensors (None entries skipped) on one GPU."""
-    ds = {t.device for t in ts if t is not None}
-    if len(ds) > 1:
-        raise ValueError("tensors on different devices: %s" % sorted(str(d) for d in ds))
-    for d in ds:
-        if d.type != "cuda":
-            raise ValueError("tensors must be GPU tensors, got device %s" % d)
+    first = None
+    for t in ts:
+        if t is None:
+            continue
+        d = t.device
+        if first is None:
+            first = d
+            if d.type != "cuda":
+                raise ValueError("tensors must be GPU tensors, got device %s" % d)
+        elif d != first:
+            raise ValueError("tensors on different devices: %s" % sorted({str(u.device) for u in ts if u is not None}))
 
 
 def _q(n_pam_bits):
@@ -149,9 +173,10 @@ def _q(n_pam_bits):
 
 
 def _stream(stream):
+    """The HIP stream handle of a torch stream (the current stream when None), for a c_void_p argument."""
     import torch
     s = stream if stream is not None else torch.cuda.current_stream()
-    return ctypes.c_void_p(s.cuda_stream)
+    return s.cuda_stream
 
 
 def _prec(p):

@@ -114,6 +114,31 @@ def mmse_stage_times(pe, cells, steps):
     return el, el16
 
 
+def call_latency(fn, iters, spin):
+    """Milliseconds from an event recorded before fn() to one recorded after it, one call at a time on an idle GPU
+    (20 untimed calls first), sorted. spin: a spin kernel is queued before the start event, so fn's launches are
+    already in the queue when it fires (device time); without it, the host's time to issue fn is included."""
+    import torch
+    out = []
+    for i in range(iters + 20):
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        if spin:
+            torch.cuda._sleep(200000)
+        a.record()
+        fn()
+        b.record()
+        b.synchronize()
+        if i >= 20:
+            out.append(a.elapsed_time(b))
+    out.sort()
+    return out
+
+
+def percentile(v, p):
+    """Nearest-rank percentile of the sorted list v (None if empty)."""
+    return v[min(len(v) - 1, max(0, int(-(-p * len(v) // 100)) - 1))] if v else None
+
+
 def cpu_model():
     """The host CPU as /proc/cpuinfo names it: model name, family / model / stepping, cache, cores per socket."""
     info = {}
@@ -1030,33 +1055,27 @@ def main():
         el_c2 / nch * 1e6, el_c1 / nch * 1e6, one_pass_identical))
     del cbufs, hfull
 
-    # single-cell slot latency: MMSE + fused apply/demap for one cell, one stream
+    # single-cell slot latency: MMSE + fused apply/demap for one cell, one stream; as the caller sees it (from an
+    # event recorded before the Python call, so the binding's argument checks and the launches are included) and
+    # on the device alone (the call queued behind a spin kernel, so its launches wait in the queue when the start
+    # event fires)
     cell = {k: (v[c1].contiguous() if torch.is_tensor(v) and v.shape[0] == nsc else v) for k, v in s0.items()}
-    lat = []
-    for i in range(args.latency_iters + 20):
-        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        a.record()
+
+    def two_calls():
         pe.gram_mmse(cell["H"], cell["s2"], unbiased=True, C=cell["C"], noise_inv=cell["ni"], flags=cell["fl"])
         fused(cell["C"], cell["Y"], cell["ni"], cell["llr"])
-        b.record()
-        b.synchronize()
-        if i >= 20:
-            lat.append(a.elapsed_time(b))
-    lat.sort()
-    pct = lambda p: lat[min(len(lat) - 1, max(0, int(-(-p * len(lat) // 100)) - 1))] if lat else None
-    lat1 = []
-    for i in range(args.latency_iters + 20):
-        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        a.record()
+
+    def one_call():
         pe.mmse_apply_demap(cell["H"], cell["s2"], cell["Y"], Q, prec=args.prec, C=cell["C"], noise_inv=cell["ni"],
                             flags=cell["fl"], out=cell["llr"])
-        b.record()
-        b.synchronize()
-        if i >= 20:
-            lat1.append(a.elapsed_time(b))
+
+    lat = call_latency(two_calls, args.latency_iters, False)
+    lat1 = call_latency(one_call, args.latency_iters, False)
+    lat_dev = call_latency(two_calls, args.latency_iters, True)
+    lat1_dev = call_latency(one_call, args.latency_iters, True)
     pe.gram_mmse(cell["H"], cell["s2"], unbiased=True, C=cell["C"], noise_inv=cell["ni"], flags=cell["fl"])
-    lat1.sort()
-    pct1 = lambda p: lat1[min(len(lat1) - 1, max(0, int(-(-p * len(lat1) // 100)) - 1))] if lat1 else None
+    pct = lambda p: percentile(lat, p)
+    pct1 = lambda p: percentile(lat1, p)
 
     cpu = None
     if not args.no_cpu:
@@ -1142,9 +1161,15 @@ def main():
                                      "hbm_gbs": bytes_c16w / percw / 1e9, "bytes_per_launch": bytes_c16w,
                                      "workload": "binary16 coefficients and binary16 LLRs"},
         "slot_latency_ms": {"p50": pct(50), "p99": pct(99), "max": lat[-1] if lat else None, "iters": len(lat),
-                            "workload": "one cell: pe_gram_mmse + fused apply/demap, events on one stream",
+                            "workload": "one cell: pe_gram_mmse + fused apply/demap, events on one stream around "
+                                        "the Python calls (argument checks and launches included)",
                             "one_pass": {"p50": pct1(50), "p99": pct1(99), "max": lat1[-1] if lat1 else None,
-                                         "workload": "one cell: pe_mmse_apply_demap"}},
+                                         "workload": "one cell: pe_mmse_apply_demap"},
+                            "device": {"p50": percentile(lat_dev, 50), "p99": percentile(lat_dev, 99),
+                                       "one_pass_p50": percentile(lat1_dev, 50),
+                                       "one_pass_p99": percentile(lat1_dev, 99),
+                                       "workload": "the same calls queued behind a spin kernel: device time from "
+                                                   "the first launch's start to the last one's end"}},
         "chain_from_channel_estimates": one_pass,
         "multicell_slot": multicell,
         "serving": serving,
