@@ -115,6 +115,15 @@ int balanced_tile(long nSC, int tmax, long resident) {
     return (int)((nSC + resident * rounds - 1) / (resident * rounds));
 }
 
+// The apply of the one-pass chain: coefficients from the records in C (or the FP64 blocks) and H.
+template <int L, int KS, int PREC>
+hipError_t launch_apply(const float2 *H, const float2 *Y, float2 *C, const float *ni, const uint8_t *flags, float *llr,
+                        int nSC, int S, int A, int npb, int f16, int write_c, hipStream_t st) {
+    hipLaunchKernelGGL((apply_solve_kernel<L, KS, PREC>), dim3((unsigned)((nSC + kWpb - 1) / kWpb)), dim3(64 * kWpb),
+                       0, st, H, Y, C, ni, flags, llr, nSC, S, A, npb, f16, write_c);
+    return hipGetLastError();
+}
+
 template <int L, int KS, int PREC>
 hipError_t launch_l(const float2 *H, const float *s2, const float2 *Y, float2 *C, float *ni, uint8_t *flags,
                     float *llr, int nSC, int S, int A, int npb, bool unbiased, float thr, int f16, int write_c,
@@ -139,9 +148,7 @@ hipError_t launch_l(const float2 *H, const float *s2, const float2 *Y, float2 *C
                        (size_t)tile_lds_bytes(L, T), st, H, s2, C, ni, flags, nSC, A, P, T, unbiased ? 1 : 0, thr);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL((apply_solve_kernel<L, KS, PREC>), dim3((unsigned)((nSC + kWpb - 1) / kWpb)), dim3(64 * kWpb),
-                       0, st, H, Y, C, ni, flags, llr, nSC, S, A, npb, f16, write_c);
-    return hipGetLastError();
+    return launch_apply<L, KS, PREC>(H, Y, C, ni, flags, llr, nSC, S, A, npb, f16, write_c, st);
 }
 
 template <int KS, int PREC>
@@ -175,7 +182,44 @@ hipError_t launch_k(const float2 *H, const float *s2, const float2 *Y, float2 *C
     }
 }
 
+template <int KS>
+hipError_t launch_apply_k(const float2 *H, const float2 *Y, float2 *C, const float *ni, const uint8_t *flags,
+                          float *llr, int nSC, int L, int S, int A, int npb, int prec, int f16, int write_c,
+                          hipStream_t st) {
+#define PE_AP_LP(LL, PP)                                                                                          \
+    case PP:                                                                                                      \
+        return launch_apply<LL, KS, PP>(H, Y, C, ni, flags, llr, nSC, S, A, npb, f16, write_c, st);
+#define PE_AP_L(LL)                                                                                               \
+    case LL:                                                                                                      \
+        switch (prec) {                                                                                           \
+            PE_AP_LP(LL, 1)                                                                                       \
+            PE_AP_LP(LL, 2)                                                                                       \
+            PE_AP_LP(LL, 3)                                                                                       \
+            PE_AP_LP(LL, 4)                                                                                       \
+        default: return hipErrorInvalidValue;                                                                     \
+        }
+    switch (L) {  // 4 <= L (record_pays)
+        PE_AP_L(4)
+        PE_AP_L(5)
+        PE_AP_L(6)
+        PE_AP_L(7)
+        PE_AP_L(8)
+    default: return hipErrorInvalidValue;
+    }
+#undef PE_AP_L
+#undef PE_AP_LP
+}
+
 }  // namespace
+
+hipError_t launch_apply_records(const float2 *H, const float2 *Y, float2 *C, const float *noise_inv,
+                                const uint8_t *flags, float *llr, int nSC, int L, int S, int A, int nPamBits, int prec,
+                                int llr_f16, int write_c, hipStream_t st) {
+    if (A > 64 || !rec::record_pays(L, A)) return hipErrorInvalidValue;
+    if (A <= 32)
+        return launch_apply_k<2>(H, Y, C, noise_inv, flags, llr, nSC, L, S, A, nPamBits, prec, llr_f16, write_c, st);
+    return launch_apply_k<4>(H, Y, C, noise_inv, flags, llr, nSC, L, S, A, nPamBits, prec, llr_f16, write_c, st);
+}
 
 hipError_t launch_mmse_apply_demap(const float2 *H, const float *sigma2, const float2 *Y, float2 *C,
                                    float *noise_inv, uint8_t *flags, void *llr, int nSC, int L, int S, int A,

@@ -643,6 +643,34 @@ pe_status pe_freq_interp(const pe_c64 *W, const pe_c64 *X, pe_c64 *H_f, int nBat
     return from_hip(e, "freq_interp launch");
 }
 
+pe_status pe_interp_mmse_apply_demap(const pe_c64 *W, const pe_c64 *X, const float *sigma2, const pe_c64 *Y_rx,
+                                     pe_c64 *H, pe_c64 *C, float *noise_inv, uint8_t *sc_flags, void *llr, int nBlk,
+                                     int nIn, int nSym, int nAnt, int nPamBits, pe_prec interp_prec, pe_prec prec,
+                                     unsigned flags, float cond_threshold, void *stream) {
+    if (nBlk < 0) return fail(PE_ERR_INVALID, "nBlk = %d < 0", nBlk);
+    if (nBlk > (1 << 24)) return fail(PE_ERR_UNSUPPORTED, "nBlk = %d > 2^24", nBlk);
+    if (nIn < 1 || nIn > 128) return fail(PE_ERR_INVALID, "nIn = %d outside [1, 128]", nIn);
+    if (interp_prec != PE_PREC_BF16 && interp_prec != PE_PREC_BF16_SPLIT)
+        return fail(PE_ERR_UNSUPPORTED, "interpolation precision %d: BF16 or BF16_SPLIT", (int)interp_prec);
+    if (nAnt < 32 || nAnt > 64 || nAnt % 4 != 0)
+        return fail(PE_ERR_UNSUPPORTED, "nAnt = %d: the fused chain takes 32 to 64 antennas, a multiple of 4", nAnt);
+    const int nSC = 64 * nBlk;
+    pe_status s = check_mmse_apply(H, sigma2, Y_rx, noise_inv, sc_flags, llr, nSC, 4, nSym, nAnt, nPamBits, prec,
+                                   flags, PE_MMSE_UNBIASED | PE_EQ_WRITE_C | PE_EQ_LLR_F16);
+    if (s != PE_OK) return s;
+    if (nBlk == 0) return ok();
+    if (!check_ptrs(W, X, C)) return fail(PE_ERR_INVALID, "null pointer");
+    if ((reinterpret_cast<uintptr_t>(W) | reinterpret_cast<uintptr_t>(X) | reinterpret_cast<uintptr_t>(C)) & 7u)
+        return fail(PE_ERR_INVALID, "W, X and C must be 8-byte aligned");
+    const float thr = cond_threshold > 0.f ? cond_threshold : 1e4f;
+    hipError_t e = pe::launch_interp_mmse_apply_demap(
+        reinterpret_cast<const float2 *>(W), reinterpret_cast<const float2 *>(X), sigma2,
+        reinterpret_cast<const float2 *>(Y_rx), reinterpret_cast<float2 *>(H), reinterpret_cast<float2 *>(C), noise_inv,
+        sc_flags, llr, nBlk, nIn, nSym, nAnt, nPamBits, (int)interp_prec, (int)prec, (flags & PE_MMSE_UNBIASED) != 0,
+        thr, (flags & PE_EQ_LLR_F16) ? 1 : 0, (flags & PE_EQ_WRITE_C) ? 1 : 0, (hipStream_t)stream);
+    return from_hip(e, "interp_mmse_apply_demap launch");
+}
+
 pe_status pe_hbm_probe(const void *src, void *dst, size_t bytes, int read_only, int grid, void *stream) {
     if (bytes % 16 != 0) return fail(PE_ERR_INVALID, "bytes = %zu is not a multiple of 16", bytes);
     if (grid < 1 || grid > (1 << 20)) return fail(PE_ERR_INVALID, "grid = %d outside [1, 2^20]", grid);

@@ -105,6 +105,17 @@ hipError_t launch_mmse_apply_demap(const float2 *H, const float *sigma2, const f
                                    float *noise_inv, uint8_t *flags, void *llr, int nSC, int L, int S, int A,
                                    int nPamBits, int prec, bool unbiased, float thr, int llr_f16, int write_c,
                                    hipStream_t st);
+// The apply of the one-pass chain alone: coefficients formed from the solve records in C (FP64-solved subcarriers:
+// their blocks in C) and the antenna rows of H (4 <= L <= 8, 32 <= A <= 64).
+hipError_t launch_apply_records(const float2 *H, const float2 *Y, float2 *C, const float *noise_inv,
+                                const uint8_t *flags, float *llr, int nSC, int L, int S, int A, int nPamBits, int prec,
+                                int llr_f16, int write_c, hipStream_t st);
+// DMRS frequency interpolation (64 x K filter W, 4 layers) fused with the MMSE solve, then the records apply: X
+// [nBlk][A][K][4] -> H [64 nBlk][A][4], records in C, noise_inv, flags, LLRs (interp_prec 2: bf16, 4: bf16 split).
+hipError_t launch_interp_mmse_apply_demap(const float2 *W, const float2 *X, const float *sigma2, const float2 *Y,
+                                          float2 *H, float2 *C, float *noise_inv, uint8_t *flags, void *llr, int nBlk,
+                                          int K, int S, int A, int nPamBits, int interp_prec, int prec, bool unbiased,
+                                          float thr, int llr_f16, int write_c, hipStream_t st);
 // Codeword output of the fused demapper (descrambling and the rate-recovery layout). Record of layer l, 32 bytes,
 // eight per launch: its codeword's LLR array and scrambling sequence, the codeword's layer count nu and the layer's
 // rank v in it, and the code-block split (blocks 0 .. r0 - 1 of e_lo bits, the rest of e_lo + nu Qm bits).

@@ -47,6 +47,7 @@ SIGNATURES = {
     "pe_hbm_probe": ([_P, _P, ctypes.c_size_t, _I, _I, _P], _I),
     "pe_bfp_decompress": ([_P, _P, _I, _I, _I, _I, ctypes.c_size_t, _F, _P], _I),
     "pe_freq_interp": ([_P, _P, _P, _I, _I, _I, _I, _I, _P], _I),
+    "pe_interp_mmse_apply_demap": ([_P, _P, _P, _P, _P, _P, _P, _P, _P, _I, _I, _I, _I, _I, _I, _I, _U, _F, _P], _I),
     "pe_coef_to_c16": ([_P, _P, _P, _I, _I, _I, _P], _I),
     "pe_gram_mmse_c16": ([_P, _P, _P, _P, _P, _P, _I, _I, _I, _U, _F, _P], _I),
     "pe_coef_apply_c16": ([_P, _P, _P, _P, _I, _I, _I, _I, _P], _I),
@@ -573,6 +574,48 @@ def freq_interp(W, X, prec="f32", out=None, stream=None):
     _same_device(W, X, out)
     _check(lib().pe_freq_interp(_ptr(W), _ptr(X), _ptr(out), nBatch, nOut, nIn, nCol, _prec(prec), _stream(stream)))
     return out
+
+
+def interp_mmse_apply_demap(W, X, sigma2, Y, n_pam_bits, interp_prec="bf16", prec="f16", unbiased=True,
+                            cond_threshold=0.0, write_c=False, llr_f16=False, H=None, C=None, noise_inv=None,
+                            flags=None, out=None, stream=None):
+    """DMRS estimates to LLRs with the frequency interpolation fused into the MMSE solve
+    (pe_interp_mmse_apply_demap): W [64, nIn], X [nBlk, nAnt, nIn, 4], sigma2 [64 nBlk], Y [64 nBlk, nSym, nAnt]
+    complex64 -> LLRs [64 nBlk, 4, nSym, 2 n_pam_bits], with H [64 nBlk, nAnt, 4] written (the values
+    freq_interp(W, X.reshape(-1, nIn, 4), interp_prec) computes, in the equalizer's layout). The LLRs, noise_inv and
+    flags are those of mmse_apply_demap on that H. Returns (llr, H, C, noise_inv, flags)."""
+    import torch
+    q = _q(n_pam_bits)
+    _need(W, "W", torch.complex64, (64, None))
+    nIn = W.shape[1]
+    nBlk, A = _need(X, "X", torch.complex64, (None, None, nIn, 4)).shape[:2]
+    nSC = 64 * nBlk
+    _need(sigma2, "sigma2", torch.float32, (nSC,))
+    S = _need(Y, "Y", torch.complex64, (nSC, None, A)).shape[1]
+    dev = X.device
+    ldt = torch.float16 if llr_f16 else torch.float32
+    if H is None:
+        H = torch.empty((nSC, A, 4), dtype=torch.complex64, device=dev)
+    if C is None:
+        C = torch.empty((nSC, 4, A), dtype=torch.complex64, device=dev)
+    if noise_inv is None:
+        noise_inv = torch.empty((nSC, 4), dtype=torch.float32, device=dev)
+    if flags is None:
+        flags = torch.empty((nSC,), dtype=torch.uint8, device=dev)
+    if out is None:
+        out = torch.empty((nSC, 4, S, 2 * q), dtype=ldt, device=dev)
+    _need(H, "H", torch.complex64, (nSC, A, 4))
+    _need(C, "C", torch.complex64, (nSC, 4, A))
+    _need(noise_inv, "noise_inv", torch.float32, (nSC, 4))
+    _need(flags, "flags", torch.uint8, (nSC,))
+    _need(out, "out", ldt, (nSC, 4, S, 2 * q))
+    _same_device(W, X, sigma2, Y, H, C, noise_inv, flags, out)
+    f = (MMSE_UNBIASED if unbiased else 0) | (EQ_WRITE_C if write_c else 0) | (EQ_LLR_F16 if llr_f16 else 0)
+    _check(lib().pe_interp_mmse_apply_demap(_ptr(W), _ptr(X), _ptr(sigma2), _ptr(Y), _ptr(H), _ptr(C),
+                                            _ptr(noise_inv), _ptr(flags), _ptr(out), nBlk, nIn, S, A, q,
+                                            _prec(interp_prec), _prec(prec), f, float(cond_threshold),
+                                            _stream(stream)))
+    return out, H, C, noise_inv, flags
 
 
 def hbm_probe(src, dst, read_only=False, grid=4096, nbytes=None, stream=None):

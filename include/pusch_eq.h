@@ -300,6 +300,23 @@ pe_status pe_srs_filter(pe_c16 *H_est, const pe_c16 *SRS, const pe_c16 *W, const
 pe_status pe_freq_interp(const pe_c64 *W, const pe_c64 *X, pe_c64 *H_f, int nBatch, int nOut, int nIn, int nCol,
                          pe_prec prec, void *stream);
 
+/* No reference kernel: the chain from the DMRS estimates to LLRs, with the frequency interpolation
+   (README.md:199-200 names it as the step upstream of the equalizer, whose fusion with the MMSE removes the round
+   trip of H through HBM) fused with the MMSE solve. 4 layers; blocks of 64 output subcarriers:
+     X [nBlk][nAnt][nIn][4]   estimates on the nIn pilot tones of block blk, antenna ant (item blk * nAnt + ant of
+                              pe_freq_interp), W [64][nIn] the filter,
+     H [64 nBlk][nAnt][4]     written: H[64 blk + m][ant][n] = the value pe_freq_interp(W, X, ..., nOut = 64,
+                              nCol = 4, interp_prec) writes at [blk * nAnt + ant][m][n], in the equalizer's layout,
+   then as pe_mmse_apply_demap(H, sigma2, Y_rx, C, noise_inv, sc_flags, llr, 64 nBlk, 4, nSym, nAnt, nPamBits,
+   prec, flags, cond_threshold): C (its workspace, [64 nBlk][4][nAnt]), noise_inv, sc_flags and llr are
+   bit-identical to that call on this H (the Gram matrices are summed in the same order). interp_prec
+   PE_PREC_BF16 or PE_PREC_BF16_SPLIT; prec as pe_mmse_apply_demap (MFMA precisions); 32 <= nAnt <= 64,
+   nAnt % 4 == 0; 1 <= nIn <= 128; H, Y_rx and llr 16-byte aligned, W, X and C 8-byte aligned. */
+pe_status pe_interp_mmse_apply_demap(const pe_c64 *W, const pe_c64 *X, const float *sigma2, const pe_c64 *Y_rx,
+                                     pe_c64 *H, pe_c64 *C, float *noise_inv, uint8_t *sc_flags, void *llr, int nBlk,
+                                     int nIn, int nSym, int nAnt, int nPamBits, pe_prec interp_prec, pe_prec prec,
+                                     unsigned flags, float cond_threshold, void *stream);
+
 /* ---- fronthaul input: block floating point U-plane data -> Y_rx ----------------------------------- */
 
 /* No reference kernel: the reference cites the O-RAN WG4 CUS-plane specification (README.md:214) for the

@@ -341,3 +341,24 @@ def test_noalloc_entry_point_validates_without_a_device(pe):
     assert na(a, a, a, a, a, a, 16, 8, 11, 64, 4, 0, 1, 0.0, None) == 2                # FP32: no MFMA
     assert na(a, a, ctypes.c_void_p(264), a, a, a, 16, 8, 11, 64, 4, 1, 1, 0.0, None) == 1  # unaligned Y
     assert pe.NOALLOC_SCRATCH_BYTES == 32 << 20
+
+
+def test_interp_mmse_entry_point_validates_without_a_device(pe):
+    """pe_interp_mmse_apply_demap: shapes outside the fused chain and bad pointers are rejected before any device
+    call; an empty batch succeeds without launching."""
+    h = pe.lib()
+    a = ctypes.c_void_p(256)
+    n = ctypes.c_void_p(0)
+    f = h.pe_interp_mmse_apply_demap
+    # (W, X, sigma2, Y, H, C, noise_inv, flags, llr, nBlk, nIn, nSym, nAnt, q, interp_prec, prec, flags, thr, stream)
+    ok = (a, a, a, a, a, a, a, a, a, 2, 96, 11, 64, 4, 2, 1, 1, 0.0, None)
+    assert f(n, n, n, n, n, n, n, n, n, 0, 96, 11, 64, 4, 2, 1, 1, 0.0, None) == 0   # empty
+    assert f(*ok[:9], 2, 129, 11, 64, 4, 2, 1, 1, 0.0, None) == 1                      # nIn > 128
+    assert f(*ok[:9], 2, 96, 11, 16, 4, 2, 1, 1, 0.0, None) == 2                       # 16 antennas
+    assert f(*ok[:9], 2, 96, 11, 66, 4, 2, 1, 1, 0.0, None) == 2                       # not a multiple of 4
+    assert f(*ok[:9], 2, 96, 11, 64, 4, 0, 1, 1, 0.0, None) == 2                       # FP32 interpolation
+    assert f(*ok[:9], 2, 96, 11, 64, 4, 2, 0, 1, 0.0, None) == 2                       # FP32 apply
+    assert f(*ok[:9], 2, 96, 11, 64, 5, 2, 1, 1, 0.0, None) == 1                       # nPamBits
+    assert f(*ok[:9], 2, 96, 11, 64, 4, 2, 1, 8, 0.0, None) == 1                       # unknown flag bit
+    assert f(n, *ok[1:], ) == 1                                                       # null W
+    assert f(a, a, a, a, ctypes.c_void_p(264), *ok[5:]) == 1                         # H not 16-byte aligned
