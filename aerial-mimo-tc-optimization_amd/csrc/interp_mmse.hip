@@ -35,9 +35,23 @@ constexpr int kIsM = 64;   // output subcarriers per block (4 waves x 16)
 constexpr int kIsNC = 4;   // layers (MFMA tile columns per item)
 constexpr int kIsIPT = 4;  // items (antennas) per tile
 
+// LDS of a workgroup: region 0 holds the two X tile buffers (and a spare word) during the tile loop, then the 64
+// ok-flag slots of the solve; without SPLIT, W follows as bf16 pairs [64][KS 16 + 4] (the pad spreads the rows
+// over the LDS banks), read there for the MFMA B operands instead of being held in registers (48 fewer VGPRs at 96
+// pilots, which lets four waves share a SIMD).
 template <int KS, bool SPLIT>
 __host__ __device__ constexpr int is_buf_bytes() {
     return 2 * (SPLIT ? 2 : 1) * KS * 64 * 16 + 16;  // two tile buffers and one spare word
+}
+template <int KS, bool SPLIT>
+__host__ __device__ constexpr int is_region0_bytes() {
+    return is_buf_bytes<KS, SPLIT>() > kIsM * Dim<kIsNC>::SLOT * 4 ? is_buf_bytes<KS, SPLIT>()
+                                                                 : kIsM * Dim<kIsNC>::SLOT * 4;
+}
+template <int KS> __host__ __device__ constexpr int is_wpitch() { return KS * 16 + 4; }
+template <int KS, bool SPLIT>
+__host__ __device__ constexpr int is_lds_bytes() {
+    return is_region0_bytes<KS, SPLIT>() + (SPLIT ? 0 : kIsM * is_wpitch<KS>() * 4);
 }
 
 template <int KS, bool SPLIT, int P>
@@ -60,7 +74,9 @@ __global__ __launch_bounds__(256) void interp_solve_kernel(const float2 *__restr
     vec *bufv = reinterpret_cast<vec *>(is_lds);
     uint32_t *bufw = reinterpret_cast<uint32_t *>(is_lds);
     constexpr int sink = 2 * TILEV * 4;       // the spare word: target of values past the tile
-    float *slots = reinterpret_cast<float *>(is_lds + is_buf_bytes<KS, SPLIT>());  // [64][SLOT]: the ok flags
+    constexpr int KP = is_wpitch<KS>();
+    float *slots = reinterpret_cast<float *>(is_lds);  // [64][SLOT]: the ok flags, after the tile loop
+    uint32_t *wl = reinterpret_cast<uint32_t *>(is_lds + is_region0_bytes<KS, SPLIT>());  // [64][KP] W (no SPLIT)
 
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const int g = lane >> 4, c = lane & 15;
@@ -68,10 +84,19 @@ __global__ __launch_bounds__(256) void interp_solve_kernel(const float2 *__restr
     const long blk = blockIdx.x;
     const long sc = blk * kIsM + m;
 
-    // B fragments of this wave's 16 subcarriers (zero past K), as in the interpolation kernel
-    vec br[LV][KS], bi[LV][KS];
+    // B fragments of this wave's 16 subcarriers (zero past K), as in the interpolation kernel: in registers with
+    // SPLIT, else formed per tile from W's bf16 pairs in LDS (the same bits: bf16(-x) = -bf16(x))
+    constexpr int KSR = SPLIT ? KS : 1;
+    vec br[LV][KSR], bi[LV][KSR];
+    if constexpr (!SPLIT) {
+        for (int i = tid; i < kIsM * KP; i += 256) {
+            const int mm = i / KP, k = i - mm * KP;
+            const float2 wv = k < K ? W[(size_t)mm * K + k] : make_float2(0.f, 0.f);
+            wl[i] = mfma::pack2<true>(wv.x, wv.y);
+        }
+    }
 #pragma unroll
-    for (int t = 0; t < KS; ++t) {
+    for (int t = 0; t < KSR && SPLIT; ++t) {
         float vr[8], vi[8];
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
@@ -162,9 +187,18 @@ __global__ __launch_bounds__(256) void interp_solve_kernel(const float2 *__restr
                     ai = Op::mfma(al, bi[0][t], ai);
                     ar = Op::mfma(ah, br[1][t], ar);
                     ai = Op::mfma(ah, bi[1][t], ai);
+                    ar = Op::mfma(ah, br[0][t], ar);
+                    ai = Op::mfma(ah, bi[0][t], ai);
+                } else {
+                    // pilots 16 t + 4 g .. + 3 of subcarrier m: words (bf16 re | bf16 im << 16); the B operands
+                    // are (re, -im) and (im, re) per pilot
+                    typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+                    const u32x4 w4 = *reinterpret_cast<const u32x4 *>(wl + m * KP + 16 * t + 4 * g);
+                    const u32x4 r4 = w4 ^ 0x80000000u;
+                    const u32x4 i4 = (w4 >> 16) | (w4 << 16);
+                    ar = Op::mfma(ah, __builtin_bit_cast(vec, r4), ar);
+                    ai = Op::mfma(ah, __builtin_bit_cast(vec, i4), ai);
                 }
-                ar = Op::mfma(ah, br[0][t], ar);
-                ai = Op::mfma(ah, bi[0][t], ai);
             }
             if (more) stage(cur ^ 1);
             // D rows 4 g .. 4 g + 3 of column m: item g's layers 0..3 = antenna 4 k + g's row of H at subcarrier sc
@@ -256,7 +290,7 @@ __global__ __launch_bounds__(256) void interp_solve_kernel(const float2 *__restr
 template <int KS, bool SPLIT>
 hipError_t launch_ks(const float2 *W, const float2 *X, const float *s2, float2 *H, float2 *C, float *ni,
                      uint8_t *flags, int nBlk, int K, int A, int P, bool unbiased, float thr, hipStream_t st) {
-    const size_t lds = (size_t)is_buf_bytes<KS, SPLIT>() + (size_t)kIsM * Dim<kIsNC>::SLOT * sizeof(float);
+    const size_t lds = (size_t)is_lds_bytes<KS, SPLIT>();
 #define PE_IS_P(PP)                                                                                                \
     case PP:                                                                                                       \
         hipLaunchKernelGGL((interp_solve_kernel<KS, SPLIT, PP>), dim3((unsigned)nBlk), dim3(256), lds, st, W, X, s2, \

@@ -554,6 +554,84 @@ FI_M, FI_K, FI_NC = 64, 96, 4           # the frequency-interpolation GEMM the r
 FI_BATCH = 14 * 64 * 64                  # 14 symbols x 64 antennas x 64 UE allocations
 
 
+def interp_chain_bench(dev, steps=40, nblk=792):
+    """DMRS estimates to LLRs for the 32-cell batch at 4 layers (50,688 subcarriers = 792 blocks of 64, 64
+    antennas, 96 pilots per block, 11 data symbols, 256-QAM, bf16 interpolation, f16 apply), two buffer sets
+    alternating: pe_interp_mmse_apply_demap (interpolation fused with the MMSE solve, H written once) against
+    pe_freq_interp + the re-layout of H_f into the equalizer's H[sc][ant][layer] + pe_mmse_apply_demap, and each
+    of those three alone. Y = H x + n with x uniform 256-QAM at 20 dB."""
+    import torch
+    import pusch_eq as pe
+    A, K, S, q = 64, 96, 11, 4
+    nsc = 64 * nblk
+    g = torch.Generator(device=dev)
+    g.manual_seed(4096)
+    W = torch.randn((64, K), dtype=torch.complex64, device=dev, generator=g) / K ** 0.5
+    s2v = 10.0 ** (-SNR_DB / 10.0)
+    sets = []
+    for k in range(2):
+        X = torch.randn((nblk, A, K, 4), dtype=torch.complex64, device=dev, generator=g) / A ** 0.5
+        s2 = torch.full((nsc,), s2v, dtype=torch.float32, device=dev)
+        H = pe.freq_interp(W, X.reshape(nblk * A, K, 4), prec="bf16").reshape(nblk, A, 64, 4)
+        H = H.permute(0, 2, 1, 3).reshape(nsc, A, 4).contiguous()
+        bits = torch.randint(0, 2, (nsc, 4, S, 2 * q), device=dev, generator=g, dtype=torch.uint8)
+        Y = torch.bmm(H, qam_map_torch(bits, q))
+        Y = (Y + torch.randn(Y.shape, dtype=torch.complex64, device=dev, generator=g) * s2v ** 0.5)
+        Y = Y.transpose(1, 2).contiguous()
+        sets.append({"X": X, "s2": s2, "Y": Y, "H": torch.empty_like(H), "Hf": torch.empty((nblk * A, 64, 4),
+                     dtype=torch.complex64, device=dev), "Hl": torch.empty_like(H),
+                     "C": torch.empty((nsc, 4, A), dtype=torch.complex64, device=dev),
+                     "ni": torch.empty((nsc, 4), dtype=torch.float32, device=dev),
+                     "fl": torch.empty((nsc,), dtype=torch.uint8, device=dev),
+                     "llr": torch.empty((nsc, 4, S, 2 * q), dtype=torch.float32, device=dev),
+                     "llr2": torch.empty((nsc, 4, S, 2 * q), dtype=torch.float32, device=dev)})
+        del H, bits
+
+    def fused(b):
+        pe.interp_mmse_apply_demap(W, b["X"], b["s2"], b["Y"], q, H=b["H"], C=b["C"], noise_inv=b["ni"],
+                                   flags=b["fl"], out=b["llr"])
+
+    def interp(b):
+        pe.freq_interp(W, b["X"].reshape(nblk * A, K, 4), prec="bf16", out=b["Hf"])
+
+    def relayout(b):
+        b["Hl"].view(nblk, 64, A, 4).copy_(b["Hf"].view(nblk, A, 64, 4).permute(0, 2, 1, 3))
+
+    def mmse(b):
+        pe.mmse_apply_demap(b["Hl"], b["s2"], b["Y"], q, C=b["C"], noise_inv=b["ni"], flags=b["fl"], out=b["llr2"])
+
+    def separate(b):
+        interp(b)
+        relayout(b)
+        mmse(b)
+
+    bufs = [(b,) for b in sets]
+    t = {}
+    for name, fn in (("fused", fused), ("separate", separate), ("interp", interp), ("relayout", relayout),
+                     ("mmse_apply_demap", mmse)):
+        el, _ = time_launches(fn, bufs, steps, 5)
+        t[name] = el / steps * 1e6
+    fused(sets[0])
+    separate(sets[0])
+    torch.cuda.synchronize()
+    identical = bool(torch.equal(sets[0]["llr"], sets[0]["llr2"]) and torch.equal(sets[0]["H"], sets[0]["Hl"]))
+    rec = 4 * (4 + 2) * 4  # solve record bytes at 4 layers
+    b_x, b_h, b_y = nblk * A * K * 4 * 8, nsc * A * 4 * 8, nsc * S * A * 8
+    b_llr, b_ni = nsc * 4 * S * 2 * q * 4, nsc * (4 * 4 + 1)
+    bytes_fused = b_x + b_h + nsc * rec + b_ni + (b_h + nsc * rec + b_y + b_llr + b_ni)
+    bytes_sep = b_x + b_h + 2 * b_h + (b_h + nsc * rec + b_ni) + (b_h + nsc * rec + b_y + b_llr + b_ni)
+    return {"workload": "%d blocks of 64 subcarriers (%d), 64 antennas, 96 pilots, 4 layers, 11 symbols, 256-QAM at "
+                        "20 dB; bf16 interpolation, f16 apply; two buffer sets alternating" % (nblk, nsc),
+            "us_fused": t["fused"], "us_separate": t["separate"], "us_interp": t["interp"],
+            "us_relayout": t["relayout"], "us_mmse_apply_demap": t["mmse_apply_demap"],
+            "speedup_vs_separate": t["separate"] / t["fused"],
+            "speedup_vs_separate_without_relayout": (t["interp"] + t["mmse_apply_demap"]) / t["fused"],
+            "bytes_fused": bytes_fused, "bytes_separate": bytes_sep,
+            "hbm_gbs_fused": bytes_fused / (t["fused"] * 1e-6) / 1e9,
+            "subcarriers_per_s_fused": nsc / (t["fused"] * 1e-6),
+            "llrs_and_h_identical": identical}
+
+
 def freq_interp_bench(dev, steps=50, cpu_seconds=2.0, precs=("bf16", "bf16_split", "f32")):
     """pe_freq_interp over FI_BATCH items of [96 pilots x 4 ports] through one [64 x 96] filter, per precision:
     mean of `steps` launches rotating over 2 buffer sets (586 MB together, beyond the Infinity Cache). Bytes per
@@ -835,6 +913,8 @@ def main():
     ap.add_argument("--serving-only", action="store_true", help="only the serving-loop section, printed as JSON")
     ap.add_argument("--codeword-only", action="store_true",
                     help="only the descrambling / codeword-layout launches over the first 32-cell batch, for profiling")
+    ap.add_argument("--interp-chain-only", action="store_true",
+                    help="only the DMRS-estimates-to-LLRs chain (fused interpolation + MMSE against separate calls)")
     ap.add_argument("--ceiling-only", action="store_true",
                     help="only the measured streaming ceilings (pe_hbm_probe), printed as JSON")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -860,6 +940,10 @@ def main():
     pe.init(0)
     if args.ceiling_only:
         print(json.dumps(hbm_ceiling(dev)), flush=True)
+        return
+    if args.interp_chain_only:
+        print(json.dumps(interp_chain_bench(dev)), flush=True)
+        print(json.dumps(interp_chain_bench(dev, nblk=768)), flush=True)
         return
     if args.serving_only:
         print(json.dumps(serving_bench(dev, args.serve_jobs)), flush=True)
@@ -1090,6 +1174,13 @@ def main():
         operators["gram_f32"]["hbm_gbs"], operators["llr_demap_f16"]["hbm_gbs"], operators["llr_demap_f32"]["hbm_gbs"]))
     log("fronthaul decompress %.1f GB/s" % fronthaul["hbm_gbs"])
     freq_interp = freq_interp_bench(dev)
+    interp_chain = interp_chain_bench(dev)
+    interp_chain["blocks_768"] = interp_chain_bench(dev, nblk=768)  # 3 blocks per compute unit: no partial round
+    log("DMRS estimates to LLRs: fused %.1f us, separate calls %.1f us (interpolation %.1f, re-layout %.1f, "
+        "MMSE + apply %.1f; identical: %s)" % (interp_chain["us_fused"], interp_chain["us_separate"],
+                                                interp_chain["us_interp"], interp_chain["us_relayout"],
+                                                interp_chain["us_mmse_apply_demap"],
+                                                interp_chain["llrs_and_h_identical"]))
     log("frequency interpolation: bf16 %.1f GB/s, bf16 split %.1f GB/s, f32 %.1f GB/s (%.1f TFLOP/s)" % (
         freq_interp["bf16"]["hbm_gbs"], freq_interp["bf16_split"]["hbm_gbs"], freq_interp["f32"]["hbm_gbs"],
         freq_interp["f32"]["tflops"]))
@@ -1175,6 +1266,7 @@ def main():
         "serving": serving,
         "srs_filter": srs,
         "fronthaul_bfp_decompress": fronthaul,
+        "interp_mmse_chain": interp_chain,
         "freq_interp": freq_interp,
         "standalone_operators": operators,
         "deployment_sweep": sweep,
