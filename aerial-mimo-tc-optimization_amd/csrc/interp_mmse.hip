@@ -54,8 +54,13 @@ __host__ __device__ constexpr int is_lds_bytes() {
     return is_region0_bytes<KS, SPLIT>() + (SPLIT ? 0 : kIsM * is_wpitch<KS>() * 4);
 }
 
+// Without SPLIT and up to 112 pilots the kernel is held to 128 VGPRs (four waves per SIMD: four workgroups per
+// compute unit, as many as the LDS allows; 124 at 96 pilots, no spills); the SPLIT operands need more registers.
+template <int KS, bool SPLIT>
+__host__ __device__ constexpr int is_min_waves() { return !SPLIT && KS <= 7 ? 4 : 1; }
+
 template <int KS, bool SPLIT, int P>
-__global__ __launch_bounds__(256) void interp_solve_kernel(const float2 *__restrict__ W, const float2 *__restrict__ X,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(is_min_waves<KS, SPLIT>()))) void interp_solve_kernel(const float2 *__restrict__ W, const float2 *__restrict__ X,
                                                            const float *__restrict__ sigma2, float2 *__restrict__ H,
                                                            float2 *__restrict__ C, float *__restrict__ noise_inv,
                                                            uint8_t *__restrict__ flags, int K, int A, int unbiased,
@@ -200,7 +205,6 @@ __global__ __launch_bounds__(256) void interp_solve_kernel(const float2 *__restr
                     ai = Op::mfma(ah, __builtin_bit_cast(vec, i4), ai);
                 }
             }
-            if (more) stage(cur ^ 1);
             // D rows 4 g .. 4 g + 3 of column m: item g's layers 0..3 = antenna 4 k + g's row of H at subcarrier sc
             const float vr[L] = {ar[0], ar[1], ar[2], ar[3]};
             const float vi[L] = {ai[0], ai[1], ai[2], ai[3]};
@@ -214,6 +218,8 @@ __global__ __launch_bounds__(256) void interp_solve_kernel(const float2 *__restr
                     pr[u][tri(i, j)] = fma_(vr[i], vr[j], fma_(vi[i], vi[j], pr[u][tri(i, j)]));
                     if (j < i) pim[u][tri(i, j)] = fma_(vr[i], vi[j], fma_(-vi[i], vr[j], pim[u][tri(i, j)]));
                 }
+            // the next tile is staged after this tile's H stores and Gram updates, which so overlap its loads
+            if (more) stage(cur ^ 1);
             __syncthreads();  // next buffer written; every wave done reading this one
             cur ^= 1;
         }
