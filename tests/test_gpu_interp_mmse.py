@@ -3,7 +3,7 @@ Its H must be bit-identical to pe_freq_interp's output at the same interpolation
 H[sc][ant][layer]), and its LLRs, noise_inv, flags and workspace C bit-identical to pe_mmse_apply_demap on that H
 (the Gram matrices are summed in the same order). Covers both interpolation precisions, the MFMA apply
 precisions, 32-64 antennas (8 and 16 lanes per subcarrier from the tile table, 4 through a table of the test's
-own), pilot counts that are and are not multiples of 16, binary16 LLRs, PE_EQ_WRITE_C, the FP64 fallback (forced
+own), pilot counts that are and are not multiples of 16, the full 32-cell batch (792 blocks), binary16 LLRs, PE_EQ_WRITE_C, the FP64 fallback (forced
 by a low condition threshold), and H against a complex128 product of the bf16-rounded operands."""
 import os
 
@@ -52,6 +52,7 @@ def _check(pe, dev, nblk, A, K, S, q, interp, prec, thr=0.0, llr_f16=False, writ
     (2, 36, 128, 14, 4, "bf16", "f16_split"),  # 9 tiles of 4 antennas; the largest pilot count
     (1, 64, 17, 1, 1, "bf16_split", "bf16_split"),
     (37, 64, 96, 11, 4, "bf16", "f16"),
+    (792, 64, 96, 11, 4, "bf16", "f16"),      # the 32-cell batch: 50,688 subcarriers, 792 workgroups
 ])
 def test_fused_equals_interp_then_mmse_apply(pe, dev, nblk, A, K, S, q, interp, prec):
     _check(pe, dev, nblk, A, K, S, q, interp, prec)
