@@ -3,8 +3,9 @@ Its H must be bit-identical to pe_freq_interp's output at the same interpolation
 H[sc][ant][layer]), and its LLRs, noise_inv, flags and workspace C bit-identical to pe_mmse_apply_demap on that H
 (the Gram matrices are summed in the same order). Covers both interpolation precisions, the MFMA apply
 precisions, 32-64 antennas (8 and 16 lanes per subcarrier from the tile table, 4 through a table of the test's
-own), pilot counts that are and are not multiples of 16, the full 32-cell batch (792 blocks), binary16 LLRs, PE_EQ_WRITE_C, the FP64 fallback (forced
-by a low condition threshold), and H against a complex128 product of the bf16-rounded operands."""
+own), pilot counts that are and are not multiples of 16, the full 32-cell batch (792 blocks), binary16 LLRs,
+PE_EQ_WRITE_C, the FP64 fallback (forced by a low condition threshold), and H against a complex128 product of the
+bf16-rounded operands."""
 import os
 
 import pytest
